@@ -1,0 +1,155 @@
+"""ctypes bindings of the native libraries.
+
+* ``host()``  -> libsofi_host.so (setup from the reference's input files, writers)
+* ``hip()``   -> libsofi_hip.so  (the time loop on the GPU; raises if absent)
+
+Both live in ``gpi_project_amd/_lib`` and are built by ``gpi_project_amd.build``.
+There is no fallback: a missing or unloadable HIP library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib")
+
+NARRAYS = 39        # enum sofi_array
+NPROFILES = 15      # enum sofi_profile
+NCOMP = 5
+COMP_NAMES = ("vx", "vy", "p", "div", "curl")
+WEQ_NAMES = ("AC_ISO", "VAC_ISO", "EL_ISO", "VEL_ISO", "EL_VTI", "VEL_VTI", "EL_TTI", "VEL_TTI")
+PAD = 8
+
+
+class SofiDesc(C.Structure):
+    """Mirror of ``sofi_desc`` (include/sofi_types.h)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("fdorder", C.c_int), ("weq", C.c_int), ("L", C.c_int),
+        ("nt", C.c_int), ("dt", C.c_float), ("dh", C.c_float), ("hc", C.c_float * 7),
+        ("fw", C.c_int), ("abs_type", C.c_int), ("free_surf", C.c_int),
+        ("gx", C.c_int * 5), ("gy", C.c_int * 5), ("source_type", C.c_int),
+        ("seismo", C.c_int), ("ndt", C.c_int), ("ns", C.c_int), ("snap", C.c_int),
+        ("snap_idx", C.c_int * 5), ("idx", C.c_int), ("idy", C.c_int),
+        ("lsnap", C.c_int), ("isnap", C.c_int), ("esnap", C.c_int), ("mode", C.c_int),
+        ("dtinv", C.c_int), ("velocity", C.c_int), ("adjoint_type", C.c_int),
+    ]
+
+    def copy(self) -> "SofiDesc":
+        d = SofiDesc()
+        C.memmove(C.byref(d), C.byref(self), C.sizeof(SofiDesc))
+        return d
+
+
+def pitch(nx: int) -> int:
+    return ((nx + 2 * PAD + 63) // 64) * 64
+
+
+def rows(ny: int) -> int:
+    return ny + 2 * PAD
+
+
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+_VP = C.c_void_p
+
+_host = None
+_hip = None
+
+
+def _load(name):
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run `python -m gpi_project_amd.build` (or __graft_entry__.build())")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+def host():
+    global _host
+    if _host is not None:
+        return _host
+    L = _load("libsofi_host.so")
+    L.sofi_host_last_error.restype = C.c_char_p
+    L.sofi_problem_load.restype = _VP
+    L.sofi_problem_load.argtypes = [C.c_char_p]
+    L.sofi_problem_free.argtypes = [_VP]
+    for fn in ("nshots", "nsrc", "source_type", "snapcheck", "ntr", "nsnap"):
+        f = getattr(L, "sofi_problem_" + fn)
+        f.restype = C.c_int
+        f.argtypes = [_VP]
+    L.sofi_problem_set_shot.argtypes = [_VP, C.c_int]
+    L.sofi_problem_desc.restype = C.POINTER(SofiDesc)
+    L.sofi_problem_desc.argtypes = [_VP]
+    L.sofi_problem_params.restype = _VP
+    L.sofi_problem_params.argtypes = [_VP]
+    for fn in ("array", "profile", "model"):
+        f = getattr(L, "sofi_problem_" + fn)
+        f.restype = _FP
+        f.argtypes = [_VP, C.c_int]
+    L.sofi_problem_arrays.restype = C.POINTER(_FP)
+    L.sofi_problem_arrays.argtypes = [_VP]
+    L.sofi_problem_profiles.restype = C.POINTER(_FP)
+    L.sofi_problem_profiles.argtypes = [_VP]
+    for fn in ("src_i", "src_j", "src_type", "rec_i", "rec_j"):
+        f = getattr(L, "sofi_problem_" + fn)
+        f.restype = _IP
+        f.argtypes = [_VP]
+    for fn in ("src_azi", "src_amp", "signals"):
+        f = getattr(L, "sofi_problem_" + fn)
+        f.restype = _FP
+        f.argtypes = [_VP]
+    L.sofi_problem_write_section.argtypes = [_VP, C.c_int, _FP]
+    L.sofi_problem_write_signal.argtypes = [_VP]
+    L.sofi_problem_write_snapshot.argtypes = [_VP, C.c_int, C.c_int, _FP, C.c_int]
+    L.sofi_problem_seismo_comps.argtypes = [_VP, _IP]
+    L.sofi_problem_snap_comps.argtypes = [_VP, _IP]
+    L.sofi_holberg.argtypes = [C.c_int, C.c_int, _FP]
+    L.sofi_outseis.argtypes = [C.c_char_p, _FP, _IP, C.c_int, _FP, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_float, C.c_float, C.c_int, _FP]
+    _host = L
+    return L
+
+
+def hip():
+    """The GPU library.  Raises when it is missing: there is no CPU path."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    L = _load("libsofi_hip.so")
+    L.sofi_last_error.restype = C.c_char_p
+    L.sofi_open.argtypes = [C.POINTER(SofiDesc), C.POINTER(_VP)]
+    L.sofi_set_medium.argtypes = [_VP, C.POINTER(_FP), C.POINTER(_FP)]
+    L.sofi_set_sources.argtypes = [_VP, C.c_int, _IP, _IP, _IP, _FP, _FP, _FP, C.c_int]
+    L.sofi_set_receivers.argtypes = [_VP, C.c_int, _IP, _IP]
+    L.sofi_set_snapshots.argtypes = [_VP, C.c_int]
+    L.sofi_reset.argtypes = [_VP]
+    L.sofi_run.argtypes = [_VP, C.c_int, C.c_int]
+    L.sofi_last_run_ms.restype = C.c_double
+    L.sofi_last_run_ms.argtypes = [_VP]
+    L.sofi_get_section.argtypes = [_VP, C.c_int, _FP]
+    L.sofi_snapshot_count.argtypes = [_VP]
+    L.sofi_get_snapshot.argtypes = [_VP, C.c_int, C.c_int, _FP]
+    L.sofi_get_field.argtypes = [_VP, C.c_int, _FP]
+    L.sofi_close.argtypes = [_VP]
+    L.sofi_footprint.restype = C.c_size_t
+    L.sofi_footprint.argtypes = [_VP]
+    _hip = L
+    return L
+
+
+def host_error() -> str:
+    return host().sofi_host_last_error().decode()
+
+
+def hip_error() -> str:
+    return hip().sofi_last_error().decode()
+
+
+def exported_symbols():
+    """Names each library must export (declared in include/*.h)."""
+    return {
+        "libsofi_hip.so": ["sofi_last_error", "sofi_open", "sofi_set_medium", "sofi_set_sources",
+                           "sofi_set_receivers", "sofi_set_snapshots", "sofi_reset", "sofi_run",
+                           "sofi_last_run_ms", "sofi_get_section", "sofi_snapshot_count",
+                           "sofi_get_snapshot", "sofi_get_field", "sofi_close", "sofi_footprint"],
+    }

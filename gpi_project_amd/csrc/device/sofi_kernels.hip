@@ -1,0 +1,1256 @@
+// sofi_kernels.hip - SOFI2D velocity-stress time loop on CDNA4 (gfx950).
+//
+// One handle = one shot's state resident in HBM, one HIP stream.  A time
+// step is (src/time_loop.c:42-322):
+//   force injection -> velocity update (interior + CPML/damping frame fused)
+//   -> stress update (v-derivatives computed in-kernel, CPML/damping fused)
+//   -> [div/curl sampling] -> explosive/moment source -> free surface
+//   -> [vx/vy/p sampling] -> [snapshot]
+// The reference's exchange_v/exchange_s and the full-grid derivative arrays
+// of v_derivatives disappear for ISO/VTI: every stress cell recomputes its
+// four velocity derivatives from vx/vy.  TTI needs neighbouring derivatives
+// after the frame CPML has modified them in place, so it keeps the four
+// derivative arrays and the reference's pass order (interior, frame CPML,
+// frame update).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sofi_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+int fail(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return -1;
+}
+
+#define HIPCHK(x)                                                                                     \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return fail("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// ------------------------------------------------------------------ params
+
+struct Grid {
+    int nx, ny, pitch, fw, abs_type, free_surf, weq, L, nd;
+    int gx2, gx3, gy2, gy3;
+    int plane;      // floats per padded plane
+    int wx, wy;     // row lengths of x-frame and y-frame CPML memory
+    float dt, dh, dtdh, dhi, dthalbe;
+    float hc[7];
+};
+
+struct Fields {
+    float *vx, *vy, *sxx, *syy, *sxy;
+    float *r, *p, *q;                          // L planes each
+    float *dvxx, *dvyy, *dvyx, *dvxy;           // TTI only
+    float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
+    float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
+};
+
+struct Mat {
+    const float *rip, *rjp;
+    const float *f, *g, *fipjp;                 // EL_ISO / VEL_ISO
+    const float *bd, *be, *br;                  // VEL_ISO defect bases: pu*taus, ppi*taup, puipjp*tausipjp
+    const float *c11, *c13, *c33, *c15, *c35, *c55i, *c15i, *c35i;   // elastic VTI/TTI (DT-scaled)
+    const float *u11, *u13, *u33, *u15, *u35, *u55i, *u15i, *u35i;   // visco VTI/TTI unrelaxed
+    const float *d11, *d13, *d33, *d15, *d35, *d55i, *d15i, *d35i;   // visco VTI/TTI defects (L planes)
+    const float *absorb;
+    const float *pu, *ppi, *ptaus, *ptaup;      // free surface, snapshots
+    const float *prof;                          // CPML profiles, 12 x (2FW+1)
+    float eta[SOFI_MAXL + 1], bip[SOFI_MAXL + 1], cip[SOFI_MAXL + 1];
+};
+
+__device__ __forceinline__ int IDX(const Grid &g, int i, int j) { return (j - 1 + SOFI_PAD) * g.pitch + (i - 1 + SOFI_PAD); }
+
+// CPML profile k (enum sofi_profile) at index h
+__device__ __forceinline__ float PRF(const Grid &g, const Mat &m, int k, int h) { return m.prof[k * (2 * g.fw + 1) + h]; }
+
+__device__ __forceinline__ bool in_xframe(const Grid &g, int i) { return i <= g.gx2 || i > g.gx3; }
+__device__ __forceinline__ bool in_yframe(const Grid &g, int j) { return j <= g.gy2 || j > g.gy3; }
+__device__ __forceinline__ int hx(const Grid &g, int i) { return i <= g.gx2 ? i : i - g.nx + 2 * g.fw; }
+__device__ __forceinline__ int hy(const Grid &g, int j) { return j <= g.gy2 ? j : j - g.ny + 2 * g.fw; }
+
+// ------------------------------------------------------------ FD stencils
+
+// op_v_fd* (src/initfd.c:97-183): stress derivatives, no 1/DH
+template <int ND>
+__device__ __forceinline__ void op_v(const Grid &g, const float *__restrict__ sxx, const float *__restrict__ syy,
+                                     const float *__restrict__ sxy, int c, float &sxx_x, float &sxy_x,
+                                     float &sxy_y, float &syy_y)
+{
+    const int P = g.pitch;
+    float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
+#pragma unroll
+    for (int m = 1; m <= ND; ++m) {
+        const float h = g.hc[m];
+        a += h * (sxx[c + m] - sxx[c - m + 1]);
+        b += h * (sxy[c + m - 1] - sxy[c - m]);
+        d += h * (sxy[c + (m - 1) * P] - sxy[c - m * P]);
+        e += h * (syy[c + m * P] - syy[c - (m - 1) * P]);
+    }
+    sxx_x = a;
+    sxy_x = b;
+    sxy_y = d;
+    syy_y = e;
+}
+
+// op_s_fd* (src/initfd.c:9-95): velocity derivatives times 1/DH
+template <int ND>
+__device__ __forceinline__ void op_s(const Grid &g, const float *__restrict__ vx, const float *__restrict__ vy, int c,
+                                     float &vxx, float &vyx, float &vxy, float &vyy)
+{
+    const int P = g.pitch;
+    float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
+#pragma unroll
+    for (int m = 1; m <= ND; ++m) {
+        const float h = g.hc[m];
+        a += h * (vx[c + m - 1] - vx[c - m]);
+        b += h * (vy[c + m] - vy[c - m + 1]);
+        d += h * (vx[c + m * P] - vx[c - (m - 1) * P]);
+        e += h * (vy[c + (m - 1) * P] - vy[c - m * P]);
+    }
+    vxx = a * g.dhi;
+    vyx = b * g.dhi;
+    vxy = d * g.dhi;
+    vyy = e * g.dhi;
+}
+
+// CPML on the stress-side derivatives of one frame cell (cpml_update_s_x/_y,
+// src/cpml_update.c:30-46); updates the psi memory and returns modified values.
+__device__ __forceinline__ void cpml_dv(const Grid &g, const Mat &m, Fields &f, int i, int j, bool xf, bool yf,
+                                        float &vxx, float &vyx, float &vxy, float &vyy)
+{
+    if (xf) {
+        const int h = hx(g, i);
+        const int o = j * g.wx + h;
+        float px = PRF(g, m, SOFI_P_B_X, h) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, h) * vxx;
+        f.psi_vxx[o] = px;
+        vxx = vxx / PRF(g, m, SOFI_P_K_X, h) + px;
+        float py = PRF(g, m, SOFI_P_B_X_HALF, h) * f.psi_vyx[o] + PRF(g, m, SOFI_P_A_X_HALF, h) * vyx;
+        f.psi_vyx[o] = py;
+        vyx = vyx / PRF(g, m, SOFI_P_K_X_HALF, h) + py;
+    }
+    if (yf) {
+        const int h = hy(g, j);
+        const int o = h * g.wy + i;
+        float py = PRF(g, m, SOFI_P_B_Y, h) * f.psi_vyy[o] + PRF(g, m, SOFI_P_A_Y, h) * vyy;
+        f.psi_vyy[o] = py;
+        vyy = vyy / PRF(g, m, SOFI_P_K_Y, h) + py;
+        float px = PRF(g, m, SOFI_P_B_Y_HALF, h) * f.psi_vxy[o] + PRF(g, m, SOFI_P_A_Y_HALF, h) * vxy;
+        f.psi_vxy[o] = px;
+        vxy = vxy / PRF(g, m, SOFI_P_K_Y_HALF, h) + px;
+    }
+}
+
+// read-only version: the modified derivative from the psi value already
+// written this step (used by the div/curl sampler)
+__device__ __forceinline__ void cpml_dv_read(const Grid &g, const Mat &m, const Fields &f, int i, int j, bool xf,
+                                             bool yf, float &vxx, float &vyx, float &vxy, float &vyy)
+{
+    if (xf) {
+        const int h = hx(g, i);
+        const int o = j * g.wx + h;
+        vxx = vxx / PRF(g, m, SOFI_P_K_X, h) + f.psi_vxx[o];
+        vyx = vyx / PRF(g, m, SOFI_P_K_X_HALF, h) + f.psi_vyx[o];
+    }
+    if (yf) {
+        const int h = hy(g, j);
+        const int o = h * g.wy + i;
+        vyy = vyy / PRF(g, m, SOFI_P_K_Y, h) + f.psi_vyy[o];
+        vxy = vxy / PRF(g, m, SOFI_P_K_Y_HALF, h) + f.psi_vxy[o];
+    }
+}
+
+// ---------------------------------------------------------- velocity kernel
+
+constexpr int BX = 64, BY = 4;
+
+// update_v_interior + update_v_PML/update_v_abs + wavefield_update_v
+// (src/update_v_interior.c:92-98, src/update_v_PML.c, src/update_v_abs.c,
+// src/wavefield_update_v.c:46-47), one thread per cell.
+template <int ND>
+__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
+    if (frame && g.abs_type != 1 && g.abs_type != 2) return;
+    const int c = IDX(g, i, j);
+    float sxx_x, sxy_x, sxy_y, syy_y;
+    op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
+    if (g.abs_type == 1 && frame) {
+        // cpml_update_v_x / _y (src/cpml_update.c:50-66)
+        if (xf) {
+            const int h = hx(g, i), o = j * g.wx + h;
+            float a = PRF(g, m, SOFI_P_B_X_HALF, h) * f.psi_sxx_x[o] + PRF(g, m, SOFI_P_A_X_HALF, h) * sxx_x;
+            f.psi_sxx_x[o] = a;
+            sxx_x = sxx_x / PRF(g, m, SOFI_P_K_X_HALF, h) + a;
+            float b = PRF(g, m, SOFI_P_B_X, h) * f.psi_sxy_x[o] + PRF(g, m, SOFI_P_A_X, h) * sxy_x;
+            f.psi_sxy_x[o] = b;
+            sxy_x = sxy_x / PRF(g, m, SOFI_P_K_X, h) + b;
+        }
+        if (yf) {
+            const int h = hy(g, j), o = h * g.wy + i;
+            float a = PRF(g, m, SOFI_P_B_Y_HALF, h) * f.psi_syy_y[o] + PRF(g, m, SOFI_P_A_Y_HALF, h) * syy_y;
+            f.psi_syy_y[o] = a;
+            syy_y = syy_y / PRF(g, m, SOFI_P_K_Y_HALF, h) + a;
+            float b = PRF(g, m, SOFI_P_B_Y, h) * f.psi_sxy_y[o] + PRF(g, m, SOFI_P_A_Y, h) * sxy_y;
+            f.psi_sxy_y[o] = b;
+            sxy_y = sxy_y / PRF(g, m, SOFI_P_K_Y, h) + b;
+        }
+    }
+    float vx = f.vx[c] + (sxx_x + sxy_y) * g.dtdh * m.rip[c];
+    float vy = f.vy[c] + (sxy_x + syy_y) * g.dtdh * m.rjp[c];
+    if (g.abs_type == 2 && frame) {
+        const float a = m.absorb[c];
+        vx *= a;
+        vy *= a;
+    }
+    f.vx[c] = vx;
+    f.vy[c] = vy;
+}
+
+// --------------------------------------------------------- stress updates
+
+// per-cell rheology updates; dv are the (possibly CPML-modified) derivatives
+
+// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48)
+__device__ __forceinline__ void upd_el(const Mat &m, Fields &f, int c, float vxx, float vyx, float vxy, float vyy)
+{
+    const float u1 = m.fipjp[c] * (vyx + vxy);
+    const float gg = m.g[c], ff = 2.0f * m.f[c];
+    const float u2 = (gg * (vxx + vyy)) - (ff * vyy);
+    const float u3 = (gg * (vxx + vyy)) - (ff * vxx);
+    f.sxy[c] += u1;
+    f.sxx[c] += u2;
+    f.syy[c] += u3;
+}
+
+// wavefield_update_s_visc (src/wavefield_update_s_visc.c:27-68); the defect
+// coefficients dip/d/e[l] = eta_l * (per-cell base) are formed in registers
+__device__ __forceinline__ void upd_visc(const Grid &g, const Mat &m, Fields &f, int c, float vxx, float vyx,
+                                         float vxy, float vyy)
+{
+    float sr0 = 0.f, sp0 = 0.f, sq0 = 0.f, sr = 0.f, sp = 0.f, sq = 0.f;
+    const float br = m.br[c], bd = m.bd[c], be = m.be[c];
+    const float sxyv = vxy + vyx, sdiag = vxx + vyy;
+#pragma unroll
+    for (int l = 1; l <= SOFI_MAXL; ++l) {
+        if (l > g.L) break;
+        const int o = (l - 1) * g.plane + c;
+        const float r0 = f.r[o], p0 = f.p[o], q0 = f.q[o];
+        sr0 += r0;
+        sp0 += p0;
+        sq0 += q0;
+        const float eta = m.eta[l], bip = m.bip[l], cip = m.cip[l];
+        const float dip = eta * br, d2 = 2.0f * (eta * bd), e = eta * be;
+        const float r1 = bip * (r0 * cip - dip * sxyv);
+        const float p1 = bip * (p0 * cip - e * sdiag + d2 * vyy);
+        const float q1 = bip * (q0 * cip - e * sdiag + d2 * vxx);
+        f.r[o] = r1;
+        f.p[o] = p1;
+        f.q[o] = q1;
+        sr += r1;
+        sp += p1;
+        sq += q1;
+    }
+    const float gg = m.g[c], ff = 2.0f * m.f[c];
+    const float u1 = (m.fipjp[c] * sxyv) + (g.dthalbe * (sr0 + sr));
+    const float u2 = (gg * sdiag) - (ff * vyy) + (g.dthalbe * (sp0 + sp));
+    const float u3 = (gg * sdiag) - (ff * vxx) + (g.dthalbe * (sq0 + sq));
+    f.sxy[c] += u1;
+    f.sxx[c] += u2;
+    f.syy[c] += u3;
+}
+
+// wavefield_update_s_el_vti (src/wavefield_update_s_el_vti.c:27-47)
+__device__ __forceinline__ void upd_el_vti(const Mat &m, Fields &f, int c, float vxx, float vyx, float vxy, float vyy)
+{
+    const float u1 = m.c55i[c] * (vxy + vyx);
+    const float u2 = (m.c11[c] * vxx) + (m.c13[c] * vyy);
+    const float u3 = (m.c13[c] * vxx) + (m.c33[c] * vyy);
+    f.sxy[c] += u1;
+    f.sxx[c] += u2;
+    f.syy[c] += u3;
+}
+
+// wavefield_update_s_visc_VTI (src/wavefield_update_s_visc_VTI.c:27-66)
+__device__ __forceinline__ void upd_visc_vti(const Grid &g, const Mat &m, Fields &f, int c, float vxx, float vyx,
+                                             float vxy, float vyy)
+{
+    float sr0 = 0.f, sp0 = 0.f, sq0 = 0.f, sr = 0.f, sp = 0.f, sq = 0.f;
+    const float sxyv = vxy + vyx;
+#pragma unroll
+    for (int l = 1; l <= SOFI_MAXL; ++l) {
+        if (l > g.L) break;
+        const int o = (l - 1) * g.plane + c;
+        const float r0 = f.r[o], p0 = f.p[o], q0 = f.q[o];
+        sr0 += r0;
+        sp0 += p0;
+        sq0 += q0;
+        const float bip = m.bip[l], cip = m.cip[l];
+        const float r1 = bip * (r0 * cip - (m.d55i[o] * sxyv));
+        const float p1 = bip * (p0 * cip - (m.d11[o] * vxx) - (m.d13[o] * vyy));
+        const float q1 = bip * (q0 * cip - (m.d13[o] * vxx) - (m.d33[o] * vyy));
+        f.r[o] = r1;
+        f.p[o] = p1;
+        f.q[o] = q1;
+        sr += r1;
+        sp += p1;
+        sq += q1;
+    }
+    const float u1 = (m.u55i[c] * sxyv) + (g.dthalbe * (sr0 + sr));
+    const float u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (g.dthalbe * (sp0 + sp));
+    const float u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (g.dthalbe * (sq0 + sq));
+    f.sxy[c] += u1;
+    f.sxx[c] += u2;
+    f.syy[c] += u3;
+}
+
+// stress update for ISO/VTI rheologies: velocity derivatives recomputed in
+// the kernel (replaces v_derivatives + exchange_v), CPML applied to them in
+// the frame, damping after the update (src/update_s_*_{interior,PML,abs}.c).
+template <int ND, int WEQ>
+__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f, Mat m)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
+    if (frame && g.abs_type != 1 && g.abs_type != 2) return;
+    const int c = IDX(g, i, j);
+    float vxx, vyx, vxy, vyy;
+    op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
+    if (g.abs_type == 1 && frame) cpml_dv(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
+    if (WEQ == SOFI_EL_ISO) upd_el(m, f, c, vxx, vyx, vxy, vyy);
+    if (WEQ == SOFI_VEL_ISO) upd_visc(g, m, f, c, vxx, vyx, vxy, vyy);
+    if (WEQ == SOFI_EL_VTI) upd_el_vti(m, f, c, vxx, vyx, vxy, vyy);
+    if (WEQ == SOFI_VEL_VTI) upd_visc_vti(g, m, f, c, vxx, vyx, vxy, vyy);
+    if (g.abs_type == 2 && frame) {
+        // abs_update_s (src/abs_update.c:39-44)
+        const float a = m.absorb[c];
+        f.sxy[c] *= a;
+        f.syy[c] *= a;
+        f.sxx[c] *= a;
+    }
+}
+
+// ------------------------------------------------------------ TTI passes
+
+// v_derivatives (src/v_derivatives.c:27-50)
+template <int ND>
+__global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    float vxx, vyx, vxy, vyy;
+    op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
+    f.dvxx[c] = vxx;
+    f.dvyy[c] = vyy;
+    f.dvyx[c] = vyx;
+    f.dvxy[c] = vxy;
+}
+
+// frame CPML of update_s_*_tti_PML, in place on the derivative arrays;
+// the elastic right-bottom corner loop stops at GY[4]-1
+// (src/update_s_elastic_TTI_PML.c:132 vs src/update_s_visc_TTI_PML.c:132)
+__global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f, Mat m)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
+    if (!(xf || yf)) return;
+    if (g.weq == SOFI_EL_TTI && i > g.gx3 && j > g.gy3 && j == g.ny) return;
+    const int c = IDX(g, i, j);
+    float vxx = f.dvxx[c], vyx = f.dvyx[c], vxy = f.dvxy[c], vyy = f.dvyy[c];
+    cpml_dv(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
+    f.dvxx[c] = vxx;
+    f.dvyx[c] = vyx;
+    f.dvxy[c] = vxy;
+    f.dvyy[c] = vyy;
+}
+
+// TTI stress update (src/wavefield_update_s_el_tti.c:27-52,
+// src/wavefield_update_s_visc_TTI.c:27-84) reading the derivative arrays.
+// PASS 0: interior cells; PASS 1: CPML frame on the ranges shrunk by two
+// cells at the outer edges (src/update_s_elastic_TTI_PML.c:37-143);
+// PASS 2: damping frame, every frame cell (src/update_s_elastic_TTI_abs.c).
+template <int WEQ, int PASS>
+__global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f, Mat m)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
+    if (PASS == 0 && frame) return;
+    if (PASS != 0) {
+        if (!frame) return;
+        if (PASS == 1) {
+            if (i <= g.gx2 && i < 1 + 2) return;
+            if (i > g.gx3 && i > g.nx - 2) return;
+            if (j <= g.gy2 && j < 1 + 2) return;
+            if (j > g.gy3 && j > g.ny - 2) return;
+        }
+    }
+    const int c = IDX(g, i, j), P = g.pitch;
+    const float *X = f.dvxx, *Y = f.dvyy, *YX = f.dvyx, *XY = f.dvxy;
+    const float vxx = X[c], vyy = Y[c];
+    const float vxxipjp = 0.25f * (X[c] + X[c + P] + X[c + 1] + X[c + 1 + P]);
+    const float vyyipjp = 0.25f * (Y[c] + Y[c + P] + Y[c + 1] + Y[c + 1 + P]);
+    const float vyxij = 0.25f * (YX[c] + YX[c - P] + YX[c - 1] + YX[c - 1 - P]);
+    const float vxyij = 0.25f * (XY[c] + XY[c - P] + XY[c - 1] + XY[c - 1 - P]);
+    const float vij = vyxij + vxyij;
+    const float v = XY[c] + YX[c];
+    float u1, u2, u3;
+    if (WEQ == SOFI_EL_TTI) {
+        u1 = (m.c55i[c] * v) + (m.c15i[c] * vxxipjp) + (m.c35i[c] * vyyipjp);
+        u2 = (m.c11[c] * vxx) + (m.c13[c] * vyy) + (m.c15[c] * vij);
+        u3 = (m.c13[c] * vxx) + (m.c33[c] * vyy) + (m.c35[c] * vij);
+    } else {
+        float sr0 = 0.f, sp0 = 0.f, sq0 = 0.f, sr = 0.f, sp = 0.f, sq = 0.f;
+#pragma unroll
+        for (int l = 1; l <= SOFI_MAXL; ++l) {
+            if (l > g.L) break;
+            const int o = (l - 1) * g.plane + c;
+            const float r0 = f.r[o], p0 = f.p[o], q0 = f.q[o];
+            sr0 += r0;
+            sp0 += p0;
+            sq0 += q0;
+            const float bip = m.bip[l], cip = m.cip[l];
+            const float r1 = bip * (r0 * cip - (m.d55i[o] * v) - (m.d15i[o] * vxxipjp) - (m.d35i[o] * vyyipjp));
+            const float p1 = bip * (p0 * cip - (m.d11[o] * vxx) - (m.d13[o] * vyy) - (m.d15[o] * vij));
+            const float q1 = bip * (q0 * cip - (m.d13[o] * vxx) - (m.d33[o] * vyy) - (m.d35[o] * vij));
+            f.r[o] = r1;
+            f.p[o] = p1;
+            f.q[o] = q1;
+            sr += r1;
+            sp += p1;
+            sq += q1;
+        }
+        u1 = (m.u55i[c] * v) + (m.u15i[c] * vxxipjp) + (m.u35i[c] * vyyipjp) + (g.dthalbe * (sr0 + sr));
+        u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (m.u15[c] * vij) + (g.dthalbe * (sp0 + sp));
+        u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (m.u35[c] * vij) + (g.dthalbe * (sq0 + sq));
+    }
+    float sxy = f.sxy[c] + u1, sxx = f.sxx[c] + u2, syy = f.syy[c] + u3;
+    if (PASS == 2) {
+        const float a = m.absorb[c];
+        sxy *= a;
+        syy *= a;
+        sxx *= a;
+    }
+    f.sxy[c] = sxy;
+    f.sxx[c] = sxx;
+    f.syy[c] = syy;
+}
+
+// ------------------------------------------------------------- sources
+
+struct Src {
+    const int *i, *j, *type;
+    const float *azi, *amp;
+    const float *sig;   // [nsrc][nt]
+    int n, nt;
+};
+
+// force sources before the velocity update (src/update_v_interior.c:49-71);
+// one thread adds all sources in order so that coincident sources sum as
+// in the reference
+__global__ void k_force(Grid g, Fields f, Mat m, Src s, int nt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int l = 0; l < s.n; ++l) {
+        const int c = IDX(g, s.i[l], s.j[l]);
+        const float amp = (g.dt * s.sig[l * s.nt + nt - 1]) / (g.dh * g.dh);
+        const int t = s.type[l];
+        if (t == 2) f.vx[c] += m.rip[c] * amp;
+        if (t == 3) f.vy[c] += m.rjp[c] * amp;
+        if (t == 4) {
+            const float azi = (float)(s.azi[l] * M_PI / 180);
+            f.vx[c] = (float)((double)f.vx[c] + sin((double)azi) * (double)(m.rip[c] * amp));
+            f.vy[c] = (float)((double)f.vy[c] + cos((double)azi) * (double)(m.rjp[c] * amp));
+        }
+    }
+}
+
+// psource (src/psource.c:27-54): centred time derivative of the signal
+__global__ void k_psource(Grid g, Fields f, Src s, int nt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int l = 0; l < s.n; ++l) {
+        const float *sg = s.sig + l * s.nt - 1;   // sg[k] = signal at step k
+        const double h2 = 2.0 * g.dh * g.dh;
+        float amp;
+        if (nt == 1) amp = s.nt >= 2 ? (float)(sg[2] / h2) : 0.0f;
+        else if (nt < s.nt) amp = (float)((sg[nt + 1] - sg[nt - 1]) / h2);
+        else amp = (float)(-sg[nt - 1] / h2);
+        const int c = IDX(g, s.i[l], s.j[l]);
+        f.sxx[c] += amp;
+        f.syy[c] += amp;
+    }
+}
+
+// eqsource (src/eqsource.c:25-52): double-couple moment tensor
+__global__ void k_eqsource(Grid g, Fields f, Src s, int nt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int l = 0; l < s.n; ++l) {
+        const int i = s.i[l], j = s.j[l];
+        const float dip = (float)(s.azi[l] * M_PI / 180.0);
+        const float scale = g.dt / (g.dh * g.dh);
+        const float amp = s.amp[l] * scale * s.sig[l * s.nt + nt - 1];
+        const float m11 = (float)(-sin(2.0 * dip)), m12 = (float)(-cos(2.0 * dip)), m22 = (float)sin(2.0 * dip);
+        const int c = IDX(g, i, j);
+        f.sxx[c] += amp * m11;
+        f.sxy[c] = (float)((double)f.sxy[c] + 0.3 * amp * m12);
+        f.sxy[c - g.pitch] = (float)((double)f.sxy[c - g.pitch] + 0.3 * amp * m12);
+        f.sxy[c - 1] = (float)((double)f.sxy[c - 1] + 0.3 * amp * m12);
+        f.syy[c] += amp * m22;
+    }
+}
+
+// --------------------------------------------------------- free surface
+
+// surface_elastic / surface at j = 1 (src/surface_elastic.c:27-90,
+// src/surface.c:27-113): image the stresses into the ghost rows, correct sxx
+// (and the p memory variables) with the horizontal derivative only; in the
+// x-CPML the second update of psi_vxx[1][h] in the same step is reproduced.
+template <int ND, bool VISCO>
+__global__ void k_surface(Grid g, Fields f, Mat m)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > g.nx) return;
+    const int j = 1, P = g.pitch, c = IDX(g, i, j);
+    const float dhi = (float)(1.0 / g.dh);
+    f.syy[c] = 0.0f;
+    if (VISCO)
+        for (int l = 1; l <= g.L; ++l) f.q[(l - 1) * g.plane + c] = 0.0f;
+    float vxx = 0.f, vyy = 0.f;
+#pragma unroll
+    for (int m1 = 1; m1 <= ND; ++m1) {
+        f.syy[c - m1 * P] = -f.syy[c + m1 * P];
+        f.sxy[c - m1 * P] = -f.sxy[c + (m1 - 1) * P];
+        vxx += g.hc[m1] * (f.vx[c + m1 - 1] - f.vx[c - m1]);
+        vyy += g.hc[m1] * (f.vy[c + (m1 - 1) * P] - f.vy[c - m1 * P]);
+    }
+    vxx *= dhi;
+    vyy *= dhi;
+    if (g.abs_type == 1) {
+        if (i <= g.fw) {
+            const int o = j * g.wx + i;
+            const float ps = PRF(g, m, SOFI_P_B_X, i) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, i) * vxx;
+            f.psi_vxx[o] = ps;
+            vxx = vxx / PRF(g, m, SOFI_P_K_X, i) + ps;
+        }
+        if (i >= g.nx - g.fw + 1) {
+            const int h = i - g.nx + 2 * g.fw, o = j * g.wx + h;
+            const float ps = PRF(g, m, SOFI_P_B_X, h) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, h) * vxx;
+            f.psi_vxx[o] = ps;
+            vxx = vxx / PRF(g, m, SOFI_P_K_X, h) + ps;
+        }
+    }
+    if (!VISCO) {
+        const float fjm = m.pu[c] * 2.0f;
+        const float gg = m.ppi[c];
+        f.sxx[c] += -(g.dt * (gg - fjm) * (gg - fjm) * vxx / gg) - (g.dt * (gg - fjm) * vyy);
+    } else {
+        float sump = 0.f;
+        for (int l = 1; l <= g.L; ++l) sump += f.p[(l - 1) * g.plane + c];
+        const float fjm = (float)(m.pu[c] * 2.0 * (1.0 + g.L * m.ptaus[c]));
+        const float gg = (float)(m.ppi[c] * (1.0 + g.L * m.ptaup[c]));
+        f.sxx[c] += -(g.dt * (gg - fjm) * (gg - fjm) * vxx / gg) - (g.dt * (gg - fjm) * vyy) - (g.dthalbe * sump);
+        sump = 0.f;
+        for (int l = 1; l <= g.L; ++l) {
+            const float eta = m.eta[l];
+            const float bjm = (float)(eta / (1.0 + (eta * 0.5)));
+            const float djm = (float)(2.0 * m.pu[c] * m.ptaus[c]);
+            const float e = m.ppi[c] * m.ptaup[c];
+            float &pv = f.p[(l - 1) * g.plane + c];
+            pv += bjm * (((djm - e) * ((fjm / gg) - 1.0f) * vxx) - ((djm - e) * vyy));
+            sump += pv;
+        }
+        f.sxx[c] += (g.dthalbe * sump);
+    }
+}
+
+// ------------------------------------------------------------- samplers
+
+struct Rec {
+    const int *i, *j;
+    const double *sq_ppi, *sq_pu, *pfac;   // sqrt(ppi), sqrt(pu), (3ppi-4pu)/(2ppi-2pu)
+    float *sec;                            // [5][ntr][ns]
+    int n, ns;
+};
+
+// seismo_ssg div/curl (src/seismo_ssg.c:51-58): the post-CPML derivatives;
+// runs right after the stress update, before the free surface touches psi.
+template <int ND, bool TTI>
+__global__ void k_seis_dv(Grid g, Fields f, Mat m, Rec r, int ins)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.n) return;
+    const int i = r.i[k], j = r.j[k], c = IDX(g, i, j);
+    float vxx, vyx, vxy, vyy;
+    if (TTI) {
+        vxx = f.dvxx[c];
+        vyy = f.dvyy[c];
+        vyx = f.dvyx[c];
+        vxy = f.dvxy[c];
+    } else {
+        op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
+        const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
+        if (g.abs_type == 1 && (xf || yf)) cpml_dv_read(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
+    }
+    const size_t s = (size_t)k * r.ns + (ins - 1), comp = (size_t)r.n * r.ns;
+    r.sec[SOFI_C_DIV * comp + s] = (float)((double)(vxx + vyy) * r.sq_ppi[k]);
+    r.sec[SOFI_C_CURL * comp + s] = (float)((double)(vxy - vyx) * r.sq_pu[k]);
+}
+
+// seismo_ssg vx / vy / p (src/seismo_ssg.c:40-64)
+__global__ void k_seis(Grid g, Fields f, Rec r, int ins, int seismo)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= r.n) return;
+    const int c = IDX(g, r.i[k], r.j[k]);
+    const size_t s = (size_t)k * r.ns + (ins - 1), comp = (size_t)r.n * r.ns;
+    if (seismo == 1 || seismo == 4) {
+        r.sec[SOFI_C_VX * comp + s] = f.vx[c];
+        r.sec[SOFI_C_VY * comp + s] = f.vy[c];
+    }
+    if (seismo == 2) r.sec[SOFI_C_P * comp + s] = -f.sxx[c] - f.syy[c];
+    if (seismo == 4) r.sec[SOFI_C_P * comp + s] = (float)(r.pfac[k] * (double)(-f.sxx[c] - f.syy[c]) / 3);
+}
+
+// snap (src/snap_ssg.c:68-205): x-major decimated window; curl and div
+// recomputed from vx, vy without CPML
+template <int ND>
+__global__ void k_snap(Grid g, Fields f, Mat m, float *out, int i0, int di, int nxs, int j0, int dj, int nys)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nxs * nys) return;
+    const int ii = k / nys, jj = k % nys;
+    const int i = i0 + ii * di, j = j0 + jj * dj, c = IDX(g, i, j), P = g.pitch;
+    const size_t per = (size_t)nxs * nys;
+    out[SOFI_C_VX * per + k] = f.vx[c];
+    out[SOFI_C_VY * per + k] = f.vy[c];
+    out[SOFI_C_P * per + k] = -f.sxx[c] - f.syy[c];
+    const float dhi = (float)(1.0 / g.dh);
+    float vyx = 0.f, vxy = 0.f, vxx = 0.f, vyy = 0.f;
+#pragma unroll
+    for (int m1 = 1; m1 <= ND; ++m1) {
+        vyx += g.hc[m1] * (f.vy[c + m1] - f.vy[c - m1 + 1]);
+        vxy += g.hc[m1] * (f.vx[c + m1 * P] - f.vx[c - (m1 - 1) * P]);
+        vxx += g.hc[m1] * (f.vx[c + m1 - 1] - f.vx[c - m1]);
+        vyy += g.hc[m1] * (f.vy[c + (m1 - 1) * P] - f.vy[c - m1 * P]);
+    }
+    vyx *= dhi;
+    vxy *= dhi;
+    vxx *= dhi;
+    vyy *= dhi;
+    out[SOFI_C_CURL * per + k] = (float)((double)(vxy - vyx) * sqrt((double)m.pu[c]));
+    out[SOFI_C_DIV * per + k] = (float)((double)(vxx + vyy) * sqrt((double)m.ppi[c]));
+}
+
+} // namespace
+
+// ====================================================================== handle
+
+struct sofi_dev {
+    sofi_desc d;
+    Grid g;
+    Fields f;
+    Mat m;
+    Src s;
+    Rec r;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    // device-side copies owned by the handle
+    float *mat[SOFI_A_NARRAYS] = {};
+    float *base_d = nullptr, *base_e = nullptr, *base_r = nullptr;
+    float *prof = nullptr;
+    int *src_i = nullptr, *src_j = nullptr, *src_t = nullptr;
+    float *src_azi = nullptr, *src_amp = nullptr, *src_sig = nullptr;
+    int source_type = 0;
+    int *rec_i = nullptr, *rec_j = nullptr;
+    double *rec_d = nullptr;
+    float *sec = nullptr;
+    float *snapbuf = nullptr;
+    int nsnap_max = 0, nsnap = 0, nxs = 0, nys = 0;
+    int lsamp = 0, lsnap = 0;
+    bool medium_set = false;
+    // host copies needed for receiver factors
+    std::vector<float> h_ppi, h_pu;
+};
+
+namespace {
+
+int dalloc(sofi_dev *h, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) return fail("hipMemset failed: %s", hipGetErrorString(e));
+    h->allocs.push_back(*p);
+    h->bytes += bytes;
+    return 0;
+}
+
+// release one allocation owned by the handle (per-shot buffers replaced on re-upload)
+template <typename T>
+void dfree(sofi_dev *h, T **p)
+{
+    if (!*p) return;
+    for (size_t k = 0; k < h->allocs.size(); ++k)
+        if (h->allocs[k] == (void *)*p) {
+            h->allocs.erase(h->allocs.begin() + k);
+            break;
+        }
+    size_t sz = 0;
+    (void)hipMemPtrGetInfo(*p, &sz);
+    h->bytes -= sz < h->bytes ? sz : h->bytes;
+    (void)hipFree(*p);
+    *p = nullptr;
+}
+
+template <typename T>
+int dalloc_t(sofi_dev *h, T **p, size_t n)
+{
+    return dalloc(h, (void **)p, n * sizeof(T));
+}
+
+int upload(sofi_dev *h, float **dst, const float *src, size_t n)
+{
+    if (!*dst && dalloc_t(h, dst, n)) return -1;
+    HIPCHK(hipMemcpy(*dst, src, n * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+bool planes_of(int id) { return id == SOFI_A_DIP || id == SOFI_A_D || id == SOFI_A_E || (id >= SOFI_A_C11D && id <= SOFI_A_C35IPJPD); }
+
+} // namespace
+
+extern "C" {
+
+const char *sofi_last_error(void) { return g_err.c_str(); }
+
+int sofi_open(const sofi_desc *desc, sofi_dev **out)
+{
+    if (!desc || !out) return fail("sofi_open: null argument");
+    const sofi_desc &d = *desc;
+    if (d.nx < 1 || d.ny < 1) return fail("sofi_open: bad grid %d x %d", d.nx, d.ny);
+    if (d.fdorder < 2 || d.fdorder > 12 || (d.fdorder & 1)) return fail("sofi_open: unsupported FDORDER %d", d.fdorder);
+    if (d.weq < SOFI_EL_ISO || d.weq > SOFI_VEL_TTI) return fail("sofi_open: unsupported WEQ %d", d.weq);
+    if (d.L < 0 || d.L > SOFI_MAXL) return fail("sofi_open: L=%d out of range", d.L);
+    bool visco = d.weq == SOFI_VEL_ISO || d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI;
+    if (visco && d.L < 1) return fail("sofi_open: L>0 required for viscoelastic rheologies");
+    if (d.fw < 0 || 2 * d.fw > d.nx || 2 * d.fw > d.ny) return fail("sofi_open: FW=%d does not fit the grid", d.fw);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("sofi_open: no HIP device available");
+    sofi_dev *h = new (std::nothrow) sofi_dev();
+    if (!h) return fail("out of memory");
+    h->d = d;
+    Grid &g = h->g;
+    g.nx = d.nx;
+    g.ny = d.ny;
+    g.pitch = sofi_pitch(d.nx);
+    g.plane = (int)sofi_plane(d.nx, d.ny);
+    g.fw = d.fw;
+    g.abs_type = d.abs_type;
+    g.free_surf = d.free_surf;
+    g.weq = d.weq;
+    g.L = visco ? d.L : 0;
+    g.nd = d.fdorder / 2;
+    g.gx2 = d.gx[2];
+    g.gx3 = d.gx[3];
+    g.gy2 = d.gy[2];
+    g.gy3 = d.gy[3];
+    g.wx = 2 * d.fw + 1;
+    g.wy = d.nx + 1;
+    g.dt = d.dt;
+    g.dh = d.dh;
+    g.dtdh = d.dt / d.dh;
+    g.dhi = 1.0f / d.dh;
+    g.dthalbe = (float)(d.dt / 2.0);
+    for (int k = 0; k < 7; ++k) g.hc[k] = d.hc[k];
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        sofi_close(h);
+        return fail("sofi_open: stream/event creation failed");
+    }
+    size_t pl = (size_t)g.plane;
+    Fields &f = h->f;
+    int rc = 0;
+    rc |= dalloc_t(h, &f.vx, pl);
+    rc |= dalloc_t(h, &f.vy, pl);
+    rc |= dalloc_t(h, &f.sxx, pl);
+    rc |= dalloc_t(h, &f.syy, pl);
+    rc |= dalloc_t(h, &f.sxy, pl);
+    if (visco) {
+        rc |= dalloc_t(h, &f.r, pl * g.L);
+        rc |= dalloc_t(h, &f.p, pl * g.L);
+        rc |= dalloc_t(h, &f.q, pl * g.L);
+    }
+    if (d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI) {
+        rc |= dalloc_t(h, &f.dvxx, pl);
+        rc |= dalloc_t(h, &f.dvyy, pl);
+        rc |= dalloc_t(h, &f.dvyx, pl);
+        rc |= dalloc_t(h, &f.dvxy, pl);
+    }
+    if (d.abs_type == 1 && d.fw > 0) {
+        size_t nxp = (size_t)(d.ny + 1) * g.wx, nyp = (size_t)g.wx * g.wy;
+        rc |= dalloc_t(h, &f.psi_sxx_x, nxp);
+        rc |= dalloc_t(h, &f.psi_sxy_x, nxp);
+        rc |= dalloc_t(h, &f.psi_vxx, nxp);
+        rc |= dalloc_t(h, &f.psi_vyx, nxp);
+        rc |= dalloc_t(h, &f.psi_syy_y, nyp);
+        rc |= dalloc_t(h, &f.psi_sxy_y, nyp);
+        rc |= dalloc_t(h, &f.psi_vyy, nyp);
+        rc |= dalloc_t(h, &f.psi_vxy, nyp);
+    }
+    if (rc) {
+        std::string e = g_err;
+        sofi_close(h);
+        g_err = e;
+        return -1;
+    }
+    h->lsamp = d.ndt;
+    h->lsnap = d.lsnap;
+    *out = h;
+    return 0;
+}
+
+int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const *profiles)
+{
+    if (!h || !arrays) return fail("sofi_set_medium: null argument");
+    const sofi_desc &d = h->d;
+    const Grid &g = h->g;
+    size_t pl = (size_t)g.plane;
+    Mat &m = h->m;
+    auto need = [&](int id) -> int {
+        if (!arrays[id]) return fail("sofi_set_medium: material array %d required for WEQ %d is missing", id, d.weq);
+        return upload(h, &h->mat[id], arrays[id], planes_of(id) ? pl * g.L : pl);
+    };
+    int rc = 0;
+    rc |= need(SOFI_A_RIP);
+    rc |= need(SOFI_A_RJP);
+    rc |= need(SOFI_A_PU);
+    rc |= need(SOFI_A_PPI);
+    if (rc) return -1;
+    switch (d.weq) {
+      case SOFI_EL_ISO:
+          rc |= need(SOFI_A_F) | need(SOFI_A_G) | need(SOFI_A_FIPJP);
+          break;
+      case SOFI_VEL_ISO: {
+          rc |= need(SOFI_A_F) | need(SOFI_A_G) | need(SOFI_A_FIPJP) | need(SOFI_A_PTAUS) | need(SOFI_A_PTAUP);
+          if (rc) return -1;
+          for (int id : { SOFI_A_PUIPJP, SOFI_A_PTAUSIPJP })
+              if (!arrays[id]) return fail("sofi_set_medium: array %d required for VEL_ISO is missing", id);
+          // per-cell defect bases: d_l = eta_l*pu*taus, e_l = eta_l*ppi*taup, dip_l = eta_l*puipjp*tausipjp
+          std::vector<float> bd(pl), be(pl), br(pl);
+          const float *pu = arrays[SOFI_A_PU], *ppi = arrays[SOFI_A_PPI], *ts = arrays[SOFI_A_PTAUS],
+                      *tp = arrays[SOFI_A_PTAUP], *pui = arrays[SOFI_A_PUIPJP], *tsi = arrays[SOFI_A_PTAUSIPJP];
+          for (size_t k = 0; k < pl; ++k) {
+              bd[k] = pu[k] * ts[k];
+              be[k] = ppi[k] * tp[k];
+              br[k] = pui[k] * tsi[k];
+          }
+          rc |= upload(h, &h->base_d, bd.data(), pl) | upload(h, &h->base_e, be.data(), pl) | upload(h, &h->base_r, br.data(), pl);
+          break;
+      }
+      case SOFI_EL_VTI:
+          rc |= need(SOFI_A_C11) | need(SOFI_A_C13) | need(SOFI_A_C33) | need(SOFI_A_C55IPJP);
+          break;
+      case SOFI_VEL_VTI:
+          rc |= need(SOFI_A_C11U) | need(SOFI_A_C13U) | need(SOFI_A_C33U) | need(SOFI_A_C55IPJPU);
+          rc |= need(SOFI_A_C11D) | need(SOFI_A_C13D) | need(SOFI_A_C33D) | need(SOFI_A_C55IPJPD);
+          break;
+      case SOFI_EL_TTI:
+          rc |= need(SOFI_A_C11) | need(SOFI_A_C13) | need(SOFI_A_C33) | need(SOFI_A_C15) | need(SOFI_A_C35);
+          rc |= need(SOFI_A_C55IPJP) | need(SOFI_A_C15IPJP) | need(SOFI_A_C35IPJP);
+          break;
+      case SOFI_VEL_TTI:
+          rc |= need(SOFI_A_C11U) | need(SOFI_A_C13U) | need(SOFI_A_C33U) | need(SOFI_A_C15U) | need(SOFI_A_C35U);
+          rc |= need(SOFI_A_C55IPJPU) | need(SOFI_A_C15IPJPU) | need(SOFI_A_C35IPJPU);
+          rc |= need(SOFI_A_C11D) | need(SOFI_A_C13D) | need(SOFI_A_C33D) | need(SOFI_A_C15D) | need(SOFI_A_C35D);
+          rc |= need(SOFI_A_C55IPJPD) | need(SOFI_A_C15IPJPD) | need(SOFI_A_C35IPJPD);
+          break;
+    }
+    if (rc) return -1;
+    if (d.abs_type == 2 && d.fw > 0) {
+        if (need(SOFI_A_ABSORB)) return -1;
+    }
+    if (d.abs_type == 1 && d.fw > 0) {
+        if (!profiles) return fail("sofi_set_medium: CPML profiles missing");
+        int n = 2 * d.fw + 1;
+        std::vector<float> pr((size_t)12 * n);
+        for (int k = 0; k < 12; ++k) {
+            if (!profiles[k]) return fail("sofi_set_medium: CPML profile %d missing", k);
+            memcpy(pr.data() + (size_t)k * n, profiles[k], n * sizeof(float));
+        }
+        if (upload(h, &h->prof, pr.data(), pr.size())) return -1;
+    }
+    m = Mat();
+    m.rip = h->mat[SOFI_A_RIP];
+    m.rjp = h->mat[SOFI_A_RJP];
+    m.f = h->mat[SOFI_A_F];
+    m.g = h->mat[SOFI_A_G];
+    m.fipjp = h->mat[SOFI_A_FIPJP];
+    m.bd = h->base_d;
+    m.be = h->base_e;
+    m.br = h->base_r;
+    m.c11 = h->mat[SOFI_A_C11];
+    m.c13 = h->mat[SOFI_A_C13];
+    m.c33 = h->mat[SOFI_A_C33];
+    m.c15 = h->mat[SOFI_A_C15];
+    m.c35 = h->mat[SOFI_A_C35];
+    m.c55i = h->mat[SOFI_A_C55IPJP];
+    m.c15i = h->mat[SOFI_A_C15IPJP];
+    m.c35i = h->mat[SOFI_A_C35IPJP];
+    m.u11 = h->mat[SOFI_A_C11U];
+    m.u13 = h->mat[SOFI_A_C13U];
+    m.u33 = h->mat[SOFI_A_C33U];
+    m.u15 = h->mat[SOFI_A_C15U];
+    m.u35 = h->mat[SOFI_A_C35U];
+    m.u55i = h->mat[SOFI_A_C55IPJPU];
+    m.u15i = h->mat[SOFI_A_C15IPJPU];
+    m.u35i = h->mat[SOFI_A_C35IPJPU];
+    m.d11 = h->mat[SOFI_A_C11D];
+    m.d13 = h->mat[SOFI_A_C13D];
+    m.d33 = h->mat[SOFI_A_C33D];
+    m.d15 = h->mat[SOFI_A_C15D];
+    m.d35 = h->mat[SOFI_A_C35D];
+    m.d55i = h->mat[SOFI_A_C55IPJPD];
+    m.d15i = h->mat[SOFI_A_C15IPJPD];
+    m.d35i = h->mat[SOFI_A_C35IPJPD];
+    m.absorb = h->mat[SOFI_A_ABSORB];
+    m.pu = h->mat[SOFI_A_PU];
+    m.ppi = h->mat[SOFI_A_PPI];
+    m.ptaus = h->mat[SOFI_A_PTAUS];
+    m.ptaup = h->mat[SOFI_A_PTAUP];
+    m.prof = h->prof;
+    if (g.L > 0) {
+        if (!profiles || !profiles[SOFI_P_ETA] || !profiles[SOFI_P_BIP] || !profiles[SOFI_P_CIP])
+            return fail("sofi_set_medium: relaxation profiles (eta, bip, cip) missing");
+        for (int l = 1; l <= g.L; ++l) {
+            m.eta[l] = profiles[SOFI_P_ETA][l];
+            m.bip[l] = profiles[SOFI_P_BIP][l];
+            m.cip[l] = profiles[SOFI_P_CIP][l];
+        }
+    }
+    h->h_ppi.assign(arrays[SOFI_A_PPI], arrays[SOFI_A_PPI] + pl);
+    h->h_pu.assign(arrays[SOFI_A_PU], arrays[SOFI_A_PU] + pl);
+    h->medium_set = true;
+    return 0;
+}
+
+int sofi_set_sources(sofi_dev *h, int nsrc, const int *src_i, const int *src_j, const int *src_type,
+                     const float *src_azi, const float *src_amp, const float *signals, int source_type)
+{
+    if (!h) return fail("sofi_set_sources: null handle");
+    if (nsrc < 0) return fail("sofi_set_sources: nsrc < 0");
+    const sofi_desc &d = h->d;
+    for (int k = 0; k < nsrc; ++k) {
+        // every injection site, including the eqsource neighbours (j-1, i-1), must lie on the grid
+        if (src_i[k] < 1 || src_i[k] > d.nx || src_j[k] < 1 || src_j[k] > d.ny)
+            return fail("sofi_set_sources: source %d at (%d, %d) is outside the %d x %d grid", k, src_i[k], src_j[k], d.nx, d.ny);
+    }
+    // (re)allocate for this shot
+    size_t n = (size_t)(nsrc > 0 ? nsrc : 1);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, &h->src_i); dfree(h, &h->src_j); dfree(h, &h->src_t);
+    dfree(h, &h->src_azi); dfree(h, &h->src_amp); dfree(h, &h->src_sig);
+    int rc = 0;
+    rc |= dalloc_t(h, &h->src_i, n) | dalloc_t(h, &h->src_j, n) | dalloc_t(h, &h->src_t, n);
+    rc |= dalloc_t(h, &h->src_azi, n) | dalloc_t(h, &h->src_amp, n) | dalloc_t(h, &h->src_sig, n * d.nt);
+    if (rc) return -1;
+    if (nsrc > 0) {
+        HIPCHK(hipMemcpy(h->src_i, src_i, nsrc * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->src_j, src_j, nsrc * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->src_t, src_type, nsrc * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->src_azi, src_azi, nsrc * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->src_amp, src_amp, nsrc * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->src_sig, signals, (size_t)nsrc * d.nt * sizeof(float), hipMemcpyHostToDevice));
+    }
+    h->s.i = h->src_i;
+    h->s.j = h->src_j;
+    h->s.type = h->src_t;
+    h->s.azi = h->src_azi;
+    h->s.amp = h->src_amp;
+    h->s.sig = h->src_sig;
+    h->s.n = nsrc;
+    h->s.nt = d.nt;
+    h->source_type = source_type;
+    return 0;
+}
+
+int sofi_set_receivers(sofi_dev *h, int ntr, const int *rec_i, const int *rec_j)
+{
+    if (!h) return fail("sofi_set_receivers: null handle");
+    if (!h->medium_set) return fail("sofi_set_receivers: call sofi_set_medium first");
+    const sofi_desc &d = h->d;
+    if (ntr < 0) return fail("sofi_set_receivers: ntr < 0");
+    std::vector<double> fac((size_t)3 * (ntr > 0 ? ntr : 1));
+    for (int k = 0; k < ntr; ++k) {
+        if (rec_i[k] < 1 || rec_i[k] > d.nx || rec_j[k] < 1 || rec_j[k] > d.ny)
+            return fail("sofi_set_receivers: receiver %d at (%d, %d) is outside the grid", k, rec_i[k], rec_j[k]);
+        size_t c = sofi_idx(h->g.pitch, rec_i[k], rec_j[k]);
+        double ppi = h->h_ppi[c], pu = h->h_pu[c];
+        fac[k] = sqrt(ppi);
+        fac[ntr + k] = sqrt(pu);
+        fac[2 * (size_t)ntr + k] = (3.0 * (float)ppi - 4.0 * (float)pu) / (2.0 * (float)ppi - 2.0 * (float)pu);
+    }
+    size_t n = (size_t)(ntr > 0 ? ntr : 1);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, &h->rec_i); dfree(h, &h->rec_j); dfree(h, &h->rec_d); dfree(h, &h->sec);
+    int rc = dalloc_t(h, &h->rec_i, n) | dalloc_t(h, &h->rec_j, n) | dalloc_t(h, &h->rec_d, 3 * n) |
+             dalloc_t(h, &h->sec, (size_t)SOFI_NCOMP * n * (d.ns > 0 ? d.ns : 1));
+    if (rc) return -1;
+    if (ntr > 0) {
+        HIPCHK(hipMemcpy(h->rec_i, rec_i, ntr * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->rec_j, rec_j, ntr * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->rec_d, fac.data(), 3 * (size_t)ntr * sizeof(double), hipMemcpyHostToDevice));
+    }
+    Rec &r = h->r;
+    r.i = h->rec_i;
+    r.j = h->rec_j;
+    r.sq_ppi = h->rec_d;
+    r.sq_pu = h->rec_d + ntr;
+    r.pfac = h->rec_d + 2 * (size_t)ntr;
+    r.sec = h->sec;
+    r.n = ntr;
+    r.ns = d.ns;
+    return 0;
+}
+
+int sofi_set_snapshots(sofi_dev *h, int nsnap)
+{
+    if (!h) return fail("sofi_set_snapshots: null handle");
+    const sofi_desc &d = h->d;
+    h->nxs = (d.snap_idx[2] - d.snap_idx[1]) / (d.idx > 0 ? d.idx : 1) + 1;
+    h->nys = (d.snap_idx[4] - d.snap_idx[3]) / (d.idy > 0 ? d.idy : 1) + 1;
+    if (d.snap_idx[1] < 1 || d.snap_idx[2] > d.nx || d.snap_idx[3] < 1 || d.snap_idx[4] > d.ny)
+        return fail("sofi_set_snapshots: snapshot window outside the grid");
+    h->nsnap_max = nsnap > 0 ? nsnap : 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, &h->snapbuf);
+    if (h->nsnap_max > 0 &&
+        dalloc_t(h, &h->snapbuf, (size_t)h->nsnap_max * SOFI_NCOMP * h->nxs * h->nys))
+        return -1;
+    h->nsnap = 0;
+    return 0;
+}
+
+int sofi_reset(sofi_dev *h)
+{
+    if (!h) return fail("sofi_reset: null handle");
+    size_t pl = (size_t)h->g.plane * sizeof(float);
+    Fields &f = h->f;
+    for (float *p : { f.vx, f.vy, f.sxx, f.syy, f.sxy, f.dvxx, f.dvyy, f.dvyx, f.dvxy })
+        if (p) HIPCHK(hipMemsetAsync(p, 0, pl, h->stream));
+    for (float *p : { f.r, f.p, f.q })
+        if (p) HIPCHK(hipMemsetAsync(p, 0, pl * h->g.L, h->stream));
+    size_t nxp = (size_t)(h->d.ny + 1) * h->g.wx * sizeof(float), nyp = (size_t)h->g.wx * h->g.wy * sizeof(float);
+    for (float *p : { f.psi_sxx_x, f.psi_sxy_x, f.psi_vxx, f.psi_vyx })
+        if (p) HIPCHK(hipMemsetAsync(p, 0, nxp, h->stream));
+    for (float *p : { f.psi_syy_y, f.psi_sxy_y, f.psi_vyy, f.psi_vxy })
+        if (p) HIPCHK(hipMemsetAsync(p, 0, nyp, h->stream));
+    if (h->sec) HIPCHK(hipMemsetAsync(h->sec, 0, (size_t)SOFI_NCOMP * (h->r.n > 0 ? h->r.n : 1) * (h->d.ns > 0 ? h->d.ns : 1) * sizeof(float), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->nsnap = 0;
+    h->lsamp = h->d.ndt;
+    h->lsnap = h->d.lsnap;
+    return 0;
+}
+
+} // extern "C"
+
+namespace {
+
+template <int ND>
+int step_nd(sofi_dev *h, int nt)
+{
+    const Grid &g = h->g;
+    Fields &f = h->f;
+    const Mat &m = h->m;
+    const sofi_desc &d = h->d;
+    hipStream_t st = h->stream;
+    dim3 blk(BX, BY), grd((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
+    if (h->s.n > 0) hipLaunchKernelGGL(k_force, dim3(1), dim3(64), 0, st, g, f, m, h->s, nt);
+    hipLaunchKernelGGL(k_velocity<ND>, grd, blk, 0, st, g, f, m);
+    const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
+    switch (d.weq) {
+      case SOFI_EL_ISO: hipLaunchKernelGGL((k_stress<ND, SOFI_EL_ISO>), grd, blk, 0, st, g, f, m); break;
+      case SOFI_VEL_ISO: hipLaunchKernelGGL((k_stress<ND, SOFI_VEL_ISO>), grd, blk, 0, st, g, f, m); break;
+      case SOFI_EL_VTI: hipLaunchKernelGGL((k_stress<ND, SOFI_EL_VTI>), grd, blk, 0, st, g, f, m); break;
+      case SOFI_VEL_VTI: hipLaunchKernelGGL((k_stress<ND, SOFI_VEL_VTI>), grd, blk, 0, st, g, f, m); break;
+      default: {
+          hipLaunchKernelGGL(k_dv<ND>, grd, blk, 0, st, g, f);
+          const bool el = d.weq == SOFI_EL_TTI;
+          if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), grd, blk, 0, st, g, f, m);
+          else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), grd, blk, 0, st, g, f, m);
+          if (d.fw > 0 && d.abs_type == 1) {
+              hipLaunchKernelGGL(k_cpml_dv_frame, grd, blk, 0, st, g, f, m);
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), grd, blk, 0, st, g, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), grd, blk, 0, st, g, f, m);
+          } else if (d.fw > 0 && d.abs_type == 2) {
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), grd, blk, 0, st, g, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), grd, blk, 0, st, g, f, m);
+          }
+      }
+    }
+    const bool sample = d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
+    const int ins = h->lsamp / (d.ndt > 0 ? d.ndt : 1);
+    if (sample && (d.seismo == 3 || d.seismo == 4)) {
+        dim3 rb(64), rg((h->r.n + 63) / 64);
+        if (tti) hipLaunchKernelGGL((k_seis_dv<ND, true>), rg, rb, 0, st, g, f, m, h->r, ins);
+        else hipLaunchKernelGGL((k_seis_dv<ND, false>), rg, rb, 0, st, g, f, m, h->r, ins);
+    }
+    if (h->s.n > 0) {
+        if (h->source_type == 1) hipLaunchKernelGGL(k_psource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
+        if (h->source_type == 5) hipLaunchKernelGGL(k_eqsource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
+    }
+    if (d.free_surf) {
+        dim3 sb(256), sg((g.nx + 255) / 256);
+        if (g.L > 0) hipLaunchKernelGGL((k_surface<ND, true>), sg, sb, 0, st, g, f, m);
+        else hipLaunchKernelGGL((k_surface<ND, false>), sg, sb, 0, st, g, f, m);
+    }
+    if (sample) {
+        if (d.seismo != 3) {
+            dim3 rb(64), rg((h->r.n + 63) / 64);
+            hipLaunchKernelGGL(k_seis, rg, rb, 0, st, g, f, h->r, ins, d.seismo);
+        }
+    }
+    if (d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
+    if (d.snap && nt == h->lsnap && nt <= d.esnap) {
+        if (h->nsnap < h->nsnap_max) {
+            int n = h->nxs * h->nys;
+            float *out = h->snapbuf + (size_t)h->nsnap * SOFI_NCOMP * n;
+            hipLaunchKernelGGL(k_snap<ND>, dim3((n + 255) / 256), dim3(256), 0, st, g, f, m, out, d.snap_idx[1],
+                               d.idx, h->nxs, d.snap_idx[3], d.idy, h->nys);
+            h->nsnap++;
+        }
+        h->lsnap += d.isnap;
+    }
+    return 0;
+}
+
+int step(sofi_dev *h, int nt)
+{
+    switch (h->g.nd) {
+      case 1: return step_nd<1>(h, nt);
+      case 2: return step_nd<2>(h, nt);
+      case 3: return step_nd<3>(h, nt);
+      case 4: return step_nd<4>(h, nt);
+      case 5: return step_nd<5>(h, nt);
+      case 6: return step_nd<6>(h, nt);
+    }
+    return fail("unsupported ND %d", h->g.nd);
+}
+
+} // namespace
+
+extern "C" {
+
+int sofi_run(sofi_dev *h, int nt_first, int nt_last)
+{
+    if (!h) return fail("sofi_run: null handle");
+    if (!h->medium_set) return fail("sofi_run: medium not set");
+    if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
+    if (h->d.seismo && h->r.n > 0 && !h->sec) return fail("sofi_run: receivers not set");
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int nt = nt_first; nt <= nt_last; ++nt)
+        if (step(h, nt)) return -1;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    return 0;
+}
+
+double sofi_last_run_ms(const sofi_dev *h) { return h ? h->last_ms : 0.0; }
+
+int sofi_get_section(sofi_dev *h, int comp, float *out)
+{
+    if (!h || !out) return fail("sofi_get_section: null argument");
+    if (comp < 0 || comp >= SOFI_NCOMP) return fail("sofi_get_section: bad component %d", comp);
+    size_t n = (size_t)h->r.n * h->d.ns;
+    if (n == 0) return 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, h->sec + (size_t)comp * n, n * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sofi_snapshot_count(const sofi_dev *h) { return h ? h->nsnap : 0; }
+
+int sofi_get_snapshot(sofi_dev *h, int k, int comp, float *out)
+{
+    if (!h || !out) return fail("sofi_get_snapshot: null argument");
+    if (k < 0 || k >= h->nsnap) return fail("sofi_get_snapshot: snapshot %d not taken (have %d)", k, h->nsnap);
+    if (comp < 0 || comp >= SOFI_NCOMP) return fail("sofi_get_snapshot: bad component %d", comp);
+    size_t n = (size_t)h->nxs * h->nys;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, h->snapbuf + ((size_t)k * SOFI_NCOMP + comp) * n, n * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sofi_get_field(sofi_dev *h, int id, float *out)
+{
+    if (!h || !out) return fail("sofi_get_field: null argument");
+    float *p = nullptr;
+    switch (id) {
+      case 0: p = h->f.vx; break;
+      case 1: p = h->f.vy; break;
+      case 2: p = h->f.sxx; break;
+      case 3: p = h->f.syy; break;
+      case 4: p = h->f.sxy; break;
+      default: return fail("sofi_get_field: bad field id %d", id);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, p, (size_t)h->g.plane * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sofi_close(sofi_dev *h)
+{
+    if (!h) return 0;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+size_t sofi_footprint(const sofi_dev *h) { return h ? h->bytes : 0; }
+
+} // extern "C"

@@ -1,0 +1,266 @@
+"""Python face of the SOFI2D time loop on MI355X.
+
+``Problem`` reads a reference parameter file (and the model/source/receiver
+files it names) through libsofi_host; ``Simulation`` runs shots on the GPU
+through the C-ABI of libsofi_hip (include/sofi_hip.h).  ``run_json`` is the
+Python counterpart of the ``sofi2D <file.json>`` driver (src/sofi2D.c:54-442
+for MODE=FW): it writes the same SU seismograms, source signal and merged
+snapshots.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _native as N
+
+
+class SofiError(RuntimeError):
+    pass
+
+
+def _chk_host(rc):
+    if rc != 0:
+        raise SofiError(N.host_error())
+
+
+def _chk_hip(rc):
+    if rc != 0:
+        raise SofiError(N.hip_error())
+
+
+def _as_int_ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _as_float_ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Problem:
+    """A run set up from a JSON parameter file (params, model, acquisition)."""
+
+    def __init__(self, json_path: str):
+        self._L = N.host()
+        p = self._L.sofi_problem_load(json_path.encode())
+        if not p:
+            raise SofiError(N.host_error())
+        self._p = p
+        self.nshots = self._L.sofi_problem_nshots(p)
+        self.set_shot(1)
+
+    def close(self):
+        if self._p:
+            self._L.sofi_problem_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ shots
+    def set_shot(self, ishot: int):
+        _chk_host(self._L.sofi_problem_set_shot(self._p, ishot))
+        self.ishot = ishot
+        self.desc = self._L.sofi_problem_desc(self._p).contents.copy()
+
+    @property
+    def nx(self):
+        return self.desc.nx
+
+    @property
+    def ny(self):
+        return self.desc.ny
+
+    @property
+    def plane(self):
+        return N.pitch(self.desc.nx) * N.rows(self.desc.ny)
+
+    def array(self, aid: int, planes: int = 1):
+        ptr = self._L.sofi_problem_array(self._p, aid)
+        if not ptr:
+            return None
+        n = self.plane * planes
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+    def arrays_ptr(self):
+        return self._L.sofi_problem_arrays(self._p)
+
+    def profiles_ptr(self):
+        return self._L.sofi_problem_profiles(self._p)
+
+    def sources(self):
+        n = self._L.sofi_problem_nsrc(self._p)
+        L, p = self._L, self._p
+        def ints(f):
+            return np.ctypeslib.as_array(f(p), shape=(max(n, 1),))[:n].astype(np.int32).copy()
+        def flts(f, m=1):
+            return np.ctypeslib.as_array(f(p), shape=(max(n, 1) * m,))[: n * m].astype(np.float32).copy()
+        return {
+            "n": n, "i": ints(L.sofi_problem_src_i), "j": ints(L.sofi_problem_src_j),
+            "type": ints(L.sofi_problem_src_type), "azi": flts(L.sofi_problem_src_azi),
+            "amp": flts(L.sofi_problem_src_amp), "signals": flts(L.sofi_problem_signals, self.desc.nt),
+            "source_type": L.sofi_problem_source_type(p),
+        }
+
+    def receivers(self):
+        n = self._L.sofi_problem_ntr(self._p)
+        ri = np.ctypeslib.as_array(self._L.sofi_problem_rec_i(self._p), shape=(max(n, 1),))[:n].copy()
+        rj = np.ctypeslib.as_array(self._L.sofi_problem_rec_j(self._p), shape=(max(n, 1),))[:n].copy()
+        return ri.astype(np.int32), rj.astype(np.int32)
+
+    @property
+    def snapcheck(self):
+        return self._L.sofi_problem_snapcheck(self._p)
+
+    @property
+    def nsnap(self):
+        return self._L.sofi_problem_nsnap(self._p)
+
+    def seismo_comps(self):
+        buf = (C.c_int * 5)()
+        n = self._L.sofi_problem_seismo_comps(self._p, buf)
+        return list(buf[:n])
+
+    def snap_comps(self):
+        buf = (C.c_int * 5)()
+        n = self._L.sofi_problem_snap_comps(self._p, buf)
+        return list(buf[:n])
+
+    def snapshot_dims(self):
+        d = self.desc
+        nxs = (d.snap_idx[2] - d.snap_idx[1]) // max(d.idx, 1) + 1
+        nys = (d.snap_idx[4] - d.snap_idx[3]) // max(d.idy, 1) + 1
+        return nxs, nys
+
+    # ------------------------------------------------------------ outputs
+    def write_section(self, comp: int, section: np.ndarray):
+        s = np.ascontiguousarray(section, dtype=np.float32)
+        _chk_host(self._L.sofi_problem_write_section(self._p, comp, _as_float_ptr(s)))
+
+    def write_signal(self):
+        _chk_host(self._L.sofi_problem_write_signal(self._p))
+
+    def write_snapshot(self, comp: int, first: bool, data: np.ndarray):
+        s = np.ascontiguousarray(data, dtype=np.float32).ravel()
+        _chk_host(self._L.sofi_problem_write_snapshot(self._p, comp, int(first), _as_float_ptr(s), s.size))
+
+
+class Simulation:
+    """Device state of one shot (a ``sofi_dev`` handle) for a Problem's grid."""
+
+    def __init__(self, problem: Problem, desc: N.SofiDesc | None = None):
+        self._L = N.hip()
+        self.problem = problem
+        self.desc = (desc or problem.desc).copy()
+        h = C.c_void_p()
+        _chk_hip(self._L.sofi_open(C.byref(self.desc), C.byref(h)))
+        self._h = h
+        _chk_hip(self._L.sofi_set_medium(h, problem.arrays_ptr(), problem.profiles_ptr()))
+        ri, rj = problem.receivers()
+        self.ntr = len(ri)
+        self._ri, self._rj = ri, rj
+        if self.desc.seismo and self.ntr:
+            _chk_hip(self._L.sofi_set_receivers(h, self.ntr, _as_int_ptr(ri), _as_int_ptr(rj)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.sofi_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def footprint(self) -> int:
+        return int(self._L.sofi_footprint(self._h))
+
+    def load_shot(self, nsnap: int | None = None):
+        """Reset the wavefield and load the problem's current shot."""
+        p = self.problem
+        s = p.sources()
+        self._src = s
+        _chk_hip(self._L.sofi_reset(self._h))
+        _chk_hip(self._L.sofi_set_sources(self._h, s["n"], _as_int_ptr(s["i"]), _as_int_ptr(s["j"]),
+                                          _as_int_ptr(s["type"]), _as_float_ptr(s["azi"]),
+                                          _as_float_ptr(s["amp"]), _as_float_ptr(s["signals"]),
+                                          s["source_type"]))
+        if nsnap is None:
+            nsnap = p.nsnap if p.snapcheck else 0
+        _chk_hip(self._L.sofi_set_snapshots(self._h, nsnap))
+
+    def run(self, nt_first: int = 1, nt_last: int | None = None) -> float:
+        nt_last = self.desc.nt if nt_last is None else nt_last
+        _chk_hip(self._L.sofi_run(self._h, nt_first, nt_last))
+        return float(self._L.sofi_last_run_ms(self._h))
+
+    def section(self, comp: int) -> np.ndarray:
+        out = np.zeros((self.ntr, self.desc.ns), dtype=np.float32)
+        if out.size:
+            _chk_hip(self._L.sofi_get_section(self._h, comp, _as_float_ptr(out)))
+        return out
+
+    def snapshots(self, comp: int) -> np.ndarray:
+        k = self._L.sofi_snapshot_count(self._h)
+        nxs, nys = self.problem.snapshot_dims()
+        out = np.zeros((k, nxs, nys), dtype=np.float32)
+        for q in range(k):
+            _chk_hip(self._L.sofi_get_snapshot(self._h, q, comp, _as_float_ptr(out[q])))
+        return out
+
+    def field(self, fid: int) -> np.ndarray:
+        out = np.zeros(self.problem.plane, dtype=np.float32)
+        _chk_hip(self._L.sofi_get_field(self._h, fid, _as_float_ptr(out)))
+        return out.reshape(N.rows(self.desc.ny), N.pitch(self.desc.nx))
+
+
+def interior(a: np.ndarray, nx: int, ny: int) -> np.ndarray:
+    """[ny][nx] view of the interior of a padded-layout plane."""
+    a = a.reshape(N.rows(ny), N.pitch(nx))
+    return a[N.PAD:N.PAD + ny, N.PAD:N.PAD + nx]
+
+
+def run_json(json_path: str, verbose: bool = True) -> dict:
+    """sofi2D <file.json> on the GPU: every shot, all output files."""
+    t0 = time.time()
+    prob = Problem(json_path)
+    sim = Simulation(prob)
+    loop_s = 0.0
+    snaps_written = 0
+    results = {}
+    for ishot in range(1, prob.nshots + 1):
+        if ishot > 1:
+            prob.set_shot(ishot)
+        prob.write_signal()
+        sim.load_shot()
+        ms = sim.run()
+        loop_s += ms * 1e-3
+        if verbose:
+            d = prob.desc
+            print(f"shot {ishot}: time loop {ms * 1e-3:.3f} s ({d.nx * d.ny * d.nt / (ms * 1e-3) / 1e9:.2f} GLUPS)")
+        secs = {}
+        for c in prob.seismo_comps():
+            secs[c] = sim.section(c)
+            prob.write_section(c, secs[c])
+        snaps = {}
+        comps = prob.snap_comps()
+        for c in comps:
+            snaps[c] = sim.snapshots(c)
+        if comps and len(snaps[comps[0]]):
+            for k in range(len(snaps[comps[0]])):
+                for c in comps:
+                    prob.write_snapshot(c, snaps_written == 0, snaps[c][k])
+                snaps_written += 1
+        results[ishot] = {"sections": secs, "snapshots": snaps, "ms": ms}
+    if verbose:
+        print(f"total time loop {loop_s:.3f} s, program {time.time() - t0:.3f} s")
+    sim.close()
+    prob.close()
+    return results

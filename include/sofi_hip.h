@@ -1,0 +1,99 @@
+/*
+ * sofi_hip.h - C-ABI of libsofi_hip.so: SOFI2D's velocity-stress time loop
+ * on one MI355X (gfx950).
+ *
+ * The reference's replaceable unit is
+ *     void time_loop(int iter, int ishot, int snapcheck, float *hc,
+ *                    float **srcpos_loc, float **signalx, float **signaly,
+ *                    int nsrc, int sw, AcqVar *acq, MemModel *mpm,
+ *                    MemWavefield *mpw, MemInv *minv, GlobVar *gv,
+ *                    GlobVarInv *vinv, Perform *perf)
+ * (src/fd.h:335-337, src/time_loop.c:27-326), called per shot from
+ * src/sofi2D.c:369 (forward) and src/inversion.c:222 (adjoint).  Its
+ * arguments are split here into plain-C calls: the grid/scheme descriptor
+ * (GlobVar scalars), the material arrays (MemModel), the sources and
+ * receivers (AcqVar, srcpos_loc, signals), and the outputs (GlobVar
+ * SECTION* buffers, snapshot files, MemInv gradients).  A handle owns the
+ * device memory of one shot's state (MemWavefield) and one HIP stream; it is
+ * not thread-safe.  Every call returns 0 on success or a negative code,
+ * with the message in sofi_last_error() (the reference aborts through
+ * log_fatal, src/logging.c:148-210).
+ *
+ * Arrays crossing this boundary are host pointers in the padded layout of
+ * sofi_types.h.
+ */
+#ifndef SOFI_HIP_H
+#define SOFI_HIP_H
+
+#include "sofi_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sofi_dev sofi_dev;
+
+/* Last error message of the calling thread. */
+const char *sofi_last_error(void);
+
+/* Allocate the device state for one grid/rheology; replaces initmem()
+ * (src/initmem.c:29-170) and initfd() (src/initfd.c:363-371) for the path. */
+int sofi_open(const sofi_desc *desc, sofi_dev **out);
+
+/* Upload the material arrays (enum sofi_array, NULL where unused) and
+ * profiles (enum sofi_profile); replaces the MemModel the stress and
+ * velocity kernels read (src/memm_struct.h:28-126) after prepmod()
+ * (src/prepmod.c:28-194) and PML_pro()/absorb(). */
+int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const *profiles);
+
+/* Sources of the shot: grid indices (1-based), per-source type
+ * (srcpos_loc[8]), azimuth/dip in degrees (srcpos_loc[7]), amplitude
+ * (srcpos_loc[6]) and signals[nsrc][nt]; replaces srcpos_loc/signals of
+ * time_loop (src/time_loop.c:27-29).  source_type is the SOURCE_TYPE that
+ * selects psource/eqsource (src/time_loop.c:248-253). */
+int sofi_set_sources(sofi_dev *h, int nsrc, const int *src_i, const int *src_j, const int *src_type,
+                     const float *src_azi, const float *src_amp, const float *signals, int source_type);
+
+/* Receivers (1-based grid indices); replaces acq->recpos_loc read by
+ * seismo_ssg (src/seismo_ssg.c:28-67).  Call after sofi_set_medium. */
+int sofi_set_receivers(sofi_dev *h, int ntr, const int *rec_i, const int *rec_j);
+
+/* Reserve device space for up to nsnap snapshots of the shot (snap(),
+ * src/snap_ssg.c:28-209). */
+int sofi_set_snapshots(sofi_dev *h, int nsnap);
+
+/* Zero the wavefield, memory variables, CPML memory, sections and the
+ * snapshot counter: zero_wavefield() (src/zero_wavefield.c:28-130). */
+int sofi_reset(sofi_dev *h);
+
+/* Advance time steps nt_first..nt_last (1-based, inclusive) of the forward
+ * loop; enqueues the work on the handle's stream and waits for it. */
+int sofi_run(sofi_dev *h, int nt_first, int nt_last);
+
+/* Device time of the last sofi_run in milliseconds (HIP events on the
+ * handle's stream). */
+double sofi_last_run_ms(const sofi_dev *h);
+
+/* Seismogram section of component comp (enum sofi_comp) as [ntr][ns]
+ * floats; replaces gv->SECTIONVX..SECTIONDIV. */
+int sofi_get_section(sofi_dev *h, int comp, float *out);
+
+/* Snapshots taken so far and one of them: component comp, x-major
+ * [nxs][nys] floats as written by snap() and merged by snapmerge. */
+int sofi_snapshot_count(const sofi_dev *h);
+int sofi_get_snapshot(sofi_dev *h, int k, int comp, float *out);
+
+/* Copy a wavefield array (0 vx, 1 vy, 2 sxx, 3 syy, 4 sxy) in the padded
+ * layout; mirrors reading mpw->pvx... after time_loop. */
+int sofi_get_field(sofi_dev *h, int id, float *out);
+
+/* Free the handle and its device memory. */
+int sofi_close(sofi_dev *h);
+
+/* Device bytes the handle holds (HBM footprint of one resident shot). */
+size_t sofi_footprint(const sofi_dev *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOFI_HIP_H */
