@@ -65,6 +65,7 @@ struct Fields {
     float *dvxx, *dvyy, *dvyx, *dvxy;           // TTI only
     float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
     float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
+    int *unstable;                             // first step whose start finds vy(NX/2, NY/2) = NaN
 };
 
 struct Mat {
@@ -189,11 +190,13 @@ constexpr int BX = 64, BY = 4;
 // (src/update_v_interior.c:92-98, src/update_v_PML.c, src/update_v_abs.c,
 // src/wavefield_update_v.c:46-47), one thread per cell.
 template <int ND>
-__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m)
+__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m, int nt)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
+    // instability guard of time_loop (src/time_loop.c:44-47), before this step's update
+    if (i == g.nx / 2 && j == g.ny / 2 && isnan(f.vy[IDX(g, i, j)]) && *f.unstable == 0) *f.unstable = nt;
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
     if (frame && g.abs_type != 1 && g.abs_type != 2) return;
     const int c = IDX(g, i, j);
@@ -374,17 +377,21 @@ __global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f)
     f.dvxy[c] = vxy;
 }
 
-// frame CPML of update_s_*_tti_PML, in place on the derivative arrays;
-// the elastic right-bottom corner loop stops at GY[4]-1
-// (src/update_s_elastic_TTI_PML.c:132 vs src/update_s_visc_TTI_PML.c:132)
+// frame CPML of update_s_*_tti_PML, in place on the derivative arrays.  The
+// reference treats the frame region by region (left, right, top, bottom, then
+// the four corners), CPML then update per region, so side cells read the
+// corner derivatives before the corner CPML.  CORNERS = false handles the
+// four sides, true the corners; the elastic right-bottom corner loop stops at
+// GY[4]-1 (src/update_s_elastic_TTI_PML.c:132 vs src/update_s_visc_TTI_PML.c:132)
+template <bool CORNERS>
 __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f, Mat m)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
-    if (!(xf || yf)) return;
-    if (g.weq == SOFI_EL_TTI && i > g.gx3 && j > g.gy3 && j == g.ny) return;
+    if (CORNERS ? !(xf && yf) : (xf == yf)) return;
+    if (CORNERS && g.weq == SOFI_EL_TTI && i > g.gx3 && j > g.gy3 && j == g.ny) return;
     const int c = IDX(g, i, j);
     float vxx = f.dvxx[c], vyx = f.dvyx[c], vxy = f.dvxy[c], vyy = f.dvyy[c];
     cpml_dv(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
@@ -396,9 +403,10 @@ __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f, Mat 
 
 // TTI stress update (src/wavefield_update_s_el_tti.c:27-52,
 // src/wavefield_update_s_visc_TTI.c:27-84) reading the derivative arrays.
-// PASS 0: interior cells; PASS 1: CPML frame on the ranges shrunk by two
-// cells at the outer edges (src/update_s_elastic_TTI_PML.c:37-143);
-// PASS 2: damping frame, every frame cell (src/update_s_elastic_TTI_abs.c).
+// PASS 0: interior cells; PASS 1 / 3: CPML frame sides / corners on the
+// ranges shrunk by two cells at the outer edges
+// (src/update_s_elastic_TTI_PML.c:37-143); PASS 2: damping frame, every frame
+// cell (src/update_s_elastic_TTI_abs.c).
 template <int WEQ, int PASS>
 __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f, Mat m)
 {
@@ -409,7 +417,9 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f, Mat m)
     if (PASS == 0 && frame) return;
     if (PASS != 0) {
         if (!frame) return;
-        if (PASS == 1) {
+        if (PASS == 1 && xf && yf) return;
+        if (PASS == 3 && !(xf && yf)) return;
+        if (PASS == 1 || PASS == 3) {
             if (i <= g.gx2 && i < 1 + 2) return;
             if (i > g.gx3 && i > g.nx - 2) return;
             if (j <= g.gy2 && j < 1 + 2) return;
@@ -771,6 +781,8 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
     bool visco = d.weq == SOFI_VEL_ISO || d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI;
     if (visco && d.L < 1) return fail("sofi_open: L>0 required for viscoelastic rheologies");
     if (d.fw < 0 || 2 * d.fw > d.nx || 2 * d.fw > d.ny) return fail("sofi_open: FW=%d does not fit the grid", d.fw);
+    if (d.free_surf && (d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI))
+        return fail("sofi_open: free surface with VEL_VTI/VEL_TTI is not supported (the reference reads unallocated tau arrays there)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("sofi_open: no HIP device available");
     sofi_dev *h = new (std::nothrow) sofi_dev();
@@ -812,6 +824,7 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
     rc |= dalloc_t(h, &f.sxx, pl);
     rc |= dalloc_t(h, &f.syy, pl);
     rc |= dalloc_t(h, &f.sxy, pl);
+    rc |= dalloc_t(h, &f.unstable, 1);
     if (visco) {
         rc |= dalloc_t(h, &f.r, pl * g.L);
         rc |= dalloc_t(h, &f.p, pl * g.L);
@@ -1082,6 +1095,7 @@ int sofi_reset(sofi_dev *h)
     for (float *p : { f.psi_syy_y, f.psi_sxy_y, f.psi_vyy, f.psi_vxy })
         if (p) HIPCHK(hipMemsetAsync(p, 0, nyp, h->stream));
     if (h->sec) HIPCHK(hipMemsetAsync(h->sec, 0, (size_t)SOFI_NCOMP * (h->r.n > 0 ? h->r.n : 1) * (h->d.ns > 0 ? h->d.ns : 1) * sizeof(float), h->stream));
+    HIPCHK(hipMemsetAsync(f.unstable, 0, sizeof(int), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->nsnap = 0;
     h->lsamp = h->d.ndt;
@@ -1103,7 +1117,7 @@ int step_nd(sofi_dev *h, int nt)
     hipStream_t st = h->stream;
     dim3 blk(BX, BY), grd((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
     if (h->s.n > 0) hipLaunchKernelGGL(k_force, dim3(1), dim3(64), 0, st, g, f, m, h->s, nt);
-    hipLaunchKernelGGL(k_velocity<ND>, grd, blk, 0, st, g, f, m);
+    hipLaunchKernelGGL(k_velocity<ND>, grd, blk, 0, st, g, f, m, nt);
     const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
     switch (d.weq) {
       case SOFI_EL_ISO: hipLaunchKernelGGL((k_stress<ND, SOFI_EL_ISO>), grd, blk, 0, st, g, f, m); break;
@@ -1116,9 +1130,12 @@ int step_nd(sofi_dev *h, int nt)
           if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), grd, blk, 0, st, g, f, m);
           else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), grd, blk, 0, st, g, f, m);
           if (d.fw > 0 && d.abs_type == 1) {
-              hipLaunchKernelGGL(k_cpml_dv_frame, grd, blk, 0, st, g, f, m);
+              hipLaunchKernelGGL(k_cpml_dv_frame<false>, grd, blk, 0, st, g, f, m);
               if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), grd, blk, 0, st, g, f, m);
               else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), grd, blk, 0, st, g, f, m);
+              hipLaunchKernelGGL(k_cpml_dv_frame<true>, grd, blk, 0, st, g, f, m);
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 3>), grd, blk, 0, st, g, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 3>), grd, blk, 0, st, g, f, m);
           } else if (d.fw > 0 && d.abs_type == 2) {
               if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), grd, blk, 0, st, g, f, m);
               else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), grd, blk, 0, st, g, f, m);
@@ -1193,6 +1210,9 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
+    int bad = 0;
+    HIPCHK(hipMemcpy(&bad, h->f.unstable, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad) return fail("Simulation is unstable: vy(NX/2, NY/2) is NaN at time step %d.", bad);
     return 0;
 }
 

@@ -1,0 +1,189 @@
+"""Synthetic inputs in the reference's file formats.
+
+``write_case`` writes one run directory: the pseudo-JSON parameter file
+(src/read_par_json.c keys), raw float32 x-major model files
+MFILE.{vp,vs,rho[,qp,qs][,epsilon,delta][,theta]} (src/readmod_*.c), a
+SOURCE_FILE ("x y td fc amp [azimuth] [type]" per line, src/sources.c:78-131)
+and, when receivers are listed explicitly, a REC_FILE ("x y" per line,
+src/receiver.c:68-74).  The ``config*`` functions build the benchmark
+workloads with seeded generators (numpy PCG64).
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+WEQ = ("AC_ISO", "VAC_ISO", "EL_ISO", "VEL_ISO", "EL_VTI", "VEL_VTI", "EL_TTI", "VEL_TTI")
+
+
+@dataclass
+class Case:
+    name: str
+    nx: int
+    ny: int
+    dh: float
+    dt: float
+    time: float
+    weq: str = "EL_ISO"
+    fdorder: int = 4
+    maxrelerror: int = 0
+    L: int = 0
+    fl: tuple = (15.0,)
+    f_ref: float = 15.0
+    abs_type: int = 1
+    fw: int = 20
+    free_surf: int = 0
+    damping: float = 8.0
+    npower: float = 4.0
+    k_max_cpml: float = 1.0
+    fpml: float = 15.0
+    vppml: float = 3500.0
+    source_type: int = 1
+    source_shape: int = 1
+    sources: list = field(default_factory=list)    # tuples (x, y, td, fc, amp[, azimuth, type])
+    receivers: list | None = None                  # explicit (x, y) list, or None for a line
+    rec_line: tuple = (0.0, 0.0, 0.0, 0.0, 1.0)      # XREC1, YREC1, XREC2, YREC2, NGEOPH
+    seismo: int = 1
+    ndt: int = 1
+    snap: int = 0
+    tsnap: tuple = (0.0, 0.0, 0.0)
+    idx: int = 1
+    idy: int = 1
+    run_multiple_shots: int = 0
+    mode: str = "FW"
+    extra: dict = field(default_factory=dict)
+
+
+def _json_lines(c: Case, d: str) -> str:
+    rel = lambda *p: "./" + "/".join(p)
+    kv = [
+        ("NPROCX", 1), ("NPROCY", 1), ("MODE", c.mode),
+        ("FDORDER", c.fdorder), ("FDORDER_TIME", 2), ("MAXRELERROR", c.maxrelerror),
+        ("NX", c.nx), ("NY", c.ny), ("DH", repr(float(c.dh))),
+        ("TIME", repr(float(c.time))), ("DT", repr(float(c.dt))),
+        ("WEQ", c.weq),
+        ("SOURCE_SHAPE", c.source_shape), ("SOURCE_TYPE", c.source_type), ("SIGOUT", 0),
+        ("SRCREC", 1), ("SOURCE_FILE", rel("sources.dat")), ("RUN_MULTIPLE_SHOTS", c.run_multiple_shots),
+        ("READMOD", 1), ("MFILE", rel("model", "m")), ("WRITE_MODELFILES", 0),
+        ("L", c.L), ("F_REF", repr(float(c.f_ref))),
+        ("FREE_SURF", c.free_surf), ("BOUNDARY", 0), ("FW", c.fw), ("ABS_TYPE", c.abs_type),
+        ("NPOWER", repr(float(c.npower))), ("K_MAX_CPML", repr(float(c.k_max_cpml))),
+        ("VPPML", repr(float(c.vppml))), ("FPML", repr(float(c.fpml))), ("DAMPING", repr(float(c.damping))),
+        ("SNAP", c.snap), ("TSNAP1", repr(float(c.tsnap[0]))), ("TSNAP2", repr(float(c.tsnap[1]))),
+        ("TSNAPINC", repr(float(c.tsnap[2]))), ("IDX", c.idx), ("IDY", c.idy), ("SNAP_FORMAT", 3),
+        ("SNAP_FILE", rel("out", "snap")),
+        ("SEISMO", c.seismo), ("READREC", 1 if c.receivers is not None else 0),
+        ("REC_FILE", rel("receivers.dat")),
+        ("XREC1", repr(float(c.rec_line[0]))), ("YREC1", repr(float(c.rec_line[1]))),
+        ("XREC2", repr(float(c.rec_line[2]))), ("YREC2", repr(float(c.rec_line[3]))),
+        ("NGEOPH", repr(float(c.rec_line[4]))),
+        ("REFRECX", "0.0"), ("REFRECY", "0.0"), ("REC_ARRAY", 0),
+        ("NDT", c.ndt), ("SEIS_FORMAT", 1), ("SEIS_FILE", rel("out", "seis")),
+        ("LOG_FILE", rel("out", "log")), ("LOG", 0), ("OUT_TIMESTEP_INFO", 1000000),
+    ]
+    for l, f in enumerate(c.fl[: c.L], start=1):
+        kv.append((f"FL{l}", repr(float(f))))
+    kv += list(c.extra.items())
+    body = ",\n".join(f'\t"{k}" : "{v}"' for k, v in kv)
+    return "{\n" + body + "\n}\n"
+
+
+def write_case(d: str, c: Case, model: dict) -> str:
+    """Write the case into directory d; model maps suffix -> [nx][ny] array."""
+    os.makedirs(os.path.join(d, "model"), exist_ok=True)
+    os.makedirs(os.path.join(d, "out"), exist_ok=True)
+    for k, a in model.items():
+        a = np.asarray(a, dtype=np.float32)
+        assert a.shape == (c.nx, c.ny), (k, a.shape)
+        a.tofile(os.path.join(d, "model", f"m.{k}"))      # x-major: NY floats per x position
+    with open(os.path.join(d, "sources.dat"), "w") as f:
+        for s in c.sources:
+            f.write(" ".join(repr(float(v)) for v in s) + "\n")
+    if c.receivers is not None:
+        with open(os.path.join(d, "receivers.dat"), "w") as f:
+            for x, y in c.receivers:
+                f.write(f"{float(x)!r} {float(y)!r}\n")
+    path = os.path.join(d, f"{c.name}.json")
+    with open(path, "w") as f:
+        f.write(_json_lines(c, d))
+    return path
+
+
+# ----------------------------------------------------------------- models
+
+def smooth_noise(rng, nx, ny, sigma_cells):
+    """White noise smoothed by a separable Gaussian (FFT), normalised to unit std."""
+    w = rng.standard_normal((nx, ny))
+    kx = np.fft.fftfreq(nx)[:, None]
+    ky = np.fft.rfftfreq(ny)[None, :]
+    g = np.exp(-2.0 * (math.pi * sigma_cells) ** 2 * (kx ** 2 + ky ** 2))
+    s = np.fft.irfft2(np.fft.rfft2(w) * g, s=(nx, ny))
+    s -= s.mean()
+    return s / (s.std() + 1e-30)
+
+
+def layered_model(nx, ny, dh, rng, nlayers=12, vp_top=1800.0, vp_bottom=4500.0, pert=0.05,
+                  corr_m=50.0, vp_max=None):
+    """Flat layers with vp linear in layer index, vs = vp/1.73, rho = 310 vp^0.25,
+    plus a smoothed Gaussian perturbation of std `pert` clipped at 3 sigma."""
+    depth = np.arange(ny) * dh
+    layer = np.minimum((depth / (ny * dh) * nlayers).astype(int), nlayers - 1)
+    vp1 = vp_top + (vp_bottom - vp_top) * layer / max(nlayers - 1, 1)
+    vp = np.repeat(vp1[None, :], nx, axis=0)
+    if pert > 0:
+        r = np.clip(smooth_noise(rng, nx, ny, corr_m / dh), -3.0, 3.0)
+        vp = vp * (1.0 + pert * r)
+    if vp_max is not None:
+        vp = np.minimum(vp, vp_max)
+    vs = vp / 1.73
+    rho = 310.0 * vp ** 0.25
+    return vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32)
+
+
+def q_from_tau(tau, L):
+    """Q giving the reference's tau = 2/(Q L) (src/readmod_visco.c:156-157)."""
+    return 2.0 / (tau * L)
+
+
+# ---------------------------------------------------------------- configs
+
+def config1(d: str) -> str:
+    """Isotropic elastic, 400 x 400, DH 5 m, FD4 Holberg, DT 5e-4 s, NT 2000,
+    homogeneous vp 3500 / vs 2000 / rho 2000, 15 Hz Ricker explosive source at
+    the centre, 20-point CPML, 100 receivers on a line 50 m below the top."""
+    nx = ny = 400
+    c = Case(name="c1", nx=nx, ny=ny, dh=5.0, dt=5e-4, time=1.0, weq="EL_ISO", fdorder=4, maxrelerror=1,
+             abs_type=1, fw=20, fpml=15.0, vppml=3500.0,
+             sources=[(997.5, 997.5, 0.0, 15.0, 1.0)], seismo=1, ndt=1,
+             rec_line=(200.0, 50.0, 1685.0, 50.0, 3.0))
+    model = {"vp": np.full((nx, ny), 3500.0), "vs": np.full((nx, ny), 2000.0), "rho": np.full((nx, ny), 2000.0)}
+    return write_case(d, c, model)
+
+
+C2_FL = (5.0, 15.0, 45.0)
+
+
+def config2(d: str, nx=4000, ny=2000, nt=8000, seed=2) -> str:
+    """Viscoelastic L=3 (tau=0.05 at 15 Hz), FD8, DH 2.5 m, DT 2.5e-4 s, free
+    surface + 20-point CPML; 12 flat layers vp 1800..4500 m/s with a 5 %
+    smoothed Gaussian perturbation (50 m correlation length, seed 2),
+    vs = vp/1.73, rho = 310 vp^0.25; 15 Hz Ricker explosive source at
+    x = 5000 m, y = 25 m; 1000 receivers at y = 12.5 m every 2 cells from 250 m.
+    The perturbed vp is clipped at 5080 m/s so the unrelaxed velocity stays
+    below the FD8 stability limit checked by checkfd."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    dh = 2.5
+    vp, vs, rho = layered_model(nx, ny, dh, rng, vp_max=5080.0)
+    L, tau = 3, 0.05
+    q = np.full((nx, ny), q_from_tau(tau, L), dtype=np.float32)
+    x_src = min(5000.0, (nx - 1) * dh * 0.5)
+    ntr = min(1000, max(1, (nx - 2 * 20 - 100) // 2))
+    x1 = 250.0 if nx > 400 else 20 * dh
+    c = Case(name="c2", nx=nx, ny=ny, dh=dh, dt=2.5e-4, time=nt * 2.5e-4, weq="VEL_ISO", fdorder=8,
+             maxrelerror=1, L=L, fl=C2_FL, f_ref=15.0, abs_type=1, fw=20, free_surf=1, fpml=15.0,
+             vppml=float(vp.max()), sources=[(x_src, 25.0, 0.0, 15.0, 1.0)], seismo=1, ndt=4,
+             rec_line=(x1, 12.5, x1 + 2 * dh * (ntr - 1), 12.5, 2.0))
+    return write_case(d, c, {"vp": vp, "vs": vs, "rho": rho, "qp": q, "qs": q})

@@ -533,26 +533,28 @@ static void update_s(oracle *o)
                     s_cell(o, i, j);
                 }
     } else if (d->abs_type == 1 && tti) {
-        /* update_s_*_tti_PML: CPML over the whole frame first (the elastic
-         * right-bottom corner loop stops at GY[4]-1), then the stress update
-         * on ranges shrunk by two cells at the outer edges
-         * (src/update_s_elastic_TTI_PML.c:32-145, src/update_s_visc_TTI_PML.c:132) */
-        for (int j = d->gy[1]; j <= d->gy[4]; j++)
-            for (int i = d->gx[1]; i <= d->gx[4]; i++) {
-                if (!in_frame(o, i, j)) continue;
-                if (d->weq == SOFI_EL_TTI && i > d->gx[3] && j > d->gy[3] && j == d->gy[4]) continue;
-                cpml_s_cell(o, i, j);
-            }
-        for (int j = d->gy[1]; j <= d->gy[4]; j++)
-            for (int i = d->gx[1]; i <= d->gx[4]; i++) {
-                if (!in_frame(o, i, j)) continue;
-                int xl = i <= d->gx[2], xr = i > d->gx[3], yt = j <= d->gy[2], yb = j > d->gy[3];
-                if (xl && i < d->gx[1] + 2) continue;
-                if (xr && i > d->gx[4] - 2) continue;
-                if (yt && j < d->gy[1] + 2) continue;
-                if (yb && j > d->gy[4] - 2) continue;
-                s_cell(o, i, j);
-            }
+        /* update_s_*_tti_PML works region by region, in the order left,
+         * right, top, bottom, left-top, left-bottom, right-top, right-bottom:
+         * the CPML over the region, then the stress update on the region
+         * shrunk by two cells at the outer edges.  Side cells therefore read
+         * corner derivatives before the corner CPML has modified them.  The
+         * elastic right-bottom CPML loop stops at GY[4]-1
+         * (src/update_s_elastic_TTI_PML.c:32-145, src/update_s_visc_TTI_PML.c:132). */
+        const int X[3][2] = { { d->gx[1], d->gx[2] }, { d->gx[2] + 1, d->gx[3] }, { d->gx[3] + 1, d->gx[4] } };
+        const int Y[3][2] = { { d->gy[1], d->gy[2] }, { d->gy[2] + 1, d->gy[3] }, { d->gy[3] + 1, d->gy[4] } };
+        /* (x band, y band): 0 = low frame, 1 = inner, 2 = high frame */
+        const int order[8][2] = { { 0, 1 }, { 2, 1 }, { 1, 0 }, { 1, 2 }, { 0, 0 }, { 0, 2 }, { 2, 0 }, { 2, 2 } };
+        for (int r = 0; r < 8; ++r) {
+            int bx = order[r][0], by = order[r][1];
+            int jc1 = Y[by][1];
+            if (d->weq == SOFI_EL_TTI && bx == 2 && by == 2) jc1 = d->gy[4] - 1;
+            for (int j = Y[by][0]; j <= jc1; j++)
+                for (int i = X[bx][0]; i <= X[bx][1]; i++) cpml_s_cell(o, i, j);
+            int i0 = X[bx][0] + (bx == 0 ? 2 : 0), i1 = X[bx][1] - (bx == 2 ? 2 : 0);
+            int j0 = Y[by][0] + (by == 0 ? 2 : 0), j1 = Y[by][1] - (by == 2 ? 2 : 0);
+            for (int j = j0; j <= j1; j++)
+                for (int i = i0; i <= i1; i++) s_cell(o, i, j);
+        }
     } else if (d->abs_type == 2) {
         for (int j = d->gy[1]; j <= d->gy[4]; j++)
             for (int i = d->gx[1]; i <= d->gx[4]; i++)

@@ -1,0 +1,163 @@
+"""Test cases shared by the fixture generator and the tests.
+
+* ``materialize_examples`` rebuilds the reference's example run directory
+  (homogeneous TTI_Thomsen_1 model from its stored bit patterns, source and
+  receiver files, weq*.json) in a scratch directory.
+* ``CASES``: small seeded cases in the reference's formats that exercise
+  what the shipped goldens do not (all goldens are FD4, damping frame, no
+  free surface).
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from gpi_project_amd import workloads as W  # noqa: E402
+
+EXAMPLES = os.path.join(HERE, "examples")
+COMPS = ("vx", "vy", "p", "div", "curl")
+
+
+def materialize_examples(ex_dir: str, td: str):
+    bits = json.load(open(os.path.join(ex_dir, "model_bits.json")))
+    os.makedirs(os.path.join(td, "model"), exist_ok=True)
+    for s in ("vp", "vs", "rho", "qp", "qs", "epsilon", "delta", "theta"):
+        np.full(bits["n"], bits[s], dtype=np.uint32).view(np.float32).tofile(os.path.join(td, "model", f"TTI_Thomsen_1.{s}"))
+    for sub in ("sources", "receiver"):
+        os.makedirs(os.path.join(td, sub), exist_ok=True)
+        for f in os.listdir(os.path.join(ex_dir, sub)):
+            shutil.copy(os.path.join(ex_dir, sub, f), os.path.join(td, sub, f))
+    for f in os.listdir(ex_dir):
+        if f.endswith(".json") and f.startswith("weq"):
+            shutil.copy(os.path.join(ex_dir, f), os.path.join(td, f))
+    os.makedirs(os.path.join(td, "log"), exist_ok=True)
+
+
+def example_json_1rank(ex_dir: str, td: str, name: str) -> str:
+    """Copy of examples/<name>.json for one process writing to td/out_<name>/."""
+    txt = open(os.path.join(ex_dir, name + ".json")).read()
+    txt = re.sub(r'"NPROCX"\s*:\s*"\d+"', '"NPROCX" : "1"', txt)
+    txt = re.sub(r'"NPROCY"\s*:\s*"\d+"', '"NPROCY" : "1"', txt)
+    base = name.replace("_fy", "")
+    txt = txt.replace(f"./{base}/", f"./out_{name}/")
+    os.makedirs(os.path.join(td, f"out_{name}"), exist_ok=True)
+    path = os.path.join(td, f"r1_{name}.json")
+    open(path, "w").write(txt)
+    return path
+
+
+# ---------------------------------------------------------------- cases
+
+NX, NY, DH = 160, 120, 5.0
+
+
+def _model(weq: str, seed: int, nx=NX, ny=NY, dh=DH, L=0, tau=0.05):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vp, vs, rho = W.layered_model(nx, ny, dh, rng, nlayers=6, vp_top=2000.0, vp_bottom=3500.0, pert=0.05, corr_m=25.0)
+    m = {"vp": vp, "vs": vs, "rho": rho}
+    if weq.startswith("VEL"):
+        q = np.full((nx, ny), W.q_from_tau(tau, L), dtype=np.float32)
+        m["qp"] = q
+        m["qs"] = (q * 0.8).astype(np.float32)
+    if "VTI" in weq or "TTI" in weq:
+        m["epsilon"] = (0.25 * (0.5 + 0.5 * np.tanh(W.smooth_noise(rng, nx, ny, 6.0)))).astype(np.float32)
+        m["delta"] = (-0.1 + 0.25 * (0.5 + 0.5 * np.tanh(W.smooth_noise(rng, nx, ny, 6.0)))).astype(np.float32)
+    if "TTI" in weq:
+        m["theta"] = (45.0 * (0.5 + 0.5 * np.tanh(W.smooth_noise(rng, nx, ny, 8.0)))).astype(np.float32)
+    return m
+
+
+def _rec_grid(nx, ny, dh, extra=()):
+    """receivers: a line at depth, points in every frame side and corner, row 1"""
+    pts = [(x * dh, 30.0) for x in range(4, nx - 4, 6)]
+    pts += [(1.0, 1.0), ((nx - 1) * dh - 1.0, (ny - 1) * dh - 1.0), (3 * dh, (ny // 2) * dh),
+            ((nx - 3) * dh, (ny // 2) * dh), ((nx // 2) * dh, 2 * dh), ((nx // 2) * dh, (ny - 2) * dh),
+            ((nx - 2) * dh, (ny - 1) * dh), (0.0, 0.0), ((nx // 3) * dh, 0.0)]
+    pts += list(extra)
+    seen, out = set(), []
+    for x, y in pts:
+        k = (int(x // dh), int(y // dh))
+        if k not in seen:
+            seen.add(k)
+            out.append((x, y))
+    return out
+
+
+def _src(nx, ny, dh, fc=20.0, amp=1.0, extra=()):
+    return [(((nx // 2) * dh + 1.0), ((ny // 2) * dh + 1.0), 0.0, fc, amp, *extra)]
+
+
+CASES = {
+    # name: (weq, fdorder, maxrelerror, L, abs_type, free_surf, source_type, extra source cols, seismo, seed)
+    "el_fd2_cpml": ("EL_ISO", 2, 0, 0, 1, 0, 1, (), 4, 11),
+    "el_fd6_h1_cpml_fs": ("EL_ISO", 6, 1, 0, 1, 1, 1, (), 4, 12),
+    "el_fd8_h2_abs_fs": ("EL_ISO", 8, 2, 0, 2, 1, 1, (), 4, 13),
+    "el_fd10_h3_cpml_fx": ("EL_ISO", 10, 3, 0, 1, 0, 2, (), 4, 14),
+    "el_fd12_h4_cpml_fy_fs": ("EL_ISO", 12, 4, 0, 1, 1, 3, (), 4, 15),
+    "el_fd4_cpml_eq": ("EL_ISO", 4, 0, 0, 1, 0, 5, (35.0, 5), 4, 16),
+    "vel_l3_fd8_cpml_fs": ("VEL_ISO", 8, 1, 3, 1, 1, 1, (), 4, 21),
+    "vel_l2_fd4_abs_custom": ("VEL_ISO", 4, 0, 2, 2, 0, 4, (60.0, 4), 4, 22),
+    "vel_l5_fd6_cpml_eq": ("VEL_ISO", 6, 2, 5, 1, 0, 5, (20.0, 5), 4, 23),
+    "vel_l1_fd2_cpml_fs": ("VEL_ISO", 2, 0, 1, 1, 1, 1, (), 4, 24),
+    "elvti_fd6_cpml": ("EL_VTI", 6, 1, 0, 1, 0, 1, (), 4, 31),
+    "velvti_l2_fd4_cpml": ("VEL_VTI", 4, 0, 2, 1, 0, 1, (), 4, 32),
+    "elvti_fd8_abs_fx": ("EL_VTI", 8, 1, 0, 2, 0, 2, (), 4, 33),
+    "eltti_fd6_h1_cpml": ("EL_TTI", 6, 1, 0, 1, 0, 1, (), 4, 41),
+    "eltti_fd4_cpml": ("EL_TTI", 4, 0, 0, 1, 0, 1, (), 4, 42),
+    "veltti_l1_fd4_cpml": ("VEL_TTI", 4, 0, 1, 1, 0, 1, (), 4, 43),
+    "veltti_l3_fd8_abs": ("VEL_TTI", 8, 1, 3, 2, 0, 3, (), 4, 44),
+    "eltti_fd2_abs": ("EL_TTI", 2, 0, 0, 2, 0, 1, (), 4, 45),
+}
+
+# EL_VTI/EL_TTI with a free surface: surface_elastic() reads pu (= c55) and
+# ppi (= c11 already multiplied by DT in prepmod), so the sxx correction is
+# off by ~1/DT and the run diverges; the reference aborts with "Simulation is
+# unstable" (src/surface_elastic.c:80-86, src/prepmod.c:134-139,
+# src/time_loop.c:44-47).  Kept as a quirk case: the step of the abort is the
+# fixture.
+UNSTABLE = {
+    "elvti_fd6_cpml_fs": ("EL_VTI", 6, 1, 0, 1, 1, 1, (), 4, 31),
+}
+
+NT_CASE = 500
+DT_CASE = 4.0e-4
+
+
+def case_def(name: str) -> tuple[W.Case, dict]:
+    weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = CASES[name] if name in CASES else UNSTABLE[name]
+    model = _model(weq, seed, L=L)
+    c = W.Case(name=name, nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdo,
+               maxrelerror=mre, L=L, fl=(4.0, 12.0, 30.0, 60.0, 90.0)[:max(L, 1)], f_ref=20.0,
+               abs_type=absn, fw=12, free_surf=fs, damping=8.0, fpml=20.0, vppml=3500.0,
+               source_type=st, sources=_src(NX, NY, DH, extra=extra), receivers=_rec_grid(NX, NY, DH),
+               seismo=seismo, ndt=2, snap=4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
+               idx=2, idy=3)
+    return c, model
+
+
+def build(name: str, td: str):
+    c, model = case_def(name)
+    return W.write_case(td, c, model), c
+
+
+def nsnap(c: W.Case) -> int:
+    it = lambda x: int(np.floor(x + 0.5))
+    ls, inc, es = it(np.float32(c.tsnap[0]) / np.float32(c.dt)), it(np.float32(c.tsnap[2]) / np.float32(c.dt)), \
+        it(np.float32(c.tsnap[1]) / np.float32(c.dt))
+    nt = it(np.float32(c.time) / np.float32(c.dt))
+    n = 0
+    for t in range(1, nt + 1):
+        if t == ls and t <= es:
+            n += 1
+            ls += inc
+    return n
