@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the SOFI2D time loop on one MI355X.
+
+Workload (headline configuration): viscoelastic isotropic, L = 3 relaxation
+mechanisms (tau = 0.05 at 15 Hz), FD order 8 (Holberg 0.1 %), 4000 x 2000
+grid, DH = 2.5 m, DT = 2.5e-4 s, NT = 8000, free surface + 20-point CPML,
+12 layers with a seeded 5 % Gaussian perturbation, 15 Hz Ricker explosive
+source, 1000 receivers (gpi_project_amd.workloads.config2).
+
+A "step" is one time step of the whole loop (velocity update, stress update,
+source, free surface, receiver sampling).  W warm-up steps run on a shot that
+is then reset; the K timed steps are steps 1..K of a fresh shot, so with the
+default K = NT the timed region is the complete time loop.  Prints one JSON
+line with GLUPS (NX*NY*K / t / 1e9), the HBM roofline of the dominant kernel
+(the stress update, HIP events on the handle's stream) and the reference's
+CPU time loop on this host's cores as a baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GLUPS & %HBM roofline (8th-order visco); batched shots/s, p50/p99 shot latency"
+
+
+def algorithmic_bytes(L: int):
+    """Per cell and step, two-sweep leapfrog, perfect stencil reuse:
+    velocity kernel reads sxx syy sxy vx vy rip rjp, writes vx vy (9 floats);
+    stress kernel reads vx vy sxx syy sxy, r p q (3L), f g fipjp and three
+    defect bases, writes sxx syy sxy and r p q (20 + 6L floats)."""
+    vel = 4 * (7 + 2)
+    stress = 4 * (5 + 3 * L + 6 + 3 + 3 * L)
+    return vel, stress
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(case_dir: str, json_name: str, nx: int, ny: int, steps: int, cores: int):
+    """The reference built from its own sources (oracle/_ref/sofi2D_ref) run
+    with MPI on `cores` host cores for `steps` time steps of the same
+    workload; falls back to the FP32 oracle on one core if it cannot run."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "sofi2D_ref")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    px = cores
+    while px > 1 and (nx % px or cores % px or ny % (cores // px)):
+        px -= 1
+    py = cores // px
+    txt = open(os.path.join(case_dir, json_name)).read()
+    dt = 2.5e-4
+    txt = txt.replace('"NPROCX" : "1"', f'"NPROCX" : "{px}"').replace('"NPROCY" : "1"', f'"NPROCY" : "{py}"')
+    import re
+    txt = re.sub(r'"TIME" : "[^"]*"', f'"TIME" : "{steps * dt!r}"', txt)
+    js = os.path.join(case_dir, "cpu_" + json_name)
+    open(js, "w").write(txt)
+    if os.path.exists(ref) and os.path.exists(mpiexec):
+        try:
+            log(f"cpu baseline: reference on {px * py} ranks ({px}x{py}), {steps} steps")
+            r = subprocess.run([mpiexec, "-np", str(px * py), ref, os.path.basename(js)], cwd=case_dir,
+                               capture_output=True, text=True, timeout=600)
+            t = [float(l.split()[2]) for l in r.stdout.splitlines() if l.startswith("REF_TIME_LOOP")]
+            if r.returncode == 0 and t:
+                return {"value": nx * ny * steps / t[0] / 1e9, "unit": "GLUPS", "cores": px * py, "kind": "reference",
+                        "sample": f"{steps} time steps of the same {nx}x{ny} workload, reference time_loop wall time "
+                                  f"{t[0]:.2f} s on {px}x{py} MPI ranks"}
+            log("reference run failed: " + (r.stdout[-500:] + r.stderr[-500:]))
+        except Exception as e:  # noqa: BLE001
+            log(f"reference run failed: {e}")
+    # fallback: the CPU oracle on one core
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from gpi_project_amd.sofi import Problem
+    cwd = os.getcwd()
+    os.chdir(case_dir)
+    try:
+        P = Problem(json_name)
+        o = pyoracle.Oracle(P, "f32")
+        n = max(2, steps // 10)
+        t0 = time.perf_counter()
+        o.run(1, n)
+        t = time.perf_counter() - t0
+        o.close()
+        P.close()
+    finally:
+        os.chdir(cwd)
+    return {"value": nx * ny * n / t / 1e9, "unit": "GLUPS", "cores": 1, "kind": "port",
+            "sample": f"{n} time steps of the same workload on the FP32 CPU oracle"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8000, help="timed time steps (default: the whole NT = 8000 loop)")
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--nx", type=int, default=4000)
+    ap.add_argument("--ny", type=int, default=2000)
+    ap.add_argument("--nt", type=int, default=8000)
+    ap.add_argument("--time-every", type=int, default=8, help="time every n-th step's kernels with HIP events")
+    ap.add_argument("--cpu-steps", type=int, default=160, help="time steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs with no "
+              "communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps > args.nt:
+        args.nt = args.steps
+
+    from gpi_project_amd import workloads
+    from gpi_project_amd.sofi import Problem, Simulation
+
+    td = tempfile.mkdtemp(prefix="sofi_bench_")
+    t0 = time.time()
+    js = workloads.config2(td, nx=args.nx, ny=args.ny, nt=args.nt)
+    log(f"workload written in {time.time() - t0:.1f} s: {js}")
+    cwd = os.getcwd()
+    os.chdir(td)
+    t0 = time.time()
+    P = Problem(os.path.basename(js))
+    d = P.desc
+    log(f"host setup {time.time() - t0:.1f} s: {d.nx}x{d.ny}, NT={d.nt}, FD{d.fdorder}, L={d.L}, {P.receivers()[0].size} receivers")
+    sim = Simulation(P)
+    log(f"device footprint {sim.footprint / 2**30:.2f} GiB")
+    # warm-up on a shot that is then discarded
+    sim.load_shot(nsnap=0)
+    if args.warmup > 0:
+        sim.run(1, min(args.warmup, d.nt))
+    sim.load_shot(nsnap=0)
+    nmax = args.steps // max(args.time_every, 1) + 1
+    sim.set_kernel_timing(args.time_every, nmax)
+    log(f"timed region: {args.steps} steps")
+    t_wall0 = time.perf_counter()
+    ms = sim.run(1, args.steps)          # returns after the handle's stream has drained
+    t_wall = time.perf_counter() - t_wall0
+    kt = sim.kernel_timing()
+    log(f"time loop {ms:.1f} ms (wall {t_wall * 1e3:.1f} ms); kernels {kt}")
+    cells = d.nx * d.ny
+    glups = cells * args.steps / (ms * 1e-3) / 1e9
+    b_vel, b_str = algorithmic_bytes(d.L)
+    achieved = b_str * cells / (kt["stress_ms"] * 1e-3) / 1e9 if kt["stress_ms"] > 0 else None
+    step_gbs = (b_vel + b_str) * cells * args.steps / (ms * 1e-3) / 1e9
+    vel_gbs = b_vel * cells / (kt["velocity_ms"] * 1e-3) / 1e9 if kt["velocity_ms"] > 0 else None
+    # a short check that the run stayed finite
+    import numpy as np
+    secs = sim.section(0)
+    assert np.isfinite(secs).all()
+    sim.close()
+    P.close()
+    os.chdir(cwd)
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(td, os.path.basename(js), d.nx, d.ny, args.cpu_steps, args.cpu_cores)
+    shutil.rmtree(td, ignore_errors=True)
+    out = {
+        "metric": METRIC,
+        "value": round(glups, 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded layered model with a 5 % Gaussian perturbation (workloads.config2)",
+        "config": {"workload": "viscoelastic L=3 FD8 4000x2000 NT=8000, free surface + CPML, 1000 receivers",
+                   "nx": d.nx, "ny": d.ny, "nt": d.nt, "fdorder": d.fdorder, "L": d.L, "weq": "VEL_ISO",
+                   "timed_steps": args.steps},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": None, "kernel": "stress update (k_stress VEL_ISO)",
+                     "algorithmic_bytes_per_cell": b_str, "kernel_ms": round(kt["stress_ms"], 5),
+                     "velocity_kernel_gbs": round(vel_gbs, 1) if vel_gbs else None,
+                     "whole_step_gbs": round(step_gbs, 1), "whole_step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,8 @@ def hip():
     L.sofi_close.argtypes = [_VP]
     L.sofi_footprint.restype = C.c_size_t
     L.sofi_footprint.argtypes = [_VP]
+    L.sofi_set_kernel_timing.argtypes = [_VP, C.c_int, C.c_int]
+    L.sofi_get_kernel_timing.argtypes = [_VP, _DP, _DP, _IP]
     _hip = L
     return L
 
@@ -151,5 +153,6 @@ def exported_symbols():
         "libsofi_hip.so": ["sofi_last_error", "sofi_open", "sofi_set_medium", "sofi_set_sources",
                            "sofi_set_receivers", "sofi_set_snapshots", "sofi_reset", "sofi_run",
                            "sofi_last_run_ms", "sofi_get_section", "sofi_snapshot_count",
-                           "sofi_get_snapshot", "sofi_get_field", "sofi_close", "sofi_footprint"],
+                           "sofi_get_snapshot", "sofi_get_field", "sofi_close", "sofi_footprint",
+                           "sofi_set_kernel_timing", "sofi_get_kernel_timing"],
     }

@@ -714,6 +714,12 @@ struct sofi_dev {
     int nsnap_max = 0, nsnap = 0, nxs = 0, nys = 0;
     int lsamp = 0, lsnap = 0;
     bool medium_set = false;
+    // sampled per-kernel timing (HIP events around the velocity and stress kernels)
+    int time_every = 0;                    // 0 = off; else every time_every-th step is timed
+    std::vector<hipEvent_t> tev;           // 4 events per timed step
+    int tev_used = 0;
+    double t_vel_ms = 0.0, t_str_ms = 0.0;
+    int t_samples = 0;
     // host copies needed for receiver factors
     std::vector<float> h_ppi, h_pu;
 };
@@ -1117,7 +1123,17 @@ int step_nd(sofi_dev *h, int nt)
     hipStream_t st = h->stream;
     dim3 blk(BX, BY), grd((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
     if (h->s.n > 0) hipLaunchKernelGGL(k_force, dim3(1), dim3(64), 0, st, g, f, m, h->s, nt);
+    hipEvent_t *te = nullptr;
+    if (h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
+        te = &h->tev[h->tev_used];
+        h->tev_used += 4;
+    }
+    if (te) (void)hipEventRecord(te[0], st);
     hipLaunchKernelGGL(k_velocity<ND>, grd, blk, 0, st, g, f, m, nt);
+    if (te) {
+        (void)hipEventRecord(te[1], st);
+        (void)hipEventRecord(te[2], st);
+    }
     const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
     switch (d.weq) {
       case SOFI_EL_ISO: hipLaunchKernelGGL((k_stress<ND, SOFI_EL_ISO>), grd, blk, 0, st, g, f, m); break;
@@ -1142,6 +1158,7 @@ int step_nd(sofi_dev *h, int nt)
           }
       }
     }
+    if (te) (void)hipEventRecord(te[3], st);
     const bool sample = d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
     const int ins = h->lsamp / (d.ndt > 0 ? d.ndt : 1);
     if (sample && (d.seismo == 3 || d.seismo == 4)) {
@@ -1210,6 +1227,15 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms;
+    for (int k = 0; k + 4 <= h->tev_used; k += 4) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, h->tev[k], h->tev[k + 1]));
+        HIPCHK(hipEventElapsedTime(&b, h->tev[k + 2], h->tev[k + 3]));
+        h->t_vel_ms += a;
+        h->t_str_ms += b;
+        h->t_samples++;
+    }
+    h->tev_used = 0;
     int bad = 0;
     HIPCHK(hipMemcpy(&bad, h->f.unstable, sizeof(int), hipMemcpyDeviceToHost));
     if (bad) return fail("Simulation is unstable: vy(NX/2, NY/2) is NaN at time step %d.", bad);
@@ -1263,6 +1289,7 @@ int sofi_close(sofi_dev *h)
 {
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1272,5 +1299,33 @@ int sofi_close(sofi_dev *h)
 }
 
 size_t sofi_footprint(const sofi_dev *h) { return h ? h->bytes : 0; }
+
+int sofi_set_kernel_timing(sofi_dev *h, int every, int max_samples)
+{
+    if (!h) return fail("sofi_set_kernel_timing: null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
+    h->tev.clear();
+    h->tev_used = 0;
+    h->t_vel_ms = h->t_str_ms = 0.0;
+    h->t_samples = 0;
+    h->time_every = every > 0 ? every : 0;
+    if (h->time_every) {
+        int n = 4 * (max_samples > 0 ? max_samples : 1);
+        h->tev.resize(n);
+        for (int k = 0; k < n; ++k) HIPCHK(hipEventCreate(&h->tev[k]));
+    }
+    return 0;
+}
+
+int sofi_get_kernel_timing(const sofi_dev *h, double *velocity_ms, double *stress_ms, int *samples)
+{
+    if (!h) return fail("sofi_get_kernel_timing: null handle");
+    int n = h->t_samples;
+    if (velocity_ms) *velocity_ms = n ? h->t_vel_ms / n : 0.0;
+    if (stress_ms) *stress_ms = n ? h->t_str_ms / n : 0.0;
+    if (samples) *samples = n;
+    return 0;
+}
 
 } // extern "C"

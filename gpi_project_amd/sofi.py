@@ -201,6 +201,14 @@ class Simulation:
         _chk_hip(self._L.sofi_run(self._h, nt_first, nt_last))
         return float(self._L.sofi_last_run_ms(self._h))
 
+    def set_kernel_timing(self, every: int, max_samples: int = 4096):
+        _chk_hip(self._L.sofi_set_kernel_timing(self._h, every, max_samples))
+
+    def kernel_timing(self):
+        v, s, n = C.c_double(), C.c_double(), C.c_int()
+        _chk_hip(self._L.sofi_get_kernel_timing(self._h, C.byref(v), C.byref(s), C.byref(n)))
+        return {"velocity_ms": v.value, "stress_ms": s.value, "samples": n.value}
+
     def section(self, comp: int) -> np.ndarray:
         out = np.zeros((self.ntr, self.desc.ns), dtype=np.float32)
         if out.size:

@@ -93,6 +93,15 @@ int sofi_close(sofi_dev *h);
 /* Device bytes the handle holds (HBM footprint of one resident shot). */
 size_t sofi_footprint(const sofi_dev *h);
 
+/* Sampled kernel timing on the handle's stream: every `every`-th step of
+ * the following sofi_run calls records HIP events around the velocity
+ * kernel and around the stress-update kernels (at most max_samples steps);
+ * the reference's counterpart is the Perform timers of time_loop
+ * (src/perform_struct.h:29-36, src/time_loop.c:49-116).  every = 0 turns it off. */
+int sofi_set_kernel_timing(sofi_dev *h, int every, int max_samples);
+/* Average device time per timed step of the velocity and stress kernels. */
+int sofi_get_kernel_timing(const sofi_dev *h, double *velocity_ms, double *stress_ms, int *samples);
+
 #ifdef __cplusplus
 }
 #endif
