@@ -98,6 +98,80 @@ def cpu_baseline(case_dir: str, json_name: str, nx: int, ny: int, steps: int, co
             "sample": f"{n} time steps of the same workload on the FP32 CPU oracle"}
 
 
+CAL_BYTES = 512 << 20   # calibration copy for the counter passes
+
+
+def pmc_child(case_dir: str, launch: str, steps: int):
+    """Run under rocprofv3 --pmc: one device-to-device copy of CAL_BYTES (a
+    plain copy kernel of known size), then `steps` time steps of the workload
+    in case_dir."""
+    import ctypes as C
+    import glob
+    from gpi_project_amd.sofi import Problem, Simulation
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(CAL_BYTES)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(CAL_BYTES)) == 0
+    assert hip.hipMemset(a, 0, C.c_size_t(CAL_BYTES)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(CAL_BYTES), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    os.chdir(case_dir)
+    js = [f for f in glob.glob("*.json") if not f.startswith("cpu_")][0]
+    P = Problem(js)
+    sim = Simulation(P)
+    sim.set_launch(*launch.split(":"))
+    sim.load_shot(nsnap=0)
+    sim.run(1, steps)
+    sim.close()
+    P.close()
+
+
+def pmc_traffic(case_dir: str, launch: str, steps: int = 20):
+    """HBM bytes per launch of the velocity and stress kernels from two
+    rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
+    each scaled by what the same counter reads for the calibration copy
+    (CAL_BYTES read and written).  Returns ({kernel: bytes}, factors) or
+    (None, reason)."""
+    import csv
+    import glob
+    import re
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+    factors = {}
+    for cnt in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="sofi_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "150", prof, "--pmc", cnt, "-f", "csv", "-d", d, "-o", "pmc", "--",
+               sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", case_dir, "--launch", launch,
+               "--pmc-steps", str(steps)]
+        log(f"counter pass {cnt}")
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd="/tmp", env=env)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"rocprofv3 --pmc {cnt} failed (rc {r.returncode}): {(r.stdout + r.stderr)[-300:]}"
+        vals = {}
+        for row in csv.DictReader(open(files[0])):
+            name = re.sub(r"\(anonymous namespace\)::", "", row["Kernel_Name"]).replace("void ", "").split("(")[0]
+            vals.setdefault(name, []).append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        cal = max(vals.get("__amd_rocclr_copyBuffer", [0.0]))
+        if cal <= 0:
+            return None, f"no calibration copy in the {cnt} pass"
+        factors[cnt] = (CAL_BYTES / 1024) / cal
+        for name, v in vals.items():
+            if name.startswith("k_velocity") or name.startswith("k_stress"):
+                kind = "velocity" if name.startswith("k_velocity") else "stress"
+                per.setdefault(kind, {})[cnt] = sum(v) / len(v) * factors[cnt] * 1024
+                log(f"  {cnt} {name}: {sum(v) / len(v):.1f} KiB per dispatch ({len(v)} dispatches)")
+        log(f"  {cnt} calibration copy of {CAL_BYTES >> 20} MiB: {cal:.1f} KiB -> factor {factors[cnt]:.4f}")
+    out = {k: v["FETCH_SIZE"] + v["WRITE_SIZE"] for k, v in per.items() if len(v) == 2}
+    return out, factors
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +183,16 @@ def main():
     ap.add_argument("--time-every", type=int, default=8, help="time every n-th step's kernels with HIP events")
     ap.add_argument("--cpu-steps", type=int, default=160, help="time steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--launch", default="auto",
+                    help="shape of the update kernels: auto, cell, vec2, vec4, or VEL:STRESS for each "
+                         "(default: chosen by grid size)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", metavar="CASE_DIR", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-steps", type=int, default=20, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args.pmc_child, args.launch, args.pmc_steps)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs with no "
               "communication, see DESIGN.md)", file=sys.stderr)
@@ -131,7 +214,8 @@ def main():
     d = P.desc
     log(f"host setup {time.time() - t0:.1f} s: {d.nx}x{d.ny}, NT={d.nt}, FD{d.fdorder}, L={d.L}, {P.receivers()[0].size} receivers")
     sim = Simulation(P)
-    log(f"device footprint {sim.footprint / 2**30:.2f} GiB")
+    sim.set_launch(*args.launch.split(":"))
+    log(f"device footprint {sim.footprint / 2**30:.2f} GiB, launch shape {args.launch}")
     # warm-up on a shot that is then discarded
     sim.load_shot(nsnap=0)
     if args.warmup > 0:
@@ -158,6 +242,10 @@ def main():
     sim.close()
     P.close()
     os.chdir(cwd)
+    traffic = None
+    if not args.no_traffic:
+        traffic, info = pmc_traffic(td, args.launch)
+        log(f"traffic per launch {traffic}, counter factors {info}")
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(td, os.path.basename(js), d.nx, d.ny, args.cpu_steps, args.cpu_cores)
@@ -180,9 +268,12 @@ def main():
                    "timed_steps": args.steps},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "stress update (k_stress VEL_ISO)",
+                     "traffic": round(traffic["stress"]) if traffic and "stress" in traffic else None,
+                     "kernel": "stress update (k_stress VEL_ISO)",
                      "algorithmic_bytes_per_cell": b_str, "kernel_ms": round(kt["stress_ms"], 5),
                      "velocity_kernel_gbs": round(vel_gbs, 1) if vel_gbs else None,
+                     "traffic_velocity": round(traffic["velocity"]) if traffic and "velocity" in traffic else None,
+                     "algorithmic_bytes_per_launch": b_str * cells,
                      "whole_step_gbs": round(step_gbs, 1), "whole_step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }

@@ -135,6 +135,7 @@ def hip():
     L.sofi_footprint.argtypes = [_VP]
     L.sofi_set_kernel_timing.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_get_kernel_timing.argtypes = [_VP, _DP, _DP, _IP]
+    L.sofi_set_launch.argtypes = [_VP, C.c_int, C.c_int]
     _hip = L
     return L
 
@@ -154,5 +155,5 @@ def exported_symbols():
                            "sofi_set_receivers", "sofi_set_snapshots", "sofi_reset", "sofi_run",
                            "sofi_last_run_ms", "sofi_get_section", "sofi_snapshot_count",
                            "sofi_get_snapshot", "sofi_get_field", "sofi_close", "sofi_footprint",
-                           "sofi_set_kernel_timing", "sofi_get_kernel_timing"],
+                           "sofi_set_kernel_timing", "sofi_get_kernel_timing", "sofi_set_launch"],
     }

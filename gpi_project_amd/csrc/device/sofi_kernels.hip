@@ -55,6 +55,7 @@ struct Grid {
     int gx2, gx3, gy2, gy3;
     int plane;      // floats per padded plane
     int wx, wy;     // row lengths of x-frame and y-frame CPML memory
+    int ntx;        // tiles along x of the one-cell-per-thread update kernels
     float dt, dh, dtdh, dhi, dthalbe;
     float hc[7];
 };
@@ -76,9 +77,11 @@ struct Mat {
     const float *u11, *u13, *u33, *u15, *u35, *u55i, *u15i, *u35i;   // visco VTI/TTI unrelaxed
     const float *d11, *d13, *d33, *d15, *d35, *d55i, *d15i, *d35i;   // visco VTI/TTI defects (L planes)
     const float *absorb;
-    const float *pu, *ppi, *ptaus, *ptaup;      // free surface, snapshots
+    const float *pu, *ppi, *ptaus, *ptaup;      // snapshots
+    const float *surf;                          // free-surface coefficients of row 1, SURF_N x NX
     const float *prof;                          // CPML profiles, 12 x (2FW+1)
     float eta[SOFI_MAXL + 1], bip[SOFI_MAXL + 1], cip[SOFI_MAXL + 1];
+    float bjm[SOFI_MAXL + 1];                   // free surface: eta_l / (1 + eta_l / 2)
 };
 
 __device__ __forceinline__ int IDX(const Grid &g, int i, int j) { return (j - 1 + SOFI_PAD) * g.pitch + (i - 1 + SOFI_PAD); }
@@ -182,26 +185,222 @@ __device__ __forceinline__ void cpml_dv_read(const Grid &g, const Mat &m, const 
     }
 }
 
-// ---------------------------------------------------------- velocity kernel
+// ------------------------------------------------------------- sources
 
+struct Src {
+    const int *i, *j, *type;
+    const float *azi, *amp;
+    const float *sig;   // [nsrc][nt]
+    const double *sc;   // [nsrc][2]: sin and cos of the azimuth (degrees -> float radians), host libm
+    int n, nt;
+};
+
+// Injection fused into the update kernels for shots with few sources: the
+// thread owning a source cell adds the sources in source order, so that
+// coincident sources sum as in the reference.
+constexpr int FUSE_MAX = 8;
+struct Fuse {
+    int nforce;         // sources whose forces the velocity kernel adds (0: none)
+    int npsrc;          // sources whose explosive term the stress kernel adds (0: none)
+    int surf;           // the stress kernel applies the free surface
+    int c[FUSE_MAX];    // cell offsets of the sources, in source order
+};
+
+// force of source l at step nt on the velocities of its cell, added before
+// the velocity update (src/update_v_interior.c:49-71)
+__device__ __forceinline__ void force_add(const Grid &g, const Src &s, int l, int nt, float rip, float rjp, float &vx,
+                                          float &vy)
+{
+    const float amp = (g.dt * s.sig[l * s.nt + nt - 1]) / (g.dh * g.dh);
+    const int t = s.type[l];
+    if (t == 2) vx += rip * amp;
+    if (t == 3) vy += rjp * amp;
+    if (t == 4) {
+        vx = (float)((double)vx + s.sc[2 * l] * (double)(rip * amp));
+        vy = (float)((double)vy + s.sc[2 * l + 1] * (double)(rjp * amp));
+    }
+}
+
+// psource (src/psource.c:27-54): centred time derivative of the signal
+__device__ __forceinline__ float psource_amp(const Grid &g, const Src &s, int l, int nt)
+{
+    const float *sg = s.sig + l * s.nt - 1;   // sg[k] = signal at step k
+    const double h2 = 2.0 * g.dh * g.dh;
+    if (nt == 1) return s.nt >= 2 ? (float)(sg[2] / h2) : 0.0f;
+    if (nt < s.nt) return (float)((sg[nt + 1] - sg[nt - 1]) / h2);
+    return (float)(-sg[nt - 1] / h2);
+}
+
+// force sources when they are not fused: one thread, source order
+__global__ void k_force(Grid g, Fields f, Mat m, Src s, int nt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int l = 0; l < s.n; ++l) {
+        const int c = IDX(g, s.i[l], s.j[l]);
+        float vx = f.vx[c], vy = f.vy[c];
+        force_add(g, s, l, nt, m.rip[c], m.rjp[c], vx, vy);
+        f.vx[c] = vx;
+        f.vy[c] = vy;
+    }
+}
+
+__global__ void k_psource(Grid g, Fields f, Src s, int nt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int l = 0; l < s.n; ++l) {
+        const float amp = psource_amp(g, s, l, nt);
+        const int c = IDX(g, s.i[l], s.j[l]);
+        f.sxx[c] += amp;
+        f.syy[c] += amp;
+    }
+}
+
+// eqsource (src/eqsource.c:25-52): double-couple moment tensor
+__global__ void k_eqsource(Grid g, Fields f, Src s, int nt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int l = 0; l < s.n; ++l) {
+        const int i = s.i[l], j = s.j[l];
+        const float dip = (float)(s.azi[l] * M_PI / 180.0);
+        const float scale = g.dt / (g.dh * g.dh);
+        const float amp = s.amp[l] * scale * s.sig[l * s.nt + nt - 1];
+        const float m11 = (float)(-sin(2.0 * dip)), m12 = (float)(-cos(2.0 * dip)), m22 = (float)sin(2.0 * dip);
+        const int c = IDX(g, i, j);
+        f.sxx[c] += amp * m11;
+        f.sxy[c] = (float)((double)f.sxy[c] + 0.3 * amp * m12);
+        f.sxy[c - g.pitch] = (float)((double)f.sxy[c - g.pitch] + 0.3 * amp * m12);
+        f.sxy[c - 1] = (float)((double)f.sxy[c - 1] + 0.3 * amp * m12);
+        f.syy[c] += amp * m22;
+    }
+}
+
+// --------------------------------------------------------- free surface
+
+// Per-column coefficients of the row-1 correction, formed on the host from
+// the material at row 1 with the reference's rounding (src/surface.c:53-58,
+// src/surface_elastic.c:52-54): with fjm = 2 mu (visco: 2 mu (1 + L tau_s))
+// and gg = pi (visco: pi (1 + L tau_p)), t1 = DT (gg - fjm)^2, t2 = DT (gg -
+// fjm); visco only: de = 2 mu tau_s - pi tau_p, fg = fjm / gg - 1.
+enum { SURF_T1 = 0, SURF_T2, SURF_G, SURF_DE, SURF_FG, SURF_N };
+
+// surface_elastic / surface at j = 1 (src/surface_elastic.c:27-90,
+// src/surface.c:27-113) for column i, except the imaging of syy and sxy into
+// the ghost rows: zero the q memory variables, correct sxx (and the p memory
+// variables) with the horizontal derivative only; in the x-CPML the second
+// update of psi_vxx[1][h] in the same step is reproduced.  sxx is the cell's
+// value after the stress update and source injection; returns the new one.
+template <int ND, bool VISCO>
+__device__ __forceinline__ float surface_sxx(const Grid &g, Fields &f, const Mat &m, int i, int c, float sxx)
+{
+    const int j = 1, P = g.pitch;
+    const float dhi = (float)(1.0 / g.dh);
+    if (VISCO)
+        for (int l = 1; l <= g.L; ++l) f.q[(l - 1) * g.plane + c] = 0.0f;
+    float vxx = 0.f, vyy = 0.f;
+#pragma unroll
+    for (int m1 = 1; m1 <= ND; ++m1) {
+        vxx += g.hc[m1] * (f.vx[c + m1 - 1] - f.vx[c - m1]);
+        vyy += g.hc[m1] * (f.vy[c + (m1 - 1) * P] - f.vy[c - m1 * P]);
+    }
+    vxx *= dhi;
+    vyy *= dhi;
+    if (g.abs_type == 1) {
+        if (i <= g.fw) {
+            const int o = j * g.wx + i;
+            const float ps = PRF(g, m, SOFI_P_B_X, i) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, i) * vxx;
+            f.psi_vxx[o] = ps;
+            vxx = vxx / PRF(g, m, SOFI_P_K_X, i) + ps;
+        }
+        if (i >= g.nx - g.fw + 1) {
+            const int h = i - g.nx + 2 * g.fw, o = j * g.wx + h;
+            const float ps = PRF(g, m, SOFI_P_B_X, h) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, h) * vxx;
+            f.psi_vxx[o] = ps;
+            vxx = vxx / PRF(g, m, SOFI_P_K_X, h) + ps;
+        }
+    }
+    const float *sc = m.surf + (i - 1);
+    const float t1 = sc[SURF_T1 * g.nx], t2 = sc[SURF_T2 * g.nx], gg = sc[SURF_G * g.nx];
+    if (!VISCO) {
+        sxx += -(t1 * vxx / gg) - (t2 * vyy);
+    } else {
+        float sump = 0.f;
+        for (int l = 1; l <= g.L; ++l) sump += f.p[(l - 1) * g.plane + c];
+        sxx += -(t1 * vxx / gg) - (t2 * vyy) - (g.dthalbe * sump);
+        const float de = sc[SURF_DE * g.nx], fg = sc[SURF_FG * g.nx];
+        sump = 0.f;
+        for (int l = 1; l <= g.L; ++l) {
+            float &pv = f.p[(l - 1) * g.plane + c];
+            pv += m.bjm[l] * ((de * fg * vxx) - (de * vyy));
+            sump += pv;
+        }
+        sxx += (g.dthalbe * sump);
+    }
+    return sxx;
+}
+
+// the free surface as a pass of its own, after the stress update and sources
+template <int ND, bool VISCO>
+__global__ void k_surface(Grid g, Fields f, Mat m)
+{
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > g.nx) return;
+    const int P = g.pitch, c = IDX(g, i, 1);
+    f.syy[c] = 0.0f;
+#pragma unroll
+    for (int m1 = 1; m1 <= ND; ++m1) {
+        f.syy[c - m1 * P] = -f.syy[c + m1 * P];
+        f.sxy[c - m1 * P] = -f.sxy[c + (m1 - 1) * P];
+    }
+    f.sxx[c] = surface_sxx<ND, VISCO>(g, f, m, i, c, f.sxx[c]);
+}
+
+// ------------------------------------------------------------ tile mapping
+
+// Tiles of the ISO/VTI update kernels are numbered row-major over the grid
+// in workgroup order.  The hardware hands workgroups round-robin to the
+// eight XCDs in that order, so at any moment the whole GPU works on one
+// narrow band of rows and a tile's vertical stencil halo was fetched moments
+// before by the tiles above it (from the memory-side cache when another XCD
+// fetched it).  Measured on the 4000 x 2000 viscoelastic workload, this
+// order beat giving each XCD a contiguous band of its own by 4 % (stress)
+// and 3 % (velocity).
+__device__ __forceinline__ int tile_id() { return blockIdx.x; }
+
+// Tile shapes: CELL, one thread per cell in BX x BY tiles (also the TTI
+// passes); VEC2 / VEC4, one thread per VW = 2 / 4 consecutive cells of a row
+// in (VW BX) x BY tiles, every stencil row and coefficient fetched as one
+// aligned VW-float vector (interior rows start on 256-B boundaries and the
+// first of a thread's cells sits at a column offset divisible by VW).
 constexpr int BX = 64, BY = 4;
 
-// update_v_interior + update_v_PML/update_v_abs + wavefield_update_v
-// (src/update_v_interior.c:92-98, src/update_v_PML.c, src/update_v_abs.c,
-// src/wavefield_update_v.c:46-47), one thread per cell.
-template <int ND>
-__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m, int nt)
+template <int VW>
+using vecf = float __attribute__((ext_vector_type(VW)));
+
+template <int VW>
+__device__ __forceinline__ vecf<VW> ldv(const float *p, int c)
 {
-    const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
-    if (i > g.nx || j > g.ny) return;
+    return *reinterpret_cast<const vecf<VW> *>(p + c);
+}
+template <int VW>
+__device__ __forceinline__ void stv(float *p, int c, vecf<VW> v)
+{
+    *reinterpret_cast<vecf<VW> *>(p + c) = v;
+}
+
+// ---------------------------------------------------------- velocity update
+
+// update_v_interior + update_v_PML/update_v_abs + wavefield_update_v
+// (src/update_v_interior.c:49-98, src/update_v_PML.c, src/update_v_abs.c,
+// src/wavefield_update_v.c:46-47) of cell (i, j) given the four stress
+// derivatives of op_v (no 1/DH), on the cell's velocities in registers.
+__device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Mat &m, const Src &s, const Fuse &fz,
+                                              int nt, int i, int j, int c, float sxx_x, float sxy_x, float sxy_y,
+                                              float syy_y, float rip, float rjp, float &vx, float &vy)
+{
     // instability guard of time_loop (src/time_loop.c:44-47), before this step's update
-    if (i == g.nx / 2 && j == g.ny / 2 && isnan(f.vy[IDX(g, i, j)]) && *f.unstable == 0) *f.unstable = nt;
+    if (i == g.nx / 2 && j == g.ny / 2 && isnan(vy) && *f.unstable == 0) *f.unstable = nt;
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
     if (frame && g.abs_type != 1 && g.abs_type != 2) return;
-    const int c = IDX(g, i, j);
-    float sxx_x, sxy_x, sxy_y, syy_y;
-    op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
     if (g.abs_type == 1 && frame) {
         // cpml_update_v_x / _y (src/cpml_update.c:50-66)
         if (xf) {
@@ -223,140 +422,457 @@ __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m, in
             sxy_y = sxy_y / PRF(g, m, SOFI_P_K_Y, h) + b;
         }
     }
-    float vx = f.vx[c] + (sxx_x + sxy_y) * g.dtdh * m.rip[c];
-    float vy = f.vy[c] + (sxy_x + syy_y) * g.dtdh * m.rjp[c];
+    for (int k = 0; k < fz.nforce; ++k)
+        if (c == fz.c[k]) force_add(g, s, k, nt, rip, rjp, vx, vy);
+    vx = vx + (sxx_x + sxy_y) * g.dtdh * rip;
+    vy = vy + (sxy_x + syy_y) * g.dtdh * rjp;
     if (g.abs_type == 2 && frame) {
         const float a = m.absorb[c];
         vx *= a;
         vy *= a;
     }
+}
+
+template <int ND>
+__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+{
+    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    float sxx_x, sxy_x, sxy_y, syy_y;
+    op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
+    float vx = f.vx[c], vy = f.vy[c];
+    velocity_core(g, f, m, s, fz, nt, i, j, c, sxx_x, sxy_x, sxy_y, syy_y, m.rip[c], m.rjp[c], vx, vy);
     f.vx[c] = vx;
     f.vy[c] = vy;
 }
 
-// --------------------------------------------------------- stress updates
-
-// per-cell rheology updates; dv are the (possibly CPML-modified) derivatives
-
-// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48)
-__device__ __forceinline__ void upd_el(const Mat &m, Fields &f, int c, float vxx, float vyx, float vxy, float vyy)
+// VW cells per thread: the x stencils come from 2G+1 vectors of the row,
+// the y stencils from one vector per row; cells past NX keep their (zero)
+// padding values
+template <int ND, int VW>
+__global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
 {
-    const float u1 = m.fipjp[c] * (vyx + vxy);
-    const float gg = m.g[c], ff = 2.0f * m.f[c];
-    const float u2 = (gg * (vxx + vyy)) - (ff * vyy);
-    const float u3 = (gg * (vxx + vyy)) - (ff * vxx);
-    f.sxy[c] += u1;
-    f.sxx[c] += u2;
-    f.syy[c] += u3;
+    constexpr int G = (ND + VW - 1) / VW;
+    using V = vecf<VW>;
+    const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
+    const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
+    const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i0 > g.nx || j > g.ny) return;
+    const int c0 = IDX(g, i0, j), P = g.pitch;
+    float xs[(2 * G + 1) * VW], xy[(2 * G + 1) * VW];   // sxx, sxy of row j: columns i0 - G VW .. i0 + (G + 1) VW - 1
+#pragma unroll
+    for (int q = -G; q <= G; ++q) {
+        const V a = ldv<VW>(f.sxx, c0 + VW * q), b = ldv<VW>(f.sxy, c0 + VW * q);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            xs[(q + G) * VW + k] = a[k];
+            xy[(q + G) * VW + k] = b[k];
+        }
+    }
+    float d[VW] = {}, e[VW] = {};
+#pragma unroll
+    for (int mm = 1; mm <= ND; ++mm) {
+        const float h = g.hc[mm];
+        // sxy rows j + m - 1 and j - m, syy rows j + m and j - m + 1
+        V a1;
+        if (mm == 1) {
+#pragma unroll
+            for (int k = 0; k < VW; ++k) a1[k] = xy[G * VW + k];
+        } else {
+            a1 = ldv<VW>(f.sxy, c0 + (mm - 1) * P);
+        }
+        const V a0 = ldv<VW>(f.sxy, c0 - mm * P);
+        const V b1 = ldv<VW>(f.syy, c0 + mm * P), b0 = ldv<VW>(f.syy, c0 - (mm - 1) * P);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            d[k] += h * (a1[k] - a0[k]);
+            e[k] += h * (b1[k] - b0[k]);
+        }
+    }
+    V vx = ldv<VW>(f.vx, c0), vy = ldv<VW>(f.vy, c0);
+    const V rip = ldv<VW>(m.rip, c0), rjp = ldv<VW>(m.rjp, c0);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+        if (i0 + k > g.nx) break;
+        float a = 0.f, b = 0.f;
+#pragma unroll
+        for (int mm = 1; mm <= ND; ++mm) {
+            const float h = g.hc[mm];
+            a += h * (xs[G * VW + k + mm] - xs[G * VW + k - mm + 1]);
+            b += h * (xy[G * VW + k + mm - 1] - xy[G * VW + k - mm]);
+        }
+        float x = vx[k], y = vy[k];
+        velocity_core(g, f, m, s, fz, nt, i0 + k, j, c0 + k, a, b, d[k], e[k], rip[k], rjp[k], x, y);
+        vx[k] = x;
+        vy[k] = y;
+    }
+    stv<VW>(f.vx, c0, vx);
+    stv<VW>(f.vy, c0, vy);
 }
 
-// wavefield_update_s_visc (src/wavefield_update_s_visc.c:27-68); the defect
-// coefficients dip/d/e[l] = eta_l * (per-cell base) are formed in registers
-__device__ __forceinline__ void upd_visc(const Grid &g, const Mat &m, Fields &f, int c, float vxx, float vyx,
-                                         float vxy, float vyy)
+// --------------------------------------------------------- stress updates
+
+// Per-cell rheology updates on registers: (sxx, syy, sxy) hold the old
+// stresses on entry and the updated ones on exit; dv are the (possibly
+// CPML-modified) derivatives.
+
+// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48)
+__device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vxx, float vyx, float vxy, float vyy,
+                                       float &sxx, float &syy, float &sxy)
 {
-    float sr0 = 0.f, sp0 = 0.f, sq0 = 0.f, sr = 0.f, sp = 0.f, sq = 0.f;
-    const float br = m.br[c], bd = m.bd[c], be = m.be[c];
-    const float sxyv = vxy + vyx, sdiag = vxx + vyy;
-#pragma unroll
-    for (int l = 1; l <= SOFI_MAXL; ++l) {
-        if (l > g.L) break;
-        const int o = (l - 1) * g.plane + c;
-        const float r0 = f.r[o], p0 = f.p[o], q0 = f.q[o];
-        sr0 += r0;
-        sp0 += p0;
-        sq0 += q0;
-        const float eta = m.eta[l], bip = m.bip[l], cip = m.cip[l];
-        const float dip = eta * br, d2 = 2.0f * (eta * bd), e = eta * be;
-        const float r1 = bip * (r0 * cip - dip * sxyv);
-        const float p1 = bip * (p0 * cip - e * sdiag + d2 * vyy);
-        const float q1 = bip * (q0 * cip - e * sdiag + d2 * vxx);
-        f.r[o] = r1;
-        f.p[o] = p1;
-        f.q[o] = q1;
-        sr += r1;
-        sp += p1;
-        sq += q1;
-    }
-    const float gg = m.g[c], ff = 2.0f * m.f[c];
-    const float u1 = (m.fipjp[c] * sxyv) + (g.dthalbe * (sr0 + sr));
-    const float u2 = (gg * sdiag) - (ff * vyy) + (g.dthalbe * (sp0 + sp));
-    const float u3 = (gg * sdiag) - (ff * vxx) + (g.dthalbe * (sq0 + sq));
-    f.sxy[c] += u1;
-    f.sxx[c] += u2;
-    f.syy[c] += u3;
+    const float u1 = fipjp * (vyx + vxy);
+    const float ff = 2.0f * f_;
+    const float u2 = (g_ * (vxx + vyy)) - (ff * vyy);
+    const float u3 = (g_ * (vxx + vyy)) - (ff * vxx);
+    sxy += u1;
+    sxx += u2;
+    syy += u3;
 }
 
 // wavefield_update_s_el_vti (src/wavefield_update_s_el_vti.c:27-47)
-__device__ __forceinline__ void upd_el_vti(const Mat &m, Fields &f, int c, float vxx, float vyx, float vxy, float vyy)
+__device__ __forceinline__ void upd_el_vti(float c11, float c13, float c33, float c55i, float vxx, float vyx,
+                                           float vxy, float vyy, float &sxx, float &syy, float &sxy)
 {
-    const float u1 = m.c55i[c] * (vxy + vyx);
-    const float u2 = (m.c11[c] * vxx) + (m.c13[c] * vyy);
-    const float u3 = (m.c13[c] * vxx) + (m.c33[c] * vyy);
-    f.sxy[c] += u1;
-    f.sxx[c] += u2;
-    f.syy[c] += u3;
+    const float u1 = c55i * (vxy + vyx);
+    const float u2 = (c11 * vxx) + (c13 * vyy);
+    const float u3 = (c13 * vxx) + (c33 * vyy);
+    sxy += u1;
+    sxx += u2;
+    syy += u3;
 }
 
-// wavefield_update_s_visc_VTI (src/wavefield_update_s_visc_VTI.c:27-66)
-__device__ __forceinline__ void upd_visc_vti(const Grid &g, const Mat &m, Fields &f, int c, float vxx, float vyx,
-                                             float vxy, float vyy)
+// Memory-variable sums of the viscoelastic updates: old (r0, p0, q0) and new
+// (r, p, q) summed over the relaxation mechanisms in their order.
+struct VSum {
+    float r0 = 0.f, p0 = 0.f, q0 = 0.f, r = 0.f, p = 0.f, q = 0.f;
+};
+
+// mechanism l of wavefield_update_s_visc (src/wavefield_update_s_visc.c:27-68);
+// the defect coefficients dip/d/e[l] = eta_l * (per-cell base) are formed in registers
+__device__ __forceinline__ void visc_level(float eta, float bip, float cip, float br, float bd, float be, float sxyv,
+                                           float sdiag, float vxx, float vyy, float &r, float &p, float &q, VSum &u)
 {
-    float sr0 = 0.f, sp0 = 0.f, sq0 = 0.f, sr = 0.f, sp = 0.f, sq = 0.f;
-    const float sxyv = vxy + vyx;
-#pragma unroll
-    for (int l = 1; l <= SOFI_MAXL; ++l) {
-        if (l > g.L) break;
-        const int o = (l - 1) * g.plane + c;
-        const float r0 = f.r[o], p0 = f.p[o], q0 = f.q[o];
-        sr0 += r0;
-        sp0 += p0;
-        sq0 += q0;
-        const float bip = m.bip[l], cip = m.cip[l];
-        const float r1 = bip * (r0 * cip - (m.d55i[o] * sxyv));
-        const float p1 = bip * (p0 * cip - (m.d11[o] * vxx) - (m.d13[o] * vyy));
-        const float q1 = bip * (q0 * cip - (m.d13[o] * vxx) - (m.d33[o] * vyy));
-        f.r[o] = r1;
-        f.p[o] = p1;
-        f.q[o] = q1;
-        sr += r1;
-        sp += p1;
-        sq += q1;
-    }
-    const float u1 = (m.u55i[c] * sxyv) + (g.dthalbe * (sr0 + sr));
-    const float u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (g.dthalbe * (sp0 + sp));
-    const float u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (g.dthalbe * (sq0 + sq));
-    f.sxy[c] += u1;
-    f.sxx[c] += u2;
-    f.syy[c] += u3;
+    u.r0 += r;
+    u.p0 += p;
+    u.q0 += q;
+    const float dip = eta * br, d2 = 2.0f * (eta * bd), e = eta * be;
+    r = bip * (r * cip - dip * sxyv);
+    p = bip * (p * cip - e * sdiag + d2 * vyy);
+    q = bip * (q * cip - e * sdiag + d2 * vxx);
+    u.r += r;
+    u.p += p;
+    u.q += q;
 }
 
-// stress update for ISO/VTI rheologies: velocity derivatives recomputed in
-// the kernel (replaces v_derivatives + exchange_v), CPML applied to them in
-// the frame, damping after the update (src/update_s_*_{interior,PML,abs}.c).
-template <int ND, int WEQ>
-__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f, Mat m)
+__device__ __forceinline__ void visc_final(float dthalbe, float f_, float g_, float fipjp, float sxyv, float sdiag,
+                                           float vxx, float vyy, const VSum &u, float &sxx, float &syy, float &sxy)
 {
-    const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
-    if (i > g.nx || j > g.ny) return;
+    const float ff = 2.0f * f_;
+    const float u1 = (fipjp * sxyv) + (dthalbe * (u.r0 + u.r));
+    const float u2 = (g_ * sdiag) - (ff * vyy) + (dthalbe * (u.p0 + u.p));
+    const float u3 = (g_ * sdiag) - (ff * vxx) + (dthalbe * (u.q0 + u.q));
+    sxy += u1;
+    sxx += u2;
+    syy += u3;
+}
+
+// mechanism l of wavefield_update_s_visc_VTI (src/wavefield_update_s_visc_VTI.c:27-66)
+__device__ __forceinline__ void visc_vti_level(float bip, float cip, float d11, float d13, float d33, float d55i,
+                                               float sxyv, float vxx, float vyy, float &r, float &p, float &q,
+                                               VSum &u)
+{
+    u.r0 += r;
+    u.p0 += p;
+    u.q0 += q;
+    r = bip * (r * cip - (d55i * sxyv));
+    p = bip * (p * cip - (d11 * vxx) - (d13 * vyy));
+    q = bip * (q * cip - (d13 * vxx) - (d33 * vyy));
+    u.r += r;
+    u.p += p;
+    u.q += q;
+}
+
+__device__ __forceinline__ void visc_vti_final(float dthalbe, float u11, float u13, float u33, float u55i, float sxyv,
+                                               float vxx, float vyy, const VSum &u, float &sxx, float &syy,
+                                               float &sxy)
+{
+    const float u1 = (u55i * sxyv) + (dthalbe * (u.r0 + u.r));
+    const float u2 = (u11 * vxx) + (u13 * vyy) + (dthalbe * (u.p0 + u.p));
+    const float u3 = (u13 * vxx) + (u33 * vyy) + (dthalbe * (u.q0 + u.q));
+    sxy += u1;
+    sxx += u2;
+    syy += u3;
+}
+
+// Derivatives of a cell from the raw op_s sums (no 1/DH): scaled, and CPML in
+// the frame (cpml_update_s_x/_y).  Returns false for a frame cell that the
+// reference never updates (ABS_TYPE other than 1 or 2).
+__device__ __forceinline__ bool stress_dv(const Grid &g, Fields &f, const Mat &m, int i, int j, float a, float b,
+                                          float d, float e, float &vxx, float &vyx, float &vxy, float &vyy)
+{
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
-    if (frame && g.abs_type != 1 && g.abs_type != 2) return;
-    const int c = IDX(g, i, j);
-    float vxx, vyx, vxy, vyy;
-    op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
+    if (frame && g.abs_type != 1 && g.abs_type != 2) return false;
+    vxx = a * g.dhi;
+    vyx = b * g.dhi;
+    vxy = d * g.dhi;
+    vyy = e * g.dhi;
     if (g.abs_type == 1 && frame) cpml_dv(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
-    if (WEQ == SOFI_EL_ISO) upd_el(m, f, c, vxx, vyx, vxy, vyy);
-    if (WEQ == SOFI_VEL_ISO) upd_visc(g, m, f, c, vxx, vyx, vxy, vyy);
-    if (WEQ == SOFI_EL_VTI) upd_el_vti(m, f, c, vxx, vyx, vxy, vyy);
-    if (WEQ == SOFI_VEL_VTI) upd_visc_vti(g, m, f, c, vxx, vyx, vxy, vyy);
-    if (g.abs_type == 2 && frame) {
-        // abs_update_s (src/abs_update.c:39-44)
-        const float a = m.absorb[c];
-        f.sxy[c] *= a;
-        f.syy[c] *= a;
-        f.sxx[c] *= a;
+    return true;
+}
+
+// What follows the rheology update for one cell: damping in the frame
+// (abs_update_s, src/abs_update.c:39-44), then what the reference does next
+// to the same cell when it is fused here: the explosive sources (psource) and
+// the free surface (imaging into the ghost rows by the cells that are imaged,
+// correction of row 1).
+template <int ND, bool VISCO>
+__device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat &m, const Src &s, const Fuse &fz,
+                                            int nt, int i, int j, int c, float &sxx, float &syy, float &sxy)
+{
+    if (g.abs_type == 2 && (in_xframe(g, i) || in_yframe(g, j))) {
+        const float q = m.absorb[c];
+        sxy *= q;
+        syy *= q;
+        sxx *= q;
     }
+    for (int k = 0; k < fz.npsrc; ++k)
+        if (c == fz.c[k]) {
+            const float amp = psource_amp(g, s, k, nt);
+            sxx += amp;
+            syy += amp;
+        }
+    if (fz.surf && j <= ND + 1) {
+        const int P = g.pitch;
+        if (j >= 2) f.syy[c + (2 - 2 * j) * P] = -syy;   // syy(1 - m) = -syy(1 + m)
+        if (j <= ND) f.sxy[c + (1 - 2 * j) * P] = -sxy;  // sxy(1 - m) = -sxy(m)
+        if (j == 1) {
+            syy = 0.0f;
+            sxx = surface_sxx<ND, VISCO>(g, f, m, i, c, sxx);
+        }
+    }
+}
+
+// one thread per cell; the velocity derivatives are recomputed from vx, vy
+// (replaces v_derivatives + exchange_v) (src/update_s_*_{interior,PML,abs}.c)
+template <int ND, int WEQ>
+__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+{
+    constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
+    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j), P = g.pitch;
+    float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
+#pragma unroll
+    for (int mm = 1; mm <= ND; ++mm) {
+        const float h = g.hc[mm];
+        a += h * (f.vx[c + mm - 1] - f.vx[c - mm]);
+        b += h * (f.vy[c + mm] - f.vy[c - mm + 1]);
+        d += h * (f.vx[c + mm * P] - f.vx[c - (mm - 1) * P]);
+        e += h * (f.vy[c + (mm - 1) * P] - f.vy[c - mm * P]);
+    }
+    float vxx, vyx, vxy, vyy;
+    if (!stress_dv(g, f, m, i, j, a, b, d, e, vxx, vyx, vxy, vyy)) return;
+    float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
+    if (WEQ == SOFI_EL_ISO) upd_el(m.f[c], m.g[c], m.fipjp[c], vxx, vyx, vxy, vyy, sxx, syy, sxy);
+    if (WEQ == SOFI_EL_VTI) upd_el_vti(m.c11[c], m.c13[c], m.c33[c], m.c55i[c], vxx, vyx, vxy, vyy, sxx, syy, sxy);
+    if (WEQ == SOFI_VEL_ISO) {
+        const float br = m.br[c], bd = m.bd[c], be = m.be[c];
+        const float sxyv = vxy + vyx, sdiag = vxx + vyy;
+        VSum u;
+#pragma unroll
+        for (int l = 1; l <= SOFI_MAXL; ++l) {
+            if (l > g.L) break;
+            const int o = (l - 1) * g.plane + c;
+            float r = f.r[o], p = f.p[o], q = f.q[o];
+            visc_level(m.eta[l], m.bip[l], m.cip[l], br, bd, be, sxyv, sdiag, vxx, vyy, r, p, q, u);
+            f.r[o] = r;
+            f.p[o] = p;
+            f.q[o] = q;
+        }
+        visc_final(g.dthalbe, m.f[c], m.g[c], m.fipjp[c], sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy);
+    }
+    if (WEQ == SOFI_VEL_VTI) {
+        const float sxyv = vxy + vyx;
+        VSum u;
+#pragma unroll
+        for (int l = 1; l <= SOFI_MAXL; ++l) {
+            if (l > g.L) break;
+            const int o = (l - 1) * g.plane + c;
+            float r = f.r[o], p = f.p[o], q = f.q[o];
+            visc_vti_level(m.bip[l], m.cip[l], m.d11[o], m.d13[o], m.d33[o], m.d55i[o], sxyv, vxx, vyy, r, p, q, u);
+            f.r[o] = r;
+            f.p[o] = p;
+            f.q[o] = q;
+        }
+        visc_vti_final(g.dthalbe, m.u11[c], m.u13[c], m.u33[c], m.u55i[c], sxyv, vxx, vyy, u, sxx, syy, sxy);
+    }
+    stress_post<ND, VISCO>(g, f, m, s, fz, nt, i, j, c, sxx, syy, sxy);
+    f.sxx[c] = sxx;
+    f.syy[c] = syy;
+    f.sxy[c] = sxy;
+}
+
+// VW cells per thread (see k_velocity_v); inactive cells (past NX, or
+// frame cells that are never updated) keep their stresses and memory
+// variables
+template <int ND, int WEQ, int VW>
+__global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+{
+    constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
+    constexpr int G = (ND + VW - 1) / VW;
+    using V = vecf<VW>;
+    const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
+    const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
+    const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i0 > g.nx || j > g.ny) return;
+    const int c0 = IDX(g, i0, j), P = g.pitch;
+    float xv[(2 * G + 1) * VW], yv[(2 * G + 1) * VW];   // vx, vy of row j: columns i0 - G VW .. i0 + (G + 1) VW - 1
+#pragma unroll
+    for (int q = -G; q <= G; ++q) {
+        const V a = ldv<VW>(f.vx, c0 + VW * q), b = ldv<VW>(f.vy, c0 + VW * q);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            xv[(q + G) * VW + k] = a[k];
+            yv[(q + G) * VW + k] = b[k];
+        }
+    }
+    float d[VW] = {}, e[VW] = {};
+#pragma unroll
+    for (int mm = 1; mm <= ND; ++mm) {
+        const float h = g.hc[mm];
+        // vx rows j + m and j - m + 1, vy rows j + m - 1 and j - m
+        const V a1 = ldv<VW>(f.vx, c0 + mm * P);
+        V a0, b1;
+        if (mm == 1) {
+#pragma unroll
+            for (int k = 0; k < VW; ++k) {
+                a0[k] = xv[G * VW + k];
+                b1[k] = yv[G * VW + k];
+            }
+        } else {
+            a0 = ldv<VW>(f.vx, c0 - (mm - 1) * P);
+            b1 = ldv<VW>(f.vy, c0 + (mm - 1) * P);
+        }
+        const V b0 = ldv<VW>(f.vy, c0 - mm * P);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            d[k] += h * (a1[k] - a0[k]);
+            e[k] += h * (b1[k] - b0[k]);
+        }
+    }
+    float vxx[VW], vyx[VW], vxy[VW], vyy[VW];
+    bool act[VW];
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+        act[k] = false;
+        vxx[k] = vyx[k] = vxy[k] = vyy[k] = 0.f;
+        if (i0 + k > g.nx) continue;
+        float a = 0.f, b = 0.f;
+#pragma unroll
+        for (int mm = 1; mm <= ND; ++mm) {
+            const float h = g.hc[mm];
+            a += h * (xv[G * VW + k + mm - 1] - xv[G * VW + k - mm]);
+            b += h * (yv[G * VW + k + mm] - yv[G * VW + k - mm + 1]);
+        }
+        act[k] = stress_dv(g, f, m, i0 + k, j, a, b, d[k], e[k], vxx[k], vyx[k], vxy[k], vyy[k]);
+    }
+    const V sxx0 = ldv<VW>(f.sxx, c0), syy0 = ldv<VW>(f.syy, c0), sxy0 = ldv<VW>(f.sxy, c0);
+    float sxx[VW], syy[VW], sxy[VW];
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+        sxx[k] = sxx0[k];
+        syy[k] = syy0[k];
+        sxy[k] = sxy0[k];
+    }
+    if (WEQ == SOFI_EL_ISO) {
+        const V c1 = ldv<VW>(m.f, c0), c2 = ldv<VW>(m.g, c0), c3 = ldv<VW>(m.fipjp, c0);
+#pragma unroll
+        for (int k = 0; k < VW; ++k) upd_el(c1[k], c2[k], c3[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k]);
+    }
+    if (WEQ == SOFI_EL_VTI) {
+        const V c1 = ldv<VW>(m.c11, c0), c2 = ldv<VW>(m.c13, c0), c3 = ldv<VW>(m.c33, c0), c4 = ldv<VW>(m.c55i, c0);
+#pragma unroll
+        for (int k = 0; k < VW; ++k)
+            upd_el_vti(c1[k], c2[k], c3[k], c4[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k]);
+    }
+    if (VISCO) {
+        VSum u[VW];
+        float sxyv[VW], sdiag[VW];
+#pragma unroll
+        for (int k = 0; k < VW; ++k) {
+            sxyv[k] = vxy[k] + vyx[k];
+            sdiag[k] = vxx[k] + vyy[k];
+        }
+        V br, bd, be;
+        if (WEQ == SOFI_VEL_ISO) {
+            br = ldv<VW>(m.br, c0);
+            bd = ldv<VW>(m.bd, c0);
+            be = ldv<VW>(m.be, c0);
+        }
+#pragma unroll
+        for (int l = 1; l <= SOFI_MAXL; ++l) {
+            if (l > g.L) break;
+            const int o = (l - 1) * g.plane + c0;
+            V r = ldv<VW>(f.r, o), p = ldv<VW>(f.p, o), q = ldv<VW>(f.q, o);
+            V d11, d13, d33, d55;
+            if (WEQ == SOFI_VEL_VTI) {
+                d11 = ldv<VW>(m.d11, o);
+                d13 = ldv<VW>(m.d13, o);
+                d33 = ldv<VW>(m.d33, o);
+                d55 = ldv<VW>(m.d55i, o);
+            }
+#pragma unroll
+            for (int k = 0; k < VW; ++k) {
+                float rr = r[k], pp = p[k], qq = q[k];
+                if (WEQ == SOFI_VEL_ISO)
+                    visc_level(m.eta[l], m.bip[l], m.cip[l], br[k], bd[k], be[k], sxyv[k], sdiag[k], vxx[k], vyy[k],
+                               rr, pp, qq, u[k]);
+                else
+                    visc_vti_level(m.bip[l], m.cip[l], d11[k], d13[k], d33[k], d55[k], sxyv[k], vxx[k], vyy[k], rr,
+                                   pp, qq, u[k]);
+                if (act[k]) {
+                    r[k] = rr;
+                    p[k] = pp;
+                    q[k] = qq;
+                }
+            }
+            stv<VW>(f.r, o, r);
+            stv<VW>(f.p, o, p);
+            stv<VW>(f.q, o, q);
+        }
+        if (WEQ == SOFI_VEL_ISO) {
+            const V c1 = ldv<VW>(m.f, c0), c2 = ldv<VW>(m.g, c0), c3 = ldv<VW>(m.fipjp, c0);
+#pragma unroll
+            for (int k = 0; k < VW; ++k)
+                visc_final(g.dthalbe, c1[k], c2[k], c3[k], sxyv[k], sdiag[k], vxx[k], vyy[k], u[k], sxx[k], syy[k],
+                           sxy[k]);
+        } else {
+            const V c1 = ldv<VW>(m.u11, c0), c2 = ldv<VW>(m.u13, c0), c3 = ldv<VW>(m.u33, c0), c4 = ldv<VW>(m.u55i, c0);
+#pragma unroll
+            for (int k = 0; k < VW; ++k)
+                visc_vti_final(g.dthalbe, c1[k], c2[k], c3[k], c4[k], sxyv[k], vxx[k], vyy[k], u[k], sxx[k], syy[k],
+                               sxy[k]);
+        }
+    }
+    V sxxn = sxx0, syyn = syy0, sxyn = sxy0;
+#pragma unroll
+    for (int k = 0; k < VW; ++k) {
+        if (!act[k]) continue;
+        stress_post<ND, VISCO>(g, f, m, s, fz, nt, i0 + k, j, c0 + k, sxx[k], syy[k], sxy[k]);
+        sxxn[k] = sxx[k];
+        syyn[k] = syy[k];
+        sxyn[k] = sxy[k];
+    }
+    stv<VW>(f.sxx, c0, sxxn);
+    stv<VW>(f.syy, c0, syyn);
+    stv<VW>(f.sxy, c0, sxyn);
 }
 
 // ------------------------------------------------------------ TTI passes
@@ -477,135 +993,6 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f, Mat m)
     f.syy[c] = syy;
 }
 
-// ------------------------------------------------------------- sources
-
-struct Src {
-    const int *i, *j, *type;
-    const float *azi, *amp;
-    const float *sig;   // [nsrc][nt]
-    int n, nt;
-};
-
-// force sources before the velocity update (src/update_v_interior.c:49-71);
-// one thread adds all sources in order so that coincident sources sum as
-// in the reference
-__global__ void k_force(Grid g, Fields f, Mat m, Src s, int nt)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int l = 0; l < s.n; ++l) {
-        const int c = IDX(g, s.i[l], s.j[l]);
-        const float amp = (g.dt * s.sig[l * s.nt + nt - 1]) / (g.dh * g.dh);
-        const int t = s.type[l];
-        if (t == 2) f.vx[c] += m.rip[c] * amp;
-        if (t == 3) f.vy[c] += m.rjp[c] * amp;
-        if (t == 4) {
-            const float azi = (float)(s.azi[l] * M_PI / 180);
-            f.vx[c] = (float)((double)f.vx[c] + sin((double)azi) * (double)(m.rip[c] * amp));
-            f.vy[c] = (float)((double)f.vy[c] + cos((double)azi) * (double)(m.rjp[c] * amp));
-        }
-    }
-}
-
-// psource (src/psource.c:27-54): centred time derivative of the signal
-__global__ void k_psource(Grid g, Fields f, Src s, int nt)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int l = 0; l < s.n; ++l) {
-        const float *sg = s.sig + l * s.nt - 1;   // sg[k] = signal at step k
-        const double h2 = 2.0 * g.dh * g.dh;
-        float amp;
-        if (nt == 1) amp = s.nt >= 2 ? (float)(sg[2] / h2) : 0.0f;
-        else if (nt < s.nt) amp = (float)((sg[nt + 1] - sg[nt - 1]) / h2);
-        else amp = (float)(-sg[nt - 1] / h2);
-        const int c = IDX(g, s.i[l], s.j[l]);
-        f.sxx[c] += amp;
-        f.syy[c] += amp;
-    }
-}
-
-// eqsource (src/eqsource.c:25-52): double-couple moment tensor
-__global__ void k_eqsource(Grid g, Fields f, Src s, int nt)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int l = 0; l < s.n; ++l) {
-        const int i = s.i[l], j = s.j[l];
-        const float dip = (float)(s.azi[l] * M_PI / 180.0);
-        const float scale = g.dt / (g.dh * g.dh);
-        const float amp = s.amp[l] * scale * s.sig[l * s.nt + nt - 1];
-        const float m11 = (float)(-sin(2.0 * dip)), m12 = (float)(-cos(2.0 * dip)), m22 = (float)sin(2.0 * dip);
-        const int c = IDX(g, i, j);
-        f.sxx[c] += amp * m11;
-        f.sxy[c] = (float)((double)f.sxy[c] + 0.3 * amp * m12);
-        f.sxy[c - g.pitch] = (float)((double)f.sxy[c - g.pitch] + 0.3 * amp * m12);
-        f.sxy[c - 1] = (float)((double)f.sxy[c - 1] + 0.3 * amp * m12);
-        f.syy[c] += amp * m22;
-    }
-}
-
-// --------------------------------------------------------- free surface
-
-// surface_elastic / surface at j = 1 (src/surface_elastic.c:27-90,
-// src/surface.c:27-113): image the stresses into the ghost rows, correct sxx
-// (and the p memory variables) with the horizontal derivative only; in the
-// x-CPML the second update of psi_vxx[1][h] in the same step is reproduced.
-template <int ND, bool VISCO>
-__global__ void k_surface(Grid g, Fields f, Mat m)
-{
-    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > g.nx) return;
-    const int j = 1, P = g.pitch, c = IDX(g, i, j);
-    const float dhi = (float)(1.0 / g.dh);
-    f.syy[c] = 0.0f;
-    if (VISCO)
-        for (int l = 1; l <= g.L; ++l) f.q[(l - 1) * g.plane + c] = 0.0f;
-    float vxx = 0.f, vyy = 0.f;
-#pragma unroll
-    for (int m1 = 1; m1 <= ND; ++m1) {
-        f.syy[c - m1 * P] = -f.syy[c + m1 * P];
-        f.sxy[c - m1 * P] = -f.sxy[c + (m1 - 1) * P];
-        vxx += g.hc[m1] * (f.vx[c + m1 - 1] - f.vx[c - m1]);
-        vyy += g.hc[m1] * (f.vy[c + (m1 - 1) * P] - f.vy[c - m1 * P]);
-    }
-    vxx *= dhi;
-    vyy *= dhi;
-    if (g.abs_type == 1) {
-        if (i <= g.fw) {
-            const int o = j * g.wx + i;
-            const float ps = PRF(g, m, SOFI_P_B_X, i) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, i) * vxx;
-            f.psi_vxx[o] = ps;
-            vxx = vxx / PRF(g, m, SOFI_P_K_X, i) + ps;
-        }
-        if (i >= g.nx - g.fw + 1) {
-            const int h = i - g.nx + 2 * g.fw, o = j * g.wx + h;
-            const float ps = PRF(g, m, SOFI_P_B_X, h) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, h) * vxx;
-            f.psi_vxx[o] = ps;
-            vxx = vxx / PRF(g, m, SOFI_P_K_X, h) + ps;
-        }
-    }
-    if (!VISCO) {
-        const float fjm = m.pu[c] * 2.0f;
-        const float gg = m.ppi[c];
-        f.sxx[c] += -(g.dt * (gg - fjm) * (gg - fjm) * vxx / gg) - (g.dt * (gg - fjm) * vyy);
-    } else {
-        float sump = 0.f;
-        for (int l = 1; l <= g.L; ++l) sump += f.p[(l - 1) * g.plane + c];
-        const float fjm = (float)(m.pu[c] * 2.0 * (1.0 + g.L * m.ptaus[c]));
-        const float gg = (float)(m.ppi[c] * (1.0 + g.L * m.ptaup[c]));
-        f.sxx[c] += -(g.dt * (gg - fjm) * (gg - fjm) * vxx / gg) - (g.dt * (gg - fjm) * vyy) - (g.dthalbe * sump);
-        sump = 0.f;
-        for (int l = 1; l <= g.L; ++l) {
-            const float eta = m.eta[l];
-            const float bjm = (float)(eta / (1.0 + (eta * 0.5)));
-            const float djm = (float)(2.0 * m.pu[c] * m.ptaus[c]);
-            const float e = m.ppi[c] * m.ptaup[c];
-            float &pv = f.p[(l - 1) * g.plane + c];
-            pv += bjm * (((djm - e) * ((fjm / gg) - 1.0f) * vxx) - ((djm - e) * vyy));
-            sump += pv;
-        }
-        f.sxx[c] += (g.dthalbe * sump);
-    }
-}
-
 // ------------------------------------------------------------- samplers
 
 struct Rec {
@@ -698,15 +1085,25 @@ struct sofi_dev {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
-    std::vector<void *> allocs;
+    struct Alloc {
+        void *user, *raw;
+        size_t bytes;
+    };
+    std::vector<Alloc> allocs;
     size_t bytes = 0;
     // device-side copies owned by the handle
     float *mat[SOFI_A_NARRAYS] = {};
     float *base_d = nullptr, *base_e = nullptr, *base_r = nullptr;
     float *prof = nullptr;
+    float *surf = nullptr;
     int *src_i = nullptr, *src_j = nullptr, *src_t = nullptr;
     float *src_azi = nullptr, *src_amp = nullptr, *src_sig = nullptr;
+    double *src_sc = nullptr;
     int source_type = 0;
+    bool any_force = false;                // a source of type 2, 3 or 4 (force)
+    Fuse fz{};                             // cells of the sources when nsrc <= FUSE_MAX (fz.nforce, npsrc unset)
+    bool fusable = false;
+    int shape_v = 0, shape_s = 0;          // velocity / stress kernel shapes: SOFI_LAUNCH_*
     int *rec_i = nullptr, *rec_j = nullptr;
     double *rec_d = nullptr;
     float *sec = nullptr;
@@ -726,15 +1123,31 @@ struct sofi_dev {
 
 namespace {
 
+// Every device array starts 224 B past a 256-B boundary: a padded plane's
+// cell (1, 1) lies 8 rows and 8 columns (32 B) past the array start and the
+// row pitch is a multiple of 64 floats, so every interior row then begins on
+// a 256-B boundary and a wavefront's 64 cells of a row are two whole 128-B
+// cache lines.
+constexpr size_t ALIGN_SHIFT = 224;
+
 int dalloc(sofi_dev *h, void **p, size_t bytes)
 {
     if (bytes == 0) bytes = 4;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    e = hipMemset(*p, 0, bytes);
-    if (e != hipSuccess) return fail("hipMemset failed: %s", hipGetErrorString(e));
-    h->allocs.push_back(*p);
-    h->bytes += bytes;
+    void *raw = nullptr;
+    hipError_t e = hipMalloc(&raw, bytes + 256);
+    if (e != hipSuccess) return fail("hipMalloc(%zu) failed: %s", bytes + 256, hipGetErrorString(e));
+    if ((uintptr_t)raw % 256) {
+        (void)hipFree(raw);
+        return fail("hipMalloc returned a pointer that is not 256-B aligned");
+    }
+    e = hipMemset(raw, 0, bytes + 256);
+    if (e != hipSuccess) {
+        (void)hipFree(raw);
+        return fail("hipMemset failed: %s", hipGetErrorString(e));
+    }
+    *p = (char *)raw + ALIGN_SHIFT;
+    h->allocs.push_back({ *p, raw, bytes + 256 });
+    h->bytes += bytes + 256;
     return 0;
 }
 
@@ -744,14 +1157,12 @@ void dfree(sofi_dev *h, T **p)
 {
     if (!*p) return;
     for (size_t k = 0; k < h->allocs.size(); ++k)
-        if (h->allocs[k] == (void *)*p) {
+        if (h->allocs[k].user == (void *)*p) {
+            h->bytes -= h->allocs[k].bytes;
+            (void)hipFree(h->allocs[k].raw);
             h->allocs.erase(h->allocs.begin() + k);
             break;
         }
-    size_t sz = 0;
-    (void)hipMemPtrGetInfo(*p, &sz);
-    h->bytes -= sz < h->bytes ? sz : h->bytes;
-    (void)hipFree(*p);
     *p = nullptr;
 }
 
@@ -811,6 +1222,7 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
     g.gy3 = d.gy[3];
     g.wx = 2 * d.fw + 1;
     g.wy = d.nx + 1;
+    g.ntx = (d.nx + BX - 1) / BX;
     g.dt = d.dt;
     g.dh = d.dh;
     g.dtdh = d.dt / d.dh;
@@ -974,6 +1386,34 @@ int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const 
     m.ptaus = h->mat[SOFI_A_PTAUS];
     m.ptaup = h->mat[SOFI_A_PTAUP];
     m.prof = h->prof;
+    if (d.free_surf) {
+        // row-1 coefficients of the free surface (see SURF_*)
+        const float *pu = arrays[SOFI_A_PU], *ppi = arrays[SOFI_A_PPI];
+        const float *ts = arrays[SOFI_A_PTAUS], *tp = arrays[SOFI_A_PTAUP];
+        const bool visco = g.L > 0;
+        if (visco && (!ts || !tp)) return fail("sofi_set_medium: the free surface needs the tau arrays");
+        std::vector<float> sc((size_t)SURF_N * d.nx, 0.0f);
+        for (int i = 1; i <= d.nx; ++i) {
+            const size_t c = sofi_idx(g.pitch, i, 1);
+            float fjm, gg;
+            if (!visco) {
+                fjm = pu[c] * 2.0f;
+                gg = ppi[c];
+            } else {
+                fjm = (float)(pu[c] * 2.0 * (1.0 + g.L * ts[c]));
+                gg = (float)(ppi[c] * (1.0 + g.L * tp[c]));
+                const float djm = (float)(2.0 * pu[c] * ts[c]);
+                const float e = ppi[c] * tp[c];
+                sc[(size_t)SURF_DE * d.nx + i - 1] = djm - e;
+                sc[(size_t)SURF_FG * d.nx + i - 1] = (fjm / gg) - 1.0f;
+            }
+            sc[(size_t)SURF_T1 * d.nx + i - 1] = g.dt * (gg - fjm) * (gg - fjm);
+            sc[(size_t)SURF_T2 * d.nx + i - 1] = g.dt * (gg - fjm);
+            sc[(size_t)SURF_G * d.nx + i - 1] = gg;
+        }
+        if (upload(h, &h->surf, sc.data(), sc.size())) return -1;
+        m.surf = h->surf;
+    }
     if (g.L > 0) {
         if (!profiles || !profiles[SOFI_P_ETA] || !profiles[SOFI_P_BIP] || !profiles[SOFI_P_CIP])
             return fail("sofi_set_medium: relaxation profiles (eta, bip, cip) missing");
@@ -981,6 +1421,8 @@ int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const 
             m.eta[l] = profiles[SOFI_P_ETA][l];
             m.bip[l] = profiles[SOFI_P_BIP][l];
             m.cip[l] = profiles[SOFI_P_CIP][l];
+            const float eta = m.eta[l];
+            m.bjm[l] = (float)(eta / (1.0 + (eta * 0.5)));
         }
     }
     h->h_ppi.assign(arrays[SOFI_A_PPI], arrays[SOFI_A_PPI] + pl);
@@ -1004,11 +1446,19 @@ int sofi_set_sources(sofi_dev *h, int nsrc, const int *src_i, const int *src_j, 
     size_t n = (size_t)(nsrc > 0 ? nsrc : 1);
     HIPCHK(hipStreamSynchronize(h->stream));
     dfree(h, &h->src_i); dfree(h, &h->src_j); dfree(h, &h->src_t);
-    dfree(h, &h->src_azi); dfree(h, &h->src_amp); dfree(h, &h->src_sig);
+    dfree(h, &h->src_azi); dfree(h, &h->src_amp); dfree(h, &h->src_sig); dfree(h, &h->src_sc);
     int rc = 0;
     rc |= dalloc_t(h, &h->src_i, n) | dalloc_t(h, &h->src_j, n) | dalloc_t(h, &h->src_t, n);
     rc |= dalloc_t(h, &h->src_azi, n) | dalloc_t(h, &h->src_amp, n) | dalloc_t(h, &h->src_sig, n * d.nt);
+    rc |= dalloc_t(h, &h->src_sc, 2 * n);
     if (rc) return -1;
+    std::vector<double> sc(2 * n, 0.0);
+    for (int k = 0; k < nsrc; ++k) {
+        const float azi = (float)(src_azi[k] * M_PI / 180);
+        sc[2 * k] = sin((double)azi);
+        sc[2 * k + 1] = cos((double)azi);
+    }
+    HIPCHK(hipMemcpy(h->src_sc, sc.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice));
     if (nsrc > 0) {
         HIPCHK(hipMemcpy(h->src_i, src_i, nsrc * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(h->src_j, src_j, nsrc * sizeof(int), hipMemcpyHostToDevice));
@@ -1023,9 +1473,17 @@ int sofi_set_sources(sofi_dev *h, int nsrc, const int *src_i, const int *src_j, 
     h->s.azi = h->src_azi;
     h->s.amp = h->src_amp;
     h->s.sig = h->src_sig;
+    h->s.sc = h->src_sc;
     h->s.n = nsrc;
     h->s.nt = d.nt;
     h->source_type = source_type;
+    h->any_force = false;
+    for (int k = 0; k < nsrc; ++k)
+        if (src_type[k] >= 2 && src_type[k] <= 4) h->any_force = true;
+    h->fusable = nsrc <= FUSE_MAX;
+    h->fz = Fuse{};
+    if (h->fusable)
+        for (int k = 0; k < nsrc; ++k) h->fz.c[k] = (int)sofi_idx(h->g.pitch, src_i[k], src_j[k]);
     return 0;
 }
 
@@ -1113,6 +1571,20 @@ int sofi_reset(sofi_dev *h)
 
 namespace {
 
+// cells per thread of an ISO/VTI update kernel of the given shape (1: CELL,
+// 2: VEC2, 4: VEC4)
+int vec_width(const Grid &g, int shape)
+{
+    if (shape == SOFI_LAUNCH_CELL) return 1;
+    if (shape == SOFI_LAUNCH_VEC2) return 2;
+    if (shape == SOFI_LAUNCH_VEC4) return 4;
+    // automatic: one cell per thread (measured fastest for both kernels on the
+    // 4000 x 2000 viscoelastic workload: the vector shapes need 80-150
+    // VGPRs and lose more to the lower occupancy than they gain in loads)
+    (void)g;
+    return 1;
+}
+
 template <int ND>
 int step_nd(sofi_dev *h, int nt)
 {
@@ -1121,65 +1593,90 @@ int step_nd(sofi_dev *h, int nt)
     const Mat &m = h->m;
     const sofi_desc &d = h->d;
     hipStream_t st = h->stream;
-    dim3 blk(BX, BY), grd((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
-    if (h->s.n > 0) hipLaunchKernelGGL(k_force, dim3(1), dim3(64), 0, st, g, f, m, h->s, nt);
+    const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
+    const bool sample = d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
+    const bool sample_dv = sample && (d.seismo == 3 || d.seismo == 4);
+    const int ins = h->lsamp / (d.ndt > 0 ? d.ndt : 1);
+    // what the update kernels do besides the update: forces before the
+    // velocity update; explosive sources and the free surface after the
+    // stress update (not in steps that sample div/curl, which read the CPML
+    // memory before the free surface updates it again)
+    Fuse fv = h->fz, fs = h->fz;
+    fv.npsrc = fs.nforce = 0;
+    fv.surf = 0;
+    fv.nforce = h->fusable && h->any_force ? h->s.n : 0;
+    const bool src_fused = !tti && h->fusable && h->source_type != 5;
+    fs.npsrc = src_fused && h->source_type == 1 ? h->s.n : 0;
+    fs.surf = src_fused && d.free_surf && !sample_dv && (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
+    if (h->s.n > 0 && h->any_force && !fv.nforce) hipLaunchKernelGGL(k_force, dim3(1), dim3(64), 0, st, g, f, m, h->s, nt);
     hipEvent_t *te = nullptr;
     if (h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
         te = &h->tev[h->tev_used];
         h->tev_used += 4;
     }
+    const int nty = (g.ny + BY - 1) / BY;
+    const dim3 blk(BX, BY);
+    int vw = vec_width(g, h->shape_v);
+    dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty);
     if (te) (void)hipEventRecord(te[0], st);
-    hipLaunchKernelGGL(k_velocity<ND>, grd, blk, 0, st, g, f, m, nt);
+    if (vw == 4) hipLaunchKernelGGL((k_velocity_v<ND, 4>), tiles, blk, 0, st, g, f, m, h->s, fv, nt);
+    else if (vw == 2) hipLaunchKernelGGL((k_velocity_v<ND, 2>), tiles, blk, 0, st, g, f, m, h->s, fv, nt);
+    else hipLaunchKernelGGL(k_velocity<ND>, tiles, blk, 0, st, g, f, m, h->s, fv, nt);
+    vw = vec_width(g, h->shape_s);
+    tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty);
     if (te) {
         (void)hipEventRecord(te[1], st);
         (void)hipEventRecord(te[2], st);
     }
-    const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
+#define SOFI_STRESS(W)                                                                                  \
+    do {                                                                                                \
+        if (vw == 4) hipLaunchKernelGGL((k_stress_v<ND, W, 4>), tiles, blk, 0, st, g, f, m, h->s, fs, nt); \
+        else if (vw == 2) hipLaunchKernelGGL((k_stress_v<ND, W, 2>), tiles, blk, 0, st, g, f, m, h->s, fs, nt); \
+        else hipLaunchKernelGGL((k_stress<ND, W>), tiles, blk, 0, st, g, f, m, h->s, fs, nt);            \
+    } while (0)
     switch (d.weq) {
-      case SOFI_EL_ISO: hipLaunchKernelGGL((k_stress<ND, SOFI_EL_ISO>), grd, blk, 0, st, g, f, m); break;
-      case SOFI_VEL_ISO: hipLaunchKernelGGL((k_stress<ND, SOFI_VEL_ISO>), grd, blk, 0, st, g, f, m); break;
-      case SOFI_EL_VTI: hipLaunchKernelGGL((k_stress<ND, SOFI_EL_VTI>), grd, blk, 0, st, g, f, m); break;
-      case SOFI_VEL_VTI: hipLaunchKernelGGL((k_stress<ND, SOFI_VEL_VTI>), grd, blk, 0, st, g, f, m); break;
+      case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
+      case SOFI_VEL_ISO: SOFI_STRESS(SOFI_VEL_ISO); break;
+      case SOFI_EL_VTI: SOFI_STRESS(SOFI_EL_VTI); break;
+      case SOFI_VEL_VTI: SOFI_STRESS(SOFI_VEL_VTI); break;
       default: {
-          hipLaunchKernelGGL(k_dv<ND>, grd, blk, 0, st, g, f);
+          dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
+          hipLaunchKernelGGL(k_dv<ND>, tg, tb, 0, st, g, f);
           const bool el = d.weq == SOFI_EL_TTI;
-          if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), grd, blk, 0, st, g, f, m);
-          else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), grd, blk, 0, st, g, f, m);
+          if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), tg, tb, 0, st, g, f, m);
+          else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), tg, tb, 0, st, g, f, m);
           if (d.fw > 0 && d.abs_type == 1) {
-              hipLaunchKernelGGL(k_cpml_dv_frame<false>, grd, blk, 0, st, g, f, m);
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), grd, blk, 0, st, g, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), grd, blk, 0, st, g, f, m);
-              hipLaunchKernelGGL(k_cpml_dv_frame<true>, grd, blk, 0, st, g, f, m);
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 3>), grd, blk, 0, st, g, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 3>), grd, blk, 0, st, g, f, m);
+              hipLaunchKernelGGL(k_cpml_dv_frame<false>, tg, tb, 0, st, g, f, m);
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), tg, tb, 0, st, g, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), tg, tb, 0, st, g, f, m);
+              hipLaunchKernelGGL(k_cpml_dv_frame<true>, tg, tb, 0, st, g, f, m);
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 3>), tg, tb, 0, st, g, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 3>), tg, tb, 0, st, g, f, m);
           } else if (d.fw > 0 && d.abs_type == 2) {
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), grd, blk, 0, st, g, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), grd, blk, 0, st, g, f, m);
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), tg, tb, 0, st, g, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), tg, tb, 0, st, g, f, m);
           }
       }
     }
+#undef SOFI_STRESS
     if (te) (void)hipEventRecord(te[3], st);
-    const bool sample = d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
-    const int ins = h->lsamp / (d.ndt > 0 ? d.ndt : 1);
-    if (sample && (d.seismo == 3 || d.seismo == 4)) {
+    if (sample_dv) {
         dim3 rb(64), rg((h->r.n + 63) / 64);
         if (tti) hipLaunchKernelGGL((k_seis_dv<ND, true>), rg, rb, 0, st, g, f, m, h->r, ins);
         else hipLaunchKernelGGL((k_seis_dv<ND, false>), rg, rb, 0, st, g, f, m, h->r, ins);
     }
-    if (h->s.n > 0) {
+    if (h->s.n > 0 && !src_fused) {
         if (h->source_type == 1) hipLaunchKernelGGL(k_psource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
         if (h->source_type == 5) hipLaunchKernelGGL(k_eqsource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
     }
-    if (d.free_surf) {
+    if (d.free_surf && !fs.surf) {
         dim3 sb(256), sg((g.nx + 255) / 256);
         if (g.L > 0) hipLaunchKernelGGL((k_surface<ND, true>), sg, sb, 0, st, g, f, m);
         else hipLaunchKernelGGL((k_surface<ND, false>), sg, sb, 0, st, g, f, m);
     }
-    if (sample) {
-        if (d.seismo != 3) {
-            dim3 rb(64), rg((h->r.n + 63) / 64);
-            hipLaunchKernelGGL(k_seis, rg, rb, 0, st, g, f, h->r, ins, d.seismo);
-        }
+    if (sample && d.seismo != 3) {
+        dim3 rb(64), rg((h->r.n + 63) / 64);
+        hipLaunchKernelGGL(k_seis, rg, rb, 0, st, g, f, h->r, ins, d.seismo);
     }
     if (d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
     if (d.snap && nt == h->lsnap && nt <= d.esnap) {
@@ -1290,7 +1787,7 @@ int sofi_close(sofi_dev *h)
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
-    for (void *p : h->allocs) (void)hipFree(p);
+    for (const auto &a : h->allocs) (void)hipFree(a.raw);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1299,6 +1796,16 @@ int sofi_close(sofi_dev *h)
 }
 
 size_t sofi_footprint(const sofi_dev *h) { return h ? h->bytes : 0; }
+
+int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape)
+{
+    if (!h) return fail("sofi_set_launch: null handle");
+    for (int sh : { velocity_shape, stress_shape })
+        if (sh < SOFI_LAUNCH_AUTO || sh > SOFI_LAUNCH_VEC4) return fail("sofi_set_launch: bad shape %d", sh);
+    h->shape_v = velocity_shape;
+    h->shape_s = stress_shape;
+    return 0;
+}
 
 int sofi_set_kernel_timing(sofi_dev *h, int every, int max_samples)
 {

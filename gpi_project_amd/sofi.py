@@ -201,6 +201,16 @@ class Simulation:
         _chk_hip(self._L.sofi_run(self._h, nt_first, nt_last))
         return float(self._L.sofi_last_run_ms(self._h))
 
+    LAUNCH = {"auto": 0, "cell": 1, "vec2": 2, "vec4": 3}
+
+    def set_launch(self, shape: str, stress_shape: str | None = None):
+        """Shapes of the isotropic/VTI update kernels: "auto", "cell" (one
+        thread per cell), "vec2" or "vec4" (one thread per two or four cells
+        of a row); `shape` for the velocity kernel and, unless
+        `stress_shape` is given, the stress kernel."""
+        st = shape if stress_shape is None else stress_shape
+        _chk_hip(self._L.sofi_set_launch(self._h, self.LAUNCH[shape], self.LAUNCH[st]))
+
     def set_kernel_timing(self, every: int, max_samples: int = 4096):
         _chk_hip(self._L.sofi_set_kernel_timing(self._h, every, max_samples))
 

@@ -93,6 +93,16 @@ int sofi_close(sofi_dev *h);
 /* Device bytes the handle holds (HBM footprint of one resident shot). */
 size_t sofi_footprint(const sofi_dev *h);
 
+/* Shapes of the isotropic/VTI velocity and stress update kernels.  No counterpart in the
+ * reference (a launch-tuning knob): SOFI_LAUNCH_CELL runs one thread per
+ * cell in 64 x 4 tiles, SOFI_LAUNCH_VEC2 / SOFI_LAUNCH_VEC4 one thread per
+ * two / four consecutive cells of a row (vector loads and stores) in
+ * 128 x 4 / 256 x 4 tiles, SOFI_LAUNCH_AUTO (the default) the shape
+ * measured fastest (currently CELL).  Results agree to rounding between
+ * shapes. */
+enum { SOFI_LAUNCH_AUTO = 0, SOFI_LAUNCH_CELL = 1, SOFI_LAUNCH_VEC2 = 2, SOFI_LAUNCH_VEC4 = 3 };
+int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape);
+
 /* Sampled kernel timing on the handle's stream: every `every`-th step of
  * the following sofi_run calls records HIP events around the velocity
  * kernel and around the stress-update kernels (at most max_samples steps);

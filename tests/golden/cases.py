@@ -161,3 +161,23 @@ def nsnap(c: W.Case) -> int:
             n += 1
             ls += inc
     return n
+
+
+def multisource_case(td: str, nsrc: int) -> str:
+    """VEL_ISO L=2, FD8 (Holberg 0.1 %), CPML + free surface with `nsrc`
+    explosive sources: the first three sit on rows 1 and 3, two of them on the
+    same cell; with more, sources on rows 2, 4 and 5, one in the left and one
+    in the right CPML frame, the rest in the interior.  Grid index of a
+    coordinate is floor(x / DH) + 1, so cell (i, j) is placed at
+    ((i - 1) DH + 1, (j - 1) DH + 1)."""
+    cells = [(80, 1), (90, 3), (80, 1), (70, 2), (100, 4), (110, 5), (5, 3), (150, 60), (60, 70),
+             (40, 30), (120, 90), (20, 110)]
+    assert 1 <= nsrc <= len(cells)
+    srcs = [((i - 1) * DH + 1.0, (j - 1) * DH + 1.0, 0.002 * k, 15.0 + 2.0 * k, 1.0 - 0.05 * k)
+            for k, (i, j) in enumerate(cells[:nsrc])]
+    L = 2
+    c = W.Case(name=f"msrc{nsrc}", nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq="VEL_ISO",
+               fdorder=8, maxrelerror=1, L=L, fl=(4.0, 30.0), f_ref=20.0, abs_type=1, fw=12, free_surf=1,
+               fpml=20.0, vppml=3500.0, source_type=1, sources=srcs, receivers=_rec_grid(NX, NY, DH),
+               seismo=1, ndt=2)
+    return W.write_case(td, c, _model("VEL_ISO", 51, L=L))
