@@ -73,7 +73,7 @@ def host():
     L.sofi_problem_load.restype = _VP
     L.sofi_problem_load.argtypes = [C.c_char_p]
     L.sofi_problem_free.argtypes = [_VP]
-    for fn in ("nshots", "nsrc", "source_type", "snapcheck", "ntr", "nsnap"):
+    for fn in ("nshots", "nsrc", "source_type", "snapcheck", "ntr", "nsnap", "itermax"):
         f = getattr(L, "sofi_problem_" + fn)
         f.restype = C.c_int
         f.argtypes = [_VP]
@@ -104,6 +104,9 @@ def host():
     L.sofi_problem_seismo_comps.argtypes = [_VP, _IP]
     L.sofi_problem_snap_comps.argtypes = [_VP, _IP]
     L.sofi_holberg.argtypes = [C.c_int, C.c_int, _FP]
+    L.sofi_read_data.argtypes = [_VP, C.c_int, C.c_int, C.c_int, _FP]
+    L.sofi_residual.argtypes = [_VP, C.c_int, _FP, _FP, _FP, _DP]
+    L.sofi_write_gradient.argtypes = [_VP, C.c_int, _FP, _FP, _FP]
     L.sofi_outseis.argtypes = [C.c_char_p, _FP, _IP, C.c_int, _FP, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_float, C.c_float, C.c_int, _FP]
     _host = L
@@ -136,6 +139,11 @@ def hip():
     L.sofi_set_kernel_timing.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_get_kernel_timing.argtypes = [_VP, _DP, _DP, _IP]
     L.sofi_set_launch.argtypes = [_VP, C.c_int, C.c_int]
+    L.sofi_fwi_set_model.argtypes = [_VP, _FP, _FP, _FP]
+    L.sofi_fwi_set_pass.argtypes = [_VP, C.c_int]
+    L.sofi_fwi_set_adjoint.argtypes = [_VP, C.c_int, _IP, _IP, _FP, _FP]
+    L.sofi_fwi_gradient.argtypes = [_VP, _FP, _FP, _FP]
+    L.sofi_fwi_gradient_sum.argtypes = [_VP, _FP, _FP, _FP, C.c_int]
     _hip = L
     return L
 
@@ -155,5 +163,7 @@ def exported_symbols():
                            "sofi_set_receivers", "sofi_set_snapshots", "sofi_reset", "sofi_run",
                            "sofi_last_run_ms", "sofi_get_section", "sofi_snapshot_count",
                            "sofi_get_snapshot", "sofi_get_field", "sofi_close", "sofi_footprint",
-                           "sofi_set_kernel_timing", "sofi_get_kernel_timing", "sofi_set_launch"],
+                           "sofi_set_kernel_timing", "sofi_get_kernel_timing", "sofi_set_launch",
+                           "sofi_fwi_set_model", "sofi_fwi_set_pass", "sofi_fwi_set_adjoint",
+                           "sofi_fwi_gradient", "sofi_fwi_gradient_sum"],
     }

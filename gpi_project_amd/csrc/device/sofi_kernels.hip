@@ -22,6 +22,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "sofi_hip.h"
@@ -204,6 +205,21 @@ struct Fuse {
     int npsrc;          // sources whose explosive term the stress kernel adds (0: none)
     int surf;           // the stress kernel applies the free surface
     int c[FUSE_MAX];    // cell offsets of the sources, in source order
+    // FWI (src/wavefield_update_v.c:31-43, src/wavefield_update_s_el.c:43-47):
+    // velocity kernel: 0 nothing, 1 forward rho-weighted accelerations,
+    // 2 adjoint time integral of v, 3 adjoint copy of v (into pvxp1/pvyp1);
+    // stress kernel: u / DT of the EL_ISO update into ux/uy/uxy
+    int fwi_acc, fwi_u;
+    float *pvxp1, *pvyp1, *ux, *uy, *uxy;
+};
+
+// Sources grouped by cell, for injection one thread per distinct cell; a
+// cell's sources are added in source order, as the reference's loop does.
+struct Inj {
+    int n;              // distinct cells
+    const int *cell;    // [n] cell offsets
+    const int *first;   // [n + 1] start of each cell's run in idx
+    const int *idx;     // source indices
 };
 
 // force of source l at step nt on the velocities of its cell, added before
@@ -231,28 +247,50 @@ __device__ __forceinline__ float psource_amp(const Grid &g, const Src &s, int l,
     return (float)(-sg[nt - 1] / h2);
 }
 
-// force sources when they are not fused: one thread, source order
-__global__ void k_force(Grid g, Fields f, Mat m, Src s, int nt)
+// force sources when they are not fused: one thread per source cell
+__global__ void k_force(Grid g, Fields f, Mat m, Src s, Inj inj, int nt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int l = 0; l < s.n; ++l) {
-        const int c = IDX(g, s.i[l], s.j[l]);
-        float vx = f.vx[c], vy = f.vy[c];
-        force_add(g, s, l, nt, m.rip[c], m.rjp[c], vx, vy);
-        f.vx[c] = vx;
-        f.vy[c] = vy;
-    }
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= inj.n) return;
+    const int c = inj.cell[k];
+    float vx = f.vx[c], vy = f.vy[c];
+    const float rip = m.rip[c], rjp = m.rjp[c];
+    for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) force_add(g, s, inj.idx[q], nt, rip, rjp, vx, vy);
+    f.vx[c] = vx;
+    f.vy[c] = vy;
 }
 
-__global__ void k_psource(Grid g, Fields f, Src s, int nt)
+// explosive sources when they are not fused: one thread per source cell
+__global__ void k_psource(Grid g, Fields f, Src s, Inj inj, int nt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int l = 0; l < s.n; ++l) {
-        const float amp = psource_amp(g, s, l, nt);
-        const int c = IDX(g, s.i[l], s.j[l]);
-        f.sxx[c] += amp;
-        f.syy[c] += amp;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= inj.n) return;
+    const int c = inj.cell[k];
+    float sxx = f.sxx[c], syy = f.syy[c];
+    for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) {
+        const float amp = psource_amp(g, s, inj.idx[q], nt);
+        sxx += amp;
+        syy += amp;
     }
+    f.sxx[c] = sxx;
+    f.syy[c] = syy;
+}
+
+// adjoint sources at the receivers (src/update_v_interior.c:72-90):
+// ADJOINT_TYPE 1 adds the x and y residuals, 2 only y, 3 only x
+__global__ void k_adjoint(Grid g, Fields f, Inj inj, const float *sx, const float *sy, int nt_total, int type, int nt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= inj.n) return;
+    const int c = inj.cell[k];
+    float vx = f.vx[c], vy = f.vy[c];
+    for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) {
+        const size_t o = (size_t)inj.idx[q] * nt_total + (nt - 1);
+        if (type == 1 || type == 3) vx += sx[o];
+        if (type == 1 || type == 2) vy += sy[o];
+    }
+    f.vx[c] = vx;
+    f.vy[c] = vy;
 }
 
 // eqsource (src/eqsource.c:25-52): double-couple moment tensor
@@ -424,6 +462,16 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
     }
     for (int k = 0; k < fz.nforce; ++k)
         if (c == fz.c[k]) force_add(g, s, k, nt, rip, rjp, vx, vy);
+    if (fz.fwi_acc == 1) {
+        fz.pvxp1[c] = rip * (sxx_x + sxy_y) / g.dh;
+        fz.pvyp1[c] = rjp * (sxy_x + syy_y) / g.dh;
+    } else if (fz.fwi_acc == 2) {
+        fz.pvxp1[c] += vx * g.dt;
+        fz.pvyp1[c] += vy * g.dt;
+    } else if (fz.fwi_acc == 3) {
+        fz.pvxp1[c] = vx;
+        fz.pvyp1[c] = vy;
+    }
     vx = vx + (sxx_x + sxy_y) * g.dtdh * rip;
     vy = vy + (sxy_x + syy_y) * g.dtdh * rjp;
     if (g.abs_type == 2 && frame) {
@@ -520,9 +568,10 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f, Mat m, 
 // stresses on entry and the updated ones on exit; dv are the (possibly
 // CPML-modified) derivatives.
 
-// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48)
+// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48); in FWI forward
+// store steps with VELOCITY=1 also the increments over DT (ux, uy, uxy)
 __device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vxx, float vyx, float vxy, float vyy,
-                                       float &sxx, float &syy, float &sxy)
+                                       float &sxx, float &syy, float &sxy, const Fuse &fz, float dt, int c)
 {
     const float u1 = fipjp * (vyx + vxy);
     const float ff = 2.0f * f_;
@@ -531,6 +580,11 @@ __device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vx
     sxy += u1;
     sxx += u2;
     syy += u3;
+    if (fz.fwi_u) {
+        fz.uxy[c] = u1 / dt;
+        fz.ux[c] = u2 / dt;
+        fz.uy[c] = u3 / dt;
+    }
 }
 
 // wavefield_update_s_el_vti (src/wavefield_update_s_el_vti.c:27-47)
@@ -679,7 +733,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f, Mat m, Src 
     float vxx, vyx, vxy, vyy;
     if (!stress_dv(g, f, m, i, j, a, b, d, e, vxx, vyx, vxy, vyy)) return;
     float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
-    if (WEQ == SOFI_EL_ISO) upd_el(m.f[c], m.g[c], m.fipjp[c], vxx, vyx, vxy, vyy, sxx, syy, sxy);
+    if (WEQ == SOFI_EL_ISO) upd_el(m.f[c], m.g[c], m.fipjp[c], vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
     if (WEQ == SOFI_EL_VTI) upd_el_vti(m.c11[c], m.c13[c], m.c33[c], m.c55i[c], vxx, vyx, vxy, vyy, sxx, syy, sxy);
     if (WEQ == SOFI_VEL_ISO) {
         const float br = m.br[c], bd = m.bd[c], be = m.be[c];
@@ -794,7 +848,8 @@ __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Sr
     if (WEQ == SOFI_EL_ISO) {
         const V c1 = ldv<VW>(m.f, c0), c2 = ldv<VW>(m.g, c0), c3 = ldv<VW>(m.fipjp, c0);
 #pragma unroll
-        for (int k = 0; k < VW; ++k) upd_el(c1[k], c2[k], c3[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k]);
+        for (int k = 0; k < VW; ++k)
+            if (act[k]) upd_el(c1[k], c2[k], c3[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fz, g.dt, c0 + k);
     }
     if (WEQ == SOFI_EL_VTI) {
         const V c1 = ldv<VW>(m.c11, c0), c2 = ldv<VW>(m.c13, c0), c3 = ldv<VW>(m.c33, c0), c4 = ldv<VW>(m.c55i, c0);
@@ -1071,6 +1126,93 @@ __global__ void k_snap(Grid g, Fields f, Mat m, float *out, int i0, int di, int 
     out[SOFI_C_DIV * per + k] = (float)((double)(vxx + vyy) * sqrt((double)m.ppi[c]));
 }
 
+// ------------------------------------------------------------------ FWI
+
+struct Fwi {
+    float *pvxp1, *pvyp1, *ux, *uy, *uxy;       // rho-weighted accelerations / adjoint v; u over DT
+    float *glam, *gmu, *grho;                  // gradLam_shot, gradMu_shot, gradRhos_shot
+    float *gsum;                               // gradVp, gradVs, gradRho summed over shots (3 planes)
+    float *gshot;                              // the shot's gradVp, gradVs, gradRho (3 planes)
+    const float *vp0, *vs0, *rho0, *muss, *lamss;
+    float *store;                              // [nstore][5][ny][nx]
+    int velocity;
+};
+
+// snap_store (src/snap_store.c:27-47): the forward fields of one store step
+// into slot `slot` of the store (compact [5][ny][nx] per slot)
+__global__ __launch_bounds__(BX *BY) void k_fwi_store(Grid g, Fields f, Fwi w, int slot)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    const size_t n = (size_t)g.nx * g.ny, o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
+    w.store[o] = w.pvxp1[c];
+    w.store[o + n] = w.pvyp1[c];
+    if (w.velocity == 0) {
+        w.store[o + 2 * n] = f.sxx[c];
+        w.store[o + 3 * n] = f.syy[c];
+        w.store[o + 4 * n] = f.sxy[c];
+    } else {
+        w.store[o + 2 * n] = w.ux[c];
+        w.store[o + 3 * n] = w.uy[c];
+        w.store[o + 4 * n] = w.uxy[c];
+    }
+}
+
+// calc_conv (src/calc_conv.c:30-73): zero-lag correlation of the adjoint
+// fields with the forward fields of slot `slot`, with the reference's double
+// promotions (1.0 / (4.0 ...), 1.0 / 4.0)
+__global__ __launch_bounds__(BX *BY) void k_fwi_conv(Grid g, Fields f, Mat m, Fwi w, int slot)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    const size_t n = (size_t)g.nx * g.ny, o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
+    const float frx = w.store[o], fry = w.store[o + n];
+    const float fx = w.store[o + 2 * n], fy = w.store[o + 3 * n], fu = w.store[o + 4 * n];
+    const float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
+    const float muss = w.muss[c], lamss = w.lamss[c], DT = g.dt;
+    w.grho[c] += -DT * ((w.pvxp1[c] * frx) + (w.pvyp1[c] * fry));
+    w.glam[c] = (float)((double)w.glam[c] +
+                        (double)(-DT) * (1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) *
+                            (double)((fx + fy) * (sxx + syy)));
+    if (m.pu[c] > 0.0f) {
+        const double a = (1.0 / (double)(muss * muss)) * (double)(fu * sxy);
+        const double b = (1.0 / 4.0) * (double)((fx + fy) * (sxx + syy)) / (double)((lamss + muss) * (lamss + muss));
+        const double e = (1.0 / 4.0) * (double)((fx - fy) * (sxx - syy)) / (double)(muss * muss);
+        w.gmu[c] = (float)((double)w.gmu[c] + (double)(-DT) * (a + b + e));
+    }
+}
+
+// gradient of the shot (src/inversion.c:226-249): vp / vs / rho' from the
+// lambda / mu / rho accumulators, added to the sums; accumulators cleared
+__global__ __launch_bounds__(BX *BY) void k_fwi_grad(Grid g, Fwi w, int idx, int idy)
+{
+    const int i = 1 + blockIdx.x * BX + threadIdx.x;
+    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    float vp = 0.f, vs = 0.f, rho = 0.f;
+    if ((i - 1) % idx == 0 && (j - 1) % idy == 0) {
+        const double Vp0 = w.vp0[c], Vs0 = w.vs0[c], Rho0 = w.rho0[c];
+        const float gl = w.glam[c], gm = w.gmu[c], gr = w.grho[c];
+        vp = (float)(2.0 * Rho0 * Vp0 * gl);
+        vs = (float)((-4.0 * Vs0 * Rho0 * gl) + 2.0 * Rho0 * Vp0 * gm);
+        rho = (float)((((double)(w.vp0[c] * w.vp0[c]) - (2.0 * Vs0 * Vs0)) * gl) + (double)(w.vs0[c] * w.vs0[c] * gm) + gr);
+        w.gsum[c] += vp;
+        w.gsum[g.plane + c] += vs;
+        w.gsum[2 * g.plane + c] += rho;
+    }
+    w.gshot[c] = vp;
+    w.gshot[g.plane + c] = vs;
+    w.gshot[2 * g.plane + c] = rho;
+    w.glam[c] = 0.f;
+    w.gmu[c] = 0.f;
+    w.grho[c] = 0.f;
+}
+
 } // namespace
 
 // ====================================================================== handle
@@ -1100,6 +1242,8 @@ struct sofi_dev {
     float *src_azi = nullptr, *src_amp = nullptr, *src_sig = nullptr;
     double *src_sc = nullptr;
     int source_type = 0;
+    Inj src_inj{};                         // the shot's sources grouped by cell
+    int *src_inj_buf = nullptr;
     bool any_force = false;                // a source of type 2, 3 or 4 (force)
     Fuse fz{};                             // cells of the sources when nsrc <= FUSE_MAX (fz.nforce, npsrc unset)
     bool fusable = false;
@@ -1119,6 +1263,15 @@ struct sofi_dev {
     int t_samples = 0;
     // host copies needed for receiver factors
     std::vector<float> h_ppi, h_pu;
+    // FWI (desc.mode == SOFI_MODE_FWI)
+    Fwi w{};
+    int nstore = 0;
+    int sw = 0;                            // 0 forward pass (store), 1 adjoint pass (correlate)
+    bool model_set = false;
+    Inj adj_inj{};
+    int *adj_inj_buf = nullptr;
+    float *adj_sx = nullptr, *adj_sy = nullptr;
+    int nadj = 0;
 };
 
 namespace {
@@ -1176,6 +1329,44 @@ int upload(sofi_dev *h, float **dst, const float *src, size_t n)
 {
     if (!*dst && dalloc_t(h, dst, n)) return -1;
     HIPCHK(hipMemcpy(*dst, src, n * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Groups sources by cell (first occurrence order of the cells, source order
+// within a cell) into one device buffer [cell | first | idx].
+int build_inj(sofi_dev *h, int n, const int *si, const int *sj, Inj *inj, int **buf)
+{
+    dfree(h, buf);
+    *inj = Inj{};
+    std::vector<int> cells, order;
+    std::vector<std::vector<int>> members;
+    std::unordered_map<int, size_t> slot;
+    for (int k = 0; k < n; ++k) {
+        const int c = (int)sofi_idx(h->g.pitch, si[k], sj[k]);
+        auto it = slot.find(c);
+        if (it == slot.end()) {
+            it = slot.emplace(c, cells.size()).first;
+            cells.push_back(c);
+            members.emplace_back();
+        }
+        members[it->second].push_back(k);
+    }
+    std::vector<int> first(1, 0);
+    for (const auto &mbr : members) {
+        order.insert(order.end(), mbr.begin(), mbr.end());
+        first.push_back((int)order.size());
+    }
+    const int nc = (int)cells.size();
+    std::vector<int> all;
+    all.insert(all.end(), cells.begin(), cells.end());
+    all.insert(all.end(), first.begin(), first.end());
+    all.insert(all.end(), order.begin(), order.end());
+    if (dalloc_t(h, buf, all.size())) return -1;
+    HIPCHK(hipMemcpy(*buf, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+    inj->n = nc;
+    inj->cell = *buf;
+    inj->first = *buf + nc;
+    inj->idx = *buf + 2 * nc + 1;
     return 0;
 }
 
@@ -1264,6 +1455,25 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
         rc |= dalloc_t(h, &f.psi_sxy_y, nyp);
         rc |= dalloc_t(h, &f.psi_vyy, nyp);
         rc |= dalloc_t(h, &f.psi_vxy, nyp);
+    }
+    if (d.mode == SOFI_MODE_FWI) {
+        if (d.weq != SOFI_EL_ISO) {
+            sofi_close(h);
+            return fail("sofi_open: MODE=FWI is supported for WEQ=EL_ISO only");
+        }
+        if (d.adjoint_type < 1 || d.adjoint_type > 3) {
+            sofi_close(h);
+            return fail("sofi_open: ADJOINT_TYPE=%d is not supported (1, 2 or 3)", d.adjoint_type);
+        }
+        const int dtinv = d.dtinv > 0 ? d.dtinv : 1;
+        h->nstore = (d.nt + dtinv - 1) / dtinv;
+        Fwi &w = h->w;
+        w.velocity = d.velocity;
+        rc |= dalloc_t(h, &w.pvxp1, pl) | dalloc_t(h, &w.pvyp1, pl);
+        rc |= dalloc_t(h, &w.ux, pl) | dalloc_t(h, &w.uy, pl) | dalloc_t(h, &w.uxy, pl);
+        rc |= dalloc_t(h, &w.glam, pl) | dalloc_t(h, &w.gmu, pl) | dalloc_t(h, &w.grho, pl);
+        rc |= dalloc_t(h, &w.gsum, 3 * pl) | dalloc_t(h, &w.gshot, 3 * pl);
+        rc |= dalloc_t(h, &w.store, (size_t)h->nstore * 5 * d.nx * d.ny);
     }
     if (rc) {
         std::string e = g_err;
@@ -1480,6 +1690,7 @@ int sofi_set_sources(sofi_dev *h, int nsrc, const int *src_i, const int *src_j, 
     h->any_force = false;
     for (int k = 0; k < nsrc; ++k)
         if (src_type[k] >= 2 && src_type[k] <= 4) h->any_force = true;
+    if (build_inj(h, nsrc, src_i, src_j, &h->src_inj, &h->src_inj_buf)) return -1;
     h->fusable = nsrc <= FUSE_MAX;
     h->fz = Fuse{};
     if (h->fusable)
@@ -1559,6 +1770,8 @@ int sofi_reset(sofi_dev *h)
     for (float *p : { f.psi_syy_y, f.psi_sxy_y, f.psi_vyy, f.psi_vxy })
         if (p) HIPCHK(hipMemsetAsync(p, 0, nyp, h->stream));
     if (h->sec) HIPCHK(hipMemsetAsync(h->sec, 0, (size_t)SOFI_NCOMP * (h->r.n > 0 ? h->r.n : 1) * (h->d.ns > 0 ? h->d.ns : 1) * sizeof(float), h->stream));
+    for (float *p : { h->w.pvxp1, h->w.pvyp1, h->w.ux, h->w.uy, h->w.uxy })
+        if (p) HIPCHK(hipMemsetAsync(p, 0, pl, h->stream));
     HIPCHK(hipMemsetAsync(f.unstable, 0, sizeof(int), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->nsnap = 0;
@@ -1594,9 +1807,18 @@ int step_nd(sofi_dev *h, int nt)
     const sofi_desc &d = h->d;
     hipStream_t st = h->stream;
     const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
-    const bool sample = d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
+    // FWI: the forward pass stores every DTINV-th step; the adjoint pass
+    // injects the residuals at the receivers and correlates at the steps whose
+    // time-reversed forward step was stored (src/time_loop.c:287-303); it
+    // records no seismograms and takes no snapshots
+    const bool fwi = d.mode == SOFI_MODE_FWI, adj = fwi && h->sw == 1;
+    const int dtinv = d.dtinv > 0 ? d.dtinv : 1, fstep = d.nt - nt + 1;
+    const bool store = fwi && !adj && (nt - 1) % dtinv == 0;
+    const bool conv = adj && (fstep - 1) % dtinv == 0;
+    const bool sample = !adj && d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
     const bool sample_dv = sample && (d.seismo == 3 || d.seismo == 4);
     const int ins = h->lsamp / (d.ndt > 0 ? d.ndt : 1);
+    const int nsrc = adj ? 0 : h->s.n;     // the shot's sources act in the forward pass only
     // what the update kernels do besides the update: forces before the
     // velocity update; explosive sources and the free surface after the
     // stress update (not in steps that sample div/curl, which read the CPML
@@ -1604,11 +1826,22 @@ int step_nd(sofi_dev *h, int nt)
     Fuse fv = h->fz, fs = h->fz;
     fv.npsrc = fs.nforce = 0;
     fv.surf = 0;
-    fv.nforce = h->fusable && h->any_force ? h->s.n : 0;
+    fv.nforce = h->fusable && h->any_force ? nsrc : 0;
     const bool src_fused = !tti && h->fusable && h->source_type != 5;
-    fs.npsrc = src_fused && h->source_type == 1 ? h->s.n : 0;
+    fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
     fs.surf = src_fused && d.free_surf && !sample_dv && (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
-    if (h->s.n > 0 && h->any_force && !fv.nforce) hipLaunchKernelGGL(k_force, dim3(1), dim3(64), 0, st, g, f, m, h->s, nt);
+    fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
+    fs.fwi_u = store && d.velocity == 1;
+    fv.pvxp1 = fs.pvxp1 = h->w.pvxp1;
+    fv.pvyp1 = fs.pvyp1 = h->w.pvyp1;
+    fv.ux = fs.ux = h->w.ux;
+    fv.uy = fs.uy = h->w.uy;
+    fv.uxy = fs.uxy = h->w.uxy;
+    if (nsrc > 0 && h->any_force && !fv.nforce)
+        hipLaunchKernelGGL(k_force, dim3((h->src_inj.n + 63) / 64), dim3(64), 0, st, g, f, m, h->s, h->src_inj, nt);
+    if (adj && h->nadj > 0)
+        hipLaunchKernelGGL(k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), 0, st, g, f, h->adj_inj, h->adj_sx,
+                           h->adj_sy, d.nt, d.adjoint_type, nt);
     hipEvent_t *te = nullptr;
     if (h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
         te = &h->tev[h->tev_used];
@@ -1665,8 +1898,9 @@ int step_nd(sofi_dev *h, int nt)
         if (tti) hipLaunchKernelGGL((k_seis_dv<ND, true>), rg, rb, 0, st, g, f, m, h->r, ins);
         else hipLaunchKernelGGL((k_seis_dv<ND, false>), rg, rb, 0, st, g, f, m, h->r, ins);
     }
-    if (h->s.n > 0 && !src_fused) {
-        if (h->source_type == 1) hipLaunchKernelGGL(k_psource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
+    if (nsrc > 0 && !src_fused) {
+        if (h->source_type == 1)
+            hipLaunchKernelGGL(k_psource, dim3((h->src_inj.n + 63) / 64), dim3(64), 0, st, g, f, h->s, h->src_inj, nt);
         if (h->source_type == 5) hipLaunchKernelGGL(k_eqsource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
     }
     if (d.free_surf && !fs.surf) {
@@ -1679,7 +1913,12 @@ int step_nd(sofi_dev *h, int nt)
         hipLaunchKernelGGL(k_seis, rg, rb, 0, st, g, f, h->r, ins, d.seismo);
     }
     if (d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
-    if (d.snap && nt == h->lsnap && nt <= d.esnap) {
+    if (store || conv) {
+        dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
+        if (store) hipLaunchKernelGGL(k_fwi_store, tg, tb, 0, st, g, f, h->w, (nt - 1) / dtinv);
+        else hipLaunchKernelGGL(k_fwi_conv, tg, tb, 0, st, g, f, m, h->w, (fstep - 1) / dtinv);
+    }
+    if (!adj && d.snap && nt == h->lsnap && nt <= d.esnap) {
         if (h->nsnap < h->nsnap_max) {
             int n = h->nxs * h->nys;
             float *out = h->snapbuf + (size_t)h->nsnap * SOFI_NCOMP * n;
@@ -1796,6 +2035,94 @@ int sofi_close(sofi_dev *h)
 }
 
 size_t sofi_footprint(const sofi_dev *h) { return h ? h->bytes : 0; }
+
+int sofi_fwi_set_model(sofi_dev *h, const float *vp0, const float *vs0, const float *rho0)
+{
+    if (!h || !vp0 || !vs0 || !rho0) return fail("sofi_fwi_set_model: null argument");
+    if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_set_model: the handle was not opened with MODE=FWI");
+    const size_t pl = (size_t)h->g.plane;
+    // the moduli calc_conv forms per cell (src/calc_conv.c:45-46), in the
+    // reference's float / double rounding
+    std::vector<float> muss(pl), lamss(pl);
+    for (size_t q = 0; q < pl; ++q) {
+        muss[q] = vs0[q] * vs0[q] * rho0[q];
+        lamss[q] = (float)(vp0[q] * vp0[q] * rho0[q] - 2.0 * muss[q]);
+    }
+    float *b[5] = { const_cast<float *>(h->w.vp0), const_cast<float *>(h->w.vs0), const_cast<float *>(h->w.rho0),
+                    const_cast<float *>(h->w.muss), const_cast<float *>(h->w.lamss) };
+    const float *src[5] = { vp0, vs0, rho0, muss.data(), lamss.data() };
+    for (int k = 0; k < 5; ++k)
+        if (upload(h, &b[k], src[k], pl)) return -1;
+    h->w.vp0 = b[0];
+    h->w.vs0 = b[1];
+    h->w.rho0 = b[2];
+    h->w.muss = b[3];
+    h->w.lamss = b[4];
+    h->model_set = true;
+    return 0;
+}
+
+int sofi_fwi_set_pass(sofi_dev *h, int sw)
+{
+    if (!h) return fail("sofi_fwi_set_pass: null handle");
+    if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_set_pass: the handle was not opened with MODE=FWI");
+    if (sw != 0 && sw != 1) return fail("sofi_fwi_set_pass: sw must be 0 (forward) or 1 (adjoint)");
+    if (sw == 1 && !h->model_set) return fail("sofi_fwi_set_pass: sofi_fwi_set_model first");
+    h->sw = sw;
+    return 0;
+}
+
+int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const float *sig_x, const float *sig_y)
+{
+    if (!h) return fail("sofi_fwi_set_adjoint: null handle");
+    if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_set_adjoint: the handle was not opened with MODE=FWI");
+    if (n < 0 || (n > 0 && (!ai || !aj || !sig_x || !sig_y))) return fail("sofi_fwi_set_adjoint: bad arguments");
+    for (int k = 0; k < n; ++k)
+        if (ai[k] < 1 || ai[k] > h->d.nx || aj[k] < 1 || aj[k] > h->d.ny)
+            return fail("sofi_fwi_set_adjoint: source %d at (%d, %d) is outside the grid", k, ai[k], aj[k]);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, &h->adj_sx);
+    dfree(h, &h->adj_sy);
+    const size_t ns = (size_t)(n > 0 ? n : 1) * h->d.nt;
+    if (dalloc_t(h, &h->adj_sx, ns) || dalloc_t(h, &h->adj_sy, ns)) return -1;
+    if (n > 0) {
+        HIPCHK(hipMemcpy(h->adj_sx, sig_x, (size_t)n * h->d.nt * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->adj_sy, sig_y, (size_t)n * h->d.nt * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (build_inj(h, n, ai, aj, &h->adj_inj, &h->adj_inj_buf)) return -1;
+    h->nadj = n;
+    return 0;
+}
+
+int sofi_fwi_gradient(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho)
+{
+    if (!h) return fail("sofi_fwi_gradient: null handle");
+    if (h->d.mode != SOFI_MODE_FWI || !h->model_set) return fail("sofi_fwi_gradient: no FWI model set");
+    const Grid &g = h->g;
+    dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
+    hipLaunchKernelGGL(k_fwi_grad, tg, tb, 0, h->stream, g, h->w, h->d.idx > 0 ? h->d.idx : 1,
+                       h->d.idy > 0 ? h->d.idy : 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t pl = (size_t)g.plane;
+    float *out[3] = { grad_vp, grad_vs, grad_rho };
+    for (int k = 0; k < 3; ++k)
+        if (out[k]) HIPCHK(hipMemcpy(out[k], h->w.gshot + k * pl, pl * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho, int reset)
+{
+    if (!h) return fail("sofi_fwi_gradient_sum: null handle");
+    if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_gradient_sum: the handle was not opened with MODE=FWI");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t pl = (size_t)h->g.plane;
+    float *out[3] = { grad_vp, grad_vs, grad_rho };
+    for (int k = 0; k < 3; ++k)
+        if (out[k]) HIPCHK(hipMemcpy(out[k], h->w.gsum + k * pl, pl * sizeof(float), hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(hipMemset(h->w.gsum, 0, 3 * pl * sizeof(float)));
+    return 0;
+}
 
 int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape)
 {

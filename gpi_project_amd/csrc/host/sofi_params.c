@@ -386,6 +386,8 @@ int sofi_read_params(const char *json_file, sofi_params *p)
         OPT_INT("LNORM", p->lnorm, 2);
         OPT_INT("DTINV", p->dtinv, 1);
         OPT_STR("GRADIENT", p->gradient_file, "gradient");
+        OPT_INT("TIME_FILT", p->time_filt, 0);
+        OPT_INT("EPRECOND", p->eprecond, 0);
         if (p->dtinv <= 0) { sofi_fail("Variable DTINV must be a positive integer!"); goto fail; }
     }
     /* NS = iround(NT / NDT) with integer division (src/read_par_json.c:516) */
@@ -422,5 +424,18 @@ int sofi_check_supported(const sofi_params *p)
     if (p->snap && p->snap_format != 2 && p->snap_format != 3)
         return sofi_fail("SNAP_FORMAT=%d is not supported (the reference supports 2 and 3, src/writedsk.c).", p->snap_format);
     if (p->source_shape == 7) return sofi_fail("SOURCE_SHAPE=7 (STF inversion) is not supported.");
+    if (p->mode == SOFI_MODE_FWI) {
+        /* the gradient path of this build: elastic isotropic, L2 residuals of vx
+         * and/or vy, no filtering or energy preconditioning */
+        if (p->weq != SOFI_EL_ISO) return sofi_fail("MODE=FWI: only WEQ=EL_ISO is supported by this build.");
+        if (p->ndt != 1)
+            return sofi_fail("MODE=FWI needs NDT=1: the adjoint sources are read at every time step (src/update_v_interior.c:79-86).");
+        if (p->adjoint_type < 1 || p->adjoint_type > 3)
+            return sofi_fail("MODE=FWI: ADJOINT_TYPE=%d is not supported (1: vx and vy, 2: vy, 3: vx).", p->adjoint_type);
+        if (p->seismo != 1 && p->seismo != 4) return sofi_fail("MODE=FWI needs SEISMO=1 or 4 (vx and vy sections).");
+        if (p->lnorm != 2) return sofi_fail("MODE=FWI: LNORM=%d is not supported by this build (L2 only).", p->lnorm);
+        if (p->time_filt) return sofi_fail("MODE=FWI: TIME_FILT is not supported by this build.");
+        if (p->eprecond) return sofi_fail("MODE=FWI: EPRECOND is not supported by this build.");
+    }
     return 0;
 }

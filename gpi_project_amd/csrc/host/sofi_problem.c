@@ -100,6 +100,8 @@ int sofi_problem_snapcheck(const sofi_problem *P) { return P->s.snapcheck; }
 int sofi_problem_ntr(const sofi_problem *P) { return P->a.ntr; }
 const int *sofi_problem_rec_i(const sofi_problem *P) { return P->rec_i; }
 const int *sofi_problem_rec_j(const sofi_problem *P) { return P->rec_j; }
+int sofi_problem_itermax(const sofi_problem *P) { return P->p.itermax; }
+
 int sofi_problem_nsnap(const sofi_problem *P) { return P->p.snap ? sofi_count_snapshots(&P->p) : 0; }
 
 int sofi_problem_write_section(const sofi_problem *P, int comp, const float *section)

@@ -1,0 +1,86 @@
+"""MODE=FWI of sofi2D on the GPU: the gradient of the misfit for every shot
+(src/sofi2D.c:280-404 with src/inversion.c:27-258).
+
+Per iteration (ITERMAX), the gradient sums are cleared (init_inv,
+src/init_inv.c:20-25); per shot:
+  forward pass  time loop with the shot's sources, storing the forward
+                fields every DTINV steps; the vx/vy seismograms are written
+                as in MODE=FW (saveseis)
+  residuals     measured data DATA_DIR_<vx|vy>.su.shot<n> (inseis), L2
+                adjoint sources, time-reversed (calc_res)
+  adjoint pass  the residuals injected at the receivers (ADJOINT_TYPE),
+                zero-lag correlation with the stored forward fields
+  gradient      vp / vs / rho' of the shot, written as
+                GRADIENT_gradient.<p>.shot<n>; after the last shot the sums
+                GRADIENT_gradient.<p> (output_gradient)
+The model update (step length, L-BFGS) is not part of this path.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .sofi import Problem, Simulation
+
+VX, VY = 0, 1
+
+
+def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True) -> dict:
+    """Forward pass, residuals, adjoint pass and gradient of one shot."""
+    d = prob.desc
+    if ishot != prob.ishot:
+        prob.set_shot(ishot)
+    t0 = time.perf_counter()
+    sim.fwi_set_pass(0)
+    sim.load_shot(nsnap=prob.nsnap if prob.snapcheck else 0)
+    ms_fwd = sim.run()
+    secs = {c: sim.section(c) for c in prob.seismo_comps()}
+    if write:
+        for c, a in secs.items():
+            prob.write_section(c, a)
+    ri, rj = prob.receivers()
+    ntr = len(ri)
+    zero = np.zeros((ntr, d.ns), dtype=np.float32)
+    diff_x, diff_y, l2 = zero, zero, 0.0
+    if d.adjoint_type in (1, 3):
+        diff_x, e = prob.residual(prob.read_data(VX, ntr), secs[VX])
+        l2 += e
+    if d.adjoint_type in (1, 2):
+        diff_y, e = prob.residual(prob.read_data(VY, ntr), secs[VY])
+        l2 += e
+    # NS == NT (NDT = 1): the adjoint signal of step nt is sample nt
+    sim.fwi_set_adjoint(ri, rj, diff_x, diff_y)
+    sim.fwi_set_pass(1)
+    sim.reset()
+    ms_adj = sim.run()
+    gvp, gvs, grho = sim.fwi_gradient()
+    if write:
+        prob.write_gradient(ishot, gvp, gvs, grho)
+    return {"gradient": (gvp, gvs, grho), "sections": secs, "l2": l2, "ms_forward": ms_fwd, "ms_adjoint": ms_adj,
+            "s": time.perf_counter() - t0}
+
+
+def gradient_json(json_path: str, verbose: bool = True, write: bool = True) -> dict:
+    """Every iteration and shot of a MODE=FWI parameter file; returns the
+    per-shot results of the last iteration and the summed gradients."""
+    prob = Problem(json_path)
+    if prob.desc.mode != 0:
+        raise ValueError(f"{json_path}: MODE is not FWI")
+    sim = Simulation(prob)
+    sim.fwi_set_model(prob.model(0), prob.model(1), prob.model(2))
+    shots = {}
+    for it in range(1, prob.itermax + 1):
+        sim.fwi_gradient_sum(reset=True)
+        for ishot in range(1, prob.nshots + 1):
+            r = shot_gradient(prob, sim, ishot, write)
+            shots[ishot] = r
+            if verbose:
+                print(f"iteration {it} shot {ishot}: forward {r['ms_forward']:.1f} ms, adjoint {r['ms_adjoint']:.1f} ms, "
+                      f"L2 {r['l2']:.6e}")
+    gsum = sim.fwi_gradient_sum()
+    if write:
+        prob.write_gradient(0, *gsum)
+    sim.close()
+    prob.close()
+    return {"shots": shots, "sum": gsum}

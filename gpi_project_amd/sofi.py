@@ -137,6 +137,38 @@ class Problem:
         nys = (d.snap_idx[4] - d.snap_idx[3]) // max(d.idy, 1) + 1
         return nxs, nys
 
+    # ------------------------------------------------------------ FWI
+    @property
+    def itermax(self) -> int:
+        return self._L.sofi_problem_itermax(self._p)
+
+    def model(self, which: int) -> np.ndarray:
+        """The model as read, padded plane: 0 vp, 1 vs, 2 rho (MODE=FWI only)."""
+        ptr = self._L.sofi_problem_model(self._p, which)
+        if not ptr:
+            raise SofiError("model planes are kept only for MODE=FWI")
+        return np.ctypeslib.as_array(ptr, shape=(self.plane,)).copy()
+
+    def read_data(self, comp: int, ntr: int) -> np.ndarray:
+        """Measured data DATA_DIR_<comp>.su.shot<ishot> as [ntr][ns] (inseis)."""
+        out = np.zeros((ntr, self.desc.ns), dtype=np.float32)
+        _chk_host(self._L.sofi_read_data(self._L.sofi_problem_params(self._p), self.ishot, comp, ntr, _as_float_ptr(out)))
+        return out
+
+    def residual(self, data: np.ndarray, calc: np.ndarray):
+        """Time-reversed L2 adjoint sources (calc_res) and the sum of their squares."""
+        d = np.ascontiguousarray(data, dtype=np.float32)
+        c = np.ascontiguousarray(calc, dtype=np.float32)
+        out = np.zeros_like(c)
+        l2 = C.c_double()
+        _chk_host(self._L.sofi_residual(self._L.sofi_problem_params(self._p), c.shape[0], _as_float_ptr(d),
+                                        _as_float_ptr(c), _as_float_ptr(out), C.byref(l2)))
+        return out, l2.value
+
+    def write_gradient(self, ishot: int, gvp: np.ndarray, gvs: np.ndarray, grho: np.ndarray):
+        a = [np.ascontiguousarray(x, dtype=np.float32).ravel() for x in (gvp, gvs, grho)]
+        _chk_host(self._L.sofi_write_gradient(self._L.sofi_problem_params(self._p), ishot, *[_as_float_ptr(x) for x in a]))
+
     # ------------------------------------------------------------ outputs
     def write_section(self, comp: int, section: np.ndarray):
         s = np.ascontiguousarray(section, dtype=np.float32)
@@ -195,6 +227,38 @@ class Simulation:
         if nsnap is None:
             nsnap = p.nsnap if p.snapcheck else 0
         _chk_hip(self._L.sofi_set_snapshots(self._h, nsnap))
+
+    def reset(self):
+        """Zero the wavefield (zero_wavefield) without touching the sources."""
+        _chk_hip(self._L.sofi_reset(self._h))
+
+    # FWI (handle opened from a MODE=FWI problem)
+    def fwi_set_model(self, vp0: np.ndarray, vs0: np.ndarray, rho0: np.ndarray):
+        self._fwi_model = [np.ascontiguousarray(x, dtype=np.float32).ravel() for x in (vp0, vs0, rho0)]
+        _chk_hip(self._L.sofi_fwi_set_model(self._h, *[_as_float_ptr(x) for x in self._fwi_model]))
+
+    def fwi_set_pass(self, sw: int):
+        _chk_hip(self._L.sofi_fwi_set_pass(self._h, sw))
+
+    def fwi_set_adjoint(self, ri: np.ndarray, rj: np.ndarray, sig_x: np.ndarray, sig_y: np.ndarray):
+        ri = np.ascontiguousarray(ri, dtype=np.int32)
+        rj = np.ascontiguousarray(rj, dtype=np.int32)
+        sx = np.ascontiguousarray(sig_x, dtype=np.float32)
+        sy = np.ascontiguousarray(sig_y, dtype=np.float32)
+        assert sx.shape == sy.shape == (len(ri), self.desc.nt), (sx.shape, len(ri), self.desc.nt)
+        _chk_hip(self._L.sofi_fwi_set_adjoint(self._h, len(ri), _as_int_ptr(ri), _as_int_ptr(rj), _as_float_ptr(sx),
+                                              _as_float_ptr(sy)))
+
+    def fwi_gradient(self):
+        """The shot's (vp, vs, rho') gradient planes; adds them to the sums."""
+        out = [np.zeros(self.problem.plane, dtype=np.float32) for _ in range(3)]
+        _chk_hip(self._L.sofi_fwi_gradient(self._h, *[_as_float_ptr(x) for x in out]))
+        return out
+
+    def fwi_gradient_sum(self, reset: bool = False):
+        out = [np.zeros(self.problem.plane, dtype=np.float32) for _ in range(3)]
+        _chk_hip(self._L.sofi_fwi_gradient_sum(self._h, *[_as_float_ptr(x) for x in out], int(reset)))
+        return out
 
     def run(self, nt_first: int = 1, nt_last: int | None = None) -> float:
         nt_last = self.desc.nt if nt_last is None else nt_last

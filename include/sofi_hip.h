@@ -93,6 +93,29 @@ int sofi_close(sofi_dev *h);
 /* Device bytes the handle holds (HBM footprint of one resident shot). */
 size_t sofi_footprint(const sofi_dev *h);
 
+/* FWI gradient path (MODE=FWI, WEQ=EL_ISO): what time_loop does with
+ * sw = 0 / 1 and MemInv (src/time_loop.c:27-326 with src/snap_store.c:27-47,
+ * src/calc_conv.c:30-73), driven per shot as src/inversion.c:27-258 does.
+ *   sofi_fwi_set_model    the iteration's model Vp0, Vs0, Rho0 (padded planes)
+ *   sofi_fwi_set_pass     sw = 0: the next sofi_run is the forward pass and
+ *                         stores the forward fields every DTINV steps;
+ *                         sw = 1: the adjoint pass, correlating with them
+ *   sofi_fwi_set_adjoint  adjoint sources: the receivers and the time-reversed
+ *                         residuals [n][nt] in x and y (ADJOINT_TYPE picks)
+ *   sofi_fwi_gradient     the shot's vp / vs / rho' gradients (padded planes,
+ *                         NULL to skip), added to the sums; clears the shot's
+ *                         accumulators (src/inversion.c:226-249)
+ *   sofi_fwi_gradient_sum the sums over shots; reset != 0 clears them
+ *                         (init_inv, src/init_inv.c:20-25)
+ * At DTINV = 1 the store and correlation steps are the reference's; with
+ * DTINV > 1 the reference indexes its store past its end, and the forward
+ * step s is kept in slot (s - 1) / DTINV. */
+int sofi_fwi_set_model(sofi_dev *h, const float *vp0, const float *vs0, const float *rho0);
+int sofi_fwi_set_pass(sofi_dev *h, int sw);
+int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *i, const int *j, const float *sig_x, const float *sig_y);
+int sofi_fwi_gradient(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho);
+int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho, int reset);
+
 /* Shapes of the isotropic/VTI velocity and stress update kernels.  No counterpart in the
  * reference (a launch-tuning knob): SOFI_LAUNCH_CELL runs one thread per
  * cell in 64 x 4 tiles, SOFI_LAUNCH_VEC2 / SOFI_LAUNCH_VEC4 one thread per

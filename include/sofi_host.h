@@ -51,7 +51,7 @@ typedef struct sofi_params {
     int L;
     float f_ref, fl[SOFI_MAXL + 1];
     /* FWI (src/read_par_json_fwi.c) */
-    int itermax, dtinv, adjoint_type, velocity, lnorm;
+    int itermax, dtinv, adjoint_type, velocity, lnorm, time_filt, eprecond;
     char data_dir[SOFI_STR], gradient_file[SOFI_STR];
     /* derived */
     int nd;
@@ -136,6 +136,7 @@ int sofi_problem_ntr(const sofi_problem *P);
 const int *sofi_problem_rec_i(const sofi_problem *P);
 const int *sofi_problem_rec_j(const sofi_problem *P);
 int sofi_problem_nsnap(const sofi_problem *P);
+int sofi_problem_itermax(const sofi_problem *P);     /* FWI iterations (ITERMAX) */
 int sofi_problem_write_section(const sofi_problem *P, int comp, const float *section);
 int sofi_problem_write_signal(const sofi_problem *P);
 int sofi_problem_write_snapshot(const sofi_problem *P, int comp, int first, const float *data, int n);
@@ -143,6 +144,15 @@ int sofi_problem_seismo_comps(const sofi_problem *P, int *comps);
 int sofi_problem_snap_comps(const sofi_problem *P, int *comps);
 
 /* Low-level writer shared with tests: one SU trace set (outseis_glob, SEIS_FORMAT 1/2/3). */
+/* FWI gradient path (src/inversion.c:27-258): measured data of a shot
+ * (inseis, src/inseis.c:28-80) as section[ntr][ns]; the L2 adjoint sources
+ * (calc_res, src/calc_res.c:26-324), time-reversed, *l2 = sum of squares;
+ * the gradient files (output_gradient, src/output_gradient.c:25-68), per shot
+ * (ishot > 0) or summed (ishot = 0), from padded planes. */
+int sofi_read_data(const sofi_params *p, int ishot, int comp, int ntr, float *section);
+int sofi_residual(const sofi_params *p, int ntr, const float *data, const float *calc, float *diff, double *l2);
+int sofi_write_gradient(const sofi_params *p, int ishot, const float *gvp, const float *gvs, const float *grho);
+
 int sofi_outseis(const char *path, const float *section, const int *recpos, int ntr,
                  const float *srcpos, int nsrc, int ns, int seis_form, int ishot, float dh,
                  float dt, int ndt, const float *refrec);

@@ -4,6 +4,7 @@ baseline leg, never by the product path.
 
     run_problem(problem, precision="f32") -> {"sections": {comp: [ntr][ns]},
                                               "snapshots": {comp: [k][nxs][nys]}}
+    fwi_gradients(problem, data, precision="f32") -> per-shot and summed gradients
 """
 from __future__ import annotations
 
@@ -40,6 +41,11 @@ def lib(precision: str = "f32"):
     L.oracle_get_snapshot.argtypes = [C.c_void_p, C.c_int, C.c_int, _DP]
     L.oracle_free.argtypes = [C.c_void_p]
     L.oracle_real_size.restype = C.c_int
+    L.oracle_fwi_set_model.argtypes = [C.c_void_p, _FP, _FP, _FP]
+    L.oracle_fwi_set_adjoint.argtypes = [C.c_void_p, C.c_int, _IP, _IP, _FP, _FP]
+    L.oracle_reset.argtypes = [C.c_void_p, C.c_int]
+    L.oracle_fwi_gradient.argtypes = [C.c_void_p, _FP, _FP, _FP]
+    L.oracle_fwi_gradient_sum.argtypes = [C.c_void_p, _DP, _DP, _DP]
     _libs[precision] = L
     return L
 
@@ -85,6 +91,32 @@ class Oracle:
         self.L.oracle_get_field_f64(self.o, fid, out.ctypes.data_as(_DP))
         return out
 
+    # FWI (src/inversion.c:27-258): forward pass with store, adjoint pass
+    # with correlation, gradient of the shot
+    def reset(self, sw: int):
+        self.L.oracle_reset(self.o, sw)
+
+    def fwi_set_model(self, vp0, vs0, rho0):
+        self._model = [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in (vp0, vs0, rho0)]
+        self.L.oracle_fwi_set_model(self.o, *[a.ctypes.data_as(_FP) for a in self._model])
+
+    def fwi_set_adjoint(self, ri, rj, sig_x, sig_y):
+        self._adj = [np.ascontiguousarray(ri, dtype=np.int32), np.ascontiguousarray(rj, dtype=np.int32),
+                     np.ascontiguousarray(sig_x, dtype=np.float32), np.ascontiguousarray(sig_y, dtype=np.float32)]
+        a = self._adj
+        self.L.oracle_fwi_set_adjoint(self.o, len(a[0]), a[0].ctypes.data_as(_IP), a[1].ctypes.data_as(_IP),
+                                      a[2].ctypes.data_as(_FP), a[3].ctypes.data_as(_FP))
+
+    def fwi_gradient(self):
+        out = [np.zeros(self.problem.plane, dtype=np.float32) for _ in range(3)]
+        self.L.oracle_fwi_gradient(self.o, *[a.ctypes.data_as(_FP) for a in out])
+        return out
+
+    def fwi_gradient_sum(self):
+        out = [np.zeros(self.problem.plane, dtype=np.float64) for _ in range(3)]
+        self.L.oracle_fwi_gradient_sum(self.o, *[a.ctypes.data_as(_DP) for a in out])
+        return out
+
     def close(self):
         if self.o:
             self.L.oracle_free(self.o)
@@ -104,3 +136,53 @@ def run_problem(problem, precision: str = "f32", nt: int | None = None):
            "snapshots": {c: o.snapshots(c) for c in problem.snap_comps()}}
     o.close()
     return res
+
+
+def l2_residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int) -> np.ndarray:
+    """calc_res for LNORM 2 (src/calc_res.c:127-143, 220-238): integrate in
+    time unless VELOCITY = 1 (float32 running sums times DT), subtract,
+    reverse in time."""
+    d = np.asarray(data, dtype=np.float32)
+    c = np.asarray(calc, dtype=np.float32)
+    if velocity == 0:
+        dt32 = np.float32(dt)
+        ic = np.empty_like(c)
+        idd = np.empty_like(d)
+        sc = np.zeros(c.shape[0], dtype=np.float32)
+        sd = np.zeros(d.shape[0], dtype=np.float32)
+        for j in range(c.shape[1]):
+            sc = (sc + c[:, j]).astype(np.float32)
+            sd = (sd + d[:, j]).astype(np.float32)
+            ic[:, j] = sc * dt32
+            idd[:, j] = sd * dt32
+        c, d = ic, idd
+    return (c - d)[:, ::-1].astype(np.float32).copy()
+
+
+def fwi_gradients(problem, data: dict, precision: str = "f32"):
+    """The FWI gradient of every shot of a MODE=FWI problem on the CPU oracle;
+    data[ishot] = (vx, vy) measured sections [ntr][ns].  The sums add the
+    shots' float32 gradients in shot order, as the reference does."""
+    d = problem.desc
+    shots = {}
+    for ishot in range(1, problem.nshots + 1):
+        problem.set_shot(ishot)
+        o = Oracle(problem, precision)
+        o.fwi_set_model(problem.model(0), problem.model(1), problem.model(2))
+        o.reset(0)
+        o.run()
+        calc_x, calc_y = o.section(0).astype(np.float32), o.section(1).astype(np.float32)
+        ri, rj = problem.receivers()
+        zero = np.zeros((len(ri), d.nt), dtype=np.float32)
+        dx = l2_residual(data[ishot][0], calc_x, d.dt, d.velocity) if d.adjoint_type in (1, 3) else zero
+        dy = l2_residual(data[ishot][1], calc_y, d.dt, d.velocity) if d.adjoint_type in (1, 2) else zero
+        o.fwi_set_adjoint(ri, rj, dx, dy)
+        o.reset(1)
+        o.run()
+        shots[ishot] = {"gradient": o.fwi_gradient(), "sections": (calc_x, calc_y)}
+        o.close()
+    gsum = [np.zeros(problem.plane, dtype=np.float32) for _ in range(3)]
+    for k in sorted(shots):
+        for q in range(3):
+            gsum[q] = (gsum[q] + shots[k]["gradient"][q]).astype(np.float32)
+    return {"shots": shots, "sum": gsum}

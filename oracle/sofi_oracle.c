@@ -76,9 +76,12 @@ typedef struct oracle {
     int sw;                            /* 0 forward, 1 adjoint */
     REAL *ux, *uy, *uxy, *pvxp1, *pvyp1;
     int nstore, hin, hin1;
-    REAL *store[5];                    /* forward store [nstore][plane] for pvxp1 pvyp1 sxx syy sxy (or ux uy uxy) */
-    REAL *grad_lam, *grad_mu, *grad_rho;
-    REAL *adj_x, *adj_y;               /* adjoint signals [nsrc][nt] */
+    REAL *store;                       /* forward store [nstore][5][ny][nx]: pvxp1 pvyp1 sxx syy sxy (or ux uy uxy) */
+    REAL *grad_lam, *grad_mu, *grad_rho;   /* gradLam_shot, gradMu_shot, gradRhos_shot */
+    REAL *gsum[3];                     /* gradVp, gradVs, gradRho summed over shots */
+    REAL *vp0, *vs0, *rho0, *muss, *lamss;
+    int nadj, *ai, *aj;                /* adjoint sources (the receivers) */
+    REAL *adj_x, *adj_y;               /* adjoint signals [nadj][nt] */
 } oracle;
 
 #define IX(i, j) ((size_t)((j) - 1 + SOFI_PAD) * (size_t)o->pitch + (size_t)((i) - 1 + SOFI_PAD))
@@ -136,6 +139,17 @@ oracle *oracle_new(const sofi_desc *d, const float *const *arrays, const float *
     o->nys = (d->snap_idx[4] - d->snap_idx[3]) / (d->idy > 0 ? d->idy : 1) + 1;
     o->hin = 1;
     o->hin1 = 1;
+    if (d->mode == SOFI_MODE_FWI) {
+        o->ux = zalloc(o->plane); o->uy = zalloc(o->plane); o->uxy = zalloc(o->plane);
+        o->pvxp1 = zalloc(o->plane); o->pvyp1 = zalloc(o->plane);
+        o->grad_lam = zalloc(o->plane); o->grad_mu = zalloc(o->plane); o->grad_rho = zalloc(o->plane);
+        for (int k = 0; k < 3; ++k) o->gsum[k] = zalloc(o->plane);
+        o->vp0 = zalloc(o->plane); o->vs0 = zalloc(o->plane); o->rho0 = zalloc(o->plane);
+        o->muss = zalloc(o->plane); o->lamss = zalloc(o->plane);
+        int dtinv = d->dtinv > 0 ? d->dtinv : 1;
+        o->nstore = (d->nt + dtinv - 1) / dtinv;
+        o->store = zalloc((size_t)o->nstore * 5 * o->nx * o->ny);
+    }
     return o;
 }
 
@@ -275,8 +289,8 @@ static void update_v(oracle *o, int nt)
             }
         }
     } else {
-        for (int l = 0; l < o->nsrc; l++) {
-            int i = o->si[l], j = o->sj[l];
+        for (int l = 0; l < o->nadj; l++) {
+            int i = o->ai[l], j = o->aj[l];
             REAL fx = o->adj_x[(size_t)l * d->nt + nt - 1], fy = o->adj_y[(size_t)l * d->nt + nt - 1];
             switch (d->adjoint_type) {
               case 1: F(o->vx, i, j) += fx; F(o->vy, i, j) += fy; break;
@@ -720,6 +734,145 @@ static void snap(oracle *o)
     o->nsnap++;
 }
 
+/* ------------------------------------------------------------- FWI */
+
+#define ST(slot, k, i, j) (o->store[(((size_t)(slot) * 5 + (k)) * o->ny + ((j) - 1)) * o->nx + ((i) - 1)])
+
+/* snap_store (src/snap_store.c:27-47): the forward fields of one step.  The
+ * reference writes slot hin = nt (src/time_loop.c:288-292); with DTINV > 1
+ * that index runs past its NT/DTINV slots, so the slot used here is the
+ * store step's rank (nt - 1) / DTINV, the same as the reference's at
+ * DTINV = 1. */
+static void fwi_store(oracle *o, int slot)
+{
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            ST(slot, 0, i, j) = F(o->pvxp1, i, j);
+            ST(slot, 1, i, j) = F(o->pvyp1, i, j);
+            if (o->d.velocity == 0) {
+                ST(slot, 2, i, j) = F(o->sxx, i, j);
+                ST(slot, 3, i, j) = F(o->syy, i, j);
+                ST(slot, 4, i, j) = F(o->sxy, i, j);
+            } else {
+                ST(slot, 2, i, j) = F(o->ux, i, j);
+                ST(slot, 3, i, j) = F(o->uy, i, j);
+                ST(slot, 4, i, j) = F(o->uxy, i, j);
+            }
+        }
+}
+
+/* calc_conv (src/calc_conv.c:30-73): zero-lag correlation of the adjoint
+ * fields with the forward fields of slot `slot` */
+static void fwi_conv(oracle *o, int slot)
+{
+    const REAL DT = o->d.dt;
+    for (int j = 1; j <= o->ny; j++)
+        for (int i = 1; i <= o->nx; i++) {
+            REAL muss = F(o->muss, i, j), lamss = F(o->lamss, i, j);
+            REAL frx = ST(slot, 0, i, j), fry = ST(slot, 1, i, j);
+            REAL fx = ST(slot, 2, i, j), fy = ST(slot, 3, i, j), fu = ST(slot, 4, i, j);
+            REAL sxx = F(o->sxx, i, j), syy = F(o->syy, i, j), sxy = F(o->sxy, i, j);
+            F(o->grad_rho, i, j) += -DT * ((F(o->pvxp1, i, j) * frx) + (F(o->pvyp1, i, j) * fry));
+            F(o->grad_lam, i, j) += -DT * (1.0 / (4.0 * (lamss + muss) * (lamss + muss))) * ((fx + fy) * (sxx + syy));
+            if (M(SOFI_A_PU, i, j) > 0.0)
+                F(o->grad_mu, i, j) += -DT * (((1.0 / (muss * muss)) * (fu * sxy)) +
+                                              ((1.0 / 4.0) * ((fx + fy) * (sxx + syy)) / ((lamss + muss) * (lamss + muss))) +
+                                              ((1.0 / 4.0) * ((fx - fy) * (sxx - syy)) / (muss * muss)));
+        }
+}
+
+/* the model of the iteration (minv->Vp0/Vs0/Rho0, src/readmod_elastic.c:126-128)
+ * and the moduli calc_conv forms from it (src/calc_conv.c:45-46) */
+void oracle_fwi_set_model(oracle *o, const float *vp0, const float *vs0, const float *rho0)
+{
+    for (size_t q = 0; q < o->plane; ++q) {
+        o->vp0[q] = vp0[q];
+        o->vs0[q] = vs0[q];
+        o->rho0[q] = rho0[q];
+        o->muss[q] = o->vs0[q] * o->vs0[q] * o->rho0[q];
+        o->lamss[q] = o->vp0[q] * o->vp0[q] * o->rho0[q] - 2.0 * o->muss[q];
+    }
+}
+
+/* adjoint sources at the receivers with signals [n][nt] (src/inversion.c:205-223) */
+void oracle_fwi_set_adjoint(oracle *o, int n, const int *ai, const int *aj, const float *sx, const float *sy)
+{
+    free(o->ai); free(o->aj); free(o->adj_x); free(o->adj_y);
+    o->nadj = n;
+    o->ai = (int *)malloc(sizeof(int) * (n + 1));
+    o->aj = (int *)malloc(sizeof(int) * (n + 1));
+    o->adj_x = zalloc((size_t)(n > 0 ? n : 1) * o->d.nt);
+    o->adj_y = zalloc((size_t)(n > 0 ? n : 1) * o->d.nt);
+    for (int k = 0; k < n; ++k) { o->ai[k] = ai[k]; o->aj[k] = aj[k]; }
+    for (size_t q = 0; q < (size_t)n * o->d.nt; ++q) { o->adj_x[q] = sx[q]; o->adj_y[q] = sy[q]; }
+}
+
+/* zero_wavefield (src/zero_wavefield.c:28-130), including the FWI fields, and
+ * the per-call counters of time_loop; sw selects the pass that follows */
+void oracle_reset(oracle *o, int sw)
+{
+    REAL *planes[] = { o->vx, o->vy, o->sxx, o->syy, o->sxy, o->ux, o->uy, o->uxy, o->pvxp1, o->pvyp1 };
+    for (size_t k = 0; k < sizeof(planes) / sizeof(planes[0]); ++k)
+        if (planes[k]) memset(planes[k], 0, o->plane * sizeof(REAL));
+    int L = o->L > 0 ? o->L : 1;
+    memset(o->r, 0, o->plane * L * sizeof(REAL));
+    memset(o->p, 0, o->plane * L * sizeof(REAL));
+    memset(o->q, 0, o->plane * L * sizeof(REAL));
+    size_t nxp = (size_t)(o->ny + 1) * o->wx, nyp = (size_t)(2 * o->d.fw + 1) * o->wy;
+    REAL *px[] = { o->psi_sxx_x, o->psi_sxy_x, o->psi_vxx, o->psi_vyx };
+    REAL *py[] = { o->psi_syy_y, o->psi_sxy_y, o->psi_vyy, o->psi_vxy };
+    for (int k = 0; k < 4; ++k) {
+        memset(px[k], 0, nxp * sizeof(REAL));
+        memset(py[k], 0, nyp * sizeof(REAL));
+    }
+    o->sw = sw;
+    o->lsamp = o->d.ndt;
+    o->lsnap = o->d.lsnap;
+    o->hin = 1;
+    o->hin1 = 1;
+}
+
+/* gradient of the shot (src/inversion.c:226-249): vp/vs/rho' from the
+ * lambda/mu/rho accumulators, added to the sums; the accumulators are then
+ * cleared (src/sofi2D.c:382-388).  Outputs the shot's gradients. */
+int oracle_fwi_gradient(oracle *o, float *gvp, float *gvs, float *grho)
+{
+    size_t n = o->plane;
+    memset(gvp, 0, n * sizeof(float));
+    memset(gvs, 0, n * sizeof(float));
+    memset(grho, 0, n * sizeof(float));
+    for (int j = 1; j <= o->ny; j += o->d.idy > 0 ? o->d.idy : 1)
+        for (int i = 1; i <= o->nx; i += o->d.idx > 0 ? o->d.idx : 1) {
+            size_t q = IX(i, j);
+            REAL Vp0 = o->vp0[q], Vs0 = o->vs0[q], Rho0 = o->rho0[q];
+            REAL gl = o->grad_lam[q], gm = o->grad_mu[q], gr = o->grad_rho[q];
+            REAL vp = 2.0 * Rho0 * Vp0 * gl;
+            REAL vs = (-4.0 * Vs0 * Rho0 * gl) + 2.0 * Rho0 * Vp0 * gm;
+            REAL rho = ((((Vp0 * Vp0) - (2.0 * Vs0 * Vs0)) * gl) + (Vs0 * Vs0 * gm) + gr);
+            gvp[q] = (float)vp;
+            gvs[q] = (float)vs;
+            grho[q] = (float)rho;
+            o->gsum[0][q] += vp;
+            o->gsum[1][q] += vs;
+            o->gsum[2][q] += rho;
+        }
+    memset(o->grad_lam, 0, n * sizeof(REAL));
+    memset(o->grad_mu, 0, n * sizeof(REAL));
+    memset(o->grad_rho, 0, n * sizeof(REAL));
+    return 0;
+}
+
+/* gradients summed over the shots so far (minv->gradVp/gradVs/gradRho) */
+int oracle_fwi_gradient_sum(const oracle *o, double *gvp, double *gvs, double *grho)
+{
+    for (size_t q = 0; q < o->plane; ++q) {
+        gvp[q] = o->gsum[0][q];
+        gvs[q] = o->gsum[1][q];
+        grho[q] = o->gsum[2][q];
+    }
+    return 0;
+}
+
 /* --------------------------------------------------------- one step */
 
 void oracle_step(oracle *o, int nt)
@@ -729,13 +882,28 @@ void oracle_step(oracle *o, int nt)
     v_derivatives(o);
     update_s(o);
     int stype = d->source_type;
-    if ((o->sw == 0 && stype == 1) || (o->sw == 1 && d->adjoint_type == 4))
-        psource(o, nt, o->sw == 0 ? o->sig : o->adj_x);
+    /* (ADJOINT_TYPE 4, a pressure adjoint source through psource, is refused
+     * by the host library: the reference injects sectionvxdiff there while
+     * calc_res fills sectionpdiff) */
+    if (o->sw == 0 && stype == 1) psource(o, nt, o->sig);
     if (o->sw == 0 && stype == 5) eqsource(o, nt);
     if (d->free_surf) free_surface(o);
     if (d->seismo && nt == o->lsamp && nt < d->nt) {
         seismo(o, o->lsamp);
         o->lsamp += d->ndt;
+    }
+    /* FWI: forward store every DTINV steps, correlation in the adjoint pass at
+     * the steps whose time-reversed forward step was stored
+     * (src/time_loop.c:287-303) */
+    if (d->mode == SOFI_MODE_FWI) {
+        int dtinv = d->dtinv > 0 ? d->dtinv : 1;
+        if (o->sw == 0 && nt == o->hin1) {
+            fwi_store(o, (nt - 1) / dtinv);
+            o->hin1 += dtinv;
+        }
+        int f = d->nt - nt + 1;
+        if (o->sw == 1 && (f - 1) % dtinv == 0) fwi_conv(o, (f - 1) / dtinv);
+        o->hin++;
     }
     if (d->snap && nt == o->lsnap && nt <= d->esnap) {
         snap(o);
@@ -815,13 +983,13 @@ void oracle_free(oracle *o)
     REAL *bufs[] = { o->vx, o->vy, o->sxx, o->syy, o->sxy, o->vxx, o->vyy, o->vyx, o->vxy, o->r, o->p, o->q,
                      o->psi_sxx_x, o->psi_sxy_x, o->psi_syy_y, o->psi_sxy_y, o->psi_vxx, o->psi_vyx, o->psi_vyy,
                      o->psi_vxy, o->sazi, o->samp, o->sig, o->snapbuf, o->ux, o->uy, o->uxy, o->pvxp1, o->pvyp1,
-                     o->grad_lam, o->grad_mu, o->grad_rho, o->adj_x, o->adj_y };
+                     o->grad_lam, o->grad_mu, o->grad_rho, o->adj_x, o->adj_y, o->store, o->gsum[0], o->gsum[1],
+                     o->gsum[2], o->vp0, o->vs0, o->rho0, o->muss, o->lamss };
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); ++k) free(bufs[k]);
     for (int k = 0; k < SOFI_A_NARRAYS; ++k) free(o->a[k]);
     for (int k = 0; k < SOFI_P_NPROFILES; ++k) free(o->prof[k]);
     for (int c = 0; c < SOFI_NCOMP; ++c) free(o->sec[c]);
-    for (int k = 0; k < 5; ++k) free(o->store[k]);
-    free(o->si); free(o->sj); free(o->stype); free(o->ri); free(o->rj);
+    free(o->si); free(o->sj); free(o->stype); free(o->ri); free(o->rj); free(o->ai); free(o->aj);
     free(o);
 }
 

@@ -71,3 +71,26 @@ def compare_component(a, g, others, tol=TOL, cancel_scale=None):
         assert float(np.abs(np.asarray(a, np.float64) - g).max()) < NOISE_FLOOR * peak
         return None
     return rel_l2(a, g)
+
+
+# ------------------------------------------------------------------- FWI
+
+def fwi_fixture(name):
+    return np.load(os.path.join(GOLDEN, "ref1", name + ".npz"))
+
+
+def fwi_measured(z):
+    """{ishot: (vx, vy)} measured sections of an FWI fixture"""
+    return {s: (z[f"data_vx_shot{s}"], z[f"data_vy_shot{s}"]) for s in (1, 2)}
+
+
+def xmajor(plane, nx, ny):
+    """padded plane -> [nx][ny] float64, the layout of the gradient files"""
+    from gpi_project_amd.sofi import interior
+    return interior(np.asarray(plane), nx, ny).T.astype(np.float64)
+
+
+def cosine(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))

@@ -181,3 +181,81 @@ def multisource_case(td: str, nsrc: int) -> str:
                fpml=20.0, vppml=3500.0, source_type=1, sources=srcs, receivers=_rec_grid(NX, NY, DH),
                seismo=1, ndt=2)
     return W.write_case(td, c, _model("VEL_ISO", 51, L=L))
+
+
+# ------------------------------------------------------------------- FWI
+
+# name: (VELOCITY, ADJOINT_TYPE)
+FWI_CASES = {
+    "fwi_el_v0": (0, 1),
+    "fwi_el_v1_y": (1, 2),
+}
+FWI_NX, FWI_NY, FWI_DH, FWI_DT, FWI_NT = 160, 100, 5.0, 4.0e-4, 700
+
+
+def _fwi_models(nx=FWI_NX, ny=FWI_NY, dh=FWI_DH):
+    """Background: vp 2000 -> 3000 m/s linear in depth, vs = vp / 1.73,
+    rho = 310 vp^0.25; the true model adds a Gaussian anomaly (+10 % vp and
+    vs, sigma 30 m) centred at (80, 25) cells, shallow enough for its
+    reflections to reach the receivers within the record."""
+    depth = np.arange(ny) * dh
+    vp = np.repeat((2000.0 + 1000.0 * depth / (depth[-1] or 1.0))[None, :], nx, axis=0)
+    x = (np.arange(nx) - 80)[:, None] * dh
+    z = (np.arange(ny) - 25)[None, :] * dh
+    bump = 1.0 + 0.10 * np.exp(-(x * x + z * z) / (2.0 * 30.0 ** 2))
+    out = {}
+    for tag, f in (("init", 1.0), ("true", bump)):
+        v = (vp * f).astype(np.float32)
+        out[tag] = {"vp": v, "vs": (v / 1.73).astype(np.float32), "rho": (310.0 * (vp ** 0.25)).astype(np.float32)}
+    return out
+
+
+def fwi_case(td: str, name: str, which: str, dtinv: int = 1) -> str:
+    """EL_ISO, FD4 (Holberg 0.1 %), CPML 12, two explosive 20 Hz Ricker shots
+    at (40, 5) and (120, 5) cells, 31 receivers on row 4 every 4 cells.
+    which = "true": MODE=FW on the true model writing the measured data
+    ./data/obs_{vx,vy}.su.shot<n>; "init": MODE=FWI on the background model
+    reading them, gradients to ./grad/g_gradient.*."""
+    velocity, adjoint_type = FWI_CASES[name]
+    dh = FWI_DH
+    srcs = [((40 - 1) * dh + 1.0, (5 - 1) * dh + 1.0, 0.0, 20.0, 1.0), ((120 - 1) * dh + 1.0, (5 - 1) * dh + 1.0, 0.0, 20.0, 1.0)]
+    recs = [((i - 1) * dh + 1.0, (4 - 1) * dh + 1.0) for i in range(20, 141, 4)]
+    fwi = which == "init"
+    extra = {}
+    if fwi:
+        extra = {"ITERMAX": 1, "DATA_DIR": "./data/obs", "PARAMETERIZATION": 1, "ADJOINT_TYPE": adjoint_type,
+                 "VELOCITY": velocity, "INV_VP_ITER": 1, "INV_VS_ITER": 1, "INV_RHO_ITER": 1, "TIME_FILT": 0,
+                 "F_HIGH_PASS": 1, "F_LOW_PASS_START": "10.0", "F_LOW_PASS_END": "75.0", "F_LOW_PASS_INCR": "10.0",
+                 "ORDER": 8, "ZERO_PHASE": 0, "FREQ_FILE": "frequencies.dat", "WRITE_FILTERED_DATA": 0,
+                 "MIN_ITER": 10, "USE_WORKFLOW": 0, "FILE_WORKFLOW": "workflow.txt", "SWS_TAPER_GRAD_SOURCES": 0,
+                 "SRTRADIUS": 0, "SRTSHAPE": 0, "FILTSIZE": 0, "INV_MODELFILE": "./grad/model", "NFSTART": 1,
+                 "NF": 1, "GRADIENT": "./grad/g", "NFSTART_GRAD": 1, "NF_GRAD": 1, "GRAD_METHOD": 2, "LNORM": 2,
+                 "NORMALIZE": 0, "DTINV": dtinv, "WRITE_DIFF": 0, "EPS_SCALE": "0.01", "STEPMAX": 4,
+                 "SCALEFAC": "4.0", "TESTSHOT_START": 1, "TESTSHOT_END": 2, "TESTSHOT_INCR": 1, "PRO": "0.01",
+                 "MISFIT_LOG_FILE": "./grad/L2_LOG.dat"}
+    c = W.Case(name=f"{name}_{which}", nx=FWI_NX, ny=FWI_NY, dh=dh, dt=FWI_DT, time=FWI_NT * FWI_DT, weq="EL_ISO",
+               fdorder=4, maxrelerror=1, abs_type=1, fw=12, free_surf=0, fpml=20.0, vppml=3000.0, source_type=1,
+               sources=srcs, receivers=recs, seismo=1, ndt=1, run_multiple_shots=1,
+               mode="FWI" if fwi else "FW", extra=extra)
+    js = W.write_case(td, c, _fwi_models()[which])
+    os.makedirs(os.path.join(td, "data"), exist_ok=True)
+    os.makedirs(os.path.join(td, "grad"), exist_ok=True)
+    if not fwi:   # the measured data are this run's seismograms
+        txt = open(js).read().replace('"SEIS_FILE" : "./out/seis"', '"SEIS_FILE" : "./data/obs"')
+        open(js, "w").write(txt)
+    return js
+
+
+def write_measured_data(td: str, data: dict):
+    """data[ishot] = (vx, vy) [ntr][ns] -> ./data/obs_{vx,vy}.su.shot<n>
+    (240-byte headers with ns set, then the samples)."""
+    os.makedirs(os.path.join(td, "data"), exist_ok=True)
+    for ishot, (vx, vy) in data.items():
+        for comp, a in (("vx", vx), ("vy", vy)):
+            a = np.asarray(a, dtype=np.float32)
+            with open(os.path.join(td, "data", f"obs_{comp}.su.shot{ishot}"), "wb") as f:
+                for tr in a:
+                    h = np.zeros(240, dtype=np.uint8)
+                    h[114:116] = np.frombuffer(np.uint16(a.shape[1]).tobytes(), np.uint8)
+                    f.write(h.tobytes())
+                    f.write(tr.tobytes())

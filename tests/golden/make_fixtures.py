@@ -14,7 +14,10 @@ has the reference at /root/reference and its build in oracle/_ref).
      orders 2-12, Holberg 1-4, L = 1..5, force/moment sources, SEISMO 4),
      defined in tests/golden/cases.py and regenerated from their seeds.
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1]
+3. ref1/fwi_*.npz: measured data and FWI gradients of the reference build
+   at DTINV = 1 (tests/golden/cases.py FWI_CASES).
+
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi]
 """
 from __future__ import annotations
 
@@ -168,7 +171,30 @@ def c1():
         print("ref1 c1 %.2fs" % t[0])
 
 
+def fwi():
+    """FWI gradients of the reference build (DTINV = 1, one process): the
+    measured data from a MODE=FW run on the true model, then MODE=FWI on the
+    background model; stores the data and every gradient file."""
+    out = os.path.join(HERE, "ref1")
+    os.makedirs(out, exist_ok=True)
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    for name in cases.FWI_CASES:
+        with tempfile.TemporaryDirectory() as td:
+            run_ref(cases.fwi_case(td, name, "true"), td)
+            r = {}
+            for s in (1, 2):
+                for comp in ("vx", "vy"):
+                    r[f"data_{comp}_shot{s}"] = read_su(os.path.join(td, "data", f"obs_{comp}.su.shot{s}"))
+            t = run_ref(cases.fwi_case(td, name, "init"), td)
+            for p in ("vp", "vs", "rho"):
+                for s in (1, 2):
+                    r[f"grad_{p}_shot{s}"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}.shot{s}"), np.float32).reshape(nx, ny)
+                r[f"grad_{p}_sum"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(nx, ny)
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, "time loops %s s" % t, flush=True)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["examples", "ref1", "c1"]
+    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi"]
     for w in what:
-        {"examples": examples, "ref1": ref1, "c1": c1}[w]()
+        {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi}[w]()
