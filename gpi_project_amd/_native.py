@@ -139,6 +139,10 @@ def hip():
     L.sofi_set_kernel_timing.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_get_kernel_timing.argtypes = [_VP, _DP, _DP, _IP]
     L.sofi_set_launch.argtypes = [_VP, C.c_int, C.c_int]
+    L.sofi_open_batch.argtypes = [C.POINTER(SofiDesc), C.c_int, C.POINTER(_VP)]
+    L.sofi_select_shot.argtypes = [_VP, C.c_int]
+    L.sofi_set_shots_per_thread.argtypes = [_VP, C.c_int]
+    L.sofi_get_shots_per_thread.argtypes = [_VP]
     L.sofi_fwi_set_model.argtypes = [_VP, _FP, _FP, _FP]
     L.sofi_fwi_set_pass.argtypes = [_VP, C.c_int]
     L.sofi_fwi_set_adjoint.argtypes = [_VP, C.c_int, _IP, _IP, _FP, _FP]
@@ -165,5 +169,6 @@ def exported_symbols():
                            "sofi_get_snapshot", "sofi_get_field", "sofi_close", "sofi_footprint",
                            "sofi_set_kernel_timing", "sofi_get_kernel_timing", "sofi_set_launch",
                            "sofi_fwi_set_model", "sofi_fwi_set_pass", "sofi_fwi_set_adjoint",
-                           "sofi_fwi_gradient", "sofi_fwi_gradient_sum"],
+                           "sofi_fwi_gradient", "sofi_fwi_gradient_sum", "sofi_open_batch", "sofi_select_shot",
+                           "sofi_set_shots_per_thread", "sofi_get_shots_per_thread"],
     }

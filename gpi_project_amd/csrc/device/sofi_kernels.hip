@@ -57,17 +57,59 @@ struct Grid {
     int plane;      // floats per padded plane
     int wx, wy;     // row lengths of x-frame and y-frame CPML memory
     int ntx;        // tiles along x of the one-cell-per-thread update kernels
+    int nb, spt;    // shot slots of a batch; slots per thread of the update kernels
+    int nxp, nyp;   // floats per slot of the x-frame and y-frame CPML memory
     float dt, dh, dtdh, dhi, dthalbe;
     float hc[7];
 };
 
+// Wavefield of a handle.  A batch of nb shots sharing one medium keeps its
+// slots one after the other in every array (slot k of a plane array at
+// k * plane, of the memory variables at k * L * plane, of the CPML memory at
+// k * nxp / k * nyp); at(k) gives slot k's view.
 struct Fields {
     float *vx, *vy, *sxx, *syy, *sxy;
     float *r, *p, *q;                          // L planes each
     float *dvxx, *dvyy, *dvyx, *dvxy;           // TTI only
     float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
     float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
-    int *unstable;                             // first step whose start finds vy(NX/2, NY/2) = NaN
+    int *unstable;                             // per slot: first step whose start finds vy(NX/2, NY/2) = NaN
+
+    __device__ __forceinline__ Fields at(int k, const Grid &g) const
+    {
+        Fields s = *this;
+        if (k == 0) return s;
+        const size_t pl = (size_t)k * g.plane, ml = pl * (size_t)(g.L > 0 ? g.L : 1);
+        const size_t xp = (size_t)k * g.nxp, yp = (size_t)k * g.nyp;
+        s.vx += pl;
+        s.vy += pl;
+        s.sxx += pl;
+        s.syy += pl;
+        s.sxy += pl;
+        if (r) {
+            s.r += ml;
+            s.p += ml;
+            s.q += ml;
+        }
+        if (dvxx) {
+            s.dvxx += pl;
+            s.dvyy += pl;
+            s.dvyx += pl;
+            s.dvxy += pl;
+        }
+        if (psi_sxx_x) {
+            s.psi_sxx_x += xp;
+            s.psi_sxy_x += xp;
+            s.psi_vxx += xp;
+            s.psi_vyx += xp;
+            s.psi_syy_y += yp;
+            s.psi_sxy_y += yp;
+            s.psi_vyy += yp;
+            s.psi_vxy += yp;
+        }
+        s.unstable += k;
+        return s;
+    }
 };
 
 struct Mat {
@@ -188,8 +230,10 @@ __device__ __forceinline__ void cpml_dv_read(const Grid &g, const Mat &m, const 
 
 // ------------------------------------------------------------- sources
 
+// Sources of every slot of the handle, one list (slot[l] = the shot slot of
+// source l); signals [n][nt].
 struct Src {
-    const int *i, *j, *type;
+    const int *i, *j, *type, *slot;
     const float *azi, *amp;
     const float *sig;   // [nsrc][nt]
     const double *sc;   // [nsrc][2]: sin and cos of the azimuth (degrees -> float radians), host libm
@@ -247,14 +291,15 @@ __device__ __forceinline__ float psource_amp(const Grid &g, const Src &s, int l,
     return (float)(-sg[nt - 1] / h2);
 }
 
-// force sources when they are not fused: one thread per source cell
+// force sources when they are not fused: one thread per source cell (cell
+// offsets count the slots of a batch; the medium is shared by the slots)
 __global__ void k_force(Grid g, Fields f, Mat m, Src s, Inj inj, int nt)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= inj.n) return;
-    const int c = inj.cell[k];
+    const int c = inj.cell[k], cm = c % g.plane;
     float vx = f.vx[c], vy = f.vy[c];
-    const float rip = m.rip[c], rjp = m.rjp[c];
+    const float rip = m.rip[cm], rjp = m.rjp[cm];
     for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) force_add(g, s, inj.idx[q], nt, rip, rjp, vx, vy);
     f.vx[c] = vx;
     f.vy[c] = vy;
@@ -293,17 +338,19 @@ __global__ void k_adjoint(Grid g, Fields f, Inj inj, const float *sx, const floa
     f.vy[c] = vy;
 }
 
-// eqsource (src/eqsource.c:25-52): double-couple moment tensor
-__global__ void k_eqsource(Grid g, Fields f, Src s, int nt)
+// eqsource (src/eqsource.c:25-52): double-couple moment tensor; the
+// sources listed (their updates touch neighbouring cells), one thread in order
+__global__ void k_eqsource(Grid g, Fields f, Src s, const int *list, int n, int nt)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int l = 0; l < s.n; ++l) {
+    for (int q = 0; q < n; ++q) {
+        const int l = list[q];
         const int i = s.i[l], j = s.j[l];
         const float dip = (float)(s.azi[l] * M_PI / 180.0);
         const float scale = g.dt / (g.dh * g.dh);
         const float amp = s.amp[l] * scale * s.sig[l * s.nt + nt - 1];
         const float m11 = (float)(-sin(2.0 * dip)), m12 = (float)(-cos(2.0 * dip)), m22 = (float)sin(2.0 * dip);
-        const int c = IDX(g, i, j);
+        const int c = s.slot[l] * g.plane + IDX(g, i, j);
         f.sxx[c] += amp * m11;
         f.sxy[c] = (float)((double)f.sxy[c] + 0.3 * amp * m12);
         f.sxy[c - g.pitch] = (float)((double)f.sxy[c - g.pitch] + 0.3 * amp * m12);
@@ -378,8 +425,9 @@ __device__ __forceinline__ float surface_sxx(const Grid &g, Fields &f, const Mat
 
 // the free surface as a pass of its own, after the stress update and sources
 template <int ND, bool VISCO>
-__global__ void k_surface(Grid g, Fields f, Mat m)
+__global__ void k_surface(Grid g, Fields f0, Mat m)
 {
+    Fields f = f0.at(blockIdx.y, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i > g.nx) return;
     const int P = g.pitch, c = IDX(g, i, 1);
@@ -481,20 +529,27 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
     }
 }
 
+// one thread per cell; blockIdx.y picks g.spt shot slots of a batch, which
+// the thread updates in turn with the cell's coefficients read once
 template <int ND>
-__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
 {
     const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
     const int j = 1 + ty * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const int c = IDX(g, i, j);
-    float sxx_x, sxy_x, sxy_y, syy_y;
-    op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
-    float vx = f.vx[c], vy = f.vy[c];
-    velocity_core(g, f, m, s, fz, nt, i, j, c, sxx_x, sxy_x, sxy_y, syy_y, m.rip[c], m.rjp[c], vx, vy);
-    f.vx[c] = vx;
-    f.vy[c] = vy;
+    const float rip = m.rip[c], rjp = m.rjp[c];
+    const int k0 = blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.nb);
+    for (int k = k0; k < k1; ++k) {
+        Fields f = f0.at(k, g);
+        float sxx_x, sxy_x, sxy_y, syy_y;
+        op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
+        float vx = f.vx[c], vy = f.vy[c];
+        velocity_core(g, f, m, s, fz, nt, i, j, c, sxx_x, sxy_x, sxy_y, syy_y, rip, rjp, vx, vy);
+        f.vx[c] = vx;
+        f.vy[c] = vy;
+    }
 }
 
 // VW cells per thread: the x stencils come from 2G+1 vectors of the row,
@@ -711,9 +766,11 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
 }
 
 // one thread per cell; the velocity derivatives are recomputed from vx, vy
-// (replaces v_derivatives + exchange_v) (src/update_s_*_{interior,PML,abs}.c)
+// (replaces v_derivatives + exchange_v) (src/update_s_*_{interior,PML,abs}.c).
+// blockIdx.y picks g.spt shot slots of a batch, updated in turn with the
+// cell's material coefficients read once.
 template <int ND, int WEQ>
-__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
 {
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
     const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
@@ -721,55 +778,83 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f, Mat m, Src 
     const int j = 1 + ty * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const int c = IDX(g, i, j), P = g.pitch;
-    float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
-#pragma unroll
-    for (int mm = 1; mm <= ND; ++mm) {
-        const float h = g.hc[mm];
-        a += h * (f.vx[c + mm - 1] - f.vx[c - mm]);
-        b += h * (f.vy[c + mm] - f.vy[c - mm + 1]);
-        d += h * (f.vx[c + mm * P] - f.vx[c - (mm - 1) * P]);
-        e += h * (f.vy[c + (mm - 1) * P] - f.vy[c - mm * P]);
+    // the cell's coefficients: EL_ISO f g fipjp; VEL_ISO + defect bases;
+    // EL_VTI c11 c13 c33 c55ipjp; VEL_VTI the unrelaxed ones (defects per level below)
+    float c1 = 0.f, c2 = 0.f, c3 = 0.f, c4 = 0.f, br = 0.f, bd = 0.f, be = 0.f;
+    if (WEQ == SOFI_EL_ISO || WEQ == SOFI_VEL_ISO) {
+        c1 = m.f[c];
+        c2 = m.g[c];
+        c3 = m.fipjp[c];
     }
-    float vxx, vyx, vxy, vyy;
-    if (!stress_dv(g, f, m, i, j, a, b, d, e, vxx, vyx, vxy, vyy)) return;
-    float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
-    if (WEQ == SOFI_EL_ISO) upd_el(m.f[c], m.g[c], m.fipjp[c], vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
-    if (WEQ == SOFI_EL_VTI) upd_el_vti(m.c11[c], m.c13[c], m.c33[c], m.c55i[c], vxx, vyx, vxy, vyy, sxx, syy, sxy);
     if (WEQ == SOFI_VEL_ISO) {
-        const float br = m.br[c], bd = m.bd[c], be = m.be[c];
-        const float sxyv = vxy + vyx, sdiag = vxx + vyy;
-        VSum u;
-#pragma unroll
-        for (int l = 1; l <= SOFI_MAXL; ++l) {
-            if (l > g.L) break;
-            const int o = (l - 1) * g.plane + c;
-            float r = f.r[o], p = f.p[o], q = f.q[o];
-            visc_level(m.eta[l], m.bip[l], m.cip[l], br, bd, be, sxyv, sdiag, vxx, vyy, r, p, q, u);
-            f.r[o] = r;
-            f.p[o] = p;
-            f.q[o] = q;
-        }
-        visc_final(g.dthalbe, m.f[c], m.g[c], m.fipjp[c], sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy);
+        br = m.br[c];
+        bd = m.bd[c];
+        be = m.be[c];
+    }
+    if (WEQ == SOFI_EL_VTI) {
+        c1 = m.c11[c];
+        c2 = m.c13[c];
+        c3 = m.c33[c];
+        c4 = m.c55i[c];
     }
     if (WEQ == SOFI_VEL_VTI) {
-        const float sxyv = vxy + vyx;
-        VSum u;
-#pragma unroll
-        for (int l = 1; l <= SOFI_MAXL; ++l) {
-            if (l > g.L) break;
-            const int o = (l - 1) * g.plane + c;
-            float r = f.r[o], p = f.p[o], q = f.q[o];
-            visc_vti_level(m.bip[l], m.cip[l], m.d11[o], m.d13[o], m.d33[o], m.d55i[o], sxyv, vxx, vyy, r, p, q, u);
-            f.r[o] = r;
-            f.p[o] = p;
-            f.q[o] = q;
-        }
-        visc_vti_final(g.dthalbe, m.u11[c], m.u13[c], m.u33[c], m.u55i[c], sxyv, vxx, vyy, u, sxx, syy, sxy);
+        c1 = m.u11[c];
+        c2 = m.u13[c];
+        c3 = m.u33[c];
+        c4 = m.u55i[c];
     }
-    stress_post<ND, VISCO>(g, f, m, s, fz, nt, i, j, c, sxx, syy, sxy);
-    f.sxx[c] = sxx;
-    f.syy[c] = syy;
-    f.sxy[c] = sxy;
+    const int k0 = blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.nb);
+    for (int k = k0; k < k1; ++k) {
+        Fields f = f0.at(k, g);
+        float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
+#pragma unroll
+        for (int mm = 1; mm <= ND; ++mm) {
+            const float h = g.hc[mm];
+            a += h * (f.vx[c + mm - 1] - f.vx[c - mm]);
+            b += h * (f.vy[c + mm] - f.vy[c - mm + 1]);
+            d += h * (f.vx[c + mm * P] - f.vx[c - (mm - 1) * P]);
+            e += h * (f.vy[c + (mm - 1) * P] - f.vy[c - mm * P]);
+        }
+        float vxx, vyx, vxy, vyy;
+        if (!stress_dv(g, f, m, i, j, a, b, d, e, vxx, vyx, vxy, vyy)) continue;
+        float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
+        if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
+        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sxx, syy, sxy);
+        if (WEQ == SOFI_VEL_ISO) {
+            const float sxyv = vxy + vyx, sdiag = vxx + vyy;
+            VSum u;
+#pragma unroll
+            for (int l = 1; l <= SOFI_MAXL; ++l) {
+                if (l > g.L) break;
+                const int o = (l - 1) * g.plane + c;
+                float r = f.r[o], p = f.p[o], q = f.q[o];
+                visc_level(m.eta[l], m.bip[l], m.cip[l], br, bd, be, sxyv, sdiag, vxx, vyy, r, p, q, u);
+                f.r[o] = r;
+                f.p[o] = p;
+                f.q[o] = q;
+            }
+            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy);
+        }
+        if (WEQ == SOFI_VEL_VTI) {
+            const float sxyv = vxy + vyx;
+            VSum u;
+#pragma unroll
+            for (int l = 1; l <= SOFI_MAXL; ++l) {
+                if (l > g.L) break;
+                const int o = (l - 1) * g.plane + c;
+                float r = f.r[o], p = f.p[o], q = f.q[o];
+                visc_vti_level(m.bip[l], m.cip[l], m.d11[o], m.d13[o], m.d33[o], m.d55i[o], sxyv, vxx, vyy, r, p, q, u);
+                f.r[o] = r;
+                f.p[o] = p;
+                f.q[o] = q;
+            }
+            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sxx, syy, sxy);
+        }
+        stress_post<ND, VISCO>(g, f, m, s, fz, nt, i, j, c, sxx, syy, sxy);
+        f.sxx[c] = sxx;
+        f.syy[c] = syy;
+        f.sxy[c] = sxy;
+    }
 }
 
 // VW cells per thread (see k_velocity_v); inactive cells (past NX, or
@@ -934,8 +1019,9 @@ __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Sr
 
 // v_derivatives (src/v_derivatives.c:27-50)
 template <int ND>
-__global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f)
+__global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f0)
 {
+    const Fields f = f0.at(blockIdx.z, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -955,8 +1041,9 @@ __global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f)
 // four sides, true the corners; the elastic right-bottom corner loop stops at
 // GY[4]-1 (src/update_s_elastic_TTI_PML.c:132 vs src/update_s_visc_TTI_PML.c:132)
 template <bool CORNERS>
-__global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f, Mat m)
+__global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f0, Mat m)
 {
+    Fields f = f0.at(blockIdx.z, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -979,8 +1066,9 @@ __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f, Mat 
 // (src/update_s_elastic_TTI_PML.c:37-143); PASS 2: damping frame, every frame
 // cell (src/update_s_elastic_TTI_abs.c).
 template <int WEQ, int PASS>
-__global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f, Mat m)
+__global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m)
 {
+    const Fields f = f0.at(blockIdx.z, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -1050,20 +1138,25 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f, Mat m)
 
 // ------------------------------------------------------------- samplers
 
+// Receivers of every slot of the handle, one list: receiver k samples slot
+// shot[k] at (i[k], j[k]); its trace of component c starts at
+// sec + off[k] + c * cst[k] (each slot's sections are [5][ntr][ns]).
 struct Rec {
-    const int *i, *j;
+    const int *i, *j, *shot;
+    const long long *off, *cst;
     const double *sq_ppi, *sq_pu, *pfac;   // sqrt(ppi), sqrt(pu), (3ppi-4pu)/(2ppi-2pu)
-    float *sec;                            // [5][ntr][ns]
+    float *sec;
     int n, ns;
 };
 
 // seismo_ssg div/curl (src/seismo_ssg.c:51-58): the post-CPML derivatives;
 // runs right after the stress update, before the free surface touches psi.
 template <int ND, bool TTI>
-__global__ void k_seis_dv(Grid g, Fields f, Mat m, Rec r, int ins)
+__global__ void k_seis_dv(Grid g, Fields f0, Mat m, Rec r, int ins)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
+    const Fields f = f0.at(r.shot[k], g);
     const int i = r.i[k], j = r.j[k], c = IDX(g, i, j);
     float vxx, vyx, vxy, vyy;
     if (TTI) {
@@ -1076,24 +1169,26 @@ __global__ void k_seis_dv(Grid g, Fields f, Mat m, Rec r, int ins)
         const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
         if (g.abs_type == 1 && (xf || yf)) cpml_dv_read(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
     }
-    const size_t s = (size_t)k * r.ns + (ins - 1), comp = (size_t)r.n * r.ns;
-    r.sec[SOFI_C_DIV * comp + s] = (float)((double)(vxx + vyy) * r.sq_ppi[k]);
-    r.sec[SOFI_C_CURL * comp + s] = (float)((double)(vxy - vyx) * r.sq_pu[k]);
+    float *t = r.sec + r.off[k] + (ins - 1);
+    t[SOFI_C_DIV * r.cst[k]] = (float)((double)(vxx + vyy) * r.sq_ppi[k]);
+    t[SOFI_C_CURL * r.cst[k]] = (float)((double)(vxy - vyx) * r.sq_pu[k]);
 }
 
 // seismo_ssg vx / vy / p (src/seismo_ssg.c:40-64)
-__global__ void k_seis(Grid g, Fields f, Rec r, int ins, int seismo)
+__global__ void k_seis(Grid g, Fields f0, Rec r, int ins, int seismo)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
+    const Fields f = f0.at(r.shot[k], g);
     const int c = IDX(g, r.i[k], r.j[k]);
-    const size_t s = (size_t)k * r.ns + (ins - 1), comp = (size_t)r.n * r.ns;
+    float *t = r.sec + r.off[k] + (ins - 1);
+    const long long cs = r.cst[k];
     if (seismo == 1 || seismo == 4) {
-        r.sec[SOFI_C_VX * comp + s] = f.vx[c];
-        r.sec[SOFI_C_VY * comp + s] = f.vy[c];
+        t[SOFI_C_VX * cs] = f.vx[c];
+        t[SOFI_C_VY * cs] = f.vy[c];
     }
-    if (seismo == 2) r.sec[SOFI_C_P * comp + s] = -f.sxx[c] - f.syy[c];
-    if (seismo == 4) r.sec[SOFI_C_P * comp + s] = (float)(r.pfac[k] * (double)(-f.sxx[c] - f.syy[c]) / 3);
+    if (seismo == 2) t[SOFI_C_P * cs] = -f.sxx[c] - f.syy[c];
+    if (seismo == 4) t[SOFI_C_P * cs] = (float)(r.pfac[k] * (double)(-f.sxx[c] - f.syy[c]) / 3);
 }
 
 // snap (src/snap_ssg.c:68-205): x-major decimated window; curl and div
@@ -1238,19 +1333,37 @@ struct sofi_dev {
     float *base_d = nullptr, *base_e = nullptr, *base_r = nullptr;
     float *prof = nullptr;
     float *surf = nullptr;
-    int *src_i = nullptr, *src_j = nullptr, *src_t = nullptr;
-    float *src_azi = nullptr, *src_amp = nullptr, *src_sig = nullptr;
+    // acquisition of every shot slot (host copies); combined into the device
+    // lists Src / Rec by sync_acq before the next time step
+    struct Slot {
+        std::vector<int> si, sj, st, ri, rj;
+        std::vector<float> azi, amp, sig;
+        int source_type = 0;
+    };
+    std::vector<Slot> slots;
+    int nb = 1, sel = 0;                   // slots of the handle; the slot the per-shot calls address
+    bool src_dirty = true, rec_dirty = true;
+    int *src_ibuf = nullptr;               // device: i, j, type, slot of every source
+    float *src_fbuf = nullptr, *src_sig = nullptr;
     double *src_sc = nullptr;
-    int source_type = 0;
-    Inj src_inj{};                         // the shot's sources grouped by cell
-    int *src_inj_buf = nullptr;
+    int source_type = 0;                   // slot 0's SOURCE_TYPE (one-shot handles)
+    Inj src_inj{}, psrc_inj{};             // all sources / explosive ones, grouped by cell
+    int *src_inj_buf = nullptr, *psrc_inj_buf = nullptr;
+    int *eq_list = nullptr;                // moment-tensor sources, in order
+    int n_eq = 0;
     bool any_force = false;                // a source of type 2, 3 or 4 (force)
     Fuse fz{};                             // cells of the sources when nsrc <= FUSE_MAX (fz.nforce, npsrc unset)
     bool fusable = false;
     int shape_v = 0, shape_s = 0;          // velocity / stress kernel shapes: SOFI_LAUNCH_*
-    int *rec_i = nullptr, *rec_j = nullptr;
+    int spt_mode = 0;                      // shot slots per thread of a batch (0: tuned in the first run)
+    bool spt_tuned = false;
+    hipEvent_t tune0 = nullptr, tune1 = nullptr;
+    int *rec_ibuf = nullptr;               // device: i, j, slot of every receiver
+    long long *rec_lbuf = nullptr;         // device: trace offset and component stride of every receiver
     double *rec_d = nullptr;
-    float *sec = nullptr;
+    float *sec = nullptr;                  // sections of all slots
+    std::vector<long long> sec_base;       // start of each slot's [5][ntr][ns] sections
+    size_t sec_len = 0;
     float *snapbuf = nullptr;
     int nsnap_max = 0, nsnap = 0, nxs = 0, nys = 0;
     int lsamp = 0, lsnap = 0;
@@ -1334,7 +1447,10 @@ int upload(sofi_dev *h, float **dst, const float *src, size_t n)
 
 // Groups sources by cell (first occurrence order of the cells, source order
 // within a cell) into one device buffer [cell | first | idx].
-int build_inj(sofi_dev *h, int n, const int *si, const int *sj, Inj *inj, int **buf)
+// Entry k is cell (si[k], sj[k]) of shot slot slots[k] (slot 0 if slots is
+// null) and records source index ids[k] (k if ids is null).
+int build_inj(sofi_dev *h, int n, const int *si, const int *sj, const int *slots, const int *ids, Inj *inj,
+              int **buf)
 {
     dfree(h, buf);
     *inj = Inj{};
@@ -1342,14 +1458,14 @@ int build_inj(sofi_dev *h, int n, const int *si, const int *sj, Inj *inj, int **
     std::vector<std::vector<int>> members;
     std::unordered_map<int, size_t> slot;
     for (int k = 0; k < n; ++k) {
-        const int c = (int)sofi_idx(h->g.pitch, si[k], sj[k]);
+        const int c = (slots ? slots[k] : 0) * h->g.plane + (int)sofi_idx(h->g.pitch, si[k], sj[k]);
         auto it = slot.find(c);
         if (it == slot.end()) {
             it = slot.emplace(c, cells.size()).first;
             cells.push_back(c);
             members.emplace_back();
         }
-        members[it->second].push_back(k);
+        members[it->second].push_back(ids ? ids[k] : k);
     }
     std::vector<int> first(1, 0);
     for (const auto &mbr : members) {
@@ -1378,10 +1494,16 @@ extern "C" {
 
 const char *sofi_last_error(void) { return g_err.c_str(); }
 
-int sofi_open(const sofi_desc *desc, sofi_dev **out)
+int sofi_open(const sofi_desc *desc, sofi_dev **out) { return sofi_open_batch(desc, 1, out); }
+
+int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
 {
     if (!desc || !out) return fail("sofi_open: null argument");
+    if (nshots < 1 || nshots > 4096) return fail("sofi_open_batch: %d shot slots (1..4096)", nshots);
     const sofi_desc &d = *desc;
+    if (nshots > 1 && d.mode == SOFI_MODE_FWI) return fail("sofi_open_batch: MODE=FWI runs one shot per handle");
+    if ((double)nshots * (double)sofi_plane(d.nx, d.ny) * (d.L > 0 ? d.L : 1) > 2147483647.0)
+        return fail("sofi_open_batch: %d slots of a %d x %d grid exceed the 2^31-float index range", nshots, d.nx, d.ny);
     if (d.nx < 1 || d.ny < 1) return fail("sofi_open: bad grid %d x %d", d.nx, d.ny);
     if (d.fdorder < 2 || d.fdorder > 12 || (d.fdorder & 1)) return fail("sofi_open: unsupported FDORDER %d", d.fdorder);
     if (d.weq < SOFI_EL_ISO || d.weq > SOFI_VEL_TTI) return fail("sofi_open: unsupported WEQ %d", d.weq);
@@ -1396,7 +1518,11 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
     sofi_dev *h = new (std::nothrow) sofi_dev();
     if (!h) return fail("out of memory");
     h->d = d;
+    h->nb = nshots;
+    h->slots.resize(nshots);
     Grid &g = h->g;
+    g.nb = nshots;
+    g.spt = 1;
     g.nx = d.nx;
     g.ny = d.ny;
     g.pitch = sofi_pitch(d.nx);
@@ -1414,6 +1540,8 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
     g.wx = 2 * d.fw + 1;
     g.wy = d.nx + 1;
     g.ntx = (d.nx + BX - 1) / BX;
+    g.nxp = (d.ny + 1) * g.wx;
+    g.nyp = g.wx * g.wy;
     g.dt = d.dt;
     g.dh = d.dh;
     g.dtdh = d.dt / d.dh;
@@ -1421,32 +1549,33 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out)
     g.dthalbe = (float)(d.dt / 2.0);
     for (int k = 0; k < 7; ++k) g.hc[k] = d.hc[k];
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+        hipEventCreate(&h->tune0) != hipSuccess || hipEventCreate(&h->tune1) != hipSuccess) {
         sofi_close(h);
         return fail("sofi_open: stream/event creation failed");
     }
-    size_t pl = (size_t)g.plane;
+    const size_t pl = (size_t)g.plane, nb = (size_t)nshots;
     Fields &f = h->f;
     int rc = 0;
-    rc |= dalloc_t(h, &f.vx, pl);
-    rc |= dalloc_t(h, &f.vy, pl);
-    rc |= dalloc_t(h, &f.sxx, pl);
-    rc |= dalloc_t(h, &f.syy, pl);
-    rc |= dalloc_t(h, &f.sxy, pl);
-    rc |= dalloc_t(h, &f.unstable, 1);
+    rc |= dalloc_t(h, &f.vx, pl * nb);
+    rc |= dalloc_t(h, &f.vy, pl * nb);
+    rc |= dalloc_t(h, &f.sxx, pl * nb);
+    rc |= dalloc_t(h, &f.syy, pl * nb);
+    rc |= dalloc_t(h, &f.sxy, pl * nb);
+    rc |= dalloc_t(h, &f.unstable, nb);
     if (visco) {
-        rc |= dalloc_t(h, &f.r, pl * g.L);
-        rc |= dalloc_t(h, &f.p, pl * g.L);
-        rc |= dalloc_t(h, &f.q, pl * g.L);
+        rc |= dalloc_t(h, &f.r, pl * g.L * nb);
+        rc |= dalloc_t(h, &f.p, pl * g.L * nb);
+        rc |= dalloc_t(h, &f.q, pl * g.L * nb);
     }
     if (d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI) {
-        rc |= dalloc_t(h, &f.dvxx, pl);
-        rc |= dalloc_t(h, &f.dvyy, pl);
-        rc |= dalloc_t(h, &f.dvyx, pl);
-        rc |= dalloc_t(h, &f.dvxy, pl);
+        rc |= dalloc_t(h, &f.dvxx, pl * nb);
+        rc |= dalloc_t(h, &f.dvyy, pl * nb);
+        rc |= dalloc_t(h, &f.dvyx, pl * nb);
+        rc |= dalloc_t(h, &f.dvxy, pl * nb);
     }
     if (d.abs_type == 1 && d.fw > 0) {
-        size_t nxp = (size_t)(d.ny + 1) * g.wx, nyp = (size_t)g.wx * g.wy;
+        const size_t nxp = (size_t)g.nxp * nb, nyp = (size_t)g.nyp * nb;
         rc |= dalloc_t(h, &f.psi_sxx_x, nxp);
         rc |= dalloc_t(h, &f.psi_sxy_x, nxp);
         rc |= dalloc_t(h, &f.psi_vxx, nxp);
@@ -1648,53 +1777,18 @@ int sofi_set_sources(sofi_dev *h, int nsrc, const int *src_i, const int *src_j, 
     if (nsrc < 0) return fail("sofi_set_sources: nsrc < 0");
     const sofi_desc &d = h->d;
     for (int k = 0; k < nsrc; ++k) {
-        // every injection site, including the eqsource neighbours (j-1, i-1), must lie on the grid
         if (src_i[k] < 1 || src_i[k] > d.nx || src_j[k] < 1 || src_j[k] > d.ny)
             return fail("sofi_set_sources: source %d at (%d, %d) is outside the %d x %d grid", k, src_i[k], src_j[k], d.nx, d.ny);
     }
-    // (re)allocate for this shot
-    size_t n = (size_t)(nsrc > 0 ? nsrc : 1);
-    HIPCHK(hipStreamSynchronize(h->stream));
-    dfree(h, &h->src_i); dfree(h, &h->src_j); dfree(h, &h->src_t);
-    dfree(h, &h->src_azi); dfree(h, &h->src_amp); dfree(h, &h->src_sig); dfree(h, &h->src_sc);
-    int rc = 0;
-    rc |= dalloc_t(h, &h->src_i, n) | dalloc_t(h, &h->src_j, n) | dalloc_t(h, &h->src_t, n);
-    rc |= dalloc_t(h, &h->src_azi, n) | dalloc_t(h, &h->src_amp, n) | dalloc_t(h, &h->src_sig, n * d.nt);
-    rc |= dalloc_t(h, &h->src_sc, 2 * n);
-    if (rc) return -1;
-    std::vector<double> sc(2 * n, 0.0);
-    for (int k = 0; k < nsrc; ++k) {
-        const float azi = (float)(src_azi[k] * M_PI / 180);
-        sc[2 * k] = sin((double)azi);
-        sc[2 * k + 1] = cos((double)azi);
-    }
-    HIPCHK(hipMemcpy(h->src_sc, sc.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice));
-    if (nsrc > 0) {
-        HIPCHK(hipMemcpy(h->src_i, src_i, nsrc * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->src_j, src_j, nsrc * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->src_t, src_type, nsrc * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->src_azi, src_azi, nsrc * sizeof(float), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->src_amp, src_amp, nsrc * sizeof(float), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->src_sig, signals, (size_t)nsrc * d.nt * sizeof(float), hipMemcpyHostToDevice));
-    }
-    h->s.i = h->src_i;
-    h->s.j = h->src_j;
-    h->s.type = h->src_t;
-    h->s.azi = h->src_azi;
-    h->s.amp = h->src_amp;
-    h->s.sig = h->src_sig;
-    h->s.sc = h->src_sc;
-    h->s.n = nsrc;
-    h->s.nt = d.nt;
-    h->source_type = source_type;
-    h->any_force = false;
-    for (int k = 0; k < nsrc; ++k)
-        if (src_type[k] >= 2 && src_type[k] <= 4) h->any_force = true;
-    if (build_inj(h, nsrc, src_i, src_j, &h->src_inj, &h->src_inj_buf)) return -1;
-    h->fusable = nsrc <= FUSE_MAX;
-    h->fz = Fuse{};
-    if (h->fusable)
-        for (int k = 0; k < nsrc; ++k) h->fz.c[k] = (int)sofi_idx(h->g.pitch, src_i[k], src_j[k]);
+    sofi_dev::Slot &sl = h->slots[h->sel];
+    sl.si.assign(src_i, src_i + nsrc);
+    sl.sj.assign(src_j, src_j + nsrc);
+    sl.st.assign(src_type, src_type + nsrc);
+    sl.azi.assign(src_azi, src_azi + nsrc);
+    sl.amp.assign(src_amp, src_amp + nsrc);
+    sl.sig.assign(signals, signals + (size_t)nsrc * d.nt);
+    sl.source_type = source_type;
+    h->src_dirty = true;
     return 0;
 }
 
@@ -1704,42 +1798,190 @@ int sofi_set_receivers(sofi_dev *h, int ntr, const int *rec_i, const int *rec_j)
     if (!h->medium_set) return fail("sofi_set_receivers: call sofi_set_medium first");
     const sofi_desc &d = h->d;
     if (ntr < 0) return fail("sofi_set_receivers: ntr < 0");
-    std::vector<double> fac((size_t)3 * (ntr > 0 ? ntr : 1));
-    for (int k = 0; k < ntr; ++k) {
+    for (int k = 0; k < ntr; ++k)
         if (rec_i[k] < 1 || rec_i[k] > d.nx || rec_j[k] < 1 || rec_j[k] > d.ny)
             return fail("sofi_set_receivers: receiver %d at (%d, %d) is outside the grid", k, rec_i[k], rec_j[k]);
-        size_t c = sofi_idx(h->g.pitch, rec_i[k], rec_j[k]);
-        double ppi = h->h_ppi[c], pu = h->h_pu[c];
-        fac[k] = sqrt(ppi);
-        fac[ntr + k] = sqrt(pu);
-        fac[2 * (size_t)ntr + k] = (3.0 * (float)ppi - 4.0 * (float)pu) / (2.0 * (float)ppi - 2.0 * (float)pu);
-    }
-    size_t n = (size_t)(ntr > 0 ? ntr : 1);
-    HIPCHK(hipStreamSynchronize(h->stream));
-    dfree(h, &h->rec_i); dfree(h, &h->rec_j); dfree(h, &h->rec_d); dfree(h, &h->sec);
-    int rc = dalloc_t(h, &h->rec_i, n) | dalloc_t(h, &h->rec_j, n) | dalloc_t(h, &h->rec_d, 3 * n) |
-             dalloc_t(h, &h->sec, (size_t)SOFI_NCOMP * n * (d.ns > 0 ? d.ns : 1));
-    if (rc) return -1;
-    if (ntr > 0) {
-        HIPCHK(hipMemcpy(h->rec_i, rec_i, ntr * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->rec_j, rec_j, ntr * sizeof(int), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->rec_d, fac.data(), 3 * (size_t)ntr * sizeof(double), hipMemcpyHostToDevice));
-    }
-    Rec &r = h->r;
-    r.i = h->rec_i;
-    r.j = h->rec_j;
-    r.sq_ppi = h->rec_d;
-    r.sq_pu = h->rec_d + ntr;
-    r.pfac = h->rec_d + 2 * (size_t)ntr;
-    r.sec = h->sec;
-    r.n = ntr;
-    r.ns = d.ns;
+    sofi_dev::Slot &sl = h->slots[h->sel];
+    sl.ri.assign(rec_i, rec_i + ntr);
+    sl.rj.assign(rec_j, rec_j + ntr);
+    h->rec_dirty = true;
     return 0;
 }
+
+int sofi_select_shot(sofi_dev *h, int k)
+{
+    if (!h) return fail("sofi_select_shot: null handle");
+    if (k < 0 || k >= h->nb) return fail("sofi_select_shot: slot %d of %d", k, h->nb);
+    h->sel = k;
+    return 0;
+}
+
+} // extern "C"
+
+namespace {
+
+// The device lists of sources and receivers of all slots (rebuilt when a
+// slot's acquisition changed).
+int sync_acq(sofi_dev *h)
+{
+    if (!h->src_dirty && !h->rec_dirty) return 0;
+    const sofi_desc &d = h->d;
+    const Grid &g = h->g;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->src_dirty) {
+        std::vector<int> ci, cj, ct, cslot, gi, gj, pi, pj, eq;
+        std::vector<float> cazi, camp, csig;
+        std::vector<double> sc;
+        for (int k = 0; k < h->nb; ++k) {
+            const sofi_dev::Slot &sl = h->slots[k];
+            for (size_t l = 0; l < sl.si.size(); ++l) {
+                const int id = (int)ci.size();
+                ci.push_back(sl.si[l]);
+                cj.push_back(sl.sj[l]);
+                ct.push_back(sl.st[l]);
+                cslot.push_back(k);
+                cazi.push_back(sl.azi[l]);
+                camp.push_back(sl.amp[l]);
+                const float azi = (float)(sl.azi[l] * M_PI / 180);
+                sc.push_back(sin((double)azi));
+                sc.push_back(cos((double)azi));
+                if (sl.source_type == 5) eq.push_back(id);
+            }
+            csig.insert(csig.end(), sl.sig.begin(), sl.sig.end());
+        }
+        const int n = (int)ci.size();
+        dfree(h, &h->src_ibuf);
+        dfree(h, &h->src_fbuf);
+        dfree(h, &h->src_sig);
+        dfree(h, &h->src_sc);
+        dfree(h, &h->eq_list);
+        const size_t nn = (size_t)(n > 0 ? n : 1);
+        int rc = dalloc_t(h, &h->src_ibuf, 4 * nn) | dalloc_t(h, &h->src_fbuf, 2 * nn) |
+                 dalloc_t(h, &h->src_sig, nn * d.nt) | dalloc_t(h, &h->src_sc, 2 * nn) |
+                 dalloc_t(h, &h->eq_list, eq.empty() ? 1 : eq.size());
+        if (rc) return -1;
+        if (n > 0) {
+            std::vector<int> ib(ci);
+            ib.insert(ib.end(), cj.begin(), cj.end());
+            ib.insert(ib.end(), ct.begin(), ct.end());
+            ib.insert(ib.end(), cslot.begin(), cslot.end());
+            std::vector<float> fb(cazi);
+            fb.insert(fb.end(), camp.begin(), camp.end());
+            HIPCHK(hipMemcpy(h->src_ibuf, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->src_fbuf, fb.data(), fb.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->src_sig, csig.data(), csig.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->src_sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (!eq.empty()) HIPCHK(hipMemcpy(h->eq_list, eq.data(), eq.size() * sizeof(int), hipMemcpyHostToDevice));
+        h->n_eq = (int)eq.size();
+        Src &sv = h->s;
+        sv.i = h->src_ibuf;
+        sv.j = h->src_ibuf + nn;
+        sv.type = h->src_ibuf + 2 * nn;
+        sv.slot = h->src_ibuf + 3 * nn;
+        sv.azi = h->src_fbuf;
+        sv.amp = h->src_fbuf + nn;
+        sv.sig = h->src_sig;
+        sv.sc = h->src_sc;
+        sv.n = n;
+        sv.nt = d.nt;
+        // injection plans over the slots' cells (cell offsets count the slots)
+        std::vector<int> all_i, all_j, ps_i, ps_j, all_id, ps_id;
+        h->any_force = false;
+        for (int l = 0; l < n; ++l) {
+            if (ct[l] >= 2 && ct[l] <= 4) h->any_force = true;
+        }
+        auto plan = [&](bool explosive_only, Inj *inj, int **buf) -> int {
+            std::vector<int> ii, jj, kk, idx;
+            for (int l = 0; l < n; ++l) {
+                if (explosive_only && h->slots[cslot[l]].source_type != 1) continue;
+                ii.push_back(ci[l]);
+                jj.push_back(cj[l]);
+                kk.push_back(cslot[l]);
+                idx.push_back(l);
+            }
+            return build_inj(h, (int)ii.size(), ii.data(), jj.data(), kk.data(), idx.data(), inj, buf);
+        };
+        if (plan(false, &h->src_inj, &h->src_inj_buf) || plan(true, &h->psrc_inj, &h->psrc_inj_buf)) return -1;
+        h->source_type = h->slots[0].source_type;
+        h->fusable = h->nb == 1 && n <= FUSE_MAX;
+        h->fz = Fuse{};
+        if (h->fusable)
+            for (int l = 0; l < n; ++l) h->fz.c[l] = (int)sofi_idx(g.pitch, ci[l], cj[l]);
+        h->src_dirty = false;
+    }
+    if (h->rec_dirty) {
+        std::vector<int> ri, rj, rs;
+        std::vector<long long> off, cst;
+        std::vector<double> sq_ppi, sq_pu, pfac;
+        h->sec_base.assign(h->nb, 0);
+        long long base = 0;
+        for (int k = 0; k < h->nb; ++k) {
+            const sofi_dev::Slot &sl = h->slots[k];
+            const long long ntr = (long long)sl.ri.size(), ns = d.ns > 0 ? d.ns : 1;
+            h->sec_base[k] = base;
+            for (long long r = 0; r < ntr; ++r) {
+                ri.push_back(sl.ri[r]);
+                rj.push_back(sl.rj[r]);
+                rs.push_back(k);
+                off.push_back(base + r * ns);
+                cst.push_back(ntr * ns);
+                const size_t c = sofi_idx(g.pitch, sl.ri[r], sl.rj[r]);
+                const double ppi = h->h_ppi[c], pu = h->h_pu[c];
+                sq_ppi.push_back(sqrt(ppi));
+                sq_pu.push_back(sqrt(pu));
+                pfac.push_back((3.0 * (float)ppi - 4.0 * (float)pu) / (2.0 * (float)ppi - 2.0 * (float)pu));
+            }
+            base += (long long)SOFI_NCOMP * ntr * ns;
+        }
+        const int n = (int)ri.size();
+        const size_t nn = (size_t)(n > 0 ? n : 1);
+        dfree(h, &h->rec_ibuf);
+        dfree(h, &h->rec_lbuf);
+        dfree(h, &h->rec_d);
+        dfree(h, &h->sec);
+        h->sec_len = (size_t)(base > 0 ? base : 1);
+        int rc = dalloc_t(h, &h->rec_ibuf, 3 * nn) | dalloc_t(h, &h->rec_lbuf, 2 * nn) |
+                 dalloc_t(h, &h->rec_d, 3 * nn) | dalloc_t(h, &h->sec, h->sec_len);
+        if (rc) return -1;
+        if (n > 0) {
+            std::vector<int> ib(ri);
+            ib.insert(ib.end(), rj.begin(), rj.end());
+            ib.insert(ib.end(), rs.begin(), rs.end());
+            std::vector<long long> lb(off);
+            lb.insert(lb.end(), cst.begin(), cst.end());
+            std::vector<double> db(sq_ppi);
+            db.insert(db.end(), sq_pu.begin(), sq_pu.end());
+            db.insert(db.end(), pfac.begin(), pfac.end());
+            HIPCHK(hipMemcpy(h->rec_ibuf, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->rec_lbuf, lb.data(), lb.size() * sizeof(long long), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->rec_d, db.data(), db.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        Rec &r = h->r;
+        r.i = h->rec_ibuf;
+        r.j = h->rec_ibuf + nn;
+        r.shot = h->rec_ibuf + 2 * nn;
+        r.off = h->rec_lbuf;
+        r.cst = h->rec_lbuf + nn;
+        r.sq_ppi = h->rec_d;
+        r.sq_pu = h->rec_d + nn;
+        r.pfac = h->rec_d + 2 * nn;
+        r.sec = h->sec;
+        r.n = n;
+        r.ns = d.ns;
+        h->rec_dirty = false;
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
 
 int sofi_set_snapshots(sofi_dev *h, int nsnap)
 {
     if (!h) return fail("sofi_set_snapshots: null handle");
+    if (nsnap > 0 && h->nb > 1) return fail("sofi_set_snapshots: snapshots are taken by one-shot handles only");
     const sofi_desc &d = h->d;
     h->nxs = (d.snap_idx[2] - d.snap_idx[1]) / (d.idx > 0 ? d.idx : 1) + 1;
     h->nys = (d.snap_idx[4] - d.snap_idx[3]) / (d.idy > 0 ? d.idy : 1) + 1;
@@ -1758,21 +2000,21 @@ int sofi_set_snapshots(sofi_dev *h, int nsnap)
 int sofi_reset(sofi_dev *h)
 {
     if (!h) return fail("sofi_reset: null handle");
-    size_t pl = (size_t)h->g.plane * sizeof(float);
+    const size_t nb = (size_t)h->nb, pl = (size_t)h->g.plane * sizeof(float);
     Fields &f = h->f;
     for (float *p : { f.vx, f.vy, f.sxx, f.syy, f.sxy, f.dvxx, f.dvyy, f.dvyx, f.dvxy })
-        if (p) HIPCHK(hipMemsetAsync(p, 0, pl, h->stream));
+        if (p) HIPCHK(hipMemsetAsync(p, 0, pl * nb, h->stream));
     for (float *p : { f.r, f.p, f.q })
-        if (p) HIPCHK(hipMemsetAsync(p, 0, pl * h->g.L, h->stream));
-    size_t nxp = (size_t)(h->d.ny + 1) * h->g.wx * sizeof(float), nyp = (size_t)h->g.wx * h->g.wy * sizeof(float);
+        if (p) HIPCHK(hipMemsetAsync(p, 0, pl * h->g.L * nb, h->stream));
+    const size_t nxp = (size_t)h->g.nxp * sizeof(float) * nb, nyp = (size_t)h->g.nyp * sizeof(float) * nb;
     for (float *p : { f.psi_sxx_x, f.psi_sxy_x, f.psi_vxx, f.psi_vyx })
         if (p) HIPCHK(hipMemsetAsync(p, 0, nxp, h->stream));
     for (float *p : { f.psi_syy_y, f.psi_sxy_y, f.psi_vyy, f.psi_vxy })
         if (p) HIPCHK(hipMemsetAsync(p, 0, nyp, h->stream));
-    if (h->sec) HIPCHK(hipMemsetAsync(h->sec, 0, (size_t)SOFI_NCOMP * (h->r.n > 0 ? h->r.n : 1) * (h->d.ns > 0 ? h->d.ns : 1) * sizeof(float), h->stream));
+    if (h->sec) HIPCHK(hipMemsetAsync(h->sec, 0, h->sec_len * sizeof(float), h->stream));
     for (float *p : { h->w.pvxp1, h->w.pvyp1, h->w.ux, h->w.uy, h->w.uxy })
         if (p) HIPCHK(hipMemsetAsync(p, 0, pl, h->stream));
-    HIPCHK(hipMemsetAsync(f.unstable, 0, sizeof(int), h->stream));
+    HIPCHK(hipMemsetAsync(f.unstable, 0, sizeof(int) * nb, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->nsnap = 0;
     h->lsamp = h->d.ndt;
@@ -1827,9 +2069,14 @@ int step_nd(sofi_dev *h, int nt)
     fv.npsrc = fs.nforce = 0;
     fv.surf = 0;
     fv.nforce = h->fusable && h->any_force ? nsrc : 0;
+    // sources fused into the update kernels (one-shot handles with few sources)
     const bool src_fused = !tti && h->fusable && h->source_type != 5;
     fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
-    fs.surf = src_fused && d.free_surf && !sample_dv && (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
+    // the free surface follows the explosive / moment sources, so it is fused
+    // when they are, or when there are none to inject this step
+    const bool no_stress_src = nsrc == 0 || (h->psrc_inj.n == 0 && h->n_eq == 0);
+    fs.surf = (src_fused || (!tti && no_stress_src)) && d.free_surf && !sample_dv &&
+              (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
     fs.fwi_u = store && d.velocity == 1;
     fv.pvxp1 = fs.pvxp1 = h->w.pvxp1;
@@ -1849,14 +2096,15 @@ int step_nd(sofi_dev *h, int nt)
     }
     const int nty = (g.ny + BY - 1) / BY;
     const dim3 blk(BX, BY);
-    int vw = vec_width(g, h->shape_v);
-    dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty);
+    const int nchunk = (h->nb + g.spt - 1) / g.spt;   // groups of g.spt slots (blockIdx.y)
+    int vw = h->nb > 1 ? 1 : vec_width(g, h->shape_v);
+    dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
     if (te) (void)hipEventRecord(te[0], st);
     if (vw == 4) hipLaunchKernelGGL((k_velocity_v<ND, 4>), tiles, blk, 0, st, g, f, m, h->s, fv, nt);
     else if (vw == 2) hipLaunchKernelGGL((k_velocity_v<ND, 2>), tiles, blk, 0, st, g, f, m, h->s, fv, nt);
     else hipLaunchKernelGGL(k_velocity<ND>, tiles, blk, 0, st, g, f, m, h->s, fv, nt);
-    vw = vec_width(g, h->shape_s);
-    tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty);
+    vw = h->nb > 1 ? 1 : vec_width(g, h->shape_s);
+    tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
     if (te) {
         (void)hipEventRecord(te[1], st);
         (void)hipEventRecord(te[2], st);
@@ -1873,7 +2121,7 @@ int step_nd(sofi_dev *h, int nt)
       case SOFI_EL_VTI: SOFI_STRESS(SOFI_EL_VTI); break;
       case SOFI_VEL_VTI: SOFI_STRESS(SOFI_VEL_VTI); break;
       default: {
-          dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
+          dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, h->nb);
           hipLaunchKernelGGL(k_dv<ND>, tg, tb, 0, st, g, f);
           const bool el = d.weq == SOFI_EL_TTI;
           if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), tg, tb, 0, st, g, f, m);
@@ -1899,12 +2147,12 @@ int step_nd(sofi_dev *h, int nt)
         else hipLaunchKernelGGL((k_seis_dv<ND, false>), rg, rb, 0, st, g, f, m, h->r, ins);
     }
     if (nsrc > 0 && !src_fused) {
-        if (h->source_type == 1)
-            hipLaunchKernelGGL(k_psource, dim3((h->src_inj.n + 63) / 64), dim3(64), 0, st, g, f, h->s, h->src_inj, nt);
-        if (h->source_type == 5) hipLaunchKernelGGL(k_eqsource, dim3(1), dim3(64), 0, st, g, f, h->s, nt);
+        if (h->psrc_inj.n > 0)
+            hipLaunchKernelGGL(k_psource, dim3((h->psrc_inj.n + 63) / 64), dim3(64), 0, st, g, f, h->s, h->psrc_inj, nt);
+        if (h->n_eq > 0) hipLaunchKernelGGL(k_eqsource, dim3(1), dim3(64), 0, st, g, f, h->s, h->eq_list, h->n_eq, nt);
     }
     if (d.free_surf && !fs.surf) {
-        dim3 sb(256), sg((g.nx + 255) / 256);
+        dim3 sb(256), sg((g.nx + 255) / 256, h->nb);
         if (g.L > 0) hipLaunchKernelGGL((k_surface<ND, true>), sg, sb, 0, st, g, f, m);
         else hipLaunchKernelGGL((k_surface<ND, false>), sg, sb, 0, st, g, f, m);
     }
@@ -1953,9 +2201,40 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     if (!h) return fail("sofi_run: null handle");
     if (!h->medium_set) return fail("sofi_run: medium not set");
     if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
-    if (h->d.seismo && h->r.n > 0 && !h->sec) return fail("sofi_run: receivers not set");
+    if (sync_acq(h)) return -1;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    for (int nt = nt_first; nt <= nt_last; ++nt)
+    int nt = nt_first;
+    // a batch's slots per thread: given, or measured on the first steps of a
+    // run long enough to try each candidate for three steps (the results do
+    // not depend on it)
+    if (h->nb > 1 && h->spt_mode == 0 && !h->spt_tuned) {
+        std::vector<int> cand;
+        for (int c : { 1, 2, 4, 8, 16, 32 })
+            if (c <= h->nb) cand.push_back(c);
+        if (nt_last - nt_first + 1 >= 3 * (int)cand.size()) {
+            float best = 1e30f;
+            int best_c = 1;
+            for (int c : cand) {
+                h->g.spt = c;
+                if (step(h, nt++)) return -1;
+                HIPCHK(hipEventRecord(h->tune0, h->stream));
+                if (step(h, nt++) || step(h, nt++)) return -1;
+                HIPCHK(hipEventRecord(h->tune1, h->stream));
+                HIPCHK(hipEventSynchronize(h->tune1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, h->tune0, h->tune1));
+                if (ms < best) {
+                    best = ms;
+                    best_c = c;
+                }
+            }
+            h->g.spt = best_c;
+            h->spt_tuned = true;
+        } else {
+            h->g.spt = h->nb < 4 ? h->nb : 4;
+        }
+    }
+    for (; nt <= nt_last; ++nt)
         if (step(h, nt)) return -1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -1972,11 +2251,27 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
         h->t_samples++;
     }
     h->tev_used = 0;
-    int bad = 0;
-    HIPCHK(hipMemcpy(&bad, h->f.unstable, sizeof(int), hipMemcpyDeviceToHost));
-    if (bad) return fail("Simulation is unstable: vy(NX/2, NY/2) is NaN at time step %d.", bad);
+    std::vector<int> bad(h->nb, 0);
+    HIPCHK(hipMemcpy(bad.data(), h->f.unstable, sizeof(int) * h->nb, hipMemcpyDeviceToHost));
+    for (int k = 0; k < h->nb; ++k)
+        if (bad[k]) {
+            if (h->nb == 1) return fail("Simulation is unstable: vy(NX/2, NY/2) is NaN at time step %d.", bad[k]);
+            return fail("Simulation is unstable: vy(NX/2, NY/2) of shot slot %d is NaN at time step %d.", k, bad[k]);
+        }
     return 0;
 }
+
+int sofi_set_shots_per_thread(sofi_dev *h, int spt)
+{
+    if (!h) return fail("sofi_set_shots_per_thread: null handle");
+    if (spt < 0 || spt > h->nb) return fail("sofi_set_shots_per_thread: %d (0..%d)", spt, h->nb);
+    h->spt_mode = spt;
+    h->spt_tuned = false;
+    h->g.spt = spt > 0 ? spt : 1;
+    return 0;
+}
+
+int sofi_get_shots_per_thread(const sofi_dev *h) { return h ? h->g.spt : 0; }
 
 double sofi_last_run_ms(const sofi_dev *h) { return h ? h->last_ms : 0.0; }
 
@@ -1984,10 +2279,11 @@ int sofi_get_section(sofi_dev *h, int comp, float *out)
 {
     if (!h || !out) return fail("sofi_get_section: null argument");
     if (comp < 0 || comp >= SOFI_NCOMP) return fail("sofi_get_section: bad component %d", comp);
-    size_t n = (size_t)h->r.n * h->d.ns;
+    if (sync_acq(h)) return -1;
+    const size_t ntr = h->slots[h->sel].ri.size(), n = ntr * h->d.ns;
     if (n == 0) return 0;
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(out, h->sec + (size_t)comp * n, n * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, h->sec + h->sec_base[h->sel] + (size_t)comp * n, n * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2017,7 +2313,7 @@ int sofi_get_field(sofi_dev *h, int id, float *out)
       default: return fail("sofi_get_field: bad field id %d", id);
     }
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(out, p, (size_t)h->g.plane * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out, p + (size_t)h->sel * h->g.plane, (size_t)h->g.plane * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2027,8 +2323,8 @@ int sofi_close(sofi_dev *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     for (const auto &a : h->allocs) (void)hipFree(a.raw);
-    if (h->ev0) (void)hipEventDestroy(h->ev0);
-    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    for (hipEvent_t e : { h->ev0, h->ev1, h->tune0, h->tune1 })
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -2089,7 +2385,7 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const
         HIPCHK(hipMemcpy(h->adj_sx, sig_x, (size_t)n * h->d.nt * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(h->adj_sy, sig_y, (size_t)n * h->d.nt * sizeof(float), hipMemcpyHostToDevice));
     }
-    if (build_inj(h, n, ai, aj, &h->adj_inj, &h->adj_inj_buf)) return -1;
+    if (build_inj(h, n, ai, aj, nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
     return 0;
 }

@@ -303,6 +303,58 @@ class Simulation:
         return out.reshape(N.rows(self.desc.ny), N.pitch(self.desc.nx))
 
 
+class Batch(Simulation):
+    """Several shots of one Problem advanced together on one medium
+    (sofi_open_batch): slot k holds shot shots[k]."""
+
+    def __init__(self, problem: Problem, nslots: int):
+        self._L = N.hip()
+        self.problem = problem
+        self.desc = problem.desc.copy()
+        self.nslots = nslots
+        h = C.c_void_p()
+        _chk_hip(self._L.sofi_open_batch(C.byref(self.desc), nslots, C.byref(h)))
+        self._h = h
+        _chk_hip(self._L.sofi_set_medium(h, problem.arrays_ptr(), problem.profiles_ptr()))
+        self.ntr = 0
+        self._keep = []
+
+    def load_shots(self, shots):
+        """Sources and receivers of the problem's shots into slots 0.. and a
+        zero wavefield in every slot."""
+        assert len(shots) <= self.nslots
+        p = self.problem
+        for k, ishot in enumerate(shots):
+            p.set_shot(ishot)
+            _chk_hip(self._L.sofi_select_shot(self._h, k))
+            s = p.sources()
+            self._keep.append(s)
+            _chk_hip(self._L.sofi_set_sources(self._h, s["n"], _as_int_ptr(s["i"]), _as_int_ptr(s["j"]),
+                                              _as_int_ptr(s["type"]), _as_float_ptr(s["azi"]),
+                                              _as_float_ptr(s["amp"]), _as_float_ptr(s["signals"]),
+                                              s["source_type"]))
+            ri, rj = p.receivers()
+            self.ntr = len(ri)
+            if self.desc.seismo and self.ntr:
+                _chk_hip(self._L.sofi_set_receivers(self._h, self.ntr, _as_int_ptr(ri), _as_int_ptr(rj)))
+        _chk_hip(self._L.sofi_select_shot(self._h, 0))
+        _chk_hip(self._L.sofi_reset(self._h))
+
+    def select(self, k: int):
+        _chk_hip(self._L.sofi_select_shot(self._h, k))
+
+    def set_shots_per_thread(self, spt: int):
+        _chk_hip(self._L.sofi_set_shots_per_thread(self._h, spt))
+
+    @property
+    def shots_per_thread(self) -> int:
+        return self._L.sofi_get_shots_per_thread(self._h)
+
+    def section_of(self, k: int, comp: int) -> np.ndarray:
+        self.select(k)
+        return self.section(comp)
+
+
 def interior(a: np.ndarray, nx: int, ny: int) -> np.ndarray:
     """[ny][nx] view of the interior of a padded-layout plane."""
     a = a.reshape(N.rows(ny), N.pitch(nx))

@@ -40,6 +40,19 @@ const char *sofi_last_error(void);
  * (src/initmem.c:29-170) and initfd() (src/initfd.c:363-371) for the path. */
 int sofi_open(const sofi_desc *desc, sofi_dev **out);
 
+/* A batch: nshots shot slots sharing the handle's medium, advanced together
+ * by every sofi_run (the reference runs its shots one after the other,
+ * src/sofi2D.c:320-395).  sofi_select_shot picks the slot that
+ * sofi_set_sources / sofi_set_receivers / sofi_get_section / sofi_get_field
+ * address (slot 0 after opening); sofi_reset clears every slot.  Snapshots
+ * and MODE=FWI need one-slot handles.  The update kernels read a cell's
+ * material once for several slots; sofi_set_shots_per_thread fixes how many
+ * (0: measured on the first steps of the next run, the default). */
+int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out);
+int sofi_select_shot(sofi_dev *h, int k);
+int sofi_set_shots_per_thread(sofi_dev *h, int spt);
+int sofi_get_shots_per_thread(const sofi_dev *h);
+
 /* Upload the material arrays (enum sofi_array, NULL where unused) and
  * profiles (enum sofi_profile); replaces the MemModel the stress and
  * velocity kernels read (src/memm_struct.h:28-126) after prepmod()

@@ -259,3 +259,22 @@ def write_measured_data(td: str, data: dict):
                     h[114:116] = np.frombuffer(np.uint16(a.shape[1]).tobytes(), np.uint8)
                     f.write(h.tobytes())
                     f.write(tr.tobytes())
+
+
+def multishot_case(td: str, weq: str, nshots: int, source_type: int = 1, free_surf: int = 0, abs_type: int = 1,
+                   fdorder: int = 6) -> str:
+    """nshots shots (RUN_MULTIPLE_SHOTS) on one 160 x 120 medium of the given
+    rheology, sources spread along x at depth 6 cells (one on row 1 when the
+    free surface is on), the common receiver set of the coverage cases."""
+    L = 2 if weq.startswith("VEL") else 0
+    xs = np.linspace(30, 130, nshots).round().astype(int)
+    srcs = []
+    for k, i in enumerate(xs):
+        j = 1 if (free_surf and k == 0) else 6 + (k % 3)
+        extra = (30.0 * k, source_type) if source_type in (4, 5) else ()
+        srcs.append(((i - 1) * DH + 1.0, (j - 1) * DH + 1.0, 0.001 * k, 15.0 + k, 1.0, *extra))
+    c = W.Case(name=f"ms_{weq.lower()}_{nshots}", nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=300 * DT_CASE, weq=weq,
+               fdorder=fdorder, maxrelerror=1, L=L, fl=(4.0, 30.0), f_ref=20.0, abs_type=abs_type, fw=12,
+               free_surf=free_surf, damping=8.0, fpml=20.0, vppml=3500.0, source_type=source_type, sources=srcs,
+               receivers=_rec_grid(NX, NY, DH), seismo=4, ndt=2, run_multiple_shots=1)
+    return W.write_case(td, c, _model(weq, 60 + nshots, L=L))
