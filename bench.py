@@ -172,6 +172,75 @@ def pmc_traffic(case_dir: str, launch: str, steps: int = 20):
     return out, factors
 
 
+def bench_batch(args, weq: str):
+    """Batched anisotropic shots on one medium (workloads.config3): the
+    whole batch advanced by the same launches vs the same shots one after the
+    other on a one-shot handle.  Prints one JSON line (shots/s)."""
+    import numpy as np
+    from gpi_project_amd import workloads
+    from gpi_project_amd.sofi import Batch, Problem, Simulation
+    td = tempfile.mkdtemp(prefix="sofi_bench_")
+    gnx, gny = (int(v) for v in args.grid.lower().split("x"))
+    js = workloads.config3(td, weq, nshots=args.shots, nx=gnx, ny=gny, nt=args.nt)
+    cwd = os.getcwd()
+    os.chdir(td)
+    P = Problem(os.path.basename(js))
+    d = P.desc
+    cells = d.nx * d.ny
+    P_ntr = P.receivers()[0].size
+    log(f"{weq}: {d.nx}x{d.ny}, NT={d.nt}, FD{d.fdorder}, {P.nshots} shots, {P_ntr} receivers")
+    # one at a time (first args.single shots, full NT each)
+    sim = Simulation(P)
+    sim.set_launch(*args.launch.split(":"))
+    sim.load_shot(nsnap=0)
+    sim.run(1, min(50, d.nt))               # warm-up
+    t_one = []
+    ref = None
+    for ishot in range(1, args.single + 1):
+        P.set_shot(ishot)
+        sim.load_shot(nsnap=0)
+        t_one.append(sim.run())
+        if ishot == 1:
+            ref = sim.section(0)
+    sim.close()
+    ms_one = float(np.mean(t_one))
+    log(f"one at a time: {ms_one:.1f} ms per shot ({cells * d.nt / (ms_one * 1e-3) / 1e9:.2f} GLUPS)")
+    B = Batch(P, P.nshots)
+    B.load_shots(list(range(1, P.nshots + 1)))
+    ms_b = B.run()
+    grp, spt = B.launch_groups
+    same = np.array_equal(B.section_of(0, 0), ref)
+    B.close()
+    P.close()
+    os.chdir(cwd)
+    shutil.rmtree(td, ignore_errors=True)
+    nb = args.shots
+    glups_b = nb * cells * d.nt / (ms_b * 1e-3) / 1e9
+    p_s = {"EL_ISO": 3, "EL_VTI": 4}.get(weq, 8)
+    bytes_b = 4 * (15 + (2 + p_s) / spt)
+    log(f"batched: {ms_b:.1f} ms for {nb} shots ({glups_b:.2f} GLUPS), {grp} slots per launch, {spt} per thread, "
+        f"slot 0 == one-shot run: {same}")
+    out = {
+        "metric": "batched shots/s", "value": round(nb / (ms_b * 1e-3), 4), "unit": "shots/s", "n_gpus": 1,
+        "steps": d.nt, "warmup": 50, "ms_per_step": round(ms_b / d.nt, 5), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: layered model with smooth random Thomsen fields (workloads.config3)",
+        "config": {"workload": f"{nb} batched {weq} shots, {d.nx}x{d.ny} FD6, NT={d.nt}, {P_ntr} receivers each",
+                   "nx": d.nx, "ny": d.ny, "nt": d.nt, "fdorder": d.fdorder, "weq": weq, "shots": nb,
+                   "slots_per_launch": grp, "slots_per_thread": spt},
+        "one_at_a_time_shots_per_s": round(1e3 / ms_one, 4),
+        "speedup_vs_one_at_a_time": round((nb / ms_b) / (1.0 / ms_one), 3),
+        "batched_glups": round(glups_b, 3),
+        "roofline": {"bound": "hbm", "achieved": round(bytes_b * nb * cells * d.nt / (ms_b * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bytes_b * nb * cells * d.nt / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "whole step (all kernels)",
+                     "algorithmic_bytes_per_shot_cell": round(bytes_b, 2)},
+        "slot0_bitwise_equal_to_one_shot_run": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +256,11 @@ def main():
                     help="shape of the update kernels: auto, cell, vec2, vec4, or VEL:STRESS for each "
                          "(default: chosen by grid size)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3-vti", "c3-tti", "c3-iso"),
+                    help="c2: the headline single-shot run (default); c3-*: 64 batched anisotropic shots")
+    ap.add_argument("--shots", type=int, default=64, help="shots of the batched workloads")
+    ap.add_argument("--single", type=int, default=4, help="shots timed one at a time for the comparison")
+    ap.add_argument("--grid", default="2000x1000", help="NXxNY of the batched workloads")
     ap.add_argument("--pmc-child", metavar="CASE_DIR", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=20, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -194,9 +268,14 @@ def main():
         pmc_child(args.pmc_child, args.launch, args.pmc_steps)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (shots shard across GPUs with no "
-              "communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: shots are independent and would "
+              "shard across GPUs with no communication during a shot", file=sys.stderr)
         sys.exit(2)
+    if args.workload.startswith("c3"):
+        if "--nt" not in sys.argv:
+            args.nt = 4000
+        bench_batch(args, {"c3-vti": "EL_VTI", "c3-tti": "EL_TTI", "c3-iso": "EL_ISO"}[args.workload])
+        return
     if args.steps > args.nt:
         args.nt = args.steps
 

@@ -141,8 +141,8 @@ def hip():
     L.sofi_set_launch.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_open_batch.argtypes = [C.POINTER(SofiDesc), C.c_int, C.POINTER(_VP)]
     L.sofi_select_shot.argtypes = [_VP, C.c_int]
-    L.sofi_set_shots_per_thread.argtypes = [_VP, C.c_int]
-    L.sofi_get_shots_per_thread.argtypes = [_VP]
+    L.sofi_set_batch_launch.argtypes = [_VP, C.c_int, C.c_int]
+    L.sofi_get_batch_launch.argtypes = [_VP, _IP, _IP]
     L.sofi_fwi_set_model.argtypes = [_VP, _FP, _FP, _FP]
     L.sofi_fwi_set_pass.argtypes = [_VP, C.c_int]
     L.sofi_fwi_set_adjoint.argtypes = [_VP, C.c_int, _IP, _IP, _FP, _FP]
@@ -170,5 +170,5 @@ def exported_symbols():
                            "sofi_set_kernel_timing", "sofi_get_kernel_timing", "sofi_set_launch",
                            "sofi_fwi_set_model", "sofi_fwi_set_pass", "sofi_fwi_set_adjoint",
                            "sofi_fwi_gradient", "sofi_fwi_gradient_sum", "sofi_open_batch", "sofi_select_shot",
-                           "sofi_set_shots_per_thread", "sofi_get_shots_per_thread"],
+                           "sofi_set_batch_launch", "sofi_get_batch_launch"],
     }

@@ -14,6 +14,7 @@
 // frame update).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -58,6 +59,7 @@ struct Grid {
     int wx, wy;     // row lengths of x-frame and y-frame CPML memory
     int ntx;        // tiles along x of the one-cell-per-thread update kernels
     int nb, spt;    // shot slots of a batch; slots per thread of the update kernels
+    int k_lo, k_hi; // the slots this launch updates (a group of the batch)
     int nxp, nyp;   // floats per slot of the x-frame and y-frame CPML memory
     float dt, dh, dtdh, dhi, dthalbe;
     float hc[7];
@@ -427,7 +429,7 @@ __device__ __forceinline__ float surface_sxx(const Grid &g, Fields &f, const Mat
 template <int ND, bool VISCO>
 __global__ void k_surface(Grid g, Fields f0, Mat m)
 {
-    Fields f = f0.at(blockIdx.y, g);   // shot slot of a batch
+    Fields f = f0.at(g.k_lo + blockIdx.y, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i > g.nx) return;
     const int P = g.pitch, c = IDX(g, i, 1);
@@ -540,7 +542,7 @@ __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, S
     if (i > g.nx || j > g.ny) return;
     const int c = IDX(g, i, j);
     const float rip = m.rip[c], rjp = m.rjp[c];
-    const int k0 = blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.nb);
+    const int k0 = g.k_lo + blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.k_hi);
     for (int k = k0; k < k1; ++k) {
         Fields f = f0.at(k, g);
         float sxx_x, sxy_x, sxy_y, syy_y;
@@ -803,7 +805,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
         c3 = m.u33[c];
         c4 = m.u55i[c];
     }
-    const int k0 = blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.nb);
+    const int k0 = g.k_lo + blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.k_hi);
     for (int k = k0; k < k1; ++k) {
         Fields f = f0.at(k, g);
         float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
@@ -1021,7 +1023,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Sr
 template <int ND>
 __global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f0)
 {
-    const Fields f = f0.at(blockIdx.z, g);   // shot slot of a batch
+    const Fields f = f0.at(g.k_lo + blockIdx.z, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -1043,7 +1045,7 @@ __global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f0)
 template <bool CORNERS>
 __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f0, Mat m)
 {
-    Fields f = f0.at(blockIdx.z, g);   // shot slot of a batch
+    Fields f = f0.at(g.k_lo + blockIdx.z, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -1068,7 +1070,7 @@ __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f0, Mat
 template <int WEQ, int PASS>
 __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m)
 {
-    const Fields f = f0.at(blockIdx.z, g);   // shot slot of a batch
+    const Fields f = f0.at(g.k_lo + blockIdx.z, g);   // shot slot of a batch
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -1355,7 +1357,8 @@ struct sofi_dev {
     Fuse fz{};                             // cells of the sources when nsrc <= FUSE_MAX (fz.nforce, npsrc unset)
     bool fusable = false;
     int shape_v = 0, shape_s = 0;          // velocity / stress kernel shapes: SOFI_LAUNCH_*
-    int spt_mode = 0;                      // shot slots per thread of a batch (0: tuned in the first run)
+    int spt_mode = 0, grp_mode = 0;        // a batch's slots per thread and per launch (0: tuned in the first run)
+    int grp = 0;                           // slots per launch of the update kernels (0: all)
     bool spt_tuned = false;
     hipEvent_t tune0 = nullptr, tune1 = nullptr;
     int *rec_ibuf = nullptr;               // device: i, j, slot of every receiver
@@ -1523,6 +1526,8 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     Grid &g = h->g;
     g.nb = nshots;
     g.spt = 1;
+    g.k_lo = 0;
+    g.k_hi = nshots;
     g.nx = d.nx;
     g.ny = d.ny;
     g.pitch = sofi_pitch(d.nx);
@@ -2033,11 +2038,13 @@ int vec_width(const Grid &g, int shape)
     if (shape == SOFI_LAUNCH_CELL) return 1;
     if (shape == SOFI_LAUNCH_VEC2) return 2;
     if (shape == SOFI_LAUNCH_VEC4) return 4;
-    // automatic: one cell per thread (measured fastest for both kernels on the
-    // 4000 x 2000 viscoelastic workload: the vector shapes need 80-150
-    // VGPRs and lose more to the lower occupancy than they gain in loads)
-    (void)g;
-    return 1;
+    // automatic, as measured on MI355X: the elastic stress updates stream
+    // only 11-12 arrays, too few bytes in flight at one float per thread, and
+    // VEC2 for both kernels won by 26 % (EL_ISO and EL_VTI, 2000 x 1000 FD6:
+    // 48.4 vs 38.5 GLUPS); the viscoelastic ones stream 20-40 arrays and one
+    // cell per thread won (VEL_ISO L=3, 4000 x 2000 FD8: the vector shapes
+    // need 80-150 VGPRs and lose more to occupancy than they gain)
+    return (g.weq == SOFI_EL_ISO || g.weq == SOFI_EL_VTI) ? 2 : 1;
 }
 
 template <int ND>
@@ -2096,48 +2103,60 @@ int step_nd(sofi_dev *h, int nt)
     }
     const int nty = (g.ny + BY - 1) / BY;
     const dim3 blk(BX, BY);
-    const int nchunk = (h->nb + g.spt - 1) / g.spt;   // groups of g.spt slots (blockIdx.y)
-    int vw = h->nb > 1 ? 1 : vec_width(g, h->shape_v);
-    dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
+    // the update kernels, launched per group of h->grp slots (all slots of a
+    // one-shot handle or of an untuned batch: one group)
+    const int grp = h->grp > 0 && h->grp < h->nb ? h->grp : h->nb;
     if (te) (void)hipEventRecord(te[0], st);
-    if (vw == 4) hipLaunchKernelGGL((k_velocity_v<ND, 4>), tiles, blk, 0, st, g, f, m, h->s, fv, nt);
-    else if (vw == 2) hipLaunchKernelGGL((k_velocity_v<ND, 2>), tiles, blk, 0, st, g, f, m, h->s, fv, nt);
-    else hipLaunchKernelGGL(k_velocity<ND>, tiles, blk, 0, st, g, f, m, h->s, fv, nt);
-    vw = h->nb > 1 ? 1 : vec_width(g, h->shape_s);
-    tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
-    if (te) {
+    for (int lo = 0; lo < h->nb; lo += grp) {
+        Grid gg = g;
+        gg.k_lo = lo;
+        gg.k_hi = lo + grp < h->nb ? lo + grp : h->nb;
+        const int nchunk = (gg.k_hi - lo + g.spt - 1) / g.spt;   // groups of g.spt slots (blockIdx.y)
+        int vw = h->nb > 1 ? 1 : vec_width(g, h->shape_v);
+        dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
+        if (vw == 4) hipLaunchKernelGGL((k_velocity_v<ND, 4>), tiles, blk, 0, st, gg, f, m, h->s, fv, nt);
+        else if (vw == 2) hipLaunchKernelGGL((k_velocity_v<ND, 2>), tiles, blk, 0, st, gg, f, m, h->s, fv, nt);
+        else hipLaunchKernelGGL(k_velocity<ND>, tiles, blk, 0, st, gg, f, m, h->s, fv, nt);
+        vw = h->nb > 1 ? 1 : vec_width(g, h->shape_s);
+        tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
+        if (te && lo == 0 && grp == h->nb) {
+            (void)hipEventRecord(te[1], st);
+            (void)hipEventRecord(te[2], st);
+        }
+#define SOFI_STRESS(W)                                                                                      \
+        do {                                                                                                \
+            if (vw == 4) hipLaunchKernelGGL((k_stress_v<ND, W, 4>), tiles, blk, 0, st, gg, f, m, h->s, fs, nt); \
+            else if (vw == 2) hipLaunchKernelGGL((k_stress_v<ND, W, 2>), tiles, blk, 0, st, gg, f, m, h->s, fs, nt); \
+            else hipLaunchKernelGGL((k_stress<ND, W>), tiles, blk, 0, st, gg, f, m, h->s, fs, nt);            \
+        } while (0)
+        switch (d.weq) {
+          case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
+          case SOFI_VEL_ISO: SOFI_STRESS(SOFI_VEL_ISO); break;
+          case SOFI_EL_VTI: SOFI_STRESS(SOFI_EL_VTI); break;
+          case SOFI_VEL_VTI: SOFI_STRESS(SOFI_VEL_VTI); break;
+          default: {
+              dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, gg.k_hi - lo);
+              hipLaunchKernelGGL(k_dv<ND>, tg, tb, 0, st, gg, f);
+              const bool el = d.weq == SOFI_EL_TTI;
+              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), tg, tb, 0, st, gg, f, m);
+              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), tg, tb, 0, st, gg, f, m);
+              if (d.fw > 0 && d.abs_type == 1) {
+                  hipLaunchKernelGGL(k_cpml_dv_frame<false>, tg, tb, 0, st, gg, f, m);
+                  if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), tg, tb, 0, st, gg, f, m);
+                  else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), tg, tb, 0, st, gg, f, m);
+                  hipLaunchKernelGGL(k_cpml_dv_frame<true>, tg, tb, 0, st, gg, f, m);
+                  if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 3>), tg, tb, 0, st, gg, f, m);
+                  else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 3>), tg, tb, 0, st, gg, f, m);
+              } else if (d.fw > 0 && d.abs_type == 2) {
+                  if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), tg, tb, 0, st, gg, f, m);
+                  else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), tg, tb, 0, st, gg, f, m);
+              }
+          }
+        }
+    }
+    if (te && grp != h->nb) {
         (void)hipEventRecord(te[1], st);
         (void)hipEventRecord(te[2], st);
-    }
-#define SOFI_STRESS(W)                                                                                  \
-    do {                                                                                                \
-        if (vw == 4) hipLaunchKernelGGL((k_stress_v<ND, W, 4>), tiles, blk, 0, st, g, f, m, h->s, fs, nt); \
-        else if (vw == 2) hipLaunchKernelGGL((k_stress_v<ND, W, 2>), tiles, blk, 0, st, g, f, m, h->s, fs, nt); \
-        else hipLaunchKernelGGL((k_stress<ND, W>), tiles, blk, 0, st, g, f, m, h->s, fs, nt);            \
-    } while (0)
-    switch (d.weq) {
-      case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
-      case SOFI_VEL_ISO: SOFI_STRESS(SOFI_VEL_ISO); break;
-      case SOFI_EL_VTI: SOFI_STRESS(SOFI_EL_VTI); break;
-      case SOFI_VEL_VTI: SOFI_STRESS(SOFI_VEL_VTI); break;
-      default: {
-          dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, h->nb);
-          hipLaunchKernelGGL(k_dv<ND>, tg, tb, 0, st, g, f);
-          const bool el = d.weq == SOFI_EL_TTI;
-          if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), tg, tb, 0, st, g, f, m);
-          else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), tg, tb, 0, st, g, f, m);
-          if (d.fw > 0 && d.abs_type == 1) {
-              hipLaunchKernelGGL(k_cpml_dv_frame<false>, tg, tb, 0, st, g, f, m);
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), tg, tb, 0, st, g, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), tg, tb, 0, st, g, f, m);
-              hipLaunchKernelGGL(k_cpml_dv_frame<true>, tg, tb, 0, st, g, f, m);
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 3>), tg, tb, 0, st, g, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 3>), tg, tb, 0, st, g, f, m);
-          } else if (d.fw > 0 && d.abs_type == 2) {
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), tg, tb, 0, st, g, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), tg, tb, 0, st, g, f, m);
-          }
-      }
     }
 #undef SOFI_STRESS
     if (te) (void)hipEventRecord(te[3], st);
@@ -2207,15 +2226,22 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     // a batch's slots per thread: given, or measured on the first steps of a
     // run long enough to try each candidate for three steps (the results do
     // not depend on it)
-    if (h->nb > 1 && h->spt_mode == 0 && !h->spt_tuned) {
-        std::vector<int> cand;
-        for (int c : { 1, 2, 4, 8, 16, 32 })
-            if (c <= h->nb) cand.push_back(c);
-        if (nt_last - nt_first + 1 >= 3 * (int)cand.size()) {
+    if (h->nb > 1 && (h->spt_mode == 0 || h->grp_mode == 0) && !h->spt_tuned) {
+        std::vector<std::pair<int, int>> cand;   // (slots per launch, slots per thread)
+        for (int gr : { h->nb, 16, 8, 4 }) {
+            if (gr > h->nb || (gr != h->nb && gr >= h->nb)) continue;
+            if (h->grp_mode > 0 && gr != h->grp_mode) continue;
+            for (int sp : { 1, 2, 4, 8 }) {
+                if (sp > gr || (h->spt_mode > 0 && sp != h->spt_mode)) continue;
+                if (std::find(cand.begin(), cand.end(), std::make_pair(gr, sp)) == cand.end()) cand.emplace_back(gr, sp);
+            }
+        }
+        if (!cand.empty() && nt_last - nt_first + 1 >= 3 * (int)cand.size()) {
             float best = 1e30f;
-            int best_c = 1;
-            for (int c : cand) {
-                h->g.spt = c;
+            std::pair<int, int> best_c = cand[0];
+            for (const auto &c : cand) {
+                h->grp = c.first;
+                h->g.spt = c.second;
                 if (step(h, nt++)) return -1;
                 HIPCHK(hipEventRecord(h->tune0, h->stream));
                 if (step(h, nt++) || step(h, nt++)) return -1;
@@ -2228,10 +2254,12 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
                     best_c = c;
                 }
             }
-            h->g.spt = best_c;
+            h->grp = best_c.first;
+            h->g.spt = best_c.second;
             h->spt_tuned = true;
         } else {
-            h->g.spt = h->nb < 4 ? h->nb : 4;
+            h->grp = h->grp_mode > 0 ? h->grp_mode : (h->nb < 4 ? h->nb : 4);
+            h->g.spt = h->spt_mode > 0 ? h->spt_mode : (h->grp < 4 ? h->grp : 4);
         }
     }
     for (; nt <= nt_last; ++nt)
@@ -2261,17 +2289,26 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     return 0;
 }
 
-int sofi_set_shots_per_thread(sofi_dev *h, int spt)
+int sofi_set_batch_launch(sofi_dev *h, int group, int spt)
 {
-    if (!h) return fail("sofi_set_shots_per_thread: null handle");
-    if (spt < 0 || spt > h->nb) return fail("sofi_set_shots_per_thread: %d (0..%d)", spt, h->nb);
+    if (!h) return fail("sofi_set_batch_launch: null handle");
+    if (group < 0 || group > h->nb) return fail("sofi_set_batch_launch: group %d (0..%d)", group, h->nb);
+    if (spt < 0 || spt > (group > 0 ? group : h->nb)) return fail("sofi_set_batch_launch: %d slots per thread", spt);
+    h->grp_mode = group;
     h->spt_mode = spt;
     h->spt_tuned = false;
+    h->grp = group;
     h->g.spt = spt > 0 ? spt : 1;
     return 0;
 }
 
-int sofi_get_shots_per_thread(const sofi_dev *h) { return h ? h->g.spt : 0; }
+int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt)
+{
+    if (!h) return fail("sofi_get_batch_launch: null handle");
+    if (group) *group = h->grp > 0 ? h->grp : h->nb;
+    if (spt) *spt = h->g.spt;
+    return 0;
+}
 
 double sofi_last_run_ms(const sofi_dev *h) { return h ? h->last_ms : 0.0; }
 

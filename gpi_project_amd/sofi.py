@@ -343,12 +343,15 @@ class Batch(Simulation):
     def select(self, k: int):
         _chk_hip(self._L.sofi_select_shot(self._h, k))
 
-    def set_shots_per_thread(self, spt: int):
-        _chk_hip(self._L.sofi_set_shots_per_thread(self._h, spt))
+    def set_launch_groups(self, group: int = 0, spt: int = 0):
+        """Slots per launch and per thread of the update kernels (0: tuned)."""
+        _chk_hip(self._L.sofi_set_batch_launch(self._h, group, spt))
 
     @property
-    def shots_per_thread(self) -> int:
-        return self._L.sofi_get_shots_per_thread(self._h)
+    def launch_groups(self):
+        g, s = C.c_int(), C.c_int()
+        _chk_hip(self._L.sofi_get_batch_launch(self._h, C.byref(g), C.byref(s)))
+        return g.value, s.value
 
     def section_of(self, k: int, comp: int) -> np.ndarray:
         self.select(k)

@@ -188,3 +188,43 @@ def config2(d: str, nx=4000, ny=2000, nt=8000, seed=2) -> str:
              vppml=float(vp.max()), sources=[(x_src, 25.0, 0.0, 15.0, 1.0)], seismo=1, ndt=4,
              rec_line=(x1, 12.5, x1 + 2 * dh * (ntr - 1), 12.5, 2.0))
     return write_case(d, c, {"vp": vp, "vs": vs, "rho": rho, "qp": q, "qs": q})
+
+
+def _rank_uniform(field: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    """Map a field to U(lo, hi) by its ranks (keeps its smooth structure)."""
+    r = np.empty(field.size)
+    r[np.argsort(field, axis=None, kind="stable")] = np.arange(field.size)
+    return (lo + (hi - lo) * (r + 0.5) / field.size).reshape(field.shape)
+
+
+def config3(d: str, weq: str = "EL_VTI", nshots: int = 64, nx: int = 2000, ny: int = 1000, nt: int = 4000,
+            seed: int = 3) -> str:
+    """Batched shots: EL_VTI or EL_TTI (or EL_ISO), 2000 x 1000, DH 5 m,
+    FD6 (Holberg 0.1 %), DT 4e-4 s, 20-point CPML, no free surface.  Shared
+    model: 12 flat layers vp 1800..4500 m/s (vs = vp/1.73, rho = 310
+    vp^0.25); epsilon ~ U(0, 0.25), delta ~ U(-0.1, 0.15), theta ~ U(0, 45)
+    degrees as smooth fields (uniform noise of seed 3 smoothed with a 100 m
+    Gaussian, rank-mapped to the ranges).  nshots shots (seed 4): x ~ U(FW
+    DH, (NX - FW) DH), y = 25 m, 10..25 Hz Ricker explosive sources; 500
+    receivers at y = 12.5 m every 3 cells from cell 250."""
+    dh = 5.0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vp, vs, rho = layered_model(nx, ny, dh, rng, pert=0.0)
+    model = {"vp": vp, "vs": vs, "rho": rho}
+    sm = lambda: smooth_noise(rng, nx, ny, 100.0 / dh)
+    if weq != "EL_ISO":
+        model["epsilon"] = _rank_uniform(sm(), 0.0, 0.25).astype(np.float32)
+        model["delta"] = _rank_uniform(sm(), -0.1, 0.15).astype(np.float32)
+    if weq == "EL_TTI":
+        model["theta"] = _rank_uniform(sm(), 0.0, 45.0).astype(np.float32)
+    srng = np.random.Generator(np.random.PCG64(4))
+    fw = 20
+    xs = srng.uniform(fw * dh, (nx - fw) * dh, nshots)
+    fcs = srng.uniform(10.0, 25.0, nshots)
+    ntr = min(500, max(1, (nx - 2 * fw - 250) // 3))
+    x1 = 250 * dh if nx > 2000 - 1 else (fw + 5) * dh
+    c = Case(name=f"c3_{weq.lower()}", nx=nx, ny=ny, dh=dh, dt=4.0e-4, time=nt * 4.0e-4, weq=weq, fdorder=6,
+             maxrelerror=1, abs_type=1, fw=fw, free_surf=0, fpml=15.0, vppml=float(vp.max()),
+             sources=[(float(x), 25.0, 0.0, float(f), 1.0) for x, f in zip(xs, fcs)], seismo=1, ndt=4,
+             rec_line=(x1, 12.5, x1 + 3 * dh * (ntr - 1), 12.5, 3.0), run_multiple_shots=1)
+    return write_case(d, c, model)

@@ -45,13 +45,14 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out);
  * src/sofi2D.c:320-395).  sofi_select_shot picks the slot that
  * sofi_set_sources / sofi_set_receivers / sofi_get_section / sofi_get_field
  * address (slot 0 after opening); sofi_reset clears every slot.  Snapshots
- * and MODE=FWI need one-slot handles.  The update kernels read a cell's
- * material once for several slots; sofi_set_shots_per_thread fixes how many
- * (0: measured on the first steps of the next run, the default). */
+ * and MODE=FWI need one-slot handles.  The update kernels run over groups of
+ * `group` slots per launch and read a cell's material once for `spt` slots
+ * per thread; sofi_set_batch_launch fixes them (0: measured on the first
+ * steps of the next run, the default).  No counterpart in the reference. */
 int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out);
 int sofi_select_shot(sofi_dev *h, int k);
-int sofi_set_shots_per_thread(sofi_dev *h, int spt);
-int sofi_get_shots_per_thread(const sofi_dev *h);
+int sofi_set_batch_launch(sofi_dev *h, int group, int spt);
+int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
 
 /* Upload the material arrays (enum sofi_array, NULL where unused) and
  * profiles (enum sofi_profile); replaces the MemModel the stress and
@@ -134,7 +135,8 @@ int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *gr
  * cell in 64 x 4 tiles, SOFI_LAUNCH_VEC2 / SOFI_LAUNCH_VEC4 one thread per
  * two / four consecutive cells of a row (vector loads and stores) in
  * 128 x 4 / 256 x 4 tiles, SOFI_LAUNCH_AUTO (the default) the shape
- * measured fastest (currently CELL).  Results agree to rounding between
+ * measured fastest for the rheology (VEC2 for EL_ISO / EL_VTI, CELL
+ * otherwise; batches always run CELL).  Results agree to rounding between
  * shapes. */
 enum { SOFI_LAUNCH_AUTO = 0, SOFI_LAUNCH_CELL = 1, SOFI_LAUNCH_VEC2 = 2, SOFI_LAUNCH_VEC4 = 3 };
 int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape);

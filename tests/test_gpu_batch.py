@@ -21,6 +21,7 @@ def _single(P, ishot):
     from gpi_project_amd.sofi import Simulation
     P.set_shot(ishot)
     sim = Simulation(P)
+    sim.set_launch("cell")      # the shape batches run; the arithmetic then matches bit for bit
     sim.load_shot(nsnap=0)
     sim.run()
     out = {c: sim.section(c) for c in COMPS4}
@@ -43,16 +44,16 @@ def test_batch_matches_one_shot_handles(weq, stype, fs, absn):
         P = Problem(os.path.basename(js))
         assert P.nshots == 4
         ref = {k: _single(P, k) for k in range(1, 5)}
-        for spt in (1, 3, 0):
+        for grp, spt in ((4, 1), (4, 3), (2, 2), (0, 0)):
             B = Batch(P, 4)
-            B.set_shots_per_thread(spt)
+            B.set_launch_groups(grp, spt)
             B.load_shots([1, 2, 3, 4])
             B.run()
             for k in range(4):
                 for c in COMPS4:
                     got = B.section_of(k, c)
-                    assert np.array_equal(got, ref[k + 1][c]), (weq, spt, k, c, np.abs(got - ref[k + 1][c]).max())
-            print(weq, "slots per thread", spt, "->", B.shots_per_thread)
+                    assert np.array_equal(got, ref[k + 1][c]), (weq, grp, spt, k, c, np.abs(got - ref[k + 1][c]).max())
+            print(weq, "slots per launch / thread", grp, spt, "->", B.launch_groups)
             B.close()
         P.close()
 
