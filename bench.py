@@ -241,6 +241,93 @@ def bench_batch(args, weq: str):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args):
+    """Served shot requests (workloads.config4_requests): each request is a
+    complete run directory (parameter file, model files, sources) of its own
+    size, order, rheology and NT.  Three passes over the same requests:
+      one at a time  every request set up, run and read back alone, in order:
+                     the service times and the one-at-a-time capacity
+      served         Poisson arrivals at 1.5x that capacity through the
+                     serving loop (admission by footprint, concurrent
+                     handles): shots/s and p50/p99 latency
+      saturated      every request waiting at t = 0: the serving loop's
+                     throughput ceiling
+    The one-at-a-time latencies under the same arrivals follow from the
+    measured service times (a FIFO single server).  Every served result is
+    compared bit for bit with the request's one-at-a-time result."""
+    import numpy as np
+    from gpi_project_amd import serve, workloads
+    td = tempfile.mkdtemp(prefix="sofi_serve_")
+    q4 = workloads.config4_requests(args.requests)
+    t0 = time.time()
+    paths = [workloads.config4_write(os.path.join(td, f"r{q.rid:03d}"), q) for q in q4]
+    cell_steps = sum(q.nx * q.ny * q.nt for q in q4)
+    byts = sum((80 if q.weq == "EL_ISO" else 164) * q.nx * q.ny * q.nt for q in q4)
+    log(f"{len(q4)} requests written in {time.time() - t0:.1f} s ({cell_steps / 1e9:.1f} G cell-steps)")
+    mk = lambda: [serve.Request(q.rid, p) for q, p in zip(q4, paths)]
+    # warm-up: the smallest request once (code objects, allocator)
+    w = min(q4, key=lambda q: q.nx * q.ny * q.nt)
+    serve.run_sequential([serve.Request(w.rid, paths[w.rid])], args.launch)
+    alone = mk()
+    seq = serve.run_sequential(alone, args.launch)
+    svc = np.diff([0.0] + [r.end for r in alone])          # all waiting at t = 0: back to back
+    mean_svc = seq["span_s"] / len(alone)
+    log(f"one at a time: {seq['span_s']:.2f} s, {seq['shots_per_s']:.2f} shots/s, {seq['glups']:.1f} GLUPS")
+    gaps = np.array([q.gap for q in q4]) * mean_svc / 1.5
+    arrivals = np.cumsum(gaps) - gaps[0]
+    fifo_end, t = [], 0.0
+    for a, s in zip(arrivals, svc):
+        t = max(t, a) + s
+        fifo_end.append(t)
+    lat_seq = np.array(fifo_end) - arrivals
+
+    def served(arr):
+        reqs = mk()
+        for r, a in zip(reqs, arr):
+            r.arrival = float(a)
+        srv = serve.Server(gpu_workers=args.workers, launch=args.launch)
+        st = srv.serve(reqs)
+        same = all(r.error is None and set(r.sections) == set(a.sections) and
+                   all(np.array_equal(r.sections[s][c], a.sections[s][c]) for s in a.sections for c in a.sections[s])
+                   for r, a in zip(reqs, alone))
+        st["peak_hbm_bytes"] = srv.peak_used
+        return st, same
+
+    st, same = served(arrivals)
+    log(f"served (Poisson, 1.5x capacity): {st['span_s']:.2f} s, {st['shots_per_s']:.2f} shots/s, "
+        f"p50 {st['p50_s']:.3f} s p99 {st['p99_s']:.3f} s, identical to alone: {same}")
+    sat, same2 = served(np.zeros(len(q4)))
+    log(f"saturated: {sat['span_s']:.2f} s, {sat['shots_per_s']:.2f} shots/s, {sat['glups']:.1f} GLUPS, "
+        f"identical to alone: {same2}")
+    shutil.rmtree(td, ignore_errors=True)
+    ach = byts / sat["span_s"] / 1e9
+    out = {
+        "metric": "served shots/s", "value": round(st["shots_per_s"], 4), "unit": "shots/s", "n_gpus": 1,
+        "steps": int(sum(q.nt for q in q4)), "warmup": 1, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: per-request seeded layered models (workloads.config4_requests, seed 5)",
+        "config": {"workload": f"{len(q4)} served requests, sides log-U[256,4096], FD4/8, NT U{{1000..8000}}, "
+                               "EL_ISO/VEL_ISO, Poisson arrivals at 1.5x the one-at-a-time capacity",
+                   "requests": len(q4), "gpu_workers": args.workers, "cell_steps": cell_steps},
+        "p50_latency_s": round(st["p50_s"], 4), "p99_latency_s": round(st["p99_s"], 4),
+        "one_at_a_time_shots_per_s": round(seq["shots_per_s"], 4),
+        "one_at_a_time_p50_latency_s": round(float(np.percentile(lat_seq, 50)), 4),
+        "one_at_a_time_p99_latency_s": round(float(np.percentile(lat_seq, 99)), 4),
+        "saturated_shots_per_s": round(sat["shots_per_s"], 4),
+        "speedup_vs_one_at_a_time": round(st["shots_per_s"] / seq["shots_per_s"], 3),
+        "saturated_speedup_vs_one_at_a_time": round(sat["shots_per_s"] / seq["shots_per_s"], 3),
+        "served_glups": round(st["glups"], 3), "saturated_glups": round(sat["glups"], 3),
+        "one_at_a_time_glups": round(seq["glups"], 3),
+        "one_at_a_time_device_s": round(sum(r.gpu_ms for r in alone) * 1e-3, 3),
+        "peak_admitted_hbm_bytes": st["peak_hbm_bytes"],
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole saturated job (80 B per EL_ISO, 164 B per VEL_ISO cell-step)"},
+        "results_identical_alone_and_in_mix": bool(same and same2),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,11 +343,14 @@ def main():
                     help="shape of the update kernels: auto, cell, vec2, vec4, or VEL:STRESS for each "
                          "(default: chosen by grid size)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3-vti", "c3-tti", "c3-iso"),
-                    help="c2: the headline single-shot run (default); c3-*: 64 batched anisotropic shots")
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3-vti", "c3-tti", "c3-iso", "c4"),
+                    help="c2: the headline single-shot run (default); c3-*: 64 batched anisotropic shots; "
+                         "c4: served requests of mixed sizes")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched workloads")
     ap.add_argument("--single", type=int, default=4, help="shots timed one at a time for the comparison")
     ap.add_argument("--grid", default="2000x1000", help="NXxNY of the batched workloads")
+    ap.add_argument("--requests", type=int, default=48, help="requests of the served workload")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent handles of the served workload")
     ap.add_argument("--pmc-child", metavar="CASE_DIR", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=20, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -275,6 +365,9 @@ def main():
         if "--nt" not in sys.argv:
             args.nt = 4000
         bench_batch(args, {"c3-vti": "EL_VTI", "c3-tti": "EL_TTI", "c3-iso": "EL_ISO"}[args.workload])
+        return
+    if args.workload == "c4":
+        bench_serve(args)
         return
     if args.steps > args.nt:
         args.nt = args.steps

@@ -2369,6 +2369,50 @@ int sofi_close(sofi_dev *h)
 
 size_t sofi_footprint(const sofi_dev *h) { return h ? h->bytes : 0; }
 
+size_t sofi_estimate_footprint(const sofi_desc *d, int nshots, int ntr, int nsnap)
+{
+    if (!d || d->nx < 1 || d->ny < 1 || nshots < 1) return 0;
+    // the allocations of sofi_open_batch, sofi_set_medium, the acquisition and
+    // sofi_set_snapshots, each rounded up by the 256 B of its alignment
+    const size_t pl = sofi_plane(d->nx, d->ny) * sizeof(float), nb = (size_t)nshots, A = 256;
+    const bool visco = d->weq == SOFI_VEL_ISO || d->weq == SOFI_VEL_VTI || d->weq == SOFI_VEL_TTI;
+    const bool tti = d->weq == SOFI_EL_TTI || d->weq == SOFI_VEL_TTI;
+    const size_t L = visco ? (size_t)d->L : 0, wx = 2 * (size_t)d->fw + 1;
+    size_t b = 5 * (pl * nb + A) + (nb * sizeof(int) + A);                  // v, s, instability flags
+    if (visco) b += 3 * (pl * L * nb + A);                                    // memory variables
+    if (tti) b += 4 * (pl * nb + A);                                          // derivative arrays
+    if (d->abs_type == 1 && d->fw > 0)
+        b += 4 * ((size_t)(d->ny + 1) * wx * 4 * nb + A) + 4 * (wx * (size_t)(d->nx + 1) * 4 * nb + A);
+    size_t arrays = 4;                                                        // rip rjp pu ppi
+    switch (d->weq) {
+      case SOFI_EL_ISO: arrays += 3; break;
+      case SOFI_VEL_ISO: arrays += 5 + 3; break;                              // f g fipjp taus taup + bases
+      case SOFI_EL_VTI: arrays += 4; break;
+      case SOFI_VEL_VTI: arrays += 4 + 4 * L; break;
+      case SOFI_EL_TTI: arrays += 8; break;
+      case SOFI_VEL_TTI: arrays += 8 + 8 * L; break;
+    }
+    if (d->abs_type == 2 && d->fw > 0) arrays += 1;
+    b += arrays * (pl + A);
+    if (d->abs_type == 1 && d->fw > 0) b += 12 * wx * 4 + A;
+    if (d->free_surf) b += 5 * (size_t)d->nx * 4 + A;
+    const size_t ns = (size_t)(d->ns > 0 ? d->ns : 1), nr = (size_t)(ntr > 0 ? ntr : 0) * nb;
+    b += (SOFI_NCOMP * nr * ns * 4 + A) + (nr * 3 * 4 + A) + (nr * 2 * 8 + A) + (nr * 3 * 8 + A);
+    if (nsnap > 0) {
+        const size_t nxs = (size_t)((d->snap_idx[2] - d->snap_idx[1]) / (d->idx > 0 ? d->idx : 1) + 1);
+        const size_t nys = (size_t)((d->snap_idx[4] - d->snap_idx[3]) / (d->idy > 0 ? d->idy : 1) + 1);
+        b += (size_t)nsnap * SOFI_NCOMP * nxs * nys * 4 + A;
+    }
+    if (d->mode == SOFI_MODE_FWI) {
+        const size_t dtinv = d->dtinv > 0 ? (size_t)d->dtinv : 1, nstore = ((size_t)d->nt + dtinv - 1) / dtinv;
+        b += 8 * (pl + A) + 2 * (3 * pl + A) + 5 * (pl + A);
+        b += nstore * 5 * (size_t)d->nx * d->ny * 4 + A;
+        b += 2 * ((size_t)(ntr > 0 ? ntr : 1) * d->nt * 4 + A);
+    }
+    b += (size_t)1 << 20;                                                     // sources and small buffers
+    return b;
+}
+
 int sofi_fwi_set_model(sofi_dev *h, const float *vp0, const float *vs0, const float *rho0)
 {
     if (!h || !vp0 || !vs0 || !rho0) return fail("sofi_fwi_set_model: null argument");

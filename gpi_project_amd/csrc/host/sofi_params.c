@@ -23,7 +23,7 @@
 #include "sofi_host.h"
 #include "sofi_host_internal.h"
 
-static char g_err[1024];
+static _Thread_local char g_err[1024];   /* per thread: the library is used from several threads */
 
 const char *sofi_host_last_error(void) { return g_err; }
 

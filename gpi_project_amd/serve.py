@@ -1,0 +1,277 @@
+"""Serving loop: shot requests (reference parameter files) admitted by their
+HBM footprint and run concurrently, each on its own handle and HIP stream.
+
+A request moves through three stages:
+  setup      on arrival, a CPU worker reads its parameter and model files
+             (libsofi_host) and computes its device footprint
+             (sofi_estimate_footprint); a request larger than the whole
+             budget is rejected here
+  admission  requests wait in arrival order; any waiting request whose
+             footprint fits the free part of the HBM budget is admitted (FIFO
+             with backfill), at most `gpu_workers` at a time
+  run        a GPU worker opens the handle, uploads the medium and advances
+             the shot in blocks of `block_steps` time steps; the wavefield,
+             medium and seismograms stay resident on the device between
+             blocks, and a cancelled request stops at the next block
+             boundary; the seismograms are copied back at the end and the
+             footprint is released when the handle closes
+Requests that arrive while others run are admitted as soon as a worker and
+memory are free, and their kernels overlap with the running ones on the
+device (one stream per handle).  Latency is completion time minus arrival
+time.  A request's results are those of the same request run alone: handles
+share nothing but the GPU, and a block boundary does not change a step.
+
+The reference has no serving layer (it runs the shots of one parameter file
+one after the other, src/sofi2D.c:320-395).
+"""
+from __future__ import annotations
+
+import contextlib
+import ctypes as C
+import os
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+from typing import Callable
+
+import numpy as np
+
+
+@dataclass
+class Request:
+    rid: int
+    json_path: str
+    arrival: float = 0.0                   # seconds after the start of the run
+    footprint: int = 0
+    ready: float | None = None             # setup done
+    admitted: float | None = None
+    end: float | None = None
+    gpu_ms: float = 0.0
+    cell_steps: float = 0.0
+    sections: dict = field(default_factory=dict)
+    error: str | None = None
+    cancelled: bool = False
+
+    @property
+    def latency(self) -> float | None:
+        return None if self.end is None else self.end - self.arrival
+
+
+_CWD_LOCK = threading.Lock()
+
+
+@contextlib.contextmanager
+def in_dir(directory: str):
+    """The reference's parameter files name their inputs relative to the run
+    directory; the host library opens them while this holds, so the working
+    directory is switched under a process-wide lock."""
+    with _CWD_LOCK:
+        cwd = os.getcwd()
+        os.chdir(directory)
+        try:
+            yield
+        finally:
+            os.chdir(cwd)
+
+
+def device_free_bytes() -> int:
+    """Free HBM of the current device (hipMemGetInfo)."""
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return int(free.value)
+
+
+def open_problem(req: Request):
+    """Read the request's parameter, model and acquisition files."""
+    from .sofi import Problem
+    d = os.path.dirname(os.path.abspath(req.json_path))
+    with in_dir(d):
+        return Problem(os.path.basename(req.json_path))
+
+
+def gpu_prepare(req: Request):
+    """Setup stage: the problem and its device footprint (one slot, the
+    receivers of the first shot, no snapshots)."""
+    from . import _native as N
+    P = open_problem(req)
+    try:
+        ntr = len(P.receivers()[0])
+        fp = int(N.hip().sofi_estimate_footprint(C.byref(P.desc), 1, ntr, 0))
+    except Exception:
+        P.close()
+        raise
+    return fp, P
+
+
+def gpu_execute(req: Request, P, should_stop: Callable[[], bool], launch: str = "auto",
+                block_steps: int = 500) -> None:
+    """Run stage: every shot of the request on one handle, in blocks of
+    block_steps time steps; fills req.sections[ishot][comp]."""
+    from .sofi import Simulation
+    d = os.path.dirname(os.path.abspath(req.json_path))
+    sim = None
+    try:
+        sim = Simulation(P)
+        sim.set_launch(launch)
+        for ishot in range(1, P.nshots + 1):
+            if ishot > 1:
+                with in_dir(d):
+                    P.set_shot(ishot)
+            sim.load_shot(nsnap=0)
+            nt = P.desc.nt
+            for t0 in range(1, nt + 1, block_steps):
+                if should_stop():
+                    return
+                req.gpu_ms += sim.run(t0, min(nt, t0 + block_steps - 1))
+            req.cell_steps += float(P.desc.nx) * P.desc.ny * nt
+            req.sections[ishot] = {c: sim.section(c) for c in P.seismo_comps()}
+    finally:
+        if sim is not None:
+            sim.close()
+        P.close()
+
+
+class Server:
+    """Admission by footprint and concurrent execution of shot requests.
+
+    prepare(req) -> (footprint_bytes, state) and execute(req, state,
+    should_stop) are the two stages (the GPU ones by default); execute owns
+    state and releases it before returning."""
+
+    def __init__(self, budget_bytes: int | None = None, gpu_workers: int = 4, setup_workers: int = 4,
+                 launch: str = "auto", block_steps: int = 500,
+                 prepare: Callable | None = None, execute: Callable | None = None):
+        self.budget = int(budget_bytes if budget_bytes is not None else 0.85 * device_free_bytes())
+        self.gpu_workers = gpu_workers
+        self.setup_workers = setup_workers
+        self.prepare = prepare or gpu_prepare
+        self.execute = execute or (lambda r, s, stop: gpu_execute(r, s, stop, launch, block_steps))
+        self.used = 0
+        self.peak_used = 0
+        self.admit_order: list[int] = []
+        self._cv = threading.Condition()
+        self._cancel: set[int] = set()
+
+    def cancel(self, rid: int) -> None:
+        """Stop request rid: dropped if it still waits, stopped at its next
+        block boundary if it runs.  Its memory returns to the budget."""
+        with self._cv:
+            self._cancel.add(rid)
+            self._cv.notify_all()
+
+    # -- stages --------------------------------------------------------------
+    def _setup(self, req: Request, t0: float, ready: list):
+        state = None
+        try:
+            req.footprint, state = self.prepare(req)
+            if req.footprint > self.budget:
+                raise RuntimeError(f"request {req.rid} needs {req.footprint} B, "
+                                   f"more than the {self.budget} B budget")
+        except Exception as e:  # noqa: BLE001
+            req.error = str(e)
+            if state is not None and hasattr(state, "close"):
+                state.close()
+            state = None
+        req.ready = time.perf_counter() - t0
+        with self._cv:
+            ready.append((req, state))
+            self._cv.notify_all()
+
+    def _run(self, req: Request, state, t0: float):
+        try:
+            self.execute(req, state, lambda: req.rid in self._cancel)
+        except Exception as e:  # noqa: BLE001
+            req.error = str(e)
+        finally:
+            req.end = time.perf_counter() - t0
+            with self._cv:
+                if req.rid in self._cancel:
+                    req.cancelled = True
+                    req.error = req.error or "cancelled"
+                self.used -= req.footprint
+                self._running -= 1
+                self._done += 1
+                self._cv.notify_all()
+
+    # -- the loop --------------------------------------------------------------
+    def serve(self, requests: list[Request]) -> dict:
+        """Run the requests, each released at its arrival time; returns the
+        throughput and latency statistics (requests are updated in place)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        ready: list = []
+        self._running = 0
+        self._done = 0
+        t0 = time.perf_counter()
+        setup_pool = ThreadPoolExecutor(self.setup_workers)
+        gpu_pool = ThreadPoolExecutor(self.gpu_workers)
+
+        def feeder():
+            for r in reqs:
+                dt = r.arrival - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                setup_pool.submit(self._setup, r, t0, ready)
+
+        threading.Thread(target=feeder, daemon=True).start()
+        with self._cv:
+            while self._done < len(reqs):
+                ready.sort(key=lambda rs: rs[0].arrival)
+                k = 0
+                while k < len(ready):
+                    r, state = ready[k]
+                    if r.error is not None or r.rid in self._cancel:
+                        ready.pop(k)
+                        if state is not None and hasattr(state, "close"):
+                            state.close()
+                        r.cancelled = r.rid in self._cancel
+                        r.error = r.error or "cancelled"
+                        r.end = time.perf_counter() - t0
+                        self._done += 1
+                        continue
+                    # first waiting request that fits, in arrival order (backfill)
+                    if self._running < self.gpu_workers and self.used + r.footprint <= self.budget:
+                        ready.pop(k)
+                        self.used += r.footprint
+                        self.peak_used = max(self.peak_used, self.used)
+                        self._running += 1
+                        r.admitted = time.perf_counter() - t0
+                        self.admit_order.append(r.rid)
+                        gpu_pool.submit(self._run, r, state, t0)
+                        continue
+                    k += 1
+                self._cv.wait(0.05)
+        setup_pool.shutdown()
+        gpu_pool.shutdown()
+        return summarize(reqs)
+
+
+def summarize(reqs: list[Request]) -> dict:
+    ok = [r for r in reqs if r.error is None]
+    lat = np.array([r.latency for r in ok]) if ok else np.zeros(1)
+    span = max(r.end for r in reqs) - min(r.arrival for r in reqs)
+    return {
+        "requests": len(reqs), "completed": len(ok), "span_s": span,
+        "shots_per_s": sum(len(r.sections) for r in ok) / span if span > 0 else 0.0,
+        "glups": sum(r.cell_steps for r in ok) / span / 1e9 if span > 0 else 0.0,
+        "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+    }
+
+
+def run_sequential(requests: list[Request], launch: str = "auto") -> dict:
+    """The same requests one after the other (setup, run, results) as they
+    arrive, with nothing overlapped: the one-at-a-time capacity."""
+    reqs = sorted(requests, key=lambda q: q.arrival)
+    t0 = time.perf_counter()
+    for r in reqs:
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        fp, P = gpu_prepare(r)
+        r.footprint = fp
+        r.admitted = time.perf_counter() - t0
+        gpu_execute(r, P, lambda: False, launch, block_steps=1 << 30)
+        r.end = time.perf_counter() - t0
+    return summarize(reqs)

@@ -166,7 +166,7 @@ def config1(d: str) -> str:
 C2_FL = (5.0, 15.0, 45.0)
 
 
-def config2(d: str, nx=4000, ny=2000, nt=8000, seed=2) -> str:
+def config2(d: str, nx=4000, ny=2000, nt=8000, seed=2, weq="VEL_ISO", fdorder=8, name="c2") -> str:
     """Viscoelastic L=3 (tau=0.05 at 15 Hz), FD8, DH 2.5 m, DT 2.5e-4 s, free
     surface + 20-point CPML; 12 flat layers vp 1800..4500 m/s with a 5 %
     smoothed Gaussian perturbation (50 m correlation length, seed 2),
@@ -174,20 +174,63 @@ def config2(d: str, nx=4000, ny=2000, nt=8000, seed=2) -> str:
     x = 5000 m, y = 25 m; 1000 receivers at y = 12.5 m every 2 cells from 250 m.
     The perturbed vp is clipped at 5060 m/s so the unrelaxed velocity
     (1.034 vp for these relaxation frequencies) stays below the FD8 stability
-    limit checkfd enforces (DT <= DH / (sqrt(2) sum|hc| vmax), 5254 m/s)."""
+    limit checkfd enforces (DT <= DH / (sqrt(2) sum|hc| vmax), 5254 m/s).
+    weq = "EL_ISO" drops the relaxation (no Q files); fdorder 4 is stable at
+    the same DT (its Holberg sum|hc| is smaller)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     dh = 2.5
     vp, vs, rho = layered_model(nx, ny, dh, rng, vp_max=5060.0)
-    L, tau = 3, 0.05
-    q = np.full((nx, ny), q_from_tau(tau, L), dtype=np.float32)
+    visco = weq == "VEL_ISO"
+    L, tau = (3, 0.05) if visco else (0, 0.0)
     x_src = min(5000.0, (nx - 1) * dh * 0.5)
     ntr = min(1000, max(1, (nx - 2 * 20 - 100) // 2))
     x1 = 250.0 if nx > 400 else 20 * dh
-    c = Case(name="c2", nx=nx, ny=ny, dh=dh, dt=2.5e-4, time=nt * 2.5e-4, weq="VEL_ISO", fdorder=8,
+    c = Case(name=name, nx=nx, ny=ny, dh=dh, dt=2.5e-4, time=nt * 2.5e-4, weq=weq, fdorder=fdorder,
              maxrelerror=1, L=L, fl=C2_FL, f_ref=15.0, abs_type=1, fw=20, free_surf=1, fpml=15.0,
              vppml=float(vp.max()), sources=[(x_src, 25.0, 0.0, 15.0, 1.0)], seismo=1, ndt=4,
              rec_line=(x1, 12.5, x1 + 2 * dh * (ntr - 1), 12.5, 2.0))
-    return write_case(d, c, {"vp": vp, "vs": vs, "rho": rho, "qp": q, "qs": q})
+    model = {"vp": vp, "vs": vs, "rho": rho}
+    if visco:
+        q = np.full((nx, ny), q_from_tau(tau, L), dtype=np.float32)
+        model.update(qp=q, qs=q)
+    return write_case(d, c, model)
+
+
+@dataclass
+class Request4:
+    rid: int
+    nx: int
+    ny: int
+    nt: int
+    weq: str
+    fdorder: int
+    gap: float          # unit-rate exponential inter-arrival gap
+
+
+def config4_requests(nreq: int, seed: int = 5, side=(256, 4096), nt=(1000, 8000)) -> list:
+    """Served shot requests: grid sides log-uniform in [256, 4096]
+    (independently in x and y), FDORDER in {4, 8}, NT ~ U{1000..8000}, WEQ in
+    {EL_ISO, VEL_ISO}, and unit-rate exponential inter-arrival gaps (Poisson
+    arrivals; multiply by the mean service time / offered load), all from one
+    generator of seed 5.  Each request's model is the C2 model with seed
+    100 + rid."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = math.log(side[0]), math.log(side[1])
+    out = []
+    for r in range(nreq):
+        nx = int(round(math.exp(rng.uniform(lo, hi))))
+        ny = int(round(math.exp(rng.uniform(lo, hi))))
+        fd = int(rng.choice([4, 8]))
+        n = int(rng.integers(nt[0], nt[1] + 1))
+        weq = ("EL_ISO", "VEL_ISO")[int(rng.integers(0, 2))]
+        out.append(Request4(r, nx, ny, n, weq, fd, float(rng.exponential(1.0))))
+    return out
+
+
+def config4_write(d: str, q: "Request4") -> str:
+    """Run directory of one served request (C2 model of seed 100 + rid)."""
+    return config2(d, nx=q.nx, ny=q.ny, nt=q.nt, seed=100 + q.rid, weq=q.weq, fdorder=q.fdorder,
+                   name=f"c4_{q.rid:03d}")
 
 
 def _rank_uniform(field: np.ndarray, lo: float, hi: float) -> np.ndarray:

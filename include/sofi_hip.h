@@ -107,6 +107,12 @@ int sofi_close(sofi_dev *h);
 /* Device bytes the handle holds (HBM footprint of one resident shot). */
 size_t sofi_footprint(const sofi_dev *h);
 
+/* The bytes a handle for desc with nshots slots, ntr receivers per slot and
+ * nsnap snapshots will hold, without allocating (admission control of a
+ * serving layer; no counterpart in the reference).  An upper bound of
+ * sofi_footprint after the same calls, by at most 1 MiB of small buffers. */
+size_t sofi_estimate_footprint(const sofi_desc *desc, int nshots, int ntr, int nsnap);
+
 /* FWI gradient path (MODE=FWI, WEQ=EL_ISO): what time_loop does with
  * sw = 0 / 1 and MemInv (src/time_loop.c:27-326 with src/snap_store.c:27-47,
  * src/calc_conv.c:30-73), driven per shot as src/inversion.c:27-258 does.

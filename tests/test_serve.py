@@ -1,0 +1,120 @@
+"""The serving loop's scheduling on the CPU: admission by footprint against
+the budget, FIFO with backfill, the worker limit, rejection of a request
+larger than the budget, and cancellation (waiting and mid-flight).  The two
+stages are stand-ins that sleep and return deterministic outputs, so the
+scheduler is exercised without a device."""
+import threading
+import time
+
+import pytest
+
+from gpi_project_amd.serve import Request, Server
+
+
+class Fake:
+    """prepare/execute stand-ins: footprint and duration per request id; the
+    run advances in blocks of 10 ms and records the largest admitted total."""
+
+    def __init__(self, fp, dur):
+        self.fp, self.dur = fp, dur
+        self.lock = threading.Lock()
+        self.live, self.peak_live, self.running, self.peak_running = 0, 0, 0, 0
+        self.started = {}
+
+    def prepare(self, req):
+        return self.fp[req.rid], {"rid": req.rid}
+
+    def execute(self, req, state, should_stop):
+        with self.lock:
+            self.live += req.footprint
+            self.running += 1
+            self.peak_live = max(self.peak_live, self.live)
+            self.peak_running = max(self.peak_running, self.running)
+            self.started[req.rid] = time.perf_counter()
+        try:
+            t_end = time.perf_counter() + self.dur[req.rid]
+            while time.perf_counter() < t_end:
+                if should_stop():
+                    return
+                time.sleep(0.01)
+            req.sections[1] = {0: [req.rid * 10 + k for k in range(3)]}
+        finally:
+            with self.lock:
+                self.live -= req.footprint
+                self.running -= 1
+
+
+def _serve(fake, reqs, budget, workers=4):
+    s = Server(budget_bytes=budget, gpu_workers=workers, prepare=fake.prepare, execute=fake.execute)
+    st = s.serve(reqs)
+    return s, st
+
+
+def test_budget_is_never_exceeded_and_all_complete():
+    fp = {i: (i % 4 + 1) * 10 for i in range(12)}
+    dur = {i: 0.03 + 0.01 * (i % 3) for i in range(12)}
+    f = Fake(fp, dur)
+    reqs = [Request(i, f"r{i}.json", arrival=0.005 * i) for i in range(12)]
+    s, st = _serve(f, reqs, budget=60, workers=8)
+    assert st["completed"] == 12 and all(r.error is None for r in reqs)
+    assert f.peak_live <= 60 and s.peak_used <= 60
+    assert s.used == 0
+    assert all(r.sections[1][0] == [r.rid * 10 + k for k in range(3)] for r in reqs)
+    assert all(r.latency >= dur[r.rid] for r in reqs)
+
+
+def test_fifo_with_backfill():
+    """A (6) runs; B (6) does not fit beside it, C (3) does and is admitted
+    ahead of B; B starts once A has released its memory."""
+    f = Fake({0: 6, 1: 6, 2: 3}, {0: 0.15, 1: 0.02, 2: 0.02})
+    reqs = [Request(0, "a", 0.0), Request(1, "b", 0.02), Request(2, "c", 0.04)]
+    s, st = _serve(f, reqs, budget=10)
+    assert s.admit_order == [0, 2, 1]
+    assert f.started[1] >= f.started[0] + 0.14
+    assert st["completed"] == 3
+
+
+def test_worker_limit():
+    f = Fake({i: 1 for i in range(6)}, {i: 0.05 for i in range(6)})
+    reqs = [Request(i, "r", 0.0) for i in range(6)]
+    _serve(f, reqs, budget=100, workers=2)
+    assert f.peak_running == 2
+
+
+def test_over_budget_request_is_rejected_and_others_run():
+    f = Fake({0: 5, 1: 50, 2: 5}, {0: 0.02, 1: 0.02, 2: 0.02})
+    reqs = [Request(i, "r", 0.0) for i in range(3)]
+    s, st = _serve(f, reqs, budget=20)
+    assert reqs[1].error and "budget" in reqs[1].error and not reqs[1].sections
+    assert reqs[0].error is None and reqs[2].error is None
+    assert st["completed"] == 2 and s.used == 0
+
+
+@pytest.mark.parametrize("when", ["running", "waiting"])
+def test_cancel_frees_memory_and_leaves_others_unchanged(when):
+    """Request 0 fills the budget; request 1 waits for it.  Cancelling 0
+    mid-flight lets 1 start at once; cancelling the waiting 1 drops it.
+    Every other request's output is what it is without the cancellation."""
+    f = Fake({0: 8, 1: 8, 2: 1}, {0: 5.0 if when == "running" else 0.3, 1: 0.05, 2: 0.05})
+    reqs = [Request(0, "r", 0.0), Request(1, "r", 0.01), Request(2, "r", 0.02)]
+    s = Server(budget_bytes=10, gpu_workers=4, prepare=f.prepare, execute=f.execute)
+    target = 0 if when == "running" else 1
+
+    def canceller():
+        while reqs[0].admitted is None or reqs[1].ready is None:
+            time.sleep(0.005)
+        time.sleep(0.05)
+        s.cancel(target)
+
+    th = threading.Thread(target=canceller)
+    th.start()
+    t0 = time.perf_counter()
+    st = s.serve(reqs)
+    th.join()
+    assert time.perf_counter() - t0 < 3.0
+    assert reqs[target].cancelled and reqs[target].error == "cancelled" and not reqs[target].sections
+    others = [r for r in reqs if r.rid != target]
+    assert all(r.error is None and r.sections[1][0] == [r.rid * 10 + k for k in range(3)] for r in others)
+    assert st["completed"] == 2 and s.used == 0 and f.live == 0
+    if when == "waiting":
+        assert 1 not in s.admit_order
