@@ -328,6 +328,55 @@ def bench_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_small(args):
+    """The reference's small CPU-runnable case (workloads.config1: EL_ISO
+    400 x 400 FD4, NT = 2000, 100 receivers): a launch-bound loop whose
+    working set (~7 MB) stays in the caches.  The whole loop step by step and
+    from captured blocks of 64 steps; prints one JSON line (GLUPS of the
+    captured run)."""
+    import numpy as np
+    from gpi_project_amd import workloads
+    from gpi_project_amd.sofi import Problem, Simulation
+    td = tempfile.mkdtemp(prefix="sofi_bench_")
+    js = workloads.config1(td)
+    cwd = os.getcwd()
+    os.chdir(td)
+    P = Problem(os.path.basename(js))
+    d = P.desc
+    res = {}
+    for mode in ("off", "on"):
+        sim = Simulation(P)
+        sim.set_graph(mode, 64)
+        sim.load_shot(nsnap=0)
+        sim.run(1, min(args.warmup, d.nt))
+        best = None
+        for _ in range(3):
+            sim.load_shot(nsnap=0)
+            ms = sim.run()
+            best = ms if best is None else min(best, ms)
+        res[mode] = (best, sim.section(1), sim.graph_blocks)
+        sim.close()
+    P.close()
+    os.chdir(cwd)
+    shutil.rmtree(td, ignore_errors=True)
+    cells = d.nx * d.ny
+    glups = {m: cells * d.nt / (res[m][0] * 1e-3) / 1e9 for m in res}
+    same = bool(np.array_equal(res["on"][1], res["off"][1]))
+    log(f"step by step {res['off'][0]:.2f} ms ({glups['off']:.2f} GLUPS), captured blocks {res['on'][0]:.2f} ms "
+        f"({glups['on']:.2f} GLUPS, {res['on'][2]} blocks), identical: {same}")
+    out = {
+        "metric": "GLUPS", "value": round(glups["on"], 3), "unit": "GLUPS", "n_gpus": 1, "steps": d.nt,
+        "warmup": args.warmup, "ms_per_step": round(res["on"][0] / d.nt, 5), "higher_is_better": True,
+        "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: homogeneous model (workloads.config1)",
+        "config": {"workload": "elastic FD4 400x400 NT=2000, CPML, 100 receivers", "nx": d.nx, "ny": d.ny,
+                   "nt": d.nt, "fdorder": d.fdorder, "weq": "EL_ISO"},
+        "step_by_step_glups": round(glups["off"], 3), "step_by_step_ms_per_step": round(res["off"][0] / d.nt, 5),
+        "graph_speedup": round(glups["on"] / glups["off"], 3), "results_identical": same,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,9 +392,9 @@ def main():
                     help="shape of the update kernels: auto, cell, vec2, vec4, or VEL:STRESS for each "
                          "(default: chosen by grid size)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3-vti", "c3-tti", "c3-iso", "c4"),
-                    help="c2: the headline single-shot run (default); c3-*: 64 batched anisotropic shots; "
-                         "c4: served requests of mixed sizes")
+    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3-vti", "c3-tti", "c3-iso", "c4"),
+                    help="c2: the headline single-shot run (default); c1: the small elastic case; c3-*: 64 "
+                         "batched anisotropic shots; c4: served requests of mixed sizes")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched workloads")
     ap.add_argument("--single", type=int, default=4, help="shots timed one at a time for the comparison")
     ap.add_argument("--grid", default="2000x1000", help="NXxNY of the batched workloads")
@@ -368,6 +417,9 @@ def main():
         return
     if args.workload == "c4":
         bench_serve(args)
+        return
+    if args.workload == "c1":
+        bench_small(args)
         return
     if args.steps > args.nt:
         args.nt = args.steps

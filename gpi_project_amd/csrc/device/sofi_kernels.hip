@@ -23,6 +23,8 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -257,7 +259,14 @@ struct Fuse {
     // stress kernel: u / DT of the EL_ISO update into ux/uy/uxy
     int fwi_acc, fwi_u;
     float *pvxp1, *pvyp1, *ux, *uy, *uxy;
+    const int *ntb;     // inside a captured block of steps: the block's base step (nt is then local)
 };
+
+// the time step of a launch: nt, or inside a captured block of steps the
+// block's local step nt plus its base step, which the device holds so that
+// one captured block serves every block of the run (the update kernels read
+// it only in the threads that inject a source or flag an instability)
+__device__ __forceinline__ int step_of(const int *ntb, int nt) { return ntb ? nt + *ntb : nt; }
 
 // Sources grouped by cell, for injection one thread per distinct cell; a
 // cell's sources are added in source order, as the reference's loop does.
@@ -295,10 +304,11 @@ __device__ __forceinline__ float psource_amp(const Grid &g, const Src &s, int l,
 
 // force sources when they are not fused: one thread per source cell (cell
 // offsets count the slots of a batch; the medium is shared by the slots)
-__global__ void k_force(Grid g, Fields f, Mat m, Src s, Inj inj, int nt)
+__global__ void k_force(Grid g, Fields f, Mat m, Src s, Inj inj, int nt, const int *ntb)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= inj.n) return;
+    nt = step_of(ntb, nt);
     const int c = inj.cell[k], cm = c % g.plane;
     float vx = f.vx[c], vy = f.vy[c];
     const float rip = m.rip[cm], rjp = m.rjp[cm];
@@ -308,10 +318,11 @@ __global__ void k_force(Grid g, Fields f, Mat m, Src s, Inj inj, int nt)
 }
 
 // explosive sources when they are not fused: one thread per source cell
-__global__ void k_psource(Grid g, Fields f, Src s, Inj inj, int nt)
+__global__ void k_psource(Grid g, Fields f, Src s, Inj inj, int nt, const int *ntb)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= inj.n) return;
+    nt = step_of(ntb, nt);
     const int c = inj.cell[k];
     float sxx = f.sxx[c], syy = f.syy[c];
     for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) {
@@ -342,9 +353,10 @@ __global__ void k_adjoint(Grid g, Fields f, Inj inj, const float *sx, const floa
 
 // eqsource (src/eqsource.c:25-52): double-couple moment tensor; the
 // sources listed (their updates touch neighbouring cells), one thread in order
-__global__ void k_eqsource(Grid g, Fields f, Src s, const int *list, int n, int nt)
+__global__ void k_eqsource(Grid g, Fields f, Src s, const int *list, int n, int nt, const int *ntb)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    nt = step_of(ntb, nt);
     for (int q = 0; q < n; ++q) {
         const int l = list[q];
         const int i = s.i[l], j = s.j[l];
@@ -486,7 +498,7 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
                                               float syy_y, float rip, float rjp, float &vx, float &vy)
 {
     // instability guard of time_loop (src/time_loop.c:44-47), before this step's update
-    if (i == g.nx / 2 && j == g.ny / 2 && isnan(vy) && *f.unstable == 0) *f.unstable = nt;
+    if (i == g.nx / 2 && j == g.ny / 2 && isnan(vy) && *f.unstable == 0) *f.unstable = step_of(fz.ntb, nt);
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
     if (frame && g.abs_type != 1 && g.abs_type != 2) return;
     if (g.abs_type == 1 && frame) {
@@ -511,7 +523,7 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
         }
     }
     for (int k = 0; k < fz.nforce; ++k)
-        if (c == fz.c[k]) force_add(g, s, k, nt, rip, rjp, vx, vy);
+        if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
     if (fz.fwi_acc == 1) {
         fz.pvxp1[c] = rip * (sxx_x + sxy_y) / g.dh;
         fz.pvyp1[c] = rjp * (sxy_x + syy_y) / g.dh;
@@ -532,8 +544,11 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
 }
 
 // one thread per cell; blockIdx.y picks g.spt shot slots of a batch, which
-// the thread updates in turn with the cell's coefficients read once
-template <int ND>
+// the thread updates in turn with the cell's coefficients read once (ONE: the
+// single slot of a one-shot handle, without the slot loop, whose run-time
+// trip count and per-slot offsets cost the one-shot kernels 10-45 % on
+// MI355X)
+template <int ND, bool ONE>
 __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
 {
     const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
@@ -542,9 +557,9 @@ __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, S
     if (i > g.nx || j > g.ny) return;
     const int c = IDX(g, i, j);
     const float rip = m.rip[c], rjp = m.rjp[c];
-    const int k0 = g.k_lo + blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.k_hi);
+    const int k0 = ONE ? 0 : g.k_lo + blockIdx.y * g.spt, k1 = ONE ? 1 : min(k0 + g.spt, g.k_hi);
     for (int k = k0; k < k1; ++k) {
-        Fields f = f0.at(k, g);
+        Fields f = ONE ? f0 : f0.at(k, g);
         float sxx_x, sxy_x, sxy_y, syy_y;
         op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
         float vx = f.vx[c], vy = f.vy[c];
@@ -752,7 +767,7 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
     }
     for (int k = 0; k < fz.npsrc; ++k)
         if (c == fz.c[k]) {
-            const float amp = psource_amp(g, s, k, nt);
+            const float amp = psource_amp(g, s, k, step_of(fz.ntb, nt));
             sxx += amp;
             syy += amp;
         }
@@ -770,8 +785,9 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
 // one thread per cell; the velocity derivatives are recomputed from vx, vy
 // (replaces v_derivatives + exchange_v) (src/update_s_*_{interior,PML,abs}.c).
 // blockIdx.y picks g.spt shot slots of a batch, updated in turn with the
-// cell's material coefficients read once.
-template <int ND, int WEQ>
+// cell's material coefficients read once (ONE: a one-shot handle's single
+// slot, as in k_velocity).
+template <int ND, int WEQ, bool ONE>
 __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
 {
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
@@ -805,9 +821,9 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
         c3 = m.u33[c];
         c4 = m.u55i[c];
     }
-    const int k0 = g.k_lo + blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.k_hi);
+    const int k0 = ONE ? 0 : g.k_lo + blockIdx.y * g.spt, k1 = ONE ? 1 : min(k0 + g.spt, g.k_hi);
     for (int k = k0; k < k1; ++k) {
-        Fields f = f0.at(k, g);
+        Fields f = ONE ? f0 : f0.at(k, g);
         float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
 #pragma unroll
         for (int mm = 1; mm <= ND; ++mm) {
@@ -1154,10 +1170,11 @@ struct Rec {
 // seismo_ssg div/curl (src/seismo_ssg.c:51-58): the post-CPML derivatives;
 // runs right after the stress update, before the free surface touches psi.
 template <int ND, bool TTI>
-__global__ void k_seis_dv(Grid g, Fields f0, Mat m, Rec r, int ins)
+__global__ void k_seis_dv(Grid g, Fields f0, Mat m, Rec r, int ins, const int *ntb, int ndt)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
+    if (ntb) ins = (ins + *ntb) / ndt;      // ins: the block's local step
     const Fields f = f0.at(r.shot[k], g);
     const int i = r.i[k], j = r.j[k], c = IDX(g, i, j);
     float vxx, vyx, vxy, vyy;
@@ -1177,10 +1194,11 @@ __global__ void k_seis_dv(Grid g, Fields f0, Mat m, Rec r, int ins)
 }
 
 // seismo_ssg vx / vy / p (src/seismo_ssg.c:40-64)
-__global__ void k_seis(Grid g, Fields f0, Rec r, int ins, int seismo)
+__global__ void k_seis(Grid g, Fields f0, Rec r, int ins, int seismo, const int *ntb, int ndt)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
+    if (ntb) ins = (ins + *ntb) / ndt;      // ins: the block's local step
     const Fields f = f0.at(r.shot[k], g);
     const int c = IDX(g, r.i[k], r.j[k]);
     float *t = r.sec + r.off[k] + (ins - 1);
@@ -1388,6 +1406,18 @@ struct sofi_dev {
     int *adj_inj_buf = nullptr;
     float *adj_sx = nullptr, *adj_sy = nullptr;
     int nadj = 0;
+    // captured blocks of time steps: one graph of graph_k steps whose kernels
+    // read the block's base step from ntb, replayed for every aligned block
+    int *ntb = nullptr;
+    int graph_mode = SOFI_GRAPH_AUTO, graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t gbuild = nullptr;           // while a block is recorded: the graph its launches go to
+    hipGraphNode_t glast = nullptr;        // the last node recorded (each node follows it)
+    hipError_t gerr = hipSuccess;
+    int graph_k = 0;
+    unsigned gen = 1, graph_gen = 0;       // launch configuration: current / of the captured graph
+    long long graph_blocks = 0;            // blocks replayed since the handle opened
 };
 
 namespace {
@@ -1568,6 +1598,7 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     rc |= dalloc_t(h, &f.syy, pl * nb);
     rc |= dalloc_t(h, &f.sxy, pl * nb);
     rc |= dalloc_t(h, &f.unstable, nb);
+    rc |= dalloc_t(h, &h->ntb, 1);
     if (visco) {
         rc |= dalloc_t(h, &f.r, pl * g.L * nb);
         rc |= dalloc_t(h, &f.p, pl * g.L * nb);
@@ -1772,6 +1803,7 @@ int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const 
     h->h_ppi.assign(arrays[SOFI_A_PPI], arrays[SOFI_A_PPI] + pl);
     h->h_pu.assign(arrays[SOFI_A_PU], arrays[SOFI_A_PU] + pl);
     h->medium_set = true;
+    h->gen++;
     return 0;
 }
 
@@ -1914,6 +1946,7 @@ int sync_acq(sofi_dev *h)
         if (h->fusable)
             for (int l = 0; l < n; ++l) h->fz.c[l] = (int)sofi_idx(g.pitch, ci[l], cj[l]);
         h->src_dirty = false;
+        h->gen++;
     }
     if (h->rec_dirty) {
         std::vector<int> ri, rj, rs;
@@ -1975,6 +2008,7 @@ int sync_acq(sofi_dev *h)
         r.n = n;
         r.ns = d.ns;
         h->rec_dirty = false;
+        h->gen++;
     }
     return 0;
 }
@@ -2047,8 +2081,47 @@ int vec_width(const Grid &g, int shape)
     return (g.weq == SOFI_EL_ISO || g.weq == SOFI_EL_VTI) ? 2 : 1;
 }
 
+// One time step.  With ntb set the step is captured into a block: nt is the
+// block's local step (1..K), the kernels add the base step held at ntb, and
+// the host-side sampling state is advanced for the whole block by the
+// caller; such a block starts on a sampling boundary, so the local step has
+// the sampling phase of the global one.
+// A launch of the time loop: on the handle's stream, or, while a block of
+// steps is recorded, a kernel node of that block's graph after the previous
+// one (graphs are built node by node, not by stream capture, so other
+// threads' synchronous HIP calls cannot invalidate a recording)
+template <typename... P, typename... A>
+void launch(sofi_dev *h, void (*k)(P...), dim3 grid, dim3 block, A &&...a)
+{
+    std::tuple<std::decay_t<P>...> v(std::forward<A>(a)...);
+    void *args[sizeof...(P) > 0 ? sizeof...(P) : 1];
+    std::apply([&](auto &...x) {
+        int q = 0;
+        ((args[q++] = (void *)&x), ...);
+    }, v);
+    if (!h->gbuild) {
+        const hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(k), grid, block, args, 0, h->stream);
+        if (e != hipSuccess && h->gerr == hipSuccess) h->gerr = e;
+        return;
+    }
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void *>(k);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = 0;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    hipGraphNode_t n = nullptr;
+    const hipError_t e = hipGraphAddKernelNode(&n, h->gbuild, h->glast ? &h->glast : nullptr, h->glast ? 1 : 0, &p);
+    if (e != hipSuccess) {
+        if (h->gerr == hipSuccess) h->gerr = e;
+    } else {
+        h->glast = n;
+    }
+}
+
 template <int ND>
-int step_nd(sofi_dev *h, int nt)
+int step_nd(sofi_dev *h, int nt, const int *ntb)
 {
     const Grid &g = h->g;
     Fields &f = h->f;
@@ -2061,12 +2134,14 @@ int step_nd(sofi_dev *h, int nt)
     // time-reversed forward step was stored (src/time_loop.c:287-303); it
     // records no seismograms and takes no snapshots
     const bool fwi = d.mode == SOFI_MODE_FWI, adj = fwi && h->sw == 1;
-    const int dtinv = d.dtinv > 0 ? d.dtinv : 1, fstep = d.nt - nt + 1;
+    const bool inblk = ntb != nullptr;
+    if (inblk && fwi) return fail("internal: FWI steps are not captured");
+    const int dtinv = d.dtinv > 0 ? d.dtinv : 1, fstep = d.nt - nt + 1, ndt = d.ndt > 0 ? d.ndt : 1;
     const bool store = fwi && !adj && (nt - 1) % dtinv == 0;
     const bool conv = adj && (fstep - 1) % dtinv == 0;
-    const bool sample = !adj && d.seismo && nt == h->lsamp && nt < d.nt && h->r.n > 0;
+    const bool sample = !adj && d.seismo && h->r.n > 0 && (inblk ? nt % ndt == 0 : (nt == h->lsamp && nt < d.nt));
     const bool sample_dv = sample && (d.seismo == 3 || d.seismo == 4);
-    const int ins = h->lsamp / (d.ndt > 0 ? d.ndt : 1);
+    const int ins = inblk ? nt : h->lsamp / ndt;       // in a block: the local step (k_seis adds the base)
     const int nsrc = adj ? 0 : h->s.n;     // the shot's sources act in the forward pass only
     // what the update kernels do besides the update: forces before the
     // velocity update; explosive sources and the free surface after the
@@ -2091,13 +2166,15 @@ int step_nd(sofi_dev *h, int nt)
     fv.ux = fs.ux = h->w.ux;
     fv.uy = fs.uy = h->w.uy;
     fv.uxy = fs.uxy = h->w.uxy;
+    fv.ntb = fs.ntb = ntb;
     if (nsrc > 0 && h->any_force && !fv.nforce)
-        hipLaunchKernelGGL(k_force, dim3((h->src_inj.n + 63) / 64), dim3(64), 0, st, g, f, m, h->s, h->src_inj, nt);
+        launch(h, k_force, dim3((h->src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, h->src_inj, nt,
+                           ntb);
     if (adj && h->nadj > 0)
-        hipLaunchKernelGGL(k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), 0, st, g, f, h->adj_inj, h->adj_sx,
+        launch(h, k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), g, f, h->adj_inj, h->adj_sx,
                            h->adj_sy, d.nt, d.adjoint_type, nt);
     hipEvent_t *te = nullptr;
-    if (h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
+    if (!inblk && h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
         te = &h->tev[h->tev_used];
         h->tev_used += 4;
     }
@@ -2114,9 +2191,10 @@ int step_nd(sofi_dev *h, int nt)
         const int nchunk = (gg.k_hi - lo + g.spt - 1) / g.spt;   // groups of g.spt slots (blockIdx.y)
         int vw = h->nb > 1 ? 1 : vec_width(g, h->shape_v);
         dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
-        if (vw == 4) hipLaunchKernelGGL((k_velocity_v<ND, 4>), tiles, blk, 0, st, gg, f, m, h->s, fv, nt);
-        else if (vw == 2) hipLaunchKernelGGL((k_velocity_v<ND, 2>), tiles, blk, 0, st, gg, f, m, h->s, fv, nt);
-        else hipLaunchKernelGGL(k_velocity<ND>, tiles, blk, 0, st, gg, f, m, h->s, fv, nt);
+        if (vw == 4) launch(h, k_velocity_v<ND, 4>, tiles, blk, gg, f, m, h->s, fv, nt);
+        else if (vw == 2) launch(h, k_velocity_v<ND, 2>, tiles, blk, gg, f, m, h->s, fv, nt);
+        else if (h->nb == 1) launch(h, k_velocity<ND, true>, tiles, blk, gg, f, m, h->s, fv, nt);
+        else launch(h, k_velocity<ND, false>, tiles, blk, gg, f, m, h->s, fv, nt);
         vw = h->nb > 1 ? 1 : vec_width(g, h->shape_s);
         tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
         if (te && lo == 0 && grp == h->nb) {
@@ -2125,9 +2203,10 @@ int step_nd(sofi_dev *h, int nt)
         }
 #define SOFI_STRESS(W)                                                                                      \
         do {                                                                                                \
-            if (vw == 4) hipLaunchKernelGGL((k_stress_v<ND, W, 4>), tiles, blk, 0, st, gg, f, m, h->s, fs, nt); \
-            else if (vw == 2) hipLaunchKernelGGL((k_stress_v<ND, W, 2>), tiles, blk, 0, st, gg, f, m, h->s, fs, nt); \
-            else hipLaunchKernelGGL((k_stress<ND, W>), tiles, blk, 0, st, gg, f, m, h->s, fs, nt);            \
+            if (vw == 4) launch(h, k_stress_v<ND, W, 4>, tiles, blk, gg, f, m, h->s, fs, nt); \
+            else if (vw == 2) launch(h, k_stress_v<ND, W, 2>, tiles, blk, gg, f, m, h->s, fs, nt); \
+            else if (h->nb == 1) launch(h, k_stress<ND, W, true>, tiles, blk, gg, f, m, h->s, fs, nt);  \
+            else launch(h, k_stress<ND, W, false>, tiles, blk, gg, f, m, h->s, fs, nt);            \
         } while (0)
         switch (d.weq) {
           case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
@@ -2136,20 +2215,20 @@ int step_nd(sofi_dev *h, int nt)
           case SOFI_VEL_VTI: SOFI_STRESS(SOFI_VEL_VTI); break;
           default: {
               dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, gg.k_hi - lo);
-              hipLaunchKernelGGL(k_dv<ND>, tg, tb, 0, st, gg, f);
+              launch(h, k_dv<ND>, tg, tb, gg, f);
               const bool el = d.weq == SOFI_EL_TTI;
-              if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 0>), tg, tb, 0, st, gg, f, m);
-              else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 0>), tg, tb, 0, st, gg, f, m);
+              if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 0>, tg, tb, gg, f, m);
+              else launch(h, k_stress_tti<SOFI_VEL_TTI, 0>, tg, tb, gg, f, m);
               if (d.fw > 0 && d.abs_type == 1) {
-                  hipLaunchKernelGGL(k_cpml_dv_frame<false>, tg, tb, 0, st, gg, f, m);
-                  if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 1>), tg, tb, 0, st, gg, f, m);
-                  else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 1>), tg, tb, 0, st, gg, f, m);
-                  hipLaunchKernelGGL(k_cpml_dv_frame<true>, tg, tb, 0, st, gg, f, m);
-                  if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 3>), tg, tb, 0, st, gg, f, m);
-                  else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 3>), tg, tb, 0, st, gg, f, m);
+                  launch(h, k_cpml_dv_frame<false>, tg, tb, gg, f, m);
+                  if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 1>, tg, tb, gg, f, m);
+                  else launch(h, k_stress_tti<SOFI_VEL_TTI, 1>, tg, tb, gg, f, m);
+                  launch(h, k_cpml_dv_frame<true>, tg, tb, gg, f, m);
+                  if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 3>, tg, tb, gg, f, m);
+                  else launch(h, k_stress_tti<SOFI_VEL_TTI, 3>, tg, tb, gg, f, m);
               } else if (d.fw > 0 && d.abs_type == 2) {
-                  if (el) hipLaunchKernelGGL((k_stress_tti<SOFI_EL_TTI, 2>), tg, tb, 0, st, gg, f, m);
-                  else hipLaunchKernelGGL((k_stress_tti<SOFI_VEL_TTI, 2>), tg, tb, 0, st, gg, f, m);
+                  if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 2>, tg, tb, gg, f, m);
+                  else launch(h, k_stress_tti<SOFI_VEL_TTI, 2>, tg, tb, gg, f, m);
               }
           }
         }
@@ -2162,34 +2241,36 @@ int step_nd(sofi_dev *h, int nt)
     if (te) (void)hipEventRecord(te[3], st);
     if (sample_dv) {
         dim3 rb(64), rg((h->r.n + 63) / 64);
-        if (tti) hipLaunchKernelGGL((k_seis_dv<ND, true>), rg, rb, 0, st, g, f, m, h->r, ins);
-        else hipLaunchKernelGGL((k_seis_dv<ND, false>), rg, rb, 0, st, g, f, m, h->r, ins);
+        if (tti) launch(h, k_seis_dv<ND, true>, rg, rb, g, f, m, h->r, ins, ntb, ndt);
+        else launch(h, k_seis_dv<ND, false>, rg, rb, g, f, m, h->r, ins, ntb, ndt);
     }
     if (nsrc > 0 && !src_fused) {
         if (h->psrc_inj.n > 0)
-            hipLaunchKernelGGL(k_psource, dim3((h->psrc_inj.n + 63) / 64), dim3(64), 0, st, g, f, h->s, h->psrc_inj, nt);
-        if (h->n_eq > 0) hipLaunchKernelGGL(k_eqsource, dim3(1), dim3(64), 0, st, g, f, h->s, h->eq_list, h->n_eq, nt);
+            launch(h, k_psource, dim3((h->psrc_inj.n + 63) / 64), dim3(64), g, f, h->s, h->psrc_inj, nt,
+                               ntb);
+        if (h->n_eq > 0)
+            launch(h, k_eqsource, dim3(1), dim3(64), g, f, h->s, h->eq_list, h->n_eq, nt, ntb);
     }
     if (d.free_surf && !fs.surf) {
         dim3 sb(256), sg((g.nx + 255) / 256, h->nb);
-        if (g.L > 0) hipLaunchKernelGGL((k_surface<ND, true>), sg, sb, 0, st, g, f, m);
-        else hipLaunchKernelGGL((k_surface<ND, false>), sg, sb, 0, st, g, f, m);
+        if (g.L > 0) launch(h, k_surface<ND, true>, sg, sb, g, f, m);
+        else launch(h, k_surface<ND, false>, sg, sb, g, f, m);
     }
     if (sample && d.seismo != 3) {
         dim3 rb(64), rg((h->r.n + 63) / 64);
-        hipLaunchKernelGGL(k_seis, rg, rb, 0, st, g, f, h->r, ins, d.seismo);
+        launch(h, k_seis, rg, rb, g, f, h->r, ins, d.seismo, ntb, ndt);
     }
-    if (d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
+    if (!inblk && d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
     if (store || conv) {
         dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
-        if (store) hipLaunchKernelGGL(k_fwi_store, tg, tb, 0, st, g, f, h->w, (nt - 1) / dtinv);
-        else hipLaunchKernelGGL(k_fwi_conv, tg, tb, 0, st, g, f, m, h->w, (fstep - 1) / dtinv);
+        if (store) launch(h, k_fwi_store, tg, tb, g, f, h->w, (nt - 1) / dtinv);
+        else launch(h, k_fwi_conv, tg, tb, g, f, m, h->w, (fstep - 1) / dtinv);
     }
-    if (!adj && d.snap && nt == h->lsnap && nt <= d.esnap) {
+    if (!inblk && !adj && d.snap && nt == h->lsnap && nt <= d.esnap) {
         if (h->nsnap < h->nsnap_max) {
             int n = h->nxs * h->nys;
             float *out = h->snapbuf + (size_t)h->nsnap * SOFI_NCOMP * n;
-            hipLaunchKernelGGL(k_snap<ND>, dim3((n + 255) / 256), dim3(256), 0, st, g, f, m, out, d.snap_idx[1],
+            launch(h, k_snap<ND>, dim3((n + 255) / 256), dim3(256), g, f, m, out, d.snap_idx[1],
                                d.idx, h->nxs, d.snap_idx[3], d.idy, h->nys);
             h->nsnap++;
         }
@@ -2198,17 +2279,94 @@ int step_nd(sofi_dev *h, int nt)
     return 0;
 }
 
-int step(sofi_dev *h, int nt)
+int step(sofi_dev *h, int nt, const int *ntb = nullptr)
 {
     switch (h->g.nd) {
-      case 1: return step_nd<1>(h, nt);
-      case 2: return step_nd<2>(h, nt);
-      case 3: return step_nd<3>(h, nt);
-      case 4: return step_nd<4>(h, nt);
-      case 5: return step_nd<5>(h, nt);
-      case 6: return step_nd<6>(h, nt);
+      case 1: return step_nd<1>(h, nt, ntb);
+      case 2: return step_nd<2>(h, nt, ntb);
+      case 3: return step_nd<3>(h, nt, ntb);
+      case 4: return step_nd<4>(h, nt, ntb);
+      case 5: return step_nd<5>(h, nt, ntb);
+      case 6: return step_nd<6>(h, nt, ntb);
     }
     return fail("unsupported ND %d", h->g.nd);
+}
+
+constexpr size_t GRAPH_AUTO_CELLS = size_t(1) << 20;
+constexpr int GRAPH_STEPS = 64;
+
+// steps of a captured block for this handle (0: run step by step)
+int graph_len(const sofi_dev *h)
+{
+    const sofi_desc &d = h->d;
+    if (h->graph_mode == SOFI_GRAPH_OFF || d.mode == SOFI_MODE_FWI || h->time_every > 0) return 0;
+    if (h->graph_mode == SOFI_GRAPH_AUTO && (size_t)d.nx * d.ny * h->nb > GRAPH_AUTO_CELLS) return 0;
+    const int ndt = d.seismo && d.ndt > 0 ? d.ndt : 1, want = h->graph_steps > 0 ? h->graph_steps : GRAPH_STEPS;
+    return (want + ndt - 1) / ndt * ndt;
+}
+
+// a block of k steps may start at nt when it ends before nt_last and the
+// loop's last step, starts on a sampling boundary with the sampling state
+// there, and holds no snapshot
+bool block_fits(const sofi_dev *h, int k, int nt, int nt_last)
+{
+    const sofi_desc &d = h->d;
+    const int last = nt + k - 1, ndt = d.ndt > 0 ? d.ndt : 1;
+    if (k <= 0 || last > nt_last || last >= d.nt) return false;
+    if (d.seismo && ((nt - 1) % ndt != 0 || h->lsamp != nt - 1 + ndt)) return false;
+    if (d.snap && h->lsnap >= nt && h->lsnap <= last && h->lsnap <= d.esnap) return false;
+    return true;
+}
+
+__global__ void k_set_base(int *ntb, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ntb = v;
+}
+
+void graph_drop(sofi_dev *h)
+{
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->graph_k = 0;
+}
+
+// steps nt .. nt + k - 1 from the captured block (captured first when the
+// launch configuration changed since)
+int run_block(sofi_dev *h, int nt, int k)
+{
+    if (!h->gexec || h->graph_gen != h->gen || h->graph_k != k) {
+        graph_drop(h);
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipGraphCreate(&gr, 0));
+        h->gbuild = gr;
+        h->glast = nullptr;
+        h->gerr = hipSuccess;
+        int rc = 0;
+        for (int l = 1; l <= k && !rc; ++l) rc = step(h, l, h->ntb);
+        h->gbuild = nullptr;
+        const hipError_t e = h->gerr;
+        h->gerr = hipSuccess;
+        if (rc || e != hipSuccess) {
+            (void)hipGraphDestroy(gr);
+            return rc ? rc : fail("recording a block of steps failed: %s", hipGetErrorString(e));
+        }
+        h->graph = gr;
+        const hipError_t ei = hipGraphInstantiate(&h->gexec, gr, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            h->gexec = nullptr;
+            graph_drop(h);
+            return fail("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        }
+        h->graph_k = k;
+        h->graph_gen = h->gen;
+    }
+    hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, h->stream, h->ntb, nt - 1);
+    HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    if (h->d.seismo) h->lsamp += k;
+    h->graph_blocks++;
+    return 0;
 }
 
 } // namespace
@@ -2221,6 +2379,7 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     if (!h->medium_set) return fail("sofi_run: medium not set");
     if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
     if (sync_acq(h)) return -1;
+    h->gerr = hipSuccess;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     int nt = nt_first;
     // a batch's slots per thread: given, or measured on the first steps of a
@@ -2257,13 +2416,28 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
             h->grp = best_c.first;
             h->g.spt = best_c.second;
             h->spt_tuned = true;
+            h->gen++;
         } else {
             h->grp = h->grp_mode > 0 ? h->grp_mode : (h->nb < 4 ? h->nb : 4);
             h->g.spt = h->spt_mode > 0 ? h->spt_mode : (h->grp < 4 ? h->grp : 4);
+            h->gen++;
         }
     }
-    for (; nt <= nt_last; ++nt)
-        if (step(h, nt)) return -1;
+    const int k = graph_len(h);
+    while (nt <= nt_last) {
+        if (block_fits(h, k, nt, nt_last)) {
+            if (run_block(h, nt, k)) return -1;
+            nt += k;
+        } else {
+            if (step(h, nt)) return -1;
+            ++nt;
+        }
+    }
+    if (h->gerr != hipSuccess) {
+        const hipError_t e = h->gerr;
+        h->gerr = hipSuccess;
+        return fail("kernel launch failed: %s", hipGetErrorString(e));
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     HIPCHK(hipEventSynchronize(h->ev1));
@@ -2299,6 +2473,7 @@ int sofi_set_batch_launch(sofi_dev *h, int group, int spt)
     h->spt_tuned = false;
     h->grp = group;
     h->g.spt = spt > 0 ? spt : 1;
+    h->gen++;
     return 0;
 }
 
@@ -2358,6 +2533,7 @@ int sofi_close(sofi_dev *h)
 {
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    graph_drop(h);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     for (const auto &a : h->allocs) (void)hipFree(a.raw);
     for (hipEvent_t e : { h->ev0, h->ev1, h->tune0, h->tune1 })
@@ -2508,8 +2684,21 @@ int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape)
         if (sh < SOFI_LAUNCH_AUTO || sh > SOFI_LAUNCH_VEC4) return fail("sofi_set_launch: bad shape %d", sh);
     h->shape_v = velocity_shape;
     h->shape_s = stress_shape;
+    h->gen++;
     return 0;
 }
+
+int sofi_set_graph(sofi_dev *h, int mode, int steps)
+{
+    if (!h) return fail("sofi_set_graph: null handle");
+    if (mode < SOFI_GRAPH_AUTO || mode > SOFI_GRAPH_ON) return fail("sofi_set_graph: bad mode %d", mode);
+    if (steps < 0 || steps > 4096) return fail("sofi_set_graph: %d steps per block (0..4096)", steps);
+    h->graph_mode = mode;
+    h->graph_steps = steps;
+    return 0;
+}
+
+long long sofi_graph_blocks(const sofi_dev *h) { return h ? h->graph_blocks : 0; }
 
 int sofi_set_kernel_timing(sofi_dev *h, int every, int max_samples)
 {

@@ -275,6 +275,17 @@ class Simulation:
         st = shape if stress_shape is None else stress_shape
         _chk_hip(self._L.sofi_set_launch(self._h, self.LAUNCH[shape], self.LAUNCH[st]))
 
+    GRAPH = {"auto": 0, "off": 1, "on": 2}
+
+    def set_graph(self, mode: str = "auto", steps: int = 0):
+        """Captured blocks of `steps` time steps (HIP graphs): "auto" (grids
+        of at most 2^20 cells), "off" or "on"."""
+        _chk_hip(self._L.sofi_set_graph(self._h, self.GRAPH[mode], steps))
+
+    @property
+    def graph_blocks(self) -> int:
+        return int(self._L.sofi_graph_blocks(self._h))
+
     def set_kernel_timing(self, every: int, max_samples: int = 4096):
         _chk_hip(self._L.sofi_set_kernel_timing(self._h, every, max_samples))
 

@@ -147,6 +147,22 @@ int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *gr
 enum { SOFI_LAUNCH_AUTO = 0, SOFI_LAUNCH_CELL = 1, SOFI_LAUNCH_VEC2 = 2, SOFI_LAUNCH_VEC4 = 3 };
 int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape);
 
+/* Captured blocks of time steps.  The reference's time loop launches every
+ * step's updates one after the other (src/time_loop.c:40-325); here sofi_run
+ * captures `steps` consecutive steps (rounded up to a multiple of NDT; 0:
+ * 64) once into a HIP graph whose kernels read the block's base step from
+ * device memory, and replays it for every block that starts on a sampling
+ * boundary and holds no snapshot and not the loop's last step; the other
+ * steps run one by one.  SOFI_GRAPH_AUTO (the default) captures for grids of
+ * at most 2^20 cells (all slots of a batch together), where launch overhead
+ * is a large part of a step; FWI handles and timed runs
+ * (sofi_set_kernel_timing) always run step by step.  Results are identical
+ * with and without graphs. */
+enum { SOFI_GRAPH_AUTO = 0, SOFI_GRAPH_OFF = 1, SOFI_GRAPH_ON = 2 };
+int sofi_set_graph(sofi_dev *h, int mode, int steps);
+/* Blocks replayed from a captured graph since the handle was opened. */
+long long sofi_graph_blocks(const sofi_dev *h);
+
 /* Sampled kernel timing on the handle's stream: every `every`-th step of
  * the following sofi_run calls records HIP events around the velocity
  * kernel and around the stress-update kernels (at most max_samples steps);

@@ -1,0 +1,87 @@
+"""Captured blocks of time steps (HIP graphs whose kernels read the block's
+base step from device memory) against the same runs step by step: the
+seismograms, snapshots and final wavefields must agree bit for bit.  Every
+run is split into several sofi_run calls, so replays alternate with eager
+steps, host synchronisations and sampling-state updates, and the handle
+then runs a second shot (sources reloaded: the block is captured again) and
+a third without a reload (the captured block is reused)."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import cases
+from conftest import chdir
+
+pytestmark = pytest.mark.gpu
+
+SEGMENTS = ((1, 37), (38, 263), (264, None))
+
+
+def _run(P, mode, steps):
+    from gpi_project_amd.sofi import Simulation
+    sim = Simulation(P)
+    sim.set_graph(mode, steps)
+    out = []
+    for rep in range(3):
+        if rep < 2:
+            sim.load_shot()
+        else:
+            sim.reset()
+        for a, b in SEGMENTS:
+            sim.run(a, b)
+        r = {("sec", c): sim.section(c) for c in P.seismo_comps()}
+        r.update({("snap", c): sim.snapshots(c) for c in P.snap_comps()})
+        r.update({("field", k): sim.field(k) for k in range(5)})
+        out.append(r)
+    blocks = sim.graph_blocks
+    sim.close()
+    return out, blocks
+
+
+def _equal(a, b, tag):
+    for k in a:
+        assert np.array_equal(a[k], b[k]), (tag, k, float(np.abs(a[k] - b[k]).max()))
+
+
+@pytest.mark.parametrize("name", list(cases.CASES))
+def test_graph_blocks_equal_step_by_step(name):
+    from gpi_project_amd.sofi import Problem
+    td = tempfile.mkdtemp(prefix="sofi_graph_")
+    js, c = cases.build(name, td)
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        ref, nb0 = _run(P, "off", 0)
+        assert nb0 == 0
+        for steps in (10, 64):
+            got, nb = _run(P, "on", steps)
+            print(name, "steps per block", steps, "blocks replayed", nb)
+            assert nb > 0
+            for rep in range(3):
+                _equal(got[rep], ref[rep], (name, steps, rep))
+        P.close()
+
+
+def test_graph_blocks_in_a_batch():
+    from gpi_project_amd.sofi import Batch, Problem
+    td = tempfile.mkdtemp(prefix="sofi_graph_")
+    js = cases.multishot_case(td, "VEL_ISO", 4, 1, 1, 1)
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        res = {}
+        for mode in ("off", "on"):
+            B = Batch(P, 4)
+            B.set_launch_groups(2, 2)
+            B.set_graph(mode, 16)
+            B.load_shots([1, 2, 3, 4])
+            B.run(1, 100)
+            B.run(101)
+            res[mode] = [[B.section_of(k, c) for c in (0, 1, 2, 3, 4)] for k in range(4)]
+            res[mode + "_blocks"] = B.graph_blocks
+            B.close()
+        P.close()
+    assert res["off_blocks"] == 0 and res["on_blocks"] > 0
+    for k in range(4):
+        for c in range(5):
+            assert np.array_equal(res["on"][k][c], res["off"][k][c]), (k, c)

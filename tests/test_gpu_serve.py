@@ -49,6 +49,7 @@ def test_served_results_equal_results_alone():
     srv = serve.Server(gpu_workers=4, block_steps=64)
     st = srv.serve(mix)
     print(st, srv.admit_order)
+    assert all(r.error is None for r in mix), [(r.rid, r.error) for r in mix]
     assert st["completed"] == len(SPECS) and srv.used == 0
     for a, b in zip(alone, mix):
         assert _same(a, b), a.rid
@@ -88,6 +89,7 @@ def test_served_request_vs_oracle():
     srv = serve.Server(gpu_workers=2)
     srv.serve(reqs)
     for r in reqs:
+        assert r.error is None, (r.rid, r.error)
         with chdir(os.path.dirname(r.json_path)):
             P = Problem(os.path.basename(r.json_path))
             o = pyoracle.run_problem(P, "f32")
