@@ -377,6 +377,117 @@ def bench_small(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_fwi(args):
+    """Elastic FWI gradient (workloads.config5: 1500 x 500 FD8, NT = 3000,
+    32 shots, 300 receivers, DTINV = 5): the measured data come from forward
+    runs on the true model; then one gradient iteration on the background
+    model, per shot a forward pass storing every 5th step, L2 residuals
+    read from the data files, an adjoint pass correlating with the store and
+    the gradient.  Shot 1 is repeated with DTINV = 1 (the reference's full
+    rate; a 45 GB store) for the direction of the DTINV = 5 gradient.
+    Prints one JSON line (gradient shots/s)."""
+    import numpy as np
+    from gpi_project_amd import fwi, workloads
+    from gpi_project_amd.sofi import Problem, Simulation
+    td = tempfile.mkdtemp(prefix="sofi_fwi_")
+    cwd = os.getcwd()
+    os.chdir(td)
+    js = workloads.config5(td, "true", nshots=args.fwi_shots)
+    P = Problem(os.path.basename(js))
+    d = P.desc
+    sim = Simulation(P)
+    t0 = time.perf_counter()
+    for ishot in range(1, P.nshots + 1):
+        P.set_shot(ishot)
+        sim.load_shot(nsnap=0)
+        sim.run()
+        for c in P.seismo_comps():
+            P.write_section(c, sim.section(c))
+    sim.close()
+    P.close()
+    log(f"measured data: {d.nx}x{d.ny} NT={d.nt}, {args.fwi_shots} shots in {time.perf_counter() - t0:.1f} s")
+    js = workloads.config5(td, "init", nshots=args.fwi_shots)
+    P = Problem(os.path.basename(js))
+    d = P.desc
+    sim = Simulation(P)
+    sim.fwi_set_model(P.model(0), P.model(1), P.model(2))
+    fwi.shot_gradient(P, sim, 1, write=False)              # warm-up (code objects, store pages)
+    sim.fwi_gradient_sum(reset=True)
+    t0 = time.perf_counter()
+    dev_ms, l2 = 0.0, 0.0
+    for ishot in range(1, P.nshots + 1):
+        r = fwi.shot_gradient(P, sim, ishot, write=False)
+        dev_ms += r["ms_forward"] + r["ms_adjoint"]
+        l2 += r["l2"]
+    t_it = time.perf_counter() - t0
+    gsum = sim.fwi_gradient_sum()
+    fp = sim.footprint
+    g5 = fwi.shot_gradient(P, sim, 1, write=False)["gradient"]
+    sim.close()
+    finite = all(np.isfinite(g).all() for g in gsum)
+    # shot 1 at the full rate
+    d1 = d.copy()
+    d1.dtinv = 1
+    sim1 = Simulation(P, desc=d1)
+    sim1.fwi_set_model(P.model(0), P.model(1), P.model(2))
+    r1 = fwi.shot_gradient(P, sim1, 1, write=False)
+    sim1.close()
+    cos = [float(np.dot(a.ravel().astype(np.float64), b.ravel()) /
+                 (np.linalg.norm(a.astype(np.float64)) * np.linalg.norm(b.astype(np.float64)) + 1e-300))
+           for a, b in zip(g5, r1["gradient"])]
+    log(f"gradient: {P.nshots} shots in {t_it:.2f} s ({P.nshots / t_it:.2f} shots/s, device {dev_ms / 1e3:.2f} s), "
+        f"L2 {l2:.4e}, finite {finite}; DTINV=5 vs 1 cosine vp/vs/rho {cos}, DTINV=1 shot "
+        f"{(r1['ms_forward'] + r1['ms_adjoint']):.0f} ms")
+    cells = d.nx * d.ny
+    # CPU: the FP32 oracle's forward (storing) and adjoint (correlating) steps on the same shot, 40 each
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        n = 40
+        ds = d.copy()
+        ds.nt = n
+        P.set_shot(1)
+        o = pyoracle.Oracle(P, "f32", desc=ds)
+        o.fwi_set_model(P.model(0), P.model(1), P.model(2))
+        ri, rj = P.receivers()
+        o.fwi_set_adjoint(ri, rj, np.zeros((len(ri), n), np.float32), np.zeros((len(ri), n), np.float32))
+        tc = time.perf_counter()
+        o.reset(0)
+        o.run()
+        o.reset(1)
+        o.run()
+        tc = time.perf_counter() - tc
+        o.close()
+        cpu = {"value": round(1.0 / (tc / n * d.nt), 6), "unit": "gradient shots/s", "cores": 1, "kind": "port",
+               "sample": f"{n} forward (storing) + {n} adjoint (correlating) steps of shot 1 on the FP32 CPU oracle, "
+                         f"scaled to NT = {d.nt}"}
+    P.close()
+    os.chdir(cwd)
+    shutil.rmtree(td, ignore_errors=True)
+    # per cell and step: forward 80 B + store 20 B every DTINV steps; adjoint 80 B + read 20 B and
+    # gradient accumulators (6 floats read + written, the correlation inputs) every DTINV steps
+    byts = cells * d.nt * (2 * 80 + (20 + 20 + 48) / d.dtinv) * P.nshots
+    out = {
+        "metric": "FWI gradient shots/s", "value": round(P.nshots / t_it, 4), "unit": "shots/s", "n_gpus": 1,
+        "steps": 2 * d.nt * P.nshots, "warmup": 1, "ms_per_step": round(t_it * 1e3 / (2 * d.nt * P.nshots), 5),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: linear-gradient background with a seeded Gaussian anomaly (workloads.config5)",
+        "config": {"workload": f"elastic FWI gradient, {d.nx}x{d.ny} FD8, NT={d.nt}, {P.nshots} shots, "
+                               f"300 receivers, DTINV={d.dtinv}", "nx": d.nx, "ny": d.ny, "nt": d.nt,
+                   "shots": P.nshots, "dtinv": d.dtinv},
+        "device_s": round(dev_ms / 1e3, 3), "iteration_s": round(t_it, 3),
+        "glups_equivalent": round(2 * cells * d.nt * P.nshots / t_it / 1e9, 3),
+        "footprint_bytes": fp, "cosine_dtinv5_vs_dtinv1": [round(c, 6) for c in cos],
+        "dtinv1_shot_ms": round(r1["ms_forward"] + r1["ms_adjoint"], 1), "gradient_finite": finite,
+        "roofline": {"bound": "hbm", "achieved": round(byts / (dev_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(byts / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "forward + adjoint passes (all kernels, device time)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,13 +503,14 @@ def main():
                     help="shape of the update kernels: auto, cell, vec2, vec4, or VEL:STRESS for each "
                          "(default: chosen by grid size)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
-    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3-vti", "c3-tti", "c3-iso", "c4"),
+    ap.add_argument("--workload", default="c2", choices=("c1", "c2", "c3-vti", "c3-tti", "c3-iso", "c4", "c5"),
                     help="c2: the headline single-shot run (default); c1: the small elastic case; c3-*: 64 "
-                         "batched anisotropic shots; c4: served requests of mixed sizes")
+                         "batched anisotropic shots; c4: served requests of mixed sizes; c5: an elastic FWI gradient")
     ap.add_argument("--shots", type=int, default=64, help="shots of the batched workloads")
     ap.add_argument("--single", type=int, default=4, help="shots timed one at a time for the comparison")
     ap.add_argument("--grid", default="2000x1000", help="NXxNY of the batched workloads")
     ap.add_argument("--requests", type=int, default=48, help="requests of the served workload")
+    ap.add_argument("--fwi-shots", type=int, default=32, help="shots of the FWI workload")
     ap.add_argument("--workers", type=int, default=4, help="concurrent handles of the served workload")
     ap.add_argument("--pmc-child", metavar="CASE_DIR", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=20, help=argparse.SUPPRESS)
@@ -420,6 +532,9 @@ def main():
         return
     if args.workload == "c1":
         bench_small(args)
+        return
+    if args.workload == "c5":
+        bench_fwi(args)
         return
     if args.steps > args.nt:
         args.nt = args.steps

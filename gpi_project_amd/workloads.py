@@ -271,3 +271,57 @@ def config3(d: str, weq: str = "EL_VTI", nshots: int = 64, nx: int = 2000, ny: i
              sources=[(float(x), 25.0, 0.0, float(f), 1.0) for x, f in zip(xs, fcs)], seismo=1, ndt=4,
              rec_line=(x1, 12.5, x1 + 3 * dh * (ntr - 1), 12.5, 3.0), run_multiple_shots=1)
     return write_case(d, c, model)
+
+
+FWI_KEYS = {"ITERMAX": 1, "PARAMETERIZATION": 1, "INV_VP_ITER": 1, "INV_VS_ITER": 1, "INV_RHO_ITER": 1,
+            "TIME_FILT": 0, "F_HIGH_PASS": 1, "F_LOW_PASS_START": "10.0", "F_LOW_PASS_END": "75.0",
+            "F_LOW_PASS_INCR": "10.0", "ORDER": 8, "ZERO_PHASE": 0, "FREQ_FILE": "frequencies.dat",
+            "WRITE_FILTERED_DATA": 0, "MIN_ITER": 10, "USE_WORKFLOW": 0, "FILE_WORKFLOW": "workflow.txt",
+            "SWS_TAPER_GRAD_SOURCES": 0, "SRTRADIUS": 0, "SRTSHAPE": 0, "FILTSIZE": 0,
+            "INV_MODELFILE": "./grad/model", "NFSTART": 1, "NF": 1, "GRADIENT": "./grad/g", "NFSTART_GRAD": 1,
+            "NF_GRAD": 1, "GRAD_METHOD": 2, "LNORM": 2, "NORMALIZE": 0, "WRITE_DIFF": 0, "EPS_SCALE": "0.01",
+            "STEPMAX": 4, "SCALEFAC": "4.0", "TESTSHOT_START": 1, "TESTSHOT_END": 2, "TESTSHOT_INCR": 1,
+            "PRO": "0.01", "MISFIT_LOG_FILE": "./grad/L2_LOG.dat"}
+
+
+def config5(d: str, which: str, nshots: int = 32, nx: int = 1500, ny: int = 500, nt: int = 3000,
+            dtinv: int = 5) -> str:
+    """Elastic FWI: 1500 x 500, DH 5 m, FD8 (Holberg 0.1 %), DT 5e-4 s,
+    NT 3000, 20-point CPML all round.  Background vp 2000 -> 4000 m/s linear
+    in depth, vs = vp/1.73, rho = 310 vp^0.25; the true model adds a
+    Gaussian anomaly (sigma 200 m, +10 % vp and vs) centred at a point of
+    seed 6 (x in the middle half, depth in the middle 40 %).  nshots
+    explosive Ricker shots evenly spaced along x at 125 m depth, fc ~ U(8, 20)
+    Hz (seed 7); 300 receivers at 115 m depth.  which = "true": MODE=FW on
+    the true model writing the measured data ./data/obs_{vx,vy}.su.shot<n>;
+    "init": MODE=FWI (LNORM 2, VELOCITY 1, ADJOINT_TYPE 1, DTINV) on the
+    background reading them."""
+    dh, dt, fw = 5.0, 5.0e-4, 20
+    depth = np.arange(ny) * dh
+    vp = np.repeat((2000.0 + 2000.0 * depth / depth[-1])[None, :], nx, axis=0)
+    rng = np.random.Generator(np.random.PCG64(6))
+    cx = rng.uniform(0.25, 0.75) * nx * dh
+    cz = rng.uniform(0.3, 0.7) * ny * dh
+    x = (np.arange(nx) * dh - cx)[:, None]
+    z = (np.arange(ny) * dh - cz)[None, :]
+    bump = 1.0 + 0.10 * np.exp(-(x * x + z * z) / (2.0 * 200.0 ** 2))
+    v = vp * (bump if which == "true" else 1.0)
+    model = {"vp": v.astype(np.float32), "vs": (v / 1.73).astype(np.float32),
+             "rho": (310.0 * vp ** 0.25).astype(np.float32)}
+    fcs = np.random.Generator(np.random.PCG64(7)).uniform(8.0, 20.0, nshots)
+    x0, x1 = (fw + 10) * dh, (nx - fw - 11) * dh
+    xs = np.linspace(x0, x1, nshots)
+    srcs = [(float(xx) + 1.0, 125.0 + 1.0, 0.0, float(f), 1.0) for xx, f in zip(xs, fcs)]
+    recs = [(float(xx) + 1.0, 115.0 + 1.0) for xx in np.linspace((fw + 5) * dh, (nx - fw - 6) * dh, 300).round() // dh * dh]
+    fwi = which == "init"
+    extra = dict(FWI_KEYS, DATA_DIR="./data/obs", ADJOINT_TYPE=1, VELOCITY=1, DTINV=dtinv) if fwi else {}
+    c = Case(name=f"c5_{which}", nx=nx, ny=ny, dh=dh, dt=dt, time=nt * dt, weq="EL_ISO", fdorder=8, maxrelerror=1,
+             abs_type=1, fw=fw, free_surf=0, fpml=14.0, vppml=4000.0, source_type=1, sources=srcs, receivers=recs,
+             seismo=1, ndt=1, run_multiple_shots=1, mode="FWI" if fwi else "FW", extra=extra)
+    js = write_case(d, c, model)
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    os.makedirs(os.path.join(d, "grad"), exist_ok=True)
+    if not fwi:   # the measured data are this run's seismograms
+        txt = open(js).read().replace('"SEIS_FILE" : "./out/seis"', '"SEIS_FILE" : "./data/obs"')
+        open(js, "w").write(txt)
+    return js
