@@ -209,6 +209,7 @@ def bench_batch(args, weq: str):
     B.load_shots(list(range(1, P.nshots + 1)))
     ms_b = B.run()
     grp, spt = B.launch_groups
+    res = B.resident
     same = np.array_equal(B.section_of(0, 0), ref)
     B.close()
     P.close()
@@ -218,7 +219,8 @@ def bench_batch(args, weq: str):
     glups_b = nb * cells * d.nt / (ms_b * 1e-3) / 1e9
     p_s = {"EL_ISO": 3, "EL_VTI": 4}.get(weq, 8)
     bytes_b = 4 * (15 + (2 + p_s) / spt)
-    log(f"batched: {ms_b:.1f} ms for {nb} shots ({glups_b:.2f} GLUPS), {grp} slots per launch, {spt} per thread, "
+    log(f"batched: {ms_b:.1f} ms for {nb} shots ({glups_b:.2f} GLUPS), resident groups of {res}, {grp} slots per "
+        f"launch, {spt} per thread, "
         f"slot 0 == one-shot run: {same}")
     out = {
         "metric": "batched shots/s", "value": round(nb / (ms_b * 1e-3), 4), "unit": "shots/s", "n_gpus": 1,
@@ -227,7 +229,7 @@ def bench_batch(args, weq: str):
         "data": "synthetic: layered model with smooth random Thomsen fields (workloads.config3)",
         "config": {"workload": f"{nb} batched {weq} shots, {d.nx}x{d.ny} FD6, NT={d.nt}, {P_ntr} receivers each",
                    "nx": d.nx, "ny": d.ny, "nt": d.nt, "fdorder": d.fdorder, "weq": weq, "shots": nb,
-                   "slots_per_launch": grp, "slots_per_thread": spt},
+                   "slots_per_launch": grp, "slots_per_thread": spt, "resident_group": res},
         "one_at_a_time_shots_per_s": round(1e3 / ms_one, 4),
         "speedup_vs_one_at_a_time": round((nb / ms_b) / (1.0 / ms_one), 3),
         "batched_glups": round(glups_b, 3),

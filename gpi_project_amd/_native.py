@@ -140,6 +140,8 @@ def hip():
     L.sofi_get_kernel_timing.argtypes = [_VP, _DP, _DP, _IP]
     L.sofi_set_launch.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_set_graph.argtypes = [_VP, C.c_int, C.c_int]
+    L.sofi_set_batch_resident.argtypes = [_VP, C.c_int]
+    L.sofi_get_batch_resident.argtypes = [_VP]
     L.sofi_graph_blocks.restype = C.c_longlong
     L.sofi_graph_blocks.argtypes = [_VP]
     L.sofi_open_batch.argtypes = [C.POINTER(SofiDesc), C.c_int, C.POINTER(_VP)]
@@ -176,5 +178,6 @@ def exported_symbols():
                            "sofi_fwi_set_model", "sofi_fwi_set_pass", "sofi_fwi_set_adjoint",
                            "sofi_fwi_gradient", "sofi_fwi_gradient_sum", "sofi_open_batch", "sofi_select_shot",
                            "sofi_set_batch_launch", "sofi_get_batch_launch", "sofi_estimate_footprint",
-                           "sofi_set_graph", "sofi_graph_blocks"],
+                           "sofi_set_graph", "sofi_graph_blocks", "sofi_set_batch_resident",
+                           "sofi_get_batch_resident"],
     }

@@ -79,7 +79,7 @@ struct Fields {
     float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
     int *unstable;                             // per slot: first step whose start finds vy(NX/2, NY/2) = NaN
 
-    __device__ __forceinline__ Fields at(int k, const Grid &g) const
+    __host__ __device__ __forceinline__ Fields at(int k, const Grid &g) const
     {
         Fields s = *this;
         if (k == 0) return s;
@@ -353,12 +353,13 @@ __global__ void k_adjoint(Grid g, Fields f, Inj inj, const float *sx, const floa
 
 // eqsource (src/eqsource.c:25-52): double-couple moment tensor; the
 // sources listed (their updates touch neighbouring cells), one thread in order
-__global__ void k_eqsource(Grid g, Fields f, Src s, const int *list, int n, int nt, const int *ntb)
+__global__ void k_eqsource(Grid g, Fields f, Src s, const int *list, int n, int nt, const int *ntb, int lo, int hi)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     nt = step_of(ntb, nt);
     for (int q = 0; q < n; ++q) {
         const int l = list[q];
+        if (s.slot[l] < lo || s.slot[l] >= hi) continue;     // slots outside the group being advanced
         const int i = s.i[l], j = s.j[l];
         const float dip = (float)(s.azi[l] * M_PI / 180.0);
         const float scale = g.dt / (g.dh * g.dh);
@@ -572,16 +573,12 @@ __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, S
 // VW cells per thread: the x stencils come from 2G+1 vectors of the row,
 // the y stencils from one vector per row; cells past NX keep their (zero)
 // padding values
-template <int ND, int VW>
-__global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+template <int ND, int VW, bool PRE>
+__device__ __forceinline__ void velocity_v_slot(const Grid &g, Fields f, const Mat &m, const Src &s, const Fuse &fz,
+                                                int nt, int i0, int j, const vecf<VW> *pre)
 {
     constexpr int G = (ND + VW - 1) / VW;
     using V = vecf<VW>;
-    const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
-    const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
-    const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
-    const int j = 1 + ty * BY + threadIdx.y;
-    if (i0 > g.nx || j > g.ny) return;
     const int c0 = IDX(g, i0, j), P = g.pitch;
     float xs[(2 * G + 1) * VW], xy[(2 * G + 1) * VW];   // sxx, sxy of row j: columns i0 - G VW .. i0 + (G + 1) VW - 1
 #pragma unroll
@@ -614,7 +611,7 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f, Mat m, 
         }
     }
     V vx = ldv<VW>(f.vx, c0), vy = ldv<VW>(f.vy, c0);
-    const V rip = ldv<VW>(m.rip, c0), rjp = ldv<VW>(m.rjp, c0);
+    const V rip = PRE ? pre[0] : ldv<VW>(m.rip, c0), rjp = PRE ? pre[1] : ldv<VW>(m.rjp, c0);
 #pragma unroll
     for (int k = 0; k < VW; ++k) {
         if (i0 + k > g.nx) break;
@@ -632,6 +629,26 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f, Mat m, 
     }
     stv<VW>(f.vx, c0, vx);
     stv<VW>(f.vy, c0, vy);
+}
+
+// one slot (ONE: a one-shot handle) or blockIdx.y's g.spt slots of a batch
+template <int ND, int VW, bool ONE>
+__global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+{
+    const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
+    const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
+    const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i0 > g.nx || j > g.ny) return;
+    if (ONE) {
+        velocity_v_slot<ND, VW, false>(g, f0, m, s, fz, nt, i0, j, nullptr);
+        return;
+    }
+    // a batch: the cells' coefficients once for the thread's slots
+    const int c0 = IDX(g, i0, j);
+    const vecf<VW> pre[2] = { ldv<VW>(m.rip, c0), ldv<VW>(m.rjp, c0) };
+    const int k0 = g.k_lo + blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.k_hi);
+    for (int k = k0; k < k1; ++k) velocity_v_slot<ND, VW, true>(g, f0.at(k, g), m, s, fz, nt, i0, j, pre);
 }
 
 // --------------------------------------------------------- stress updates
@@ -878,17 +895,13 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
 // VW cells per thread (see k_velocity_v); inactive cells (past NX, or
 // frame cells that are never updated) keep their stresses and memory
 // variables
-template <int ND, int WEQ, int VW>
-__global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+template <int ND, int WEQ, int VW, bool PRE>
+__device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat &m, const Src &s, const Fuse &fz,
+                                              int nt, int i0, int j, const vecf<VW> *pre)
 {
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
     constexpr int G = (ND + VW - 1) / VW;
     using V = vecf<VW>;
-    const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
-    const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
-    const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
-    const int j = 1 + ty * BY + threadIdx.y;
-    if (i0 > g.nx || j > g.ny) return;
     const int c0 = IDX(g, i0, j), P = g.pitch;
     float xv[(2 * G + 1) * VW], yv[(2 * G + 1) * VW];   // vx, vy of row j: columns i0 - G VW .. i0 + (G + 1) VW - 1
 #pragma unroll
@@ -949,13 +962,15 @@ __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Sr
         sxy[k] = sxy0[k];
     }
     if (WEQ == SOFI_EL_ISO) {
-        const V c1 = ldv<VW>(m.f, c0), c2 = ldv<VW>(m.g, c0), c3 = ldv<VW>(m.fipjp, c0);
+        const V c1 = PRE ? pre[0] : ldv<VW>(m.f, c0), c2 = PRE ? pre[1] : ldv<VW>(m.g, c0),
+                c3 = PRE ? pre[2] : ldv<VW>(m.fipjp, c0);
 #pragma unroll
         for (int k = 0; k < VW; ++k)
             if (act[k]) upd_el(c1[k], c2[k], c3[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fz, g.dt, c0 + k);
     }
     if (WEQ == SOFI_EL_VTI) {
-        const V c1 = ldv<VW>(m.c11, c0), c2 = ldv<VW>(m.c13, c0), c3 = ldv<VW>(m.c33, c0), c4 = ldv<VW>(m.c55i, c0);
+        const V c1 = PRE ? pre[0] : ldv<VW>(m.c11, c0), c2 = PRE ? pre[1] : ldv<VW>(m.c13, c0),
+                c3 = PRE ? pre[2] : ldv<VW>(m.c33, c0), c4 = PRE ? pre[3] : ldv<VW>(m.c55i, c0);
 #pragma unroll
         for (int k = 0; k < VW; ++k)
             upd_el_vti(c1[k], c2[k], c3[k], c4[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k]);
@@ -1031,6 +1046,38 @@ __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f, Mat m, Sr
     stv<VW>(f.sxx, c0, sxxn);
     stv<VW>(f.syy, c0, syyn);
     stv<VW>(f.sxy, c0, sxyn);
+}
+
+// one slot (ONE: a one-shot handle) or blockIdx.y's g.spt slots of a batch
+template <int ND, int WEQ, int VW, bool ONE>
+__global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+{
+    const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
+    const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
+    const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i0 > g.nx || j > g.ny) return;
+    if (ONE) {
+        stress_v_slot<ND, WEQ, VW, false>(g, f0, m, s, fz, nt, i0, j, nullptr);
+        return;
+    }
+    // a batch: the elastic coefficients once for the thread's slots
+    const int c0 = IDX(g, i0, j);
+    vecf<VW> pre[4];
+    if (WEQ == SOFI_EL_ISO) {
+        pre[0] = ldv<VW>(m.f, c0);
+        pre[1] = ldv<VW>(m.g, c0);
+        pre[2] = ldv<VW>(m.fipjp, c0);
+    }
+    if (WEQ == SOFI_EL_VTI) {
+        pre[0] = ldv<VW>(m.c11, c0);
+        pre[1] = ldv<VW>(m.c13, c0);
+        pre[2] = ldv<VW>(m.c33, c0);
+        pre[3] = ldv<VW>(m.c55i, c0);
+    }
+    constexpr bool P = WEQ == SOFI_EL_ISO || WEQ == SOFI_EL_VTI;
+    const int k0 = g.k_lo + blockIdx.y * g.spt, k1 = min(k0 + g.spt, g.k_hi);
+    for (int k = k0; k < k1; ++k) stress_v_slot<ND, WEQ, VW, P>(g, f0.at(k, g), m, s, fz, nt, i0, j, pre);
 }
 
 // ------------------------------------------------------------ TTI passes
@@ -1418,6 +1465,13 @@ struct sofi_dev {
     int graph_k = 0;
     unsigned gen = 1, graph_gen = 0;       // launch configuration: current / of the captured graph
     long long graph_blocks = 0;            // blocks replayed since the handle opened
+    // a batch's resident groups: res_mode slots (0: chosen from the cache
+    // budget) advanced through a run's whole step range before the next
+    // group; the slots of the group being advanced, and where each slot's
+    // entries start in the injection and receiver lists
+    int res_mode = 0;
+    int cur_lo = 0, cur_hi = 1;
+    std::vector<int> src_inj_start, psrc_inj_start, rec_start;
 };
 
 namespace {
@@ -1483,7 +1537,7 @@ int upload(sofi_dev *h, float **dst, const float *src, size_t n)
 // Entry k is cell (si[k], sj[k]) of shot slot slots[k] (slot 0 if slots is
 // null) and records source index ids[k] (k if ids is null).
 int build_inj(sofi_dev *h, int n, const int *si, const int *sj, const int *slots, const int *ids, Inj *inj,
-              int **buf)
+              int **buf, std::vector<int> *starts = nullptr)
 {
     dfree(h, buf);
     *inj = Inj{};
@@ -1516,6 +1570,13 @@ int build_inj(sofi_dev *h, int n, const int *si, const int *sj, const int *slots
     inj->cell = *buf;
     inj->first = *buf + nc;
     inj->idx = *buf + 2 * nc + 1;
+    if (starts) {
+        // cells appear slot by slot (the slots' sources are listed in slot
+        // order): where each slot's cells start
+        starts->assign(h->nb + 1, nc);
+        for (int q = nc - 1; q >= 0; --q) (*starts)[cells[q] / h->g.plane] = q;
+        for (int k = h->nb - 1; k >= 0; --k) (*starts)[k] = std::min((*starts)[k], (*starts)[k + 1]);
+    }
     return 0;
 }
 
@@ -1552,6 +1613,7 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     if (!h) return fail("out of memory");
     h->d = d;
     h->nb = nshots;
+    h->cur_hi = nshots;
     h->slots.resize(nshots);
     Grid &g = h->g;
     g.nb = nshots;
@@ -1928,7 +1990,7 @@ int sync_acq(sofi_dev *h)
         for (int l = 0; l < n; ++l) {
             if (ct[l] >= 2 && ct[l] <= 4) h->any_force = true;
         }
-        auto plan = [&](bool explosive_only, Inj *inj, int **buf) -> int {
+        auto plan = [&](bool explosive_only, Inj *inj, int **buf, std::vector<int> *starts) -> int {
             std::vector<int> ii, jj, kk, idx;
             for (int l = 0; l < n; ++l) {
                 if (explosive_only && h->slots[cslot[l]].source_type != 1) continue;
@@ -1937,9 +1999,11 @@ int sync_acq(sofi_dev *h)
                 kk.push_back(cslot[l]);
                 idx.push_back(l);
             }
-            return build_inj(h, (int)ii.size(), ii.data(), jj.data(), kk.data(), idx.data(), inj, buf);
+            return build_inj(h, (int)ii.size(), ii.data(), jj.data(), kk.data(), idx.data(), inj, buf, starts);
         };
-        if (plan(false, &h->src_inj, &h->src_inj_buf) || plan(true, &h->psrc_inj, &h->psrc_inj_buf)) return -1;
+        if (plan(false, &h->src_inj, &h->src_inj_buf, &h->src_inj_start) ||
+            plan(true, &h->psrc_inj, &h->psrc_inj_buf, &h->psrc_inj_start))
+            return -1;
         h->source_type = h->slots[0].source_type;
         h->fusable = h->nb == 1 && n <= FUSE_MAX;
         h->fz = Fuse{};
@@ -1953,9 +2017,11 @@ int sync_acq(sofi_dev *h)
         std::vector<long long> off, cst;
         std::vector<double> sq_ppi, sq_pu, pfac;
         h->sec_base.assign(h->nb, 0);
+        h->rec_start.assign(h->nb + 1, 0);
         long long base = 0;
         for (int k = 0; k < h->nb; ++k) {
             const sofi_dev::Slot &sl = h->slots[k];
+            h->rec_start[k] = (int)ri.size();
             const long long ntr = (long long)sl.ri.size(), ns = d.ns > 0 ? d.ns : 1;
             h->sec_base[k] = base;
             for (long long r = 0; r < ntr; ++r) {
@@ -1972,6 +2038,7 @@ int sync_acq(sofi_dev *h)
             }
             base += (long long)SOFI_NCOMP * ntr * ns;
         }
+        h->rec_start[h->nb] = (int)ri.size();
         const int n = (int)ri.size();
         const size_t nn = (size_t)(n > 0 ? n : 1);
         dfree(h, &h->rec_ibuf);
@@ -2086,6 +2153,35 @@ int vec_width(const Grid &g, int shape)
 // the host-side sampling state is advanced for the whole block by the
 // caller; such a block starts on a sampling boundary, so the local step has
 // the sampling phase of the global one.
+// the entries of slots lo..hi-1 of a batch's injection / receiver lists
+// (each slot's entries are contiguous, starting at st[slot])
+Inj inj_part(const Inj &a, const std::vector<int> &st, int lo, int hi)
+{
+    Inj b = a;
+    if (st.empty()) return b;
+    b.n = st[hi] - st[lo];
+    b.cell = a.cell + st[lo];
+    b.first = a.first + st[lo];
+    return b;
+}
+
+Rec rec_part(const Rec &a, const std::vector<int> &st, int lo, int hi)
+{
+    Rec b = a;
+    if (st.empty()) return b;
+    const int o = st[lo];
+    b.n = st[hi] - o;
+    b.i = a.i + o;
+    b.j = a.j + o;
+    b.shot = a.shot + o;
+    b.off = a.off + o;
+    b.cst = a.cst + o;
+    b.sq_ppi = a.sq_ppi + o;
+    b.sq_pu = a.sq_pu + o;
+    b.pfac = a.pfac + o;
+    return b;
+}
+
 // A launch of the time loop: on the handle's stream, or, while a block of
 // steps is recorded, a kernel node of that block's graph after the previous
 // one (graphs are built node by node, not by stream capture, so other
@@ -2139,7 +2235,13 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     const int dtinv = d.dtinv > 0 ? d.dtinv : 1, fstep = d.nt - nt + 1, ndt = d.ndt > 0 ? d.ndt : 1;
     const bool store = fwi && !adj && (nt - 1) % dtinv == 0;
     const bool conv = adj && (fstep - 1) % dtinv == 0;
-    const bool sample = !adj && d.seismo && h->r.n > 0 && (inblk ? nt % ndt == 0 : (nt == h->lsamp && nt < d.nt));
+    // the slots this step advances: all, or the resident group of a batch
+    const int lo0 = h->cur_lo, hi0 = h->cur_hi;
+    const bool part = lo0 > 0 || hi0 < h->nb;
+    const Rec rr = part ? rec_part(h->r, h->rec_start, lo0, hi0) : h->r;
+    const Inj src_inj = part ? inj_part(h->src_inj, h->src_inj_start, lo0, hi0) : h->src_inj;
+    const Inj psrc_inj = part ? inj_part(h->psrc_inj, h->psrc_inj_start, lo0, hi0) : h->psrc_inj;
+    const bool sample = !adj && d.seismo && rr.n > 0 && (inblk ? nt % ndt == 0 : (nt == h->lsamp && nt < d.nt));
     const bool sample_dv = sample && (d.seismo == 3 || d.seismo == 4);
     const int ins = inblk ? nt : h->lsamp / ndt;       // in a block: the local step (k_seis adds the base)
     const int nsrc = adj ? 0 : h->s.n;     // the shot's sources act in the forward pass only
@@ -2167,9 +2269,8 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     fv.uy = fs.uy = h->w.uy;
     fv.uxy = fs.uxy = h->w.uxy;
     fv.ntb = fs.ntb = ntb;
-    if (nsrc > 0 && h->any_force && !fv.nforce)
-        launch(h, k_force, dim3((h->src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, h->src_inj, nt,
-                           ntb);
+    if (nsrc > 0 && h->any_force && !fv.nforce && src_inj.n > 0)
+        launch(h, k_force, dim3((src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, src_inj, nt, ntb);
     if (adj && h->nadj > 0)
         launch(h, k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), g, f, h->adj_inj, h->adj_sx,
                            h->adj_sy, d.nt, d.adjoint_type, nt);
@@ -2182,31 +2283,38 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     const dim3 blk(BX, BY);
     // the update kernels, launched per group of h->grp slots (all slots of a
     // one-shot handle or of an untuned batch: one group)
-    const int grp = h->grp > 0 && h->grp < h->nb ? h->grp : h->nb;
+    const int span = hi0 - lo0, grp = h->grp > 0 && h->grp < span ? h->grp : span;
     if (te) (void)hipEventRecord(te[0], st);
-    for (int lo = 0; lo < h->nb; lo += grp) {
+    for (int lo = lo0; lo < hi0; lo += grp) {
         Grid gg = g;
         gg.k_lo = lo;
-        gg.k_hi = lo + grp < h->nb ? lo + grp : h->nb;
+        gg.k_hi = lo + grp < hi0 ? lo + grp : hi0;
         const int nchunk = (gg.k_hi - lo + g.spt - 1) / g.spt;   // groups of g.spt slots (blockIdx.y)
-        int vw = h->nb > 1 ? 1 : vec_width(g, h->shape_v);
+        // a launch of one slot runs the one-shot kernels on that slot's fields
+        const bool one = gg.k_hi - gg.k_lo == 1;
+        const Fields fk = one ? f.at(lo, g) : f;
+        int vw = vec_width(g, h->shape_v);
         dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
-        if (vw == 4) launch(h, k_velocity_v<ND, 4>, tiles, blk, gg, f, m, h->s, fv, nt);
-        else if (vw == 2) launch(h, k_velocity_v<ND, 2>, tiles, blk, gg, f, m, h->s, fv, nt);
-        else if (h->nb == 1) launch(h, k_velocity<ND, true>, tiles, blk, gg, f, m, h->s, fv, nt);
-        else launch(h, k_velocity<ND, false>, tiles, blk, gg, f, m, h->s, fv, nt);
-        vw = h->nb > 1 ? 1 : vec_width(g, h->shape_s);
+        if (vw == 4 && one) launch(h, k_velocity_v<ND, 4, true>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        else if (vw == 4) launch(h, k_velocity_v<ND, 4, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        else if (vw == 2 && one) launch(h, k_velocity_v<ND, 2, true>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        else if (vw == 2) launch(h, k_velocity_v<ND, 2, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        else if (one) launch(h, k_velocity<ND, true>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        else launch(h, k_velocity<ND, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        vw = vec_width(g, h->shape_s);
         tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
-        if (te && lo == 0 && grp == h->nb) {
+        if (te && lo == lo0 && grp == span) {
             (void)hipEventRecord(te[1], st);
             (void)hipEventRecord(te[2], st);
         }
 #define SOFI_STRESS(W)                                                                                      \
         do {                                                                                                \
-            if (vw == 4) launch(h, k_stress_v<ND, W, 4>, tiles, blk, gg, f, m, h->s, fs, nt); \
-            else if (vw == 2) launch(h, k_stress_v<ND, W, 2>, tiles, blk, gg, f, m, h->s, fs, nt); \
-            else if (h->nb == 1) launch(h, k_stress<ND, W, true>, tiles, blk, gg, f, m, h->s, fs, nt);  \
-            else launch(h, k_stress<ND, W, false>, tiles, blk, gg, f, m, h->s, fs, nt);            \
+            if (vw == 4 && one) launch(h, k_stress_v<ND, W, 4, true>, tiles, blk, gg, fk, m, h->s, fs, nt);   \
+            else if (vw == 4) launch(h, k_stress_v<ND, W, 4, false>, tiles, blk, gg, fk, m, h->s, fs, nt);    \
+            else if (vw == 2 && one) launch(h, k_stress_v<ND, W, 2, true>, tiles, blk, gg, fk, m, h->s, fs, nt);   \
+            else if (vw == 2) launch(h, k_stress_v<ND, W, 2, false>, tiles, blk, gg, fk, m, h->s, fs, nt);    \
+            else if (one) launch(h, k_stress<ND, W, true>, tiles, blk, gg, fk, m, h->s, fs, nt);              \
+            else launch(h, k_stress<ND, W, false>, tiles, blk, gg, fk, m, h->s, fs, nt);            \
         } while (0)
         switch (d.weq) {
           case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
@@ -2233,32 +2341,33 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
           }
         }
     }
-    if (te && grp != h->nb) {
+    if (te && grp != span) {
         (void)hipEventRecord(te[1], st);
         (void)hipEventRecord(te[2], st);
     }
 #undef SOFI_STRESS
     if (te) (void)hipEventRecord(te[3], st);
     if (sample_dv) {
-        dim3 rb(64), rg((h->r.n + 63) / 64);
-        if (tti) launch(h, k_seis_dv<ND, true>, rg, rb, g, f, m, h->r, ins, ntb, ndt);
-        else launch(h, k_seis_dv<ND, false>, rg, rb, g, f, m, h->r, ins, ntb, ndt);
+        dim3 rb(64), rg((rr.n + 63) / 64);
+        if (tti) launch(h, k_seis_dv<ND, true>, rg, rb, g, f, m, rr, ins, ntb, ndt);
+        else launch(h, k_seis_dv<ND, false>, rg, rb, g, f, m, rr, ins, ntb, ndt);
     }
     if (nsrc > 0 && !src_fused) {
-        if (h->psrc_inj.n > 0)
-            launch(h, k_psource, dim3((h->psrc_inj.n + 63) / 64), dim3(64), g, f, h->s, h->psrc_inj, nt,
-                               ntb);
+        if (psrc_inj.n > 0)
+            launch(h, k_psource, dim3((psrc_inj.n + 63) / 64), dim3(64), g, f, h->s, psrc_inj, nt, ntb);
         if (h->n_eq > 0)
-            launch(h, k_eqsource, dim3(1), dim3(64), g, f, h->s, h->eq_list, h->n_eq, nt, ntb);
+            launch(h, k_eqsource, dim3(1), dim3(64), g, f, h->s, h->eq_list, h->n_eq, nt, ntb, lo0, hi0);
     }
     if (d.free_surf && !fs.surf) {
-        dim3 sb(256), sg((g.nx + 255) / 256, h->nb);
-        if (g.L > 0) launch(h, k_surface<ND, true>, sg, sb, g, f, m);
-        else launch(h, k_surface<ND, false>, sg, sb, g, f, m);
+        Grid gs = g;
+        gs.k_lo = lo0;
+        dim3 sb(256), sg((g.nx + 255) / 256, span);
+        if (g.L > 0) launch(h, k_surface<ND, true>, sg, sb, gs, f, m);
+        else launch(h, k_surface<ND, false>, sg, sb, gs, f, m);
     }
     if (sample && d.seismo != 3) {
-        dim3 rb(64), rg((h->r.n + 63) / 64);
-        launch(h, k_seis, rg, rb, g, f, h->r, ins, d.seismo, ntb, ndt);
+        dim3 rb(64), rg((rr.n + 63) / 64);
+        launch(h, k_seis, rg, rb, g, f, rr, ins, d.seismo, ntb, ndt);
     }
     if (!inblk && d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
     if (store || conv) {
@@ -2300,6 +2409,7 @@ int graph_len(const sofi_dev *h)
 {
     const sofi_desc &d = h->d;
     if (h->graph_mode == SOFI_GRAPH_OFF || d.mode == SOFI_MODE_FWI || h->time_every > 0) return 0;
+    if (h->cur_lo > 0 || h->cur_hi < h->nb) return 0;       // a batch advanced by resident groups
     if (h->graph_mode == SOFI_GRAPH_AUTO && (size_t)d.nx * d.ny * h->nb > GRAPH_AUTO_CELLS) return 0;
     const int ndt = d.seismo && d.ndt > 0 ? d.ndt : 1, want = h->graph_steps > 0 ? h->graph_steps : GRAPH_STEPS;
     return (want + ndt - 1) / ndt * ndt;
@@ -2369,26 +2479,48 @@ int run_block(sofi_dev *h, int nt, int k)
     return 0;
 }
 
-} // namespace
-
-extern "C" {
-
-int sofi_run(sofi_dev *h, int nt_first, int nt_last)
+// slots of a batch advanced together through a run's whole step range: the
+// largest group whose wavefields and the shared medium fit the budget of the
+// memory-side cache (256 MB Infinity Cache; 192 MB used), so that every step
+// of the group reads what the previous kernels left there; all slots when
+// even one does not fit (the medium is then shared by all of them instead)
+int resident_slots(const sofi_dev *h)
 {
-    if (!h) return fail("sofi_run: null handle");
-    if (!h->medium_set) return fail("sofi_run: medium not set");
-    if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
-    if (sync_acq(h)) return -1;
-    h->gerr = hipSuccess;
-    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    if (h->nb == 1) return 1;
+    if (h->res_mode > 0) return h->res_mode < h->nb ? h->res_mode : h->nb;
+    const sofi_desc &d = h->d;
+    const size_t pl = (size_t)h->g.plane * sizeof(float);
+    const bool visco = d.weq == SOFI_VEL_ISO || d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI;
+    const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
+    const size_t L = visco ? (size_t)d.L : 0;
+    const size_t slot = pl * (5 + 3 * L + (tti ? 4 : 0));
+    size_t arrays = 4;
+    switch (d.weq) {
+      case SOFI_EL_ISO: arrays += 3; break;
+      case SOFI_VEL_ISO: arrays += 6; break;
+      case SOFI_EL_VTI: arrays += 4; break;
+      case SOFI_VEL_VTI: arrays += 4 + 4 * L; break;
+      case SOFI_EL_TTI: arrays += 8; break;
+      default: arrays += 8 + 8 * L; break;
+    }
+    const size_t budget = (size_t)192 << 20, medium = arrays * pl;
+    if (medium + slot > budget) return h->nb;
+    const size_t n = (budget - medium) / slot;
+    return n < (size_t)h->nb ? (int)n : h->nb;
+}
+
+// steps nt_first..nt_last of the slots cur_lo..cur_hi-1: a batch's launch
+// groups and slots per thread, when not fixed, measured on the first steps
+// (three per candidate, within the range; the results do not depend on
+// them), then captured blocks where they apply, else step by step
+int run_range(sofi_dev *h, int nt_first, int nt_last)
+{
     int nt = nt_first;
-    // a batch's slots per thread: given, or measured on the first steps of a
-    // run long enough to try each candidate for three steps (the results do
-    // not depend on it)
+    const int span = h->cur_hi - h->cur_lo;
     if (h->nb > 1 && (h->spt_mode == 0 || h->grp_mode == 0) && !h->spt_tuned) {
         std::vector<std::pair<int, int>> cand;   // (slots per launch, slots per thread)
-        for (int gr : { h->nb, 16, 8, 4 }) {
-            if (gr > h->nb || (gr != h->nb && gr >= h->nb)) continue;
+        for (int gr : { span, 16, 8, 4, 1 }) {
+            if (gr > span || (gr != span && gr >= span)) continue;
             if (h->grp_mode > 0 && gr != h->grp_mode) continue;
             for (int sp : { 1, 2, 4, 8 }) {
                 if (sp > gr || (h->spt_mode > 0 && sp != h->spt_mode)) continue;
@@ -2418,7 +2550,7 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
             h->spt_tuned = true;
             h->gen++;
         } else {
-            h->grp = h->grp_mode > 0 ? h->grp_mode : (h->nb < 4 ? h->nb : 4);
+            h->grp = h->grp_mode > 0 ? h->grp_mode : (span < 4 ? span : 4);
             h->g.spt = h->spt_mode > 0 ? h->spt_mode : (h->grp < 4 ? h->grp : 4);
             h->gen++;
         }
@@ -2433,6 +2565,37 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
             ++nt;
         }
     }
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sofi_run(sofi_dev *h, int nt_first, int nt_last)
+{
+    if (!h) return fail("sofi_run: null handle");
+    if (!h->medium_set) return fail("sofi_run: medium not set");
+    if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
+    if (sync_acq(h)) return -1;
+    h->gerr = hipSuccess;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    // a batch advances its slots in resident groups (all slots together when
+    // one group holds them): each group through the whole step range, with
+    // the sampling state of the range's start
+    const int G = resident_slots(h), lsamp0 = h->lsamp;
+    for (int lo = 0; lo < h->nb; lo += G) {
+        h->cur_lo = lo;
+        h->cur_hi = lo + G < h->nb ? lo + G : h->nb;
+        h->lsamp = lsamp0;
+        if (run_range(h, nt_first, nt_last)) {
+            h->cur_lo = 0;
+            h->cur_hi = h->nb;
+            return -1;
+        }
+    }
+    h->cur_lo = 0;
+    h->cur_hi = h->nb;
     if (h->gerr != hipSuccess) {
         const hipError_t e = h->gerr;
         h->gerr = hipSuccess;
@@ -2699,6 +2862,18 @@ int sofi_set_graph(sofi_dev *h, int mode, int steps)
 }
 
 long long sofi_graph_blocks(const sofi_dev *h) { return h ? h->graph_blocks : 0; }
+
+int sofi_set_batch_resident(sofi_dev *h, int slots)
+{
+    if (!h) return fail("sofi_set_batch_resident: null handle");
+    if (slots < 0 || slots > h->nb) return fail("sofi_set_batch_resident: %d slots (0..%d)", slots, h->nb);
+    h->res_mode = slots;
+    h->spt_tuned = false;
+    h->gen++;
+    return 0;
+}
+
+int sofi_get_batch_resident(const sofi_dev *h) { return h ? resident_slots(h) : 0; }
 
 int sofi_set_kernel_timing(sofi_dev *h, int every, int max_samples)
 {

@@ -358,6 +358,15 @@ class Batch(Simulation):
         """Slots per launch and per thread of the update kernels (0: tuned)."""
         _chk_hip(self._L.sofi_set_batch_launch(self._h, group, spt))
 
+    def set_resident(self, slots: int = 0):
+        """Slots advanced through a run's whole step range before the next
+        group (0: the largest group that fits the memory-side cache)."""
+        _chk_hip(self._L.sofi_set_batch_resident(self._h, slots))
+
+    @property
+    def resident(self) -> int:
+        return int(self._L.sofi_get_batch_resident(self._h))
+
     @property
     def launch_groups(self):
         g, s = C.c_int(), C.c_int()

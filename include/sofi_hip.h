@@ -14,8 +14,9 @@
  * (GlobVar scalars), the material arrays (MemModel), the sources and
  * receivers (AcqVar, srcpos_loc, signals), and the outputs (GlobVar
  * SECTION* buffers, snapshot files, MemInv gradients).  A handle owns the
- * device memory of one shot's state (MemWavefield) and one HIP stream; it is
- * not thread-safe.  Every call returns 0 on success or a negative code,
+ * device memory of one shot's state (MemWavefield) and one HIP stream; a
+ * handle is not thread-safe, different handles may be driven from different
+ * threads at once.  Every call returns 0 on success or a negative code,
  * with the message in sofi_last_error() (the reference aborts through
  * log_fatal, src/logging.c:148-210).
  *
@@ -46,13 +47,23 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out);
  * sofi_set_sources / sofi_set_receivers / sofi_get_section / sofi_get_field
  * address (slot 0 after opening); sofi_reset clears every slot.  Snapshots
  * and MODE=FWI need one-slot handles.  The update kernels run over groups of
- * `group` slots per launch and read a cell's material once for `spt` slots
- * per thread; sofi_set_batch_launch fixes them (0: measured on the first
- * steps of the next run, the default).  No counterpart in the reference. */
+ * `group` slots per launch and `spt` slots per thread (the CELL shape reads
+ * a cell's material once for them); sofi_set_batch_launch fixes them (0:
+ * measured on the first steps of the next run, the default; the results do
+ * not depend on them).  No counterpart in the reference. */
 int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out);
 int sofi_select_shot(sofi_dev *h, int k);
 int sofi_set_batch_launch(sofi_dev *h, int group, int spt);
 int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
+/* Resident groups of a batch: sofi_run advances `slots` slots through its
+ * whole step range before the next group, so that a group's wavefields and
+ * the shared medium stay in the 256 MB memory-side cache from one kernel to
+ * the next (0, the default: the largest group that fits 192 MB, or all
+ * slots when one slot does not fit).  The results do not depend on it.
+ * sofi_get_batch_resident returns the group size the next run uses.  No
+ * counterpart in the reference. */
+int sofi_set_batch_resident(sofi_dev *h, int slots);
+int sofi_get_batch_resident(const sofi_dev *h);
 
 /* Upload the material arrays (enum sofi_array, NULL where unused) and
  * profiles (enum sofi_profile); replaces the MemModel the stress and
@@ -142,8 +153,9 @@ int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *gr
  * two / four consecutive cells of a row (vector loads and stores) in
  * 128 x 4 / 256 x 4 tiles, SOFI_LAUNCH_AUTO (the default) the shape
  * measured fastest for the rheology (VEC2 for EL_ISO / EL_VTI, CELL
- * otherwise; batches always run CELL).  Results agree to rounding between
- * shapes. */
+ * otherwise), for one-shot handles and batches alike.  Results agree to
+ * rounding between shapes and bit for bit between a batch slot and a
+ * one-shot handle of the same shape. */
 enum { SOFI_LAUNCH_AUTO = 0, SOFI_LAUNCH_CELL = 1, SOFI_LAUNCH_VEC2 = 2, SOFI_LAUNCH_VEC4 = 3 };
 int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape);
 

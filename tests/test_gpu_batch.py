@@ -1,8 +1,9 @@
 """Batches: several shots on one medium advanced by the same launches
-(sofi_open_batch).  The update kernels read a cell's material once for
-g.spt slots and loop over them; each slot's arithmetic is the one-shot
-handle's, so a batch reproduces one-shot runs bit for bit, whatever the
-slots per thread, the rheology, the sources and the boundaries."""
+(sofi_open_batch).  The update kernels loop over g.spt slots per thread;
+each slot's arithmetic is the one-shot handle's in the same launch shape,
+so a batch reproduces one-shot runs bit for bit, whatever the slots per
+launch and per thread, the shape, the rheology, the sources and the
+boundaries."""
 import os
 import tempfile
 
@@ -17,11 +18,11 @@ pytestmark = pytest.mark.gpu
 COMPS4 = (0, 1, 2, 3, 4)
 
 
-def _single(P, ishot):
+def _single(P, ishot, shape):
     from gpi_project_amd.sofi import Simulation
     P.set_shot(ishot)
     sim = Simulation(P)
-    sim.set_launch("cell")      # the shape batches run; the arithmetic then matches bit for bit
+    sim.set_launch(shape)       # batch and one-shot handle in the same shape: the arithmetic matches bit for bit
     sim.load_shot(nsnap=0)
     sim.run()
     out = {c: sim.section(c) for c in COMPS4}
@@ -29,6 +30,7 @@ def _single(P, ishot):
     return out
 
 
+@pytest.mark.parametrize("shape", ["auto", "cell", "vec4"])
 @pytest.mark.parametrize("weq,stype,fs,absn", [
     ("VEL_ISO", 1, 1, 1),     # explosive, free surface, CPML (visco memory variables per slot)
     ("EL_VTI", 1, 0, 2),      # damping frame
@@ -36,24 +38,32 @@ def _single(P, ishot):
     ("EL_ISO", 4, 1, 1),      # custom forces (injected per slot), free surface fused
     ("EL_ISO", 5, 0, 1),      # moment-tensor sources
 ])
-def test_batch_matches_one_shot_handles(weq, stype, fs, absn):
+def test_batch_matches_one_shot_handles(weq, stype, fs, absn, shape):
     from gpi_project_amd.sofi import Batch, Problem
     td = tempfile.mkdtemp(prefix="sofi_batch_")
     js = cases.multishot_case(td, weq, 4, stype, fs, absn)
     with chdir(td):
         P = Problem(os.path.basename(js))
         assert P.nshots == 4
-        ref = {k: _single(P, k) for k in range(1, 5)}
-        for grp, spt in ((4, 1), (4, 3), (2, 2), (0, 0)):
+        ref = {k: _single(P, k, shape) for k in range(1, 5)}
+        # (slots per launch, per thread, resident group; the last two runs are split in two calls)
+        for grp, spt, res in ((4, 1, 0), (4, 3, 0), (2, 2, 0), (1, 1, 0), (0, 0, 0), (2, 1, 3), (0, 0, 1)):
             B = Batch(P, 4)
+            B.set_launch(shape)
             B.set_launch_groups(grp, spt)
+            B.set_resident(res)
             B.load_shots([1, 2, 3, 4])
-            B.run()
+            if res:
+                B.run(1, 123)
+                B.run(124)
+            else:
+                B.run()
             for k in range(4):
                 for c in COMPS4:
                     got = B.section_of(k, c)
                     assert np.array_equal(got, ref[k + 1][c]), (weq, grp, spt, k, c, np.abs(got - ref[k + 1][c]).max())
-            print(weq, "slots per launch / thread", grp, spt, "->", B.launch_groups)
+            print(weq, shape, "slots per launch / thread / resident", grp, spt, res, "->", B.launch_groups,
+                  B.resident)
             B.close()
         P.close()
 
