@@ -247,6 +247,17 @@ struct Src {
 // Injection fused into the update kernels for shots with few sources: the
 // thread owning a source cell adds the sources in source order, so that
 // coincident sources sum as in the reference.
+// Receivers of every slot of the handle, one list: receiver k samples slot
+// shot[k] at (i[k], j[k]); its trace of component c starts at
+// sec + off[k] + c * cst[k] (each slot's sections are [5][ntr][ns]).
+struct Rec {
+    const int *i, *j, *shot;
+    const long long *off, *cst;
+    const double *sq_ppi, *sq_pu, *pfac;   // sqrt(ppi), sqrt(pu), (3ppi-4pu)/(2ppi-2pu)
+    float *sec;
+    int n, ns;
+};
+
 constexpr int FUSE_MAX = 8;
 struct Fuse {
     int nforce;         // sources whose forces the velocity kernel adds (0: none)
@@ -260,6 +271,11 @@ struct Fuse {
     int fwi_acc, fwi_u;
     float *pvxp1, *pvyp1, *ux, *uy, *uxy;
     const int *ntb;     // inside a captured block of steps: the block's base step (nt is then local)
+    // vx / vy sampling carried by the stress launch of a one-shot handle
+    // (SEISMO = 1): blocks ntile.. of the launch sample the receivers, whose
+    // velocities the stress update does not change
+    int sblk, ntile, ins, ndt;
+    Rec rec;
 };
 
 // the time step of a launch: nt, or inside a captured block of steps the
@@ -473,6 +489,30 @@ __device__ __forceinline__ int tile_id() { return blockIdx.x; }
 // aligned VW-float vector (interior rows start on 256-B boundaries and the
 // first of a thread's cells sits at a column offset divisible by VW).
 constexpr int BX = 64, BY = 4;
+
+// seismo_ssg vx / vy / p (src/seismo_ssg.c:40-64) of receiver k
+__device__ __forceinline__ void seis_sample(const Grid &g, const Fields &f0, const Rec &r, int k, int ins, int seismo)
+{
+    const Fields f = f0.at(r.shot[k], g);
+    const int c = IDX(g, r.i[k], r.j[k]);
+    float *t = r.sec + r.off[k] + (ins - 1);
+    const long long cs = r.cst[k];
+    if (seismo == 1 || seismo == 4) {
+        t[SOFI_C_VX * cs] = f.vx[c];
+        t[SOFI_C_VY * cs] = f.vy[c];
+    }
+    if (seismo == 2) t[SOFI_C_P * cs] = -f.sxx[c] - f.syy[c];
+    if (seismo == 4) t[SOFI_C_P * cs] = (float)(r.pfac[k] * (double)(-f.sxx[c] - f.syy[c]) / 3);
+}
+
+// the sampling blocks of a stress launch (Fuse::sblk); true when this block is one
+__device__ __forceinline__ bool fused_seis(const Grid &g, const Fields &f0, const Fuse &fz)
+{
+    if (fz.sblk == 0 || (int)blockIdx.x < fz.ntile) return false;
+    const int k = ((int)blockIdx.x - fz.ntile) * (BX * BY) + threadIdx.y * BX + threadIdx.x;
+    if (k < fz.rec.n) seis_sample(g, f0, fz.rec, k, fz.ntb ? (fz.ins + *fz.ntb) / fz.ndt : fz.ins, 1);
+    return true;
+}
 
 template <int VW>
 using vecf = float __attribute__((ext_vector_type(VW)));
@@ -807,6 +847,7 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
 template <int ND, int WEQ, bool ONE>
 __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
 {
+    if (ONE && fused_seis(g, f0, fz)) return;
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
     const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
@@ -1052,6 +1093,7 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
 template <int ND, int WEQ, int VW, bool ONE>
 __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
 {
+    if (ONE && fused_seis(g, f0, fz)) return;
     const int ntxv = (g.nx + BX * VW - 1) / (BX * VW);
     const int t = tile_id(), ty = t / ntxv, tx = t - ty * ntxv;
     const int i0 = 1 + (tx * BX + threadIdx.x) * VW;
@@ -1203,17 +1245,6 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m)
 
 // ------------------------------------------------------------- samplers
 
-// Receivers of every slot of the handle, one list: receiver k samples slot
-// shot[k] at (i[k], j[k]); its trace of component c starts at
-// sec + off[k] + c * cst[k] (each slot's sections are [5][ntr][ns]).
-struct Rec {
-    const int *i, *j, *shot;
-    const long long *off, *cst;
-    const double *sq_ppi, *sq_pu, *pfac;   // sqrt(ppi), sqrt(pu), (3ppi-4pu)/(2ppi-2pu)
-    float *sec;
-    int n, ns;
-};
-
 // seismo_ssg div/curl (src/seismo_ssg.c:51-58): the post-CPML derivatives;
 // runs right after the stress update, before the free surface touches psi.
 template <int ND, bool TTI>
@@ -1246,17 +1277,9 @@ __global__ void k_seis(Grid g, Fields f0, Rec r, int ins, int seismo, const int 
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= r.n) return;
     if (ntb) ins = (ins + *ntb) / ndt;      // ins: the block's local step
-    const Fields f = f0.at(r.shot[k], g);
-    const int c = IDX(g, r.i[k], r.j[k]);
-    float *t = r.sec + r.off[k] + (ins - 1);
-    const long long cs = r.cst[k];
-    if (seismo == 1 || seismo == 4) {
-        t[SOFI_C_VX * cs] = f.vx[c];
-        t[SOFI_C_VY * cs] = f.vy[c];
-    }
-    if (seismo == 2) t[SOFI_C_P * cs] = -f.sxx[c] - f.syy[c];
-    if (seismo == 4) t[SOFI_C_P * cs] = (float)(r.pfac[k] * (double)(-f.sxx[c] - f.syy[c]) / 3);
+    seis_sample(g, f0, r, k, ins, seismo);
 }
+
 
 // snap (src/snap_ssg.c:68-205): x-major decimated window; curl and div
 // recomputed from vx, vy without CPML
@@ -2269,6 +2292,10 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     fv.uy = fs.uy = h->w.uy;
     fv.uxy = fs.uxy = h->w.uxy;
     fv.ntb = fs.ntb = ntb;
+    fv.sblk = fs.sblk = 0;
+    // vx / vy sampling rides on the stress launch of a one-shot handle
+    // (SEISMO = 1, no TTI; the sampler addresses slot 0 of the handle's fields)
+    const bool seis_fused = sample && d.seismo == 1 && !tti && h->nb == 1;
     if (nsrc > 0 && h->any_force && !fv.nforce && src_inj.n > 0)
         launch(h, k_force, dim3((src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, src_inj, nt, ntb);
     if (adj && h->nadj > 0)
@@ -2303,18 +2330,28 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         else launch(h, k_velocity<ND, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
         vw = vec_width(g, h->shape_s);
         tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
+        // vx / vy sampling in extra blocks of a one-slot stress launch
+        Fuse fss = fs;
+        if (seis_fused && one) {
+            fss.sblk = (rr.n + BX * BY - 1) / (BX * BY);
+            fss.ntile = (int)tiles.x;
+            fss.ins = ins;
+            fss.ndt = ndt;
+            fss.rec = rr;
+            tiles.x += fss.sblk;
+        }
         if (te && lo == lo0 && grp == span) {
             (void)hipEventRecord(te[1], st);
             (void)hipEventRecord(te[2], st);
         }
 #define SOFI_STRESS(W)                                                                                      \
         do {                                                                                                \
-            if (vw == 4 && one) launch(h, k_stress_v<ND, W, 4, true>, tiles, blk, gg, fk, m, h->s, fs, nt);   \
-            else if (vw == 4) launch(h, k_stress_v<ND, W, 4, false>, tiles, blk, gg, fk, m, h->s, fs, nt);    \
-            else if (vw == 2 && one) launch(h, k_stress_v<ND, W, 2, true>, tiles, blk, gg, fk, m, h->s, fs, nt);   \
-            else if (vw == 2) launch(h, k_stress_v<ND, W, 2, false>, tiles, blk, gg, fk, m, h->s, fs, nt);    \
-            else if (one) launch(h, k_stress<ND, W, true>, tiles, blk, gg, fk, m, h->s, fs, nt);              \
-            else launch(h, k_stress<ND, W, false>, tiles, blk, gg, fk, m, h->s, fs, nt);            \
+            if (vw == 4 && one) launch(h, k_stress_v<ND, W, 4, true>, tiles, blk, gg, fk, m, h->s, fss, nt);   \
+            else if (vw == 4) launch(h, k_stress_v<ND, W, 4, false>, tiles, blk, gg, fk, m, h->s, fss, nt);    \
+            else if (vw == 2 && one) launch(h, k_stress_v<ND, W, 2, true>, tiles, blk, gg, fk, m, h->s, fss, nt);   \
+            else if (vw == 2) launch(h, k_stress_v<ND, W, 2, false>, tiles, blk, gg, fk, m, h->s, fss, nt);    \
+            else if (one) launch(h, k_stress<ND, W, true>, tiles, blk, gg, fk, m, h->s, fss, nt);              \
+            else launch(h, k_stress<ND, W, false>, tiles, blk, gg, fk, m, h->s, fss, nt);            \
         } while (0)
         switch (d.weq) {
           case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
@@ -2365,7 +2402,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         if (g.L > 0) launch(h, k_surface<ND, true>, sg, sb, gs, f, m);
         else launch(h, k_surface<ND, false>, sg, sb, gs, f, m);
     }
-    if (sample && d.seismo != 3) {
+    if (sample && d.seismo != 3 && !seis_fused) {
         dim3 rb(64), rg((rr.n + 63) / 64);
         launch(h, k_seis, rg, rb, g, f, rr, ins, d.seismo, ntb, ndt);
     }
