@@ -352,10 +352,12 @@ __global__ void k_psource(Grid g, Fields f, Src s, Inj inj, int nt, const int *n
 
 // adjoint sources at the receivers (src/update_v_interior.c:72-90):
 // ADJOINT_TYPE 1 adds the x and y residuals, 2 only y, 3 only x
-__global__ void k_adjoint(Grid g, Fields f, Inj inj, const float *sx, const float *sy, int nt_total, int type, int nt)
+__global__ void k_adjoint(Grid g, Fields f, Inj inj, const float *sx, const float *sy, int nt_total, int type, int nt,
+                          const int *ntb)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= inj.n) return;
+    nt = step_of(ntb, nt);
     const int c = inj.cell[k];
     float vx = f.vx[c], vy = f.vy[c];
     for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) {
@@ -1325,11 +1327,12 @@ struct Fwi {
 
 // snap_store (src/snap_store.c:27-47): the forward fields of one store step
 // into slot `slot` of the store (compact [5][ny][nx] per slot)
-__global__ __launch_bounds__(BX *BY) void k_fwi_store(Grid g, Fields f, Fwi w, int slot)
+__global__ __launch_bounds__(BX *BY) void k_fwi_store(Grid g, Fields f, Fwi w, int slot, const int *ntb, int dtinv)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
+    if (ntb) slot += *ntb / dtinv;          // in a captured block (its base step is a multiple of DTINV)
     const int c = IDX(g, i, j);
     const size_t n = (size_t)g.nx * g.ny, o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
     w.store[o] = w.pvxp1[c];
@@ -1348,11 +1351,12 @@ __global__ __launch_bounds__(BX *BY) void k_fwi_store(Grid g, Fields f, Fwi w, i
 // calc_conv (src/calc_conv.c:30-73): zero-lag correlation of the adjoint
 // fields with the forward fields of slot `slot`, with the reference's double
 // promotions (1.0 / (4.0 ...), 1.0 / 4.0)
-__global__ __launch_bounds__(BX *BY) void k_fwi_conv(Grid g, Fields f, Mat m, Fwi w, int slot)
+__global__ __launch_bounds__(BX *BY) void k_fwi_conv(Grid g, Fields f, Mat m, Fwi w, int slot, const int *ntb, int dtinv)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
+    if (ntb) slot -= *ntb / dtinv;          // in a captured block: the forward step runs backwards
     const int c = IDX(g, i, j);
     const size_t n = (size_t)g.nx * g.ny, o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
     const float frx = w.store[o], fry = w.store[o + n];
@@ -2253,8 +2257,10 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // time-reversed forward step was stored (src/time_loop.c:287-303); it
     // records no seismograms and takes no snapshots
     const bool fwi = d.mode == SOFI_MODE_FWI, adj = fwi && h->sw == 1;
+    // in a captured block nt is local and the block's base step a multiple
+    // of DTINV, so the store / correlation pattern of the local step is the
+    // global one
     const bool inblk = ntb != nullptr;
-    if (inblk && fwi) return fail("internal: FWI steps are not captured");
     const int dtinv = d.dtinv > 0 ? d.dtinv : 1, fstep = d.nt - nt + 1, ndt = d.ndt > 0 ? d.ndt : 1;
     const bool store = fwi && !adj && (nt - 1) % dtinv == 0;
     const bool conv = adj && (fstep - 1) % dtinv == 0;
@@ -2300,7 +2306,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         launch(h, k_force, dim3((src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, src_inj, nt, ntb);
     if (adj && h->nadj > 0)
         launch(h, k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), g, f, h->adj_inj, h->adj_sx,
-                           h->adj_sy, d.nt, d.adjoint_type, nt);
+                           h->adj_sy, d.nt, d.adjoint_type, nt, ntb);
     hipEvent_t *te = nullptr;
     if (!inblk && h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
         te = &h->tev[h->tev_used];
@@ -2409,8 +2415,8 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     if (!inblk && d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
     if (store || conv) {
         dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
-        if (store) launch(h, k_fwi_store, tg, tb, g, f, h->w, (nt - 1) / dtinv);
-        else launch(h, k_fwi_conv, tg, tb, g, f, m, h->w, (fstep - 1) / dtinv);
+        if (store) launch(h, k_fwi_store, tg, tb, g, f, h->w, (nt - 1) / dtinv, ntb, dtinv);
+        else launch(h, k_fwi_conv, tg, tb, g, f, m, h->w, (fstep - 1) / dtinv, ntb, dtinv);
     }
     if (!inblk && !adj && d.snap && nt == h->lsnap && nt <= d.esnap) {
         if (h->nsnap < h->nsnap_max) {
@@ -2445,11 +2451,18 @@ constexpr int GRAPH_STEPS = 64;
 int graph_len(const sofi_dev *h)
 {
     const sofi_desc &d = h->d;
-    if (h->graph_mode == SOFI_GRAPH_OFF || d.mode == SOFI_MODE_FWI || h->time_every > 0) return 0;
+    if (h->graph_mode == SOFI_GRAPH_OFF || h->time_every > 0) return 0;
     if (h->cur_lo > 0 || h->cur_hi < h->nb) return 0;       // a batch advanced by resident groups
     if (h->graph_mode == SOFI_GRAPH_AUTO && (size_t)d.nx * d.ny * h->nb > GRAPH_AUTO_CELLS) return 0;
-    const int ndt = d.seismo && d.ndt > 0 ? d.ndt : 1, want = h->graph_steps > 0 ? h->graph_steps : GRAPH_STEPS;
-    return (want + ndt - 1) / ndt * ndt;
+    int unit = d.seismo && d.ndt > 0 ? d.ndt : 1;
+    if (d.mode == SOFI_MODE_FWI) {
+        const int dtinv = d.dtinv > 0 ? d.dtinv : 1;
+        int a = unit, b = dtinv;                          // unit = lcm(NDT, DTINV)
+        while (b) { const int t = a % b; a = b; b = t; }
+        unit = unit / a * dtinv;
+    }
+    const int want = h->graph_steps > 0 ? h->graph_steps : GRAPH_STEPS;
+    return (want + unit - 1) / unit * unit;
 }
 
 // a block of k steps may start at nt when it ends before nt_last and the
@@ -2462,6 +2475,7 @@ bool block_fits(const sofi_dev *h, int k, int nt, int nt_last)
     if (k <= 0 || last > nt_last || last >= d.nt) return false;
     if (d.seismo && ((nt - 1) % ndt != 0 || h->lsamp != nt - 1 + ndt)) return false;
     if (d.snap && h->lsnap >= nt && h->lsnap <= last && h->lsnap <= d.esnap) return false;
+    if (d.mode == SOFI_MODE_FWI && (nt - 1) % (d.dtinv > 0 ? d.dtinv : 1) != 0) return false;
     return true;
 }
 
@@ -2812,6 +2826,7 @@ int sofi_fwi_set_model(sofi_dev *h, const float *vp0, const float *vs0, const fl
     h->w.muss = b[3];
     h->w.lamss = b[4];
     h->model_set = true;
+    h->gen++;
     return 0;
 }
 
@@ -2821,6 +2836,7 @@ int sofi_fwi_set_pass(sofi_dev *h, int sw)
     if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_set_pass: the handle was not opened with MODE=FWI");
     if (sw != 0 && sw != 1) return fail("sofi_fwi_set_pass: sw must be 0 (forward) or 1 (adjoint)");
     if (sw == 1 && !h->model_set) return fail("sofi_fwi_set_pass: sofi_fwi_set_model first");
+    if (h->sw != sw) h->gen++;            // the passes launch different kernels
     h->sw = sw;
     return 0;
 }
@@ -2844,6 +2860,7 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const
     }
     if (build_inj(h, n, ai, aj, nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
+    h->gen++;
     return 0;
 }
 

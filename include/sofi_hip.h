@@ -161,15 +161,15 @@ int sofi_set_launch(sofi_dev *h, int velocity_shape, int stress_shape);
 
 /* Captured blocks of time steps.  The reference's time loop launches every
  * step's updates one after the other (src/time_loop.c:40-325); here sofi_run
- * captures `steps` consecutive steps (rounded up to a multiple of NDT; 0:
- * 64) once into a HIP graph whose kernels read the block's base step from
- * device memory, and replays it for every block that starts on a sampling
- * boundary and holds no snapshot and not the loop's last step; the other
- * steps run one by one.  SOFI_GRAPH_AUTO (the default) captures for grids of
- * at most 2^20 cells (all slots of a batch together), where launch overhead
- * is a large part of a step; FWI handles and timed runs
- * (sofi_set_kernel_timing) always run step by step.  Results are identical
- * with and without graphs. */
+ * records `steps` consecutive steps (rounded up to a multiple of NDT, and of
+ * DTINV for FWI; 0: 64) once into a HIP graph whose kernels read the block's
+ * base step from device memory, and replays it for every block that starts
+ * on a sampling (and FWI store) boundary and holds no snapshot and not the
+ * loop's last step; the other steps run one by one.  SOFI_GRAPH_AUTO (the
+ * default) does so for grids of at most 2^20 cells (all slots of a batch
+ * together), where launch overhead is a large part of a step; timed runs
+ * (sofi_set_kernel_timing) and batches advanced by resident groups run step
+ * by step.  Results are identical with and without graphs. */
 enum { SOFI_GRAPH_AUTO = 0, SOFI_GRAPH_OFF = 1, SOFI_GRAPH_ON = 2 };
 int sofi_set_graph(sofi_dev *h, int mode, int steps);
 /* Blocks replayed from a captured graph since the handle was opened. */

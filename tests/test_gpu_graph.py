@@ -85,3 +85,35 @@ def test_graph_blocks_in_a_batch():
     for k in range(4):
         for c in range(5):
             assert np.array_equal(res["on"][k][c], res["off"][k][c]), (k, c)
+
+
+@pytest.mark.parametrize("dtinv", [1, 5])
+def test_graph_blocks_in_fwi(dtinv):
+    """Forward store and adjoint correlation from captured blocks (their base
+    step a multiple of DTINV) give the step-by-step gradients bit for bit."""
+    import common
+    from gpi_project_amd import fwi
+    from gpi_project_amd.sofi import Problem, Simulation
+    z = common.fwi_fixture("fwi_el_v0")
+    td = tempfile.mkdtemp(prefix="sofi_gfwi_")
+    js = cases.fwi_case(td, "fwi_el_v0", "init", dtinv=dtinv)
+    cases.write_measured_data(td, common.fwi_measured(z))
+    res = {}
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        for mode in ("off", "on"):
+            sim = Simulation(P)
+            sim.set_graph(mode, 16)
+            sim.fwi_set_model(P.model(0), P.model(1), P.model(2))
+            res[mode] = [fwi.shot_gradient(P, sim, s, write=False) for s in (1, 2)]
+            res[mode + "_blocks"] = sim.graph_blocks
+            sim.close()
+        P.close()
+    assert res["off_blocks"] == 0 and res["on_blocks"] > 0
+    for s in range(2):
+        for q in range(3):
+            a, b = res["on"][s]["gradient"][q], res["off"][s]["gradient"][q]
+            assert np.abs(b).max() > 0
+            assert np.array_equal(a, b), (dtinv, s, q, float(np.abs(a - b).max()))
+        for c in res["off"][s]["sections"]:
+            assert np.array_equal(res["on"][s]["sections"][c], res["off"][s]["sections"][c])
