@@ -414,19 +414,23 @@ def bench_fwi(args):
     sim = Simulation(P)
     sim.fwi_set_model(P.model(0), P.model(1), P.model(2))
     fwi.shot_gradient(P, sim, 1, write=False)              # warm-up (code objects, store pages)
-    sim.fwi_gradient_sum(reset=True)
-    t0 = time.perf_counter()
-    dev_ms, l2 = 0.0, 0.0
-    for ishot in range(1, P.nshots + 1):
-        r = fwi.shot_gradient(P, sim, ishot, write=False)
-        dev_ms += r["ms_forward"] + r["ms_adjoint"]
-        l2 += r["l2"]
-    t_it = time.perf_counter() - t0
-    gsum = sim.fwi_gradient_sum()
     fp = sim.footprint
-    g5 = fwi.shot_gradient(P, sim, 1, write=False)["gradient"]
     sim.close()
+    # one shot after the other on one handle, then args.fwi_workers shots at once
+    t0 = time.perf_counter()
+    rs = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=1)
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=args.fwi_workers)
+    t_it = time.perf_counter() - t0
+    same = all(np.array_equal(a, b) for a, b in zip(rs["sum"], rc["sum"]))
+    dev_ms = sum(r["ms_forward"] + r["ms_adjoint"] for r in rs["shots"].values())
+    l2 = sum(r["l2"] for r in rs["shots"].values())
+    gsum = rc["sum"]
+    g5 = rs["shots"][1]["gradient"]
     finite = all(np.isfinite(g).all() for g in gsum)
+    log(f"one handle: {P.nshots} shots in {t_seq:.2f} s ({P.nshots / t_seq:.2f} shots/s, device {dev_ms / 1e3:.2f} s); "
+        f"{args.fwi_workers} handles at once: {t_it:.2f} s ({P.nshots / t_it:.2f} shots/s), sums identical: {same}")
     # shot 1 at the full rate
     d1 = d.copy()
     d1.dtinv = 1
@@ -437,8 +441,7 @@ def bench_fwi(args):
     cos = [float(np.dot(a.ravel().astype(np.float64), b.ravel()) /
                  (np.linalg.norm(a.astype(np.float64)) * np.linalg.norm(b.astype(np.float64)) + 1e-300))
            for a, b in zip(g5, r1["gradient"])]
-    log(f"gradient: {P.nshots} shots in {t_it:.2f} s ({P.nshots / t_it:.2f} shots/s, device {dev_ms / 1e3:.2f} s), "
-        f"L2 {l2:.4e}, finite {finite}; DTINV=5 vs 1 cosine vp/vs/rho {cos}, DTINV=1 shot "
+    log(f"L2 {l2:.4e}, finite {finite}; DTINV=5 vs 1 cosine vp/vs/rho {cos}, DTINV=1 shot "
         f"{(r1['ms_forward'] + r1['ms_adjoint']):.0f} ms")
     cells = d.nx * d.ny
     # CPU: the FP32 oracle's forward (storing) and adjoint (correlating) steps on the same shot, 40 each
@@ -478,7 +481,8 @@ def bench_fwi(args):
         "config": {"workload": f"elastic FWI gradient, {d.nx}x{d.ny} FD8, NT={d.nt}, {P.nshots} shots, "
                                f"300 receivers, DTINV={d.dtinv}", "nx": d.nx, "ny": d.ny, "nt": d.nt,
                    "shots": P.nshots, "dtinv": d.dtinv},
-        "device_s": round(dev_ms / 1e3, 3), "iteration_s": round(t_it, 3),
+        "device_s": round(dev_ms / 1e3, 3), "iteration_s": round(t_it, 3), "workers": args.fwi_workers,
+        "one_handle_shots_per_s": round(P.nshots / t_seq, 4), "sums_identical_to_one_handle": bool(same),
         "glups_equivalent": round(2 * cells * d.nt * P.nshots / t_it / 1e9, 3),
         "footprint_bytes": fp, "cosine_dtinv5_vs_dtinv1": [round(c, 6) for c in cos],
         "dtinv1_shot_ms": round(r1["ms_forward"] + r1["ms_adjoint"], 1), "gradient_finite": finite,
@@ -513,6 +517,7 @@ def main():
     ap.add_argument("--grid", default="2000x1000", help="NXxNY of the batched workloads")
     ap.add_argument("--requests", type=int, default=48, help="requests of the served workload")
     ap.add_argument("--fwi-shots", type=int, default=32, help="shots of the FWI workload")
+    ap.add_argument("--fwi-workers", type=int, default=8, help="FWI shots computed at once (one handle each)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent handles of the served workload")
     ap.add_argument("--pmc-child", metavar="CASE_DIR", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=20, help=argparse.SUPPRESS)

@@ -61,9 +61,17 @@ def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True
             "s": time.perf_counter() - t0}
 
 
-def gradient_json(json_path: str, verbose: bool = True, write: bool = True) -> dict:
+def gradient_json(json_path: str, verbose: bool = True, write: bool = True, workers: int = 1) -> dict:
     """Every iteration and shot of a MODE=FWI parameter file; returns the
-    per-shot results of the last iteration and the summed gradients."""
+    per-shot results of the last iteration and the summed gradients.
+
+    workers > 1 computes that many shots at once, each on its own Problem,
+    handle and HIP stream (one host thread each), so that the kernels of
+    several shots overlap on the device; the shots' float32 gradients are
+    then summed in shot order, the order of the one-handle path, which gives
+    the same sums bit for bit."""
+    if workers > 1:
+        return _gradient_json_concurrent(json_path, verbose, write, workers)
     prob = Problem(json_path)
     if prob.desc.mode != 0:
         raise ValueError(f"{json_path}: MODE is not FWI")
@@ -83,4 +91,62 @@ def gradient_json(json_path: str, verbose: bool = True, write: bool = True) -> d
         prob.write_gradient(0, *gsum)
     sim.close()
     prob.close()
+    return {"shots": shots, "sum": gsum}
+
+
+def _gradient_json_concurrent(json_path: str, verbose: bool, write: bool, workers: int) -> dict:
+    import threading
+    probe = Problem(json_path)
+    if probe.desc.mode != 0:
+        probe.close()
+        raise ValueError(f"{json_path}: MODE is not FWI")
+    nshots, itermax = probe.nshots, probe.itermax
+    workers = min(workers, nshots)
+    lock = threading.Lock()
+    shots: dict = {}
+    errors: list = []
+    for it in range(1, itermax + 1):
+        queue = list(range(1, nshots + 1))
+
+        def worker():
+            prob = sim = None
+            try:
+                prob = Problem(json_path)
+                sim = Simulation(prob)
+                sim.fwi_set_model(prob.model(0), prob.model(1), prob.model(2))
+                while True:
+                    with lock:
+                        if not queue or errors:
+                            return
+                        ishot = queue.pop(0)
+                    r = shot_gradient(prob, sim, ishot, write)
+                    with lock:
+                        shots[ishot] = r
+                    if verbose:
+                        print(f"iteration {it} shot {ishot}: forward {r['ms_forward']:.1f} ms, "
+                              f"adjoint {r['ms_adjoint']:.1f} ms, L2 {r['l2']:.6e}")
+            except Exception as e:  # noqa: BLE001
+                with lock:
+                    errors.append(e)
+            finally:
+                if sim is not None:
+                    sim.close()
+                if prob is not None:
+                    prob.close()
+
+        threads = [threading.Thread(target=worker) for _ in range(workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            probe.close()
+            raise errors[0]
+    gsum = [np.zeros(probe.plane, dtype=np.float32) for _ in range(3)]
+    for ishot in range(1, nshots + 1):
+        for q in range(3):
+            gsum[q] = gsum[q] + shots[ishot]["gradient"][q]
+    if write:
+        probe.write_gradient(0, *gsum)
+    probe.close()
     return {"shots": shots, "sum": gsum}

@@ -92,3 +92,20 @@ def test_fwi_cli_writes_gradient_files():
             key = f"grad_{p}_" + (suffix[1:] if suffix else "sum")
             assert common.cosine(f, z[key]) >= COS_REF, (p, suffix)
     assert os.path.exists(os.path.join(td, "out", "seis_vy.su.shot2"))
+
+
+def test_fwi_concurrent_shots_equal_one_handle():
+    """Shots on several handles at once (one stream each) give the one-handle
+    per-shot gradients bit for bit, and their shot-order float32 sum the
+    one-handle sums."""
+    from gpi_project_amd import fwi
+    z, td, js = _setup("fwi_el_v1_y", dtinv=5)
+    with chdir(td):
+        a = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=1)
+        b = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=2)
+    for s in a["shots"]:
+        for q in range(3):
+            assert np.array_equal(a["shots"][s]["gradient"][q], b["shots"][s]["gradient"][q]), (s, q)
+    for q in range(3):
+        assert np.abs(a["sum"][q]).max() > 0
+        assert np.array_equal(a["sum"][q], b["sum"][q]), q
