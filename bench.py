@@ -206,6 +206,8 @@ def bench_batch(args, weq: str):
     ms_one = float(np.mean(t_one))
     log(f"one at a time: {ms_one:.1f} ms per shot ({cells * d.nt / (ms_one * 1e-3) / 1e9:.2f} GLUPS)")
     B = Batch(P, P.nshots)
+    B.set_resident(args.resident)
+    B.set_streams(args.streams)
     B.load_shots(list(range(1, P.nshots + 1)))
     ms_b = B.run()
     grp, spt = B.launch_groups
@@ -229,7 +231,8 @@ def bench_batch(args, weq: str):
         "data": "synthetic: layered model with smooth random Thomsen fields (workloads.config3)",
         "config": {"workload": f"{nb} batched {weq} shots, {d.nx}x{d.ny} FD6, NT={d.nt}, {P_ntr} receivers each",
                    "nx": d.nx, "ny": d.ny, "nt": d.nt, "fdorder": d.fdorder, "weq": weq, "shots": nb,
-                   "slots_per_launch": grp, "slots_per_thread": spt, "resident_group": res},
+                   "slots_per_launch": grp, "slots_per_thread": spt, "resident_group": res,
+                   "streams": args.streams or 2},
         "one_at_a_time_shots_per_s": round(1e3 / ms_one, 4),
         "speedup_vs_one_at_a_time": round((nb / ms_b) / (1.0 / ms_one), 3),
         "batched_glups": round(glups_b, 3),
@@ -516,6 +519,8 @@ def main():
     ap.add_argument("--single", type=int, default=4, help="shots timed one at a time for the comparison")
     ap.add_argument("--grid", default="2000x1000", help="NXxNY of the batched workloads")
     ap.add_argument("--requests", type=int, default=48, help="requests of the served workload")
+    ap.add_argument("--resident", type=int, default=0, help="slots per resident group of a batch (0: automatic)")
+    ap.add_argument("--streams", type=int, default=0, help="streams of a batch's resident groups (0: automatic)")
     ap.add_argument("--fwi-shots", type=int, default=32, help="shots of the FWI workload")
     ap.add_argument("--fwi-workers", type=int, default=8, help="FWI shots computed at once (one handle each)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent handles of the served workload")

@@ -142,6 +142,7 @@ def hip():
     L.sofi_set_graph.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_set_batch_resident.argtypes = [_VP, C.c_int]
     L.sofi_get_batch_resident.argtypes = [_VP]
+    L.sofi_set_batch_streams.argtypes = [_VP, C.c_int]
     L.sofi_graph_blocks.restype = C.c_longlong
     L.sofi_graph_blocks.argtypes = [_VP]
     L.sofi_open_batch.argtypes = [C.POINTER(SofiDesc), C.c_int, C.POINTER(_VP)]
@@ -179,5 +180,5 @@ def exported_symbols():
                            "sofi_fwi_gradient", "sofi_fwi_gradient_sum", "sofi_open_batch", "sofi_select_shot",
                            "sofi_set_batch_launch", "sofi_get_batch_launch", "sofi_estimate_footprint",
                            "sofi_set_graph", "sofi_graph_blocks", "sofi_set_batch_resident",
-                           "sofi_get_batch_resident"],
+                           "sofi_get_batch_resident", "sofi_set_batch_streams"],
     }

@@ -1498,6 +1498,13 @@ struct sofi_dev {
     // entries start in the injection and receiver lists
     int res_mode = 0;
     int cur_lo = 0, cur_hi = 1;
+    // resident groups on several streams at once: the extra streams (the
+    // handle's own is the first), their join events, the fork event
+    int str_mode = 0;
+    hipStream_t cur = nullptr;             // the stream launches go to (null: the handle's)
+    std::vector<hipStream_t> xs;
+    std::vector<hipEvent_t> xev;
+    hipEvent_t evfork = nullptr;
     std::vector<int> src_inj_start, psrc_inj_start, rec_start;
 };
 
@@ -2223,7 +2230,8 @@ void launch(sofi_dev *h, void (*k)(P...), dim3 grid, dim3 block, A &&...a)
         ((args[q++] = (void *)&x), ...);
     }, v);
     if (!h->gbuild) {
-        const hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(k), grid, block, args, 0, h->stream);
+        const hipError_t e =
+            hipLaunchKernel(reinterpret_cast<const void *>(k), grid, block, args, 0, h->cur ? h->cur : h->stream);
         if (e != hipSuccess && h->gerr == hipSuccess) h->gerr = e;
         return;
     }
@@ -2250,7 +2258,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     Fields &f = h->f;
     const Mat &m = h->m;
     const sofi_desc &d = h->d;
-    hipStream_t st = h->stream;
+    hipStream_t st = h->cur ? h->cur : h->stream;
     const bool tti = d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI;
     // FWI: the forward pass stores every DTINV-th step; the adjoint pass
     // injects the residuals at the receivers and correlates at the steps whose
@@ -2560,6 +2568,15 @@ int resident_slots(const sofi_dev *h)
     return n < (size_t)h->nb ? (int)n : h->nb;
 }
 
+// streams a batch's resident groups run on at once (1 when the batch is one
+// group)
+int batch_streams(const sofi_dev *h, int ngroups)
+{
+    if (ngroups <= 1) return 1;
+    const int s = h->str_mode > 0 ? h->str_mode : 2;
+    return s < ngroups ? s : ngroups;
+}
+
 // steps nt_first..nt_last of the slots cur_lo..cur_hi-1: a batch's launch
 // groups and slots per thread, when not fixed, measured on the first steps
 // (three per candidate, within the range; the results do not depend on
@@ -2585,9 +2602,9 @@ int run_range(sofi_dev *h, int nt_first, int nt_last)
                 h->grp = c.first;
                 h->g.spt = c.second;
                 if (step(h, nt++)) return -1;
-                HIPCHK(hipEventRecord(h->tune0, h->stream));
+                HIPCHK(hipEventRecord(h->tune0, h->cur ? h->cur : h->stream));
                 if (step(h, nt++) || step(h, nt++)) return -1;
-                HIPCHK(hipEventRecord(h->tune1, h->stream));
+                HIPCHK(hipEventRecord(h->tune1, h->cur ? h->cur : h->stream));
                 HIPCHK(hipEventSynchronize(h->tune1));
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, h->tune0, h->tune1));
@@ -2634,19 +2651,39 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
     // a batch advances its slots in resident groups (all slots together when
     // one group holds them): each group through the whole step range, with
     // the sampling state of the range's start
-    const int G = resident_slots(h), lsamp0 = h->lsamp;
-    for (int lo = 0; lo < h->nb; lo += G) {
-        h->cur_lo = lo;
-        h->cur_hi = lo + G < h->nb ? lo + G : h->nb;
-        h->lsamp = lsamp0;
-        if (run_range(h, nt_first, nt_last)) {
-            h->cur_lo = 0;
-            h->cur_hi = h->nb;
-            return -1;
+    const int G = resident_slots(h), lsamp0 = h->lsamp, ngroups = (h->nb + G - 1) / G;
+    // groups on S streams at once: the extra streams wait for the work
+    // enqueued so far on the handle's stream, which waits for them at the end
+    const int S = batch_streams(h, ngroups);
+    if (S > 1) {
+        while ((int)h->xs.size() < S - 1) {
+            hipStream_t x = nullptr;
+            hipEvent_t e = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+            h->xs.push_back(x);
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            h->xev.push_back(e);
         }
+        if (!h->evfork) HIPCHK(hipEventCreateWithFlags(&h->evfork, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(h->evfork, h->stream));
+        for (int q = 0; q < S - 1; ++q) HIPCHK(hipStreamWaitEvent(h->xs[q], h->evfork, 0));
     }
+    int rc = 0;
+    for (int gi = 0; gi < ngroups && !rc; ++gi) {
+        h->cur_lo = gi * G;
+        h->cur_hi = h->cur_lo + G < h->nb ? h->cur_lo + G : h->nb;
+        h->cur = gi % S == 0 ? nullptr : h->xs[gi % S - 1];
+        h->lsamp = lsamp0;
+        rc = run_range(h, nt_first, nt_last);
+    }
+    h->cur = nullptr;
     h->cur_lo = 0;
     h->cur_hi = h->nb;
+    for (int q = 0; q < S - 1; ++q) {
+        HIPCHK(hipEventRecord(h->xev[q], h->xs[q]));
+        HIPCHK(hipStreamWaitEvent(h->stream, h->xev[q], 0));
+    }
+    if (rc) return -1;
     if (h->gerr != hipSuccess) {
         const hipError_t e = h->gerr;
         h->gerr = hipSuccess;
@@ -2747,7 +2784,11 @@ int sofi_close(sofi_dev *h)
 {
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (hipStream_t x : h->xs) (void)hipStreamSynchronize(x);
     graph_drop(h);
+    for (hipStream_t x : h->xs) (void)hipStreamDestroy(x);
+    for (hipEvent_t e : h->xev) (void)hipEventDestroy(e);
+    if (h->evfork) (void)hipEventDestroy(h->evfork);
     for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
     for (const auto &a : h->allocs) (void)hipFree(a.raw);
     for (hipEvent_t e : { h->ev0, h->ev1, h->tune0, h->tune1 })
@@ -2928,6 +2969,14 @@ int sofi_set_batch_resident(sofi_dev *h, int slots)
 }
 
 int sofi_get_batch_resident(const sofi_dev *h) { return h ? resident_slots(h) : 0; }
+
+int sofi_set_batch_streams(sofi_dev *h, int streams)
+{
+    if (!h) return fail("sofi_set_batch_streams: null handle");
+    if (streams < 0 || streams > 8) return fail("sofi_set_batch_streams: %d streams (0..8)", streams);
+    h->str_mode = streams;
+    return 0;
+}
 
 int sofi_set_kernel_timing(sofi_dev *h, int every, int max_samples)
 {

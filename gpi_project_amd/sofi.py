@@ -363,6 +363,10 @@ class Batch(Simulation):
         group (0: the largest group that fits the memory-side cache)."""
         _chk_hip(self._L.sofi_set_batch_resident(self._h, slots))
 
+    def set_streams(self, streams: int = 0):
+        """Streams the resident groups run on at once (0: 2)."""
+        _chk_hip(self._L.sofi_set_batch_streams(self._h, streams))
+
     @property
     def resident(self) -> int:
         return int(self._L.sofi_get_batch_resident(self._h))

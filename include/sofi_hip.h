@@ -64,6 +64,11 @@ int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
  * counterpart in the reference. */
 int sofi_set_batch_resident(sofi_dev *h, int slots);
 int sofi_get_batch_resident(const sofi_dev *h);
+/* Streams a batch's resident groups run on at once when it has several
+ * groups (0, the default: 2; at most 8): each group's steps go to one of
+ * them, joined to the handle's stream before sofi_run returns.  The results
+ * do not depend on it.  No counterpart in the reference. */
+int sofi_set_batch_streams(sofi_dev *h, int streams);
 
 /* Upload the material arrays (enum sofi_array, NULL where unused) and
  * profiles (enum sofi_profile); replaces the MemModel the stress and

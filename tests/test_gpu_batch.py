@@ -47,11 +47,14 @@ def test_batch_matches_one_shot_handles(weq, stype, fs, absn, shape):
         assert P.nshots == 4
         ref = {k: _single(P, k, shape) for k in range(1, 5)}
         # (slots per launch, per thread, resident group; the last two runs are split in two calls)
-        for grp, spt, res in ((4, 1, 0), (4, 3, 0), (2, 2, 0), (1, 1, 0), (0, 0, 0), (2, 1, 3), (0, 0, 1)):
+        # (streams: resident groups run on that many streams at once)
+        for grp, spt, res, nst in ((4, 1, 0, 0), (4, 3, 0, 0), (2, 2, 0, 0), (1, 1, 0, 0), (0, 0, 0, 0),
+                                   (2, 1, 3, 1), (0, 0, 1, 0), (1, 1, 1, 3), (2, 2, 2, 2)):
             B = Batch(P, 4)
             B.set_launch(shape)
             B.set_launch_groups(grp, spt)
             B.set_resident(res)
+            B.set_streams(nst)
             B.load_shots([1, 2, 3, 4])
             if res:
                 B.run(1, 123)
@@ -62,8 +65,8 @@ def test_batch_matches_one_shot_handles(weq, stype, fs, absn, shape):
                 for c in COMPS4:
                     got = B.section_of(k, c)
                     assert np.array_equal(got, ref[k + 1][c]), (weq, grp, spt, k, c, np.abs(got - ref[k + 1][c]).max())
-            print(weq, shape, "slots per launch / thread / resident", grp, spt, res, "->", B.launch_groups,
-                  B.resident)
+            print(weq, shape, "slots per launch / thread / resident / streams", grp, spt, res, nst, "->",
+                  B.launch_groups, B.resident)
             B.close()
         P.close()
 
