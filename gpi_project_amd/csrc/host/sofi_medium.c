@@ -188,6 +188,11 @@ static int build_el_iso(const sofi_params *p, sofi_model *m, float **para)
     av_mue(pu, puipjp, m->nx, m->ny, pitch_);
     av_rho(rho, NEW(m, SOFI_A_RIP, 1), NEW(m, SOFI_A_RJP, 1), m->nx, m->ny, pitch_);
     float *fipjp = NEW(m, SOFI_A_FIPJP, 1), *f = NEW(m, SOFI_A_F, 1), *g = NEW(m, SOFI_A_G, 1);
+    /* with FDORDER_TIME = 4, prepare_update_s_el_4 computes these into locals
+     * and stores nothing (src/prepare_update_s_el_4.c:33-45): f, g and fipjp
+     * keep the zeros of their allocation, and the stresses change only
+     * through the explosive / moment sources */
+    if (p->fdorder_time == 4) return 0;
     for (int j = 1; j <= m->ny; j++)
         for (int i = 1; i <= m->nx; i++) {
             AT(fipjp, i, j) = AT(puipjp, i, j) * p->dt;
@@ -250,8 +255,16 @@ static int build_vel_iso(const sofi_params *p, sofi_model *m, float **para)
             AT(f, i, j) = (float)(AT(pu, i, j) * p->dt * (1.0 + L * AT(taus, i, j)));
             AT(g, i, j) = (float)(AT(ppi, i, j) * p->dt * (1.0 + L * AT(taup, i, j)));
             for (int l = 1; l <= L; l++) {
-                bip[l] = (float)(1.0 / (1.0 + (eta[l] * 0.5)));
-                cip[l] = (float)(1.0 - (eta[l] * 0.5));
+                if (p->fdorder_time == 4) {
+                    /* prepare_update_s_visc_4 (src/prepare_update_s_visc_4.c:36-48):
+                     * the relaxation factors carry the first Adams-Bashforth weight */
+                    const float c1 = (float)(13.0 / 12.0);
+                    bip[l] = (float)(1.0 / (1.0 + (c1 * eta[l] * 0.5)));
+                    cip[l] = (float)(1.0 - (c1 * eta[l] * 0.5));
+                } else {
+                    bip[l] = (float)(1.0 / (1.0 + (eta[l] * 0.5)));
+                    cip[l] = (float)(1.0 - (eta[l] * 0.5));
+                }
                 size_t o = (size_t)(l - 1) * pl + sofi_idx(pitch_, i, j);
                 dip[o] = AT(puipjp, i, j) * eta[l] * AT(tausipjp, i, j);
                 d[o] = AT(pu, i, j) * eta[l] * AT(taus, i, j);

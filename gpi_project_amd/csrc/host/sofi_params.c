@@ -411,7 +411,15 @@ int sofi_check_supported(const sofi_params *p)
 {
     if (p->weq < SOFI_EL_ISO || p->weq > SOFI_VEL_TTI)
         return sofi_fail("WEQ %s is not supported by this build (elastic and viscoelastic ISO/VTI/TTI only).", weq_names[p->weq]);
-    if (p->fdorder_time != 2) return sofi_fail("FDORDER_TIME=4 is not supported by this build.");
+    if (p->fdorder_time == 4) {
+        /* the reference's 4th-order-in-time loop exists for the isotropic
+         * rheologies only (src/prepmod.c:162-187, src/time_loop.c:215-245) */
+        if (p->weq != SOFI_EL_ISO && p->weq != SOFI_VEL_ISO)
+            return sofi_fail("WEQ not yet implemented for 4th order in time.");
+        if (p->fw > 0 && p->abs_type != 1 && p->abs_type != 2)
+            return sofi_fail("FDORDER_TIME=4 needs ABS_TYPE 1 or 2 with FW > 0 (the reference damps the stresses with "
+                             "an absorbing frame it does not allocate otherwise).");
+    }
     if (p->fdorder != 2 && p->fdorder != 4 && p->fdorder != 6 && p->fdorder != 8 && p->fdorder != 10 && p->fdorder != 12)
         return sofi_fail("Error in selection of FD coefficients: wrong FDORDER!");
     if (p->maxrelerror < 0 || p->maxrelerror > 4)

@@ -86,7 +86,7 @@ int sofi_checkfd(sofi_params *p, const sofi_model *m, const sofi_acq *a)
     sofi_holberg(p->fdorder, p->maxrelerror, hc);
     float cmax = m->vsmax > m->vpmax ? m->vsmax : m->vpmax;
     float cmin = m->vsmin < m->vpmin ? m->vsmin : m->vpmin;
-    float temporal = 1.0f;
+    float temporal = p->fdorder_time == 4 ? 1.5f : 1.0f;   /* 3.0 / 2.0 for Adams-Bashforth (src/checkfd.c:48-52) */
     float fmax = (float)(2.0 / p->ts);
     float dhstab = (cmin / (hc[0] * fmax));
     float gamma = (float)(fabs(hc[1]) + fabs(hc[2]) + fabs(hc[3]) + fabs(hc[4]) + fabs(hc[5]) + fabs(hc[6]));
@@ -172,4 +172,5 @@ void sofi_fill_desc(const sofi_params *p, const sofi_shot *s, sofi_desc *d)
     d->dtinv = p->dtinv;
     d->velocity = p->velocity;
     d->adjoint_type = p->adjoint_type;
+    d->fdorder_time = p->fdorder_time;
 }

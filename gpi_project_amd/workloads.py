@@ -29,6 +29,7 @@ class Case:
     time: float
     weq: str = "EL_ISO"
     fdorder: int = 4
+    fdorder_time: int = 2
     maxrelerror: int = 0
     L: int = 0
     fl: tuple = (15.0,)
@@ -61,7 +62,7 @@ def _json_lines(c: Case, d: str) -> str:
     rel = lambda *p: "./" + "/".join(p)
     kv = [
         ("NPROCX", 1), ("NPROCY", 1), ("MODE", c.mode),
-        ("FDORDER", c.fdorder), ("FDORDER_TIME", 2), ("MAXRELERROR", c.maxrelerror),
+        ("FDORDER", c.fdorder), ("FDORDER_TIME", c.fdorder_time), ("MAXRELERROR", c.maxrelerror),
         ("NX", c.nx), ("NY", c.ny), ("DH", repr(float(c.dh))),
         ("TIME", repr(float(c.time))), ("DT", repr(float(c.dt))),
         ("WEQ", c.weq),

@@ -25,6 +25,11 @@
  *   seismo_ssg            src/seismo_ssg.c:28-67
  *   snap                  src/snap_ssg.c:28-209
  *   snap_store / calc_conv src/snap_store.c:27-47, src/calc_conv.c:30-73
+ *   FDORDER_TIME = 4      src/time_loop.c:78-92,215-245, src/update_v_interior_4.c,
+ *                         src/update_v_{PML,abs}_4.c, src/wavefield_update_v_4.c,
+ *                         src/update_s_{elastic,visc}_{interior,PML,abs}_4.c,
+ *                         src/wavefield_update_s_{el,visc}_4.c,
+ *                         src/prepare_update_s_{el,visc}_4.c, src/util.c:89-109
  *
  * Parity status of each piece is listed in DESIGN.md; the pinned ones are
  * checked in tests/test_oracle.py against the reference's golden files, the
@@ -82,6 +87,12 @@ typedef struct oracle {
     REAL *vp0, *vs0, *rho0, *muss, *lamss;
     int nadj, *ai, *aj;                /* adjoint sources (the receivers) */
     REAL *adj_x, *adj_y;               /* adjoint signals [nadj][nt] */
+    /* FDORDER_TIME = 4: the Adams-Bashforth histories, [0] the newest;
+     * rotated after each update as shift_var2 / shift_var3 rotate pointers */
+    int fdt4;
+    REAL *sv[2][4];                    /* svx_1..4, svy_1..4 */
+    REAL *dvh[4][4];                   /* vxx_1..4, vyy_1..4, vxy_1..4, vyx_1..4 */
+    REAL *mh[3][4];                    /* (pr, pr_2..4), (pp, ..), (pq, ..): L planes each */
 } oracle;
 
 #define IX(i, j) ((size_t)((j) - 1 + SOFI_PAD) * (size_t)o->pitch + (size_t)((i) - 1 + SOFI_PAD))
@@ -139,6 +150,18 @@ oracle *oracle_new(const sofi_desc *d, const float *const *arrays, const float *
     o->nys = (d->snap_idx[4] - d->snap_idx[3]) / (d->idy > 0 ? d->idy : 1) + 1;
     o->hin = 1;
     o->hin1 = 1;
+    o->fdt4 = d->fdorder_time == 4;
+    if (o->fdt4) {
+        for (int a = 0; a < 2; ++a)
+            for (int k = 0; k < 4; ++k) o->sv[a][k] = zalloc(o->plane);
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 4; ++k) o->dvh[a][k] = zalloc(o->plane);
+        REAL *cur[3] = { o->r, o->p, o->q };
+        for (int a = 0; a < 3; ++a) {
+            o->mh[a][0] = cur[a];
+            for (int k = 1; k < 4; ++k) o->mh[a][k] = zalloc(o->plane * L);
+        }
+    }
     if (d->mode == SOFI_MODE_FWI) {
         o->ux = zalloc(o->plane); o->uy = zalloc(o->plane); o->uxy = zalloc(o->plane);
         o->pvxp1 = zalloc(o->plane); o->pvyp1 = zalloc(o->plane);
@@ -825,6 +848,14 @@ void oracle_reset(oracle *o, int sw)
         memset(px[k], 0, nxp * sizeof(REAL));
         memset(py[k], 0, nyp * sizeof(REAL));
     }
+    if (o->fdt4) {   /* zero_wavefield clears every history (src/zero_wavefield.c:88-127) */
+        for (int a = 0; a < 2; ++a)
+            for (int k = 0; k < 4; ++k) memset(o->sv[a][k], 0, o->plane * sizeof(REAL));
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 4; ++k) memset(o->dvh[a][k], 0, o->plane * sizeof(REAL));
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 4; ++k) memset(o->mh[a][k], 0, o->plane * L * sizeof(REAL));
+    }
     o->sw = sw;
     o->lsamp = o->d.ndt;
     o->lsnap = o->d.lsnap;
@@ -873,14 +904,188 @@ int oracle_fwi_gradient_sum(const oracle *o, double *gvp, double *gvs, double *g
     return 0;
 }
 
+/* ------------------------------------------------ 4th order in time */
+
+/* Adams-Bashforth coefficients (src/wavefield_update_v_4.c:29-33: floats) */
+#define AB1 ((REAL)(float)(13.0 / 12.0))
+#define AB2 ((REAL)(float)(-5.0 / 24.0))
+#define AB3 ((REAL)(float)(1.0 / 6.0))
+#define AB4 ((REAL)(float)(-1.0 / 24.0))
+
+/* shift_var2 / shift_var3 (src/util.c:89-109): the oldest buffer becomes
+ * the newest, to be overwritten by the next update */
+static void ab_shift(REAL **v)
+{
+    REAL *t = v[3];
+    v[3] = v[2];
+    v[2] = v[1];
+    v[1] = v[0];
+    v[0] = t;
+}
+
+/* wavefield_update_v_4 (src/wavefield_update_v_4.c:25-47) */
+static void v4_update(oracle *o, int i, int j, REAL sxx_x, REAL sxy_x, REAL sxy_y, REAL syy_y)
+{
+    REAL dtdh = (REAL)o->d.dt / (REAL)o->d.dh;
+    REAL **sx = o->sv[0], **sy = o->sv[1];
+    F(sx[0], i, j) = sxx_x + sxy_y;
+    F(sy[0], i, j) = sxy_x + syy_y;
+    F(o->vx, i, j) += (AB1 * F(sx[0], i, j) + AB2 * F(sx[1], i, j) + AB3 * F(sx[2], i, j) + AB4 * F(sx[3], i, j)) *
+                      dtdh * M(SOFI_A_RIP, i, j);
+    F(o->vy, i, j) += (AB1 * F(sy[0], i, j) + AB2 * F(sy[1], i, j) + AB3 * F(sy[2], i, j) + AB4 * F(sy[3], i, j)) *
+                      dtdh * M(SOFI_A_RJP, i, j);
+}
+
+/* update_v_interior_4 + update_v_PML_4 / update_v_abs_4: sources first
+ * (no adjoint sources in this loop), the interior, then the frame.  In the
+ * CPML frame the left side and the four corners only advance the CPML
+ * memory; their velocities are never updated (src/update_v_PML_4.c:51-57,
+ * 91-136).  Any ABS_TYPE other than 1 or 2 leaves the frame alone. */
+static void update_v4(oracle *o, int nt)
+{
+    const sofi_desc *d = &o->d;
+    if (o->sw == 0)
+        for (int l = 0; l < o->nsrc; l++) {
+            int i = o->si[l], j = o->sj[l];
+            float azi_rad = (float)(o->sazi[l] * M_PI / 180);
+            REAL amp = (REAL)((d->dt * o->sig[(size_t)l * d->nt + nt - 1]) / (d->dh * d->dh));
+            switch (o->stype[l]) {
+              case 2: F(o->vx, i, j) += M(SOFI_A_RIP, i, j) * amp; break;
+              case 3: F(o->vy, i, j) += M(SOFI_A_RJP, i, j) * amp; break;
+              case 4:
+                  F(o->vx, i, j) += sin(azi_rad) * (M(SOFI_A_RIP, i, j) * amp);
+                  F(o->vy, i, j) += cos(azi_rad) * (M(SOFI_A_RJP, i, j) * amp);
+                  break;
+            }
+        }
+    for (int j = d->gy[2] + 1; j <= d->gy[3]; j++)
+        for (int i = d->gx[2] + 1; i <= d->gx[3]; i++) {
+            REAL a, b, c, e;
+            op_v(o, i, j, &a, &b, &c, &e);
+            v4_update(o, i, j, a, b, c, e);
+        }
+    if (d->fw && (d->abs_type == 1 || d->abs_type == 2))
+        for (int j = d->gy[1]; j <= d->gy[4]; j++)
+            for (int i = d->gx[1]; i <= d->gx[4]; i++) {
+                if (!in_frame(o, i, j)) continue;
+                REAL a, b, c, e;
+                op_v(o, i, j, &a, &b, &c, &e);
+                int xl = i <= d->gx[2], xr = i > d->gx[3], yt = j <= d->gy[2], yb = j > d->gy[3];
+                if (d->abs_type == 1) {
+                    if (xl || xr) cpml_v_x(o, xl ? i : i - o->nx + 2 * d->fw, j, &a, &b);
+                    if (yt || yb) cpml_v_y(o, i, yt ? j : j - o->ny + 2 * d->fw, &c, &e);
+                    /* right side, top and bottom update; left side and corners do not */
+                    if (!xl && !(xr && (yt || yb))) v4_update(o, i, j, a, b, c, e);
+                } else {
+                    v4_update(o, i, j, a, b, c, e);
+                    F(o->vx, i, j) *= M(SOFI_A_ABSORB, i, j);
+                    F(o->vy, i, j) *= M(SOFI_A_ABSORB, i, j);
+                }
+            }
+    ab_shift(o->sv[0]);
+    ab_shift(o->sv[1]);
+}
+
+/* wavefield_update_s_el_4 (src/wavefield_update_s_el_4.c:27-70) and
+ * wavefield_update_s_visc_4 (src/wavefield_update_s_visc_4.c:27-98) on the
+ * (CPML-modified) derivatives of the cell */
+static void s4_cell(oracle *o, int i, int j)
+{
+    const REAL dh = (REAL)o->d.dh, dhi = (REAL)(1.0 / o->d.dh), dthalbe = (REAL)(o->d.dt / 2.0);
+    REAL *src[4] = { o->vxx, o->vyy, o->vxy, o->vyx };
+    REAL sum[4];
+    for (int a = 0; a < 4; ++a) {
+        REAL **h = o->dvh[a];
+        F(h[0], i, j) = F(src[a], i, j) * dh;
+        sum[a] = AB1 * F(h[0], i, j) + AB2 * F(h[1], i, j) + AB3 * F(h[2], i, j) + AB4 * F(h[3], i, j);
+    }
+    const REAL sumxx = sum[0], sumyy = sum[1], sumxy = sum[2], sumyx = sum[3];
+    REAL u1, u2, u3;
+    if (o->d.weq != SOFI_VEL_ISO) {
+        u1 = M(SOFI_A_FIPJP, i, j) * (sumxy + sumyx) * dhi;
+        u2 = ((M(SOFI_A_G, i, j) * (sumxx + sumyy)) - (2.0 * M(SOFI_A_F, i, j) * sumyy)) * dhi;
+        u3 = ((M(SOFI_A_G, i, j) * (sumxx + sumyy)) - (2.0 * M(SOFI_A_F, i, j) * sumxx)) * dhi;
+    } else {
+        const REAL *bip = o->prof[SOFI_P_BIP], *cip = o->prof[SOFI_P_CIP];
+        REAL **r = o->mh[0], **p = o->mh[1], **q = o->mh[2];
+        REAL sumr_old = 0, sump_old = 0, sumq_old = 0, sumr = 0, sump = 0, sumq = 0;
+        for (int l = 1; l <= o->L; l++) {
+            sumr_old += AB1 * MV(r[0], i, j, l) + AB2 * MV(r[1], i, j, l) + AB3 * MV(r[2], i, j, l) + AB4 * MV(r[3], i, j, l);
+            sump_old += AB1 * MV(p[0], i, j, l) + AB2 * MV(p[1], i, j, l) + AB3 * MV(p[2], i, j, l) + AB4 * MV(p[3], i, j, l);
+            sumq_old += AB1 * MV(q[0], i, j, l) + AB2 * MV(q[1], i, j, l) + AB3 * MV(q[2], i, j, l) + AB4 * MV(q[3], i, j, l);
+            REAL ctemp = 2 * (1 - cip[l]) / AB1;
+            MV(r[3], i, j, l) = bip[l] * (MV(r[0], i, j, l) * cip[l] - ctemp * AB2 * (MV(r[0], i, j, l) + MV(r[1], i, j, l)) -
+                                          ctemp * AB3 * (MV(r[1], i, j, l) + MV(r[2], i, j, l)) -
+                                          ctemp * AB4 * (MV(r[2], i, j, l) + MV(r[3], i, j, l)) -
+                                          (M3(SOFI_A_DIP, i, j, l) * (sumxy + sumyx) * dhi));
+            MV(p[3], i, j, l) = bip[l] * (MV(p[0], i, j, l) * cip[l] - ctemp * AB2 * (MV(p[0], i, j, l) + MV(p[1], i, j, l)) -
+                                          ctemp * AB3 * (MV(p[1], i, j, l) + MV(p[2], i, j, l)) -
+                                          ctemp * AB4 * (MV(p[2], i, j, l) + MV(p[3], i, j, l)) -
+                                          (M3(SOFI_A_E, i, j, l) * (sumxx + sumyy) * dhi) +
+                                          (2.0 * M3(SOFI_A_D, i, j, l) * sumyy * dhi));
+            MV(q[3], i, j, l) = bip[l] * (MV(q[0], i, j, l) * cip[l] - ctemp * AB2 * (MV(q[0], i, j, l) + MV(q[1], i, j, l)) -
+                                          ctemp * AB3 * (MV(q[1], i, j, l) + MV(q[2], i, j, l)) -
+                                          ctemp * AB4 * (MV(q[2], i, j, l) + MV(q[3], i, j, l)) -
+                                          (M3(SOFI_A_E, i, j, l) * (sumxx + sumyy) * dhi) +
+                                          (2.0 * M3(SOFI_A_D, i, j, l) * sumxx * dhi));
+            sumr += AB1 * MV(r[3], i, j, l) + AB2 * MV(r[0], i, j, l) + AB3 * MV(r[1], i, j, l) + AB4 * MV(r[2], i, j, l);
+            sump += AB1 * MV(p[3], i, j, l) + AB2 * MV(p[0], i, j, l) + AB3 * MV(p[1], i, j, l) + AB4 * MV(p[2], i, j, l);
+            sumq += AB1 * MV(q[3], i, j, l) + AB2 * MV(q[0], i, j, l) + AB3 * MV(q[1], i, j, l) + AB4 * MV(q[2], i, j, l);
+        }
+        u1 = (M(SOFI_A_FIPJP, i, j) * (sumxy + sumyx)) * dhi + (dthalbe * (sumr_old + sumr));
+        u2 = (M(SOFI_A_G, i, j) * (sumxx + sumyy) * dhi) - (2.0 * M(SOFI_A_F, i, j) * sumyy * dhi) +
+             (dthalbe * (sump_old + sump));
+        u3 = (M(SOFI_A_G, i, j) * (sumxx + sumyy) * dhi) - (2.0 * M(SOFI_A_F, i, j) * sumxx * dhi) +
+             (dthalbe * (sumq_old + sumq));
+    }
+    F(o->sxy, i, j) += u1;
+    F(o->sxx, i, j) += u2;
+    F(o->syy, i, j) += u3;
+}
+
+/* update_s_{elastic,visc}_{interior,PML,abs}_4, then the shifts of the
+ * histories (src/time_loop.c:215-245): the memory variables' newest buffer
+ * is the one just written */
+static void update_s4(oracle *o)
+{
+    const sofi_desc *d = &o->d;
+    for (int j = d->gy[2] + 1; j <= d->gy[3]; j++)
+        for (int i = d->gx[2] + 1; i <= d->gx[3]; i++) s4_cell(o, i, j);
+    if (d->fw)
+        for (int j = d->gy[1]; j <= d->gy[4]; j++)
+            for (int i = d->gx[1]; i <= d->gx[4]; i++) {
+                if (!in_frame(o, i, j)) continue;
+                if (d->abs_type == 1) {
+                    cpml_s_cell(o, i, j);
+                    s4_cell(o, i, j);
+                } else {
+                    s4_cell(o, i, j);
+                    abs_s(o, i, j);
+                }
+            }
+    if (d->weq == SOFI_VEL_ISO) {
+        for (int a = 0; a < 3; ++a) ab_shift(o->mh[a]);
+        o->r = o->mh[0][0];
+        o->p = o->mh[1][0];
+        o->q = o->mh[2][0];
+    }
+    for (int a = 0; a < 4; ++a) ab_shift(o->dvh[a]);
+}
+
 /* --------------------------------------------------------- one step */
 
 void oracle_step(oracle *o, int nt)
 {
     const sofi_desc *d = &o->d;
-    update_v(o, nt);
-    v_derivatives(o);
-    update_s(o);
+    if (o->fdt4) {
+        update_v4(o, nt);
+        v_derivatives(o);
+        update_s4(o);
+    } else {
+        update_v(o, nt);
+        v_derivatives(o);
+        update_s(o);
+    }
     int stype = d->source_type;
     /* (ADJOINT_TYPE 4, a pressure adjoint source through psource, is refused
      * by the host library: the reference injects sectionvxdiff there while
@@ -986,6 +1191,16 @@ void oracle_free(oracle *o)
                      o->grad_lam, o->grad_mu, o->grad_rho, o->adj_x, o->adj_y, o->store, o->gsum[0], o->gsum[1],
                      o->gsum[2], o->vp0, o->vs0, o->rho0, o->muss, o->lamss };
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); ++k) free(bufs[k]);
+    if (o->fdt4) {   /* o->r, o->p, o->q (freed above) are one of each history */
+        for (int a = 0; a < 2; ++a)
+            for (int k = 0; k < 4; ++k) free(o->sv[a][k]);
+        for (int a = 0; a < 4; ++a)
+            for (int k = 0; k < 4; ++k) free(o->dvh[a][k]);
+        REAL *cur[3] = { o->r, o->p, o->q };
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 4; ++k)
+                if (o->mh[a][k] != cur[a]) free(o->mh[a][k]);
+    }
     for (int k = 0; k < SOFI_A_NARRAYS; ++k) free(o->a[k]);
     for (int k = 0; k < SOFI_P_NPROFILES; ++k) free(o->prof[k]);
     for (int c = 0; c < SOFI_NCOMP; ++c) free(o->sec[c]);

@@ -48,7 +48,7 @@ def example_json_1rank(ex_dir: str, td: str, name: str) -> str:
     txt = open(os.path.join(ex_dir, name + ".json")).read()
     txt = re.sub(r'"NPROCX"\s*:\s*"\d+"', '"NPROCX" : "1"', txt)
     txt = re.sub(r'"NPROCY"\s*:\s*"\d+"', '"NPROCY" : "1"', txt)
-    base = name.replace("_fy", "")
+    base = name.replace("_fy", "").replace("_fdt4", "")
     txt = txt.replace(f"./{base}/", f"./out_{name}/")
     os.makedirs(os.path.join(td, f"out_{name}"), exist_ok=True)
     path = os.path.join(td, f"r1_{name}.json")
@@ -119,6 +119,18 @@ CASES = {
     "eltti_fd2_abs": ("EL_TTI", 2, 0, 0, 2, 0, 1, (), 4, 45),
 }
 
+# 4th order in time (Adams-Bashforth, FDORDER_TIME = 4): the reference runs
+# it for EL_ISO and VEL_ISO only.  EL_ISO keeps f = g = fipjp = 0 there
+# (src/prepare_update_s_el_4.c stores nothing), so only the sources move its
+# stresses; the force and moment-source cases show that.
+FDT4_CASES = {
+    "vel_fdt4_l2_fd6_cpml": ("VEL_ISO", 6, 1, 2, 1, 0, 1, (), 4, 51),
+    "vel_fdt4_l1_fd4_cpml_fs": ("VEL_ISO", 4, 0, 1, 1, 1, 1, (), 4, 52),
+    "vel_fdt4_l3_fd8_abs_fs_fy": ("VEL_ISO", 8, 2, 3, 2, 1, 3, (), 4, 53),
+    "el_fdt4_fd4_cpml_fx": ("EL_ISO", 4, 1, 0, 1, 0, 2, (), 4, 54),
+    "el_fdt4_fd2_abs_eq": ("EL_ISO", 2, 0, 0, 2, 0, 5, (25.0, 5), 4, 55),
+}
+
 # EL_VTI/EL_TTI with a free surface: surface_elastic() reads pu (= c55) and
 # ppi (= c11 already multiplied by DT in prepmod), so the sxx correction is
 # off by ~1/DT and the run diverges; the reference aborts with "Simulation is
@@ -134,14 +146,15 @@ DT_CASE = 4.0e-4
 
 
 def case_def(name: str) -> tuple[W.Case, dict]:
-    weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = CASES[name] if name in CASES else UNSTABLE[name]
+    table = CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else UNSTABLE
+    weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = table[name]
     model = _model(weq, seed, L=L)
     c = W.Case(name=name, nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdo,
                maxrelerror=mre, L=L, fl=(4.0, 12.0, 30.0, 60.0, 90.0)[:max(L, 1)], f_ref=20.0,
                abs_type=absn, fw=12, free_surf=fs, damping=8.0, fpml=20.0, vppml=3500.0,
                source_type=st, sources=_src(NX, NY, DH, extra=extra), receivers=_rec_grid(NX, NY, DH),
                seismo=seismo, ndt=2, snap=4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
-               idx=2, idy=3)
+               idx=2, idy=3, fdorder_time=4 if name in FDT4_CASES else 2)
     return c, model
 
 

@@ -56,7 +56,7 @@ def examples():
     out = os.path.join(HERE, "examples")
     os.makedirs(os.path.join(out, "golden"), exist_ok=True)
     for f in sorted(os.listdir(REF)):
-        if f.startswith("weq") and f.endswith(".json") and "fwi" not in f and "fdt4" not in f and f[3] in "345678":
+        if f.startswith("weq") and f.endswith(".json") and "fwi" not in f and f[3] in "345678":
             shutil.copy(os.path.join(REF, f), os.path.join(out, f))
     os.makedirs(os.path.join(out, "sources"), exist_ok=True)
     os.makedirs(os.path.join(out, "receiver"), exist_ok=True)
@@ -72,7 +72,7 @@ def examples():
     json.dump(consts, open(os.path.join(out, "model_bits.json"), "w"), indent=1)
     for t in ("weq3", "weq4", "weq5", "weq6", "weq7", "weq8"):
         src = os.path.join(REF, f"{t}_ref")
-        for suf in ("", "_fy"):
+        for suf in ("", "_fy", "_fdt4"):
             if not os.path.exists(os.path.join(src, f"seis{suf}_vx.su")):
                 continue
             for c in COMPS:
@@ -124,8 +124,10 @@ def ref1():
             arrs = {k: (v[:, ::SUB, ::SUB] if k.startswith("snap") else v) for k, v in r.items()}
             np.savez_compressed(os.path.join(out, f"{t}.npz"), **arrs)
             print("ref1", t, flush=True)
-    # coverage cases
-    for name in cases.CASES:
+    # coverage cases (FDORDER_TIME 2, then 4)
+    for name in list(cases.CASES) + list(cases.FDT4_CASES):
+        if os.path.exists(os.path.join(out, f"{name}.npz")):
+            continue
         with tempfile.TemporaryDirectory() as td:
             js, c = cases.build(name, td)
             t = run_ref(js, td)
@@ -145,6 +147,19 @@ def ref1():
                     shutil.copy(os.path.join(td, "out", f"seis_{comp}.su"), os.path.join(out, f"{name}_su", f"seis_{comp}.su"))
             np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
             print("ref1", name, "%.2fs" % t[0], flush=True)
+    # the 4th-order-in-time examples on one process (weq3_fdt4: the current
+    # reference sources give zeros, unlike the shipped golden)
+    with tempfile.TemporaryDirectory() as td:
+        cases.materialize_examples(ex, td)
+        for t in ("weq3_fdt4", "weq4_fdt4"):
+            if os.path.exists(os.path.join(out, f"{t}.npz")):
+                continue
+            js = cases.example_json_1rank(ex, td, t)
+            run_ref(js, td)
+            r = collect(os.path.join(td, "out_" + t, ""), (200, 200), 4, 4, 4, "_fdt4")
+            arrs = {k: (v[:, ::SUB, ::SUB] if k.startswith("snap") else v) for k, v in r.items()}
+            np.savez_compressed(os.path.join(out, f"{t}.npz"), **arrs)
+            print("ref1", t, flush=True)
     # quirk: the step at which the reference aborts as unstable
     for name in cases.UNSTABLE:
         with tempfile.TemporaryDirectory() as td:
