@@ -64,6 +64,7 @@ struct Grid {
     int k_lo, k_hi; // the slots this launch updates (a group of the batch)
     int nxp, nyp;   // floats per slot of the x-frame and y-frame CPML memory
     float dt, dh, dtdh, dhi, dthalbe;
+    float dhi4;     // 1.0 / DH rounded from double (the 4th-order stress updates' dhi)
     float hc[7];
 };
 
@@ -536,6 +537,31 @@ __device__ __forceinline__ void stv(float *p, int c, vecf<VW> v)
 // (src/update_v_interior.c:49-98, src/update_v_PML.c, src/update_v_abs.c,
 // src/wavefield_update_v.c:46-47) of cell (i, j) given the four stress
 // derivatives of op_v (no 1/DH), on the cell's velocities in registers.
+// cpml_update_v_x / _y (src/cpml_update.c:50-66) on the stress derivatives of
+// a frame cell
+__device__ __forceinline__ void cpml_v(const Grid &g, Fields &f, const Mat &m, int i, int j, bool xf, bool yf,
+                                       float &sxx_x, float &sxy_x, float &sxy_y, float &syy_y)
+{
+    if (xf) {
+        const int h = hx(g, i), o = j * g.wx + h;
+        float a = PRF(g, m, SOFI_P_B_X_HALF, h) * f.psi_sxx_x[o] + PRF(g, m, SOFI_P_A_X_HALF, h) * sxx_x;
+        f.psi_sxx_x[o] = a;
+        sxx_x = sxx_x / PRF(g, m, SOFI_P_K_X_HALF, h) + a;
+        float b = PRF(g, m, SOFI_P_B_X, h) * f.psi_sxy_x[o] + PRF(g, m, SOFI_P_A_X, h) * sxy_x;
+        f.psi_sxy_x[o] = b;
+        sxy_x = sxy_x / PRF(g, m, SOFI_P_K_X, h) + b;
+    }
+    if (yf) {
+        const int h = hy(g, j), o = h * g.wy + i;
+        float a = PRF(g, m, SOFI_P_B_Y_HALF, h) * f.psi_syy_y[o] + PRF(g, m, SOFI_P_A_Y_HALF, h) * syy_y;
+        f.psi_syy_y[o] = a;
+        syy_y = syy_y / PRF(g, m, SOFI_P_K_Y_HALF, h) + a;
+        float b = PRF(g, m, SOFI_P_B_Y, h) * f.psi_sxy_y[o] + PRF(g, m, SOFI_P_A_Y, h) * sxy_y;
+        f.psi_sxy_y[o] = b;
+        sxy_y = sxy_y / PRF(g, m, SOFI_P_K_Y, h) + b;
+    }
+}
+
 __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Mat &m, const Src &s, const Fuse &fz,
                                               int nt, int i, int j, int c, float sxx_x, float sxy_x, float sxy_y,
                                               float syy_y, float rip, float rjp, float &vx, float &vy)
@@ -543,28 +569,14 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
     // instability guard of time_loop (src/time_loop.c:44-47), before this step's update
     if (i == g.nx / 2 && j == g.ny / 2 && isnan(vy) && *f.unstable == 0) *f.unstable = step_of(fz.ntb, nt);
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
-    if (frame && g.abs_type != 1 && g.abs_type != 2) return;
-    if (g.abs_type == 1 && frame) {
-        // cpml_update_v_x / _y (src/cpml_update.c:50-66)
-        if (xf) {
-            const int h = hx(g, i), o = j * g.wx + h;
-            float a = PRF(g, m, SOFI_P_B_X_HALF, h) * f.psi_sxx_x[o] + PRF(g, m, SOFI_P_A_X_HALF, h) * sxx_x;
-            f.psi_sxx_x[o] = a;
-            sxx_x = sxx_x / PRF(g, m, SOFI_P_K_X_HALF, h) + a;
-            float b = PRF(g, m, SOFI_P_B_X, h) * f.psi_sxy_x[o] + PRF(g, m, SOFI_P_A_X, h) * sxy_x;
-            f.psi_sxy_x[o] = b;
-            sxy_x = sxy_x / PRF(g, m, SOFI_P_K_X, h) + b;
-        }
-        if (yf) {
-            const int h = hy(g, j), o = h * g.wy + i;
-            float a = PRF(g, m, SOFI_P_B_Y_HALF, h) * f.psi_syy_y[o] + PRF(g, m, SOFI_P_A_Y_HALF, h) * syy_y;
-            f.psi_syy_y[o] = a;
-            syy_y = syy_y / PRF(g, m, SOFI_P_K_Y_HALF, h) + a;
-            float b = PRF(g, m, SOFI_P_B_Y, h) * f.psi_sxy_y[o] + PRF(g, m, SOFI_P_A_Y, h) * sxy_y;
-            f.psi_sxy_y[o] = b;
-            sxy_y = sxy_y / PRF(g, m, SOFI_P_K_Y, h) + b;
-        }
+    if (frame && g.abs_type != 1 && g.abs_type != 2) {
+        // a frame that is never updated still receives its forces (the
+        // reference adds them to every source cell first)
+        for (int k = 0; k < fz.nforce; ++k)
+            if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
+        return;
     }
+    if (g.abs_type == 1 && frame) cpml_v(g, f, m, i, j, xf, yf, sxx_x, sxy_x, sxy_y, syy_y);
     for (int k = 0; k < fz.nforce; ++k)
         if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
     if (fz.fwi_acc == 1) {
@@ -1124,6 +1136,158 @@ __global__ __launch_bounds__(BX *BY) void k_stress_v(Grid g, Fields f0, Mat m, S
     for (int k = k0; k < k1; ++k) stress_v_slot<ND, WEQ, VW, P>(g, f0.at(k, g), m, s, fz, nt, i0, j, pre);
 }
 
+// ------------------------------------ 4th order in time (Adams-Bashforth)
+
+// the reference's weights, float constants (src/wavefield_update_v_4.c:29-33)
+constexpr float AB1 = (float)(13.0 / 12.0), AB2 = (float)(-5.0 / 24.0), AB3 = (float)(1.0 / 6.0),
+                AB4 = (float)(-1.0 / 24.0);
+
+// The histories of the 4th-order loop, [0] the newest: the velocity
+// kernel writes sx/sy[0], the stress kernel dv[.][0] and the memory
+// variables' [3]; the host then rotates the pointers as shift_var2 /
+// shift_var3 do (src/util.c:89-109), so that a buffer is reused four steps
+// later.
+struct Hist4 {
+    float *sx[4], *sy[4];          // svx_1..4, svy_1..4: sxx_x + sxy_y, sxy_x + syy_y
+    float *dv[4][4];               // vxx, vyy, vxy, vyx times DH
+    float *r[4], *p[4], *q[4];     // memory variables, L planes each
+};
+
+// update_v_interior_4 + update_v_PML_4 / update_v_abs_4 with
+// wavefield_update_v_4 (src/update_v_interior_4.c:32-368,
+// src/update_v_PML_4.c:33-136, src/update_v_abs_4.c:32-192,
+// src/wavefield_update_v_4.c:25-47), one thread per cell.  Forces first; in
+// the CPML frame the left side and the four corners only advance the CPML
+// memory and keep their velocities, as the reference's loops do.
+template <int ND>
+__global__ __launch_bounds__(BX *BY) void k_velocity4(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt, Hist4 H)
+{
+    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    const float rip = m.rip[c], rjp = m.rjp[c];
+    float sxx_x, sxy_x, sxy_y, syy_y;
+    op_v<ND>(g, f.sxx, f.syy, f.sxy, c, sxx_x, sxy_x, sxy_y, syy_y);
+    float vx = f.vx[c], vy = f.vy[c];
+    if (i == g.nx / 2 && j == g.ny / 2 && isnan(vy) && *f.unstable == 0) *f.unstable = step_of(fz.ntb, nt);
+    for (int k = 0; k < fz.nforce; ++k)
+        if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
+    bool upd = !frame || g.abs_type == 2;
+    if (frame && g.abs_type == 1) {
+        cpml_v(g, f, m, i, j, xf, yf, sxx_x, sxy_x, sxy_y, syy_y);
+        upd = i > g.gx2 && !(i > g.gx3 && yf);     // right side, top and bottom only
+    }
+    if (upd) {
+        const float sx = sxx_x + sxy_y, sy = sxy_x + syy_y;
+        H.sx[0][c] = sx;
+        H.sy[0][c] = sy;
+        vx += (AB1 * sx + AB2 * H.sx[1][c] + AB3 * H.sx[2][c] + AB4 * H.sx[3][c]) * g.dtdh * rip;
+        vy += (AB1 * sy + AB2 * H.sy[1][c] + AB3 * H.sy[2][c] + AB4 * H.sy[3][c]) * g.dtdh * rjp;
+        if (frame) {     // ABS_TYPE 2: damping (abs_update_v, src/abs_update.c:46-50)
+            if (g.abs_type == 2) {
+                const float a = m.absorb[c];
+                vx *= a;
+                vy *= a;
+            }
+        }
+    }
+    f.vx[c] = vx;
+    f.vy[c] = vy;
+}
+
+// update_s_{elastic,visc}_{interior,PML,abs}_4 with
+// wavefield_update_s_el_4 / wavefield_update_s_visc_4
+// (src/wavefield_update_s_el_4.c:27-70, src/wavefield_update_s_visc_4.c:27-98),
+// one thread per cell: the derivatives (CPML-modified in the frame) times DH
+// into the histories, their Adams-Bashforth sums, the memory variables with
+// their own histories (written into [3]), the stress update, then damping,
+// the fused sources and the free surface as in the 2nd-order kernel, which
+// see the memory variables just written.
+template <int ND, bool VISCO>
+__global__ __launch_bounds__(BX *BY) void k_stress4(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt, Hist4 H)
+{
+    if (fused_seis(g, f, fz)) return;
+    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j), P = g.pitch;
+    float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
+#pragma unroll
+    for (int mm = 1; mm <= ND; ++mm) {
+        const float h = g.hc[mm];
+        a += h * (f.vx[c + mm - 1] - f.vx[c - mm]);
+        b += h * (f.vy[c + mm] - f.vy[c - mm + 1]);
+        d += h * (f.vx[c + mm * P] - f.vx[c - (mm - 1) * P]);
+        e += h * (f.vy[c + (mm - 1) * P] - f.vy[c - mm * P]);
+    }
+    float vxx, vyx, vxy, vyy;
+    if (!stress_dv(g, f, m, i, j, a, b, d, e, vxx, vyx, vxy, vyy)) return;
+    const float dhi = g.dhi4;
+    const float cur[4] = { vxx * g.dh, vyy * g.dh, vxy * g.dh, vyx * g.dh };
+    float sum[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        H.dv[q][0][c] = cur[q];
+        sum[q] = AB1 * cur[q] + AB2 * H.dv[q][1][c] + AB3 * H.dv[q][2][c] + AB4 * H.dv[q][3][c];
+    }
+    const float sumxx = sum[0], sumyy = sum[1], sumxy = sum[2], sumyx = sum[3];
+    const float f_ = m.f[c], g_ = m.g[c], fipjp = m.fipjp[c];
+    float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
+    Fields fo = f;
+    float u1, u2, u3;
+    if (!VISCO) {
+        u1 = fipjp * (sumxy + sumyx) * dhi;
+        u2 = ((g_ * (sumxx + sumyy)) - (2.0f * f_ * sumyy)) * dhi;
+        u3 = ((g_ * (sumxx + sumyy)) - (2.0f * f_ * sumxx)) * dhi;
+    } else {
+        fo.r = H.r[3];      // the memory variables after this step
+        fo.p = H.p[3];
+        fo.q = H.q[3];
+        const float br = m.br[c], bd = m.bd[c], be = m.be[c];
+        float r_old = 0.f, p_old = 0.f, q_old = 0.f, r_new = 0.f, p_new = 0.f, q_new = 0.f;
+#pragma unroll
+        for (int l = 1; l <= SOFI_MAXL; ++l) {
+            if (l > g.L) break;
+            const int o = (l - 1) * g.plane + c;
+            const float eta = m.eta[l], bip = m.bip[l], cip = m.cip[l];
+            const float dip = eta * br, dd = eta * bd, ee = eta * be;
+            const float ctemp = 2.0f * (1.0f - cip) / AB1;
+            const float r0 = H.r[0][o], r1 = H.r[1][o], r2 = H.r[2][o], r3 = H.r[3][o];
+            const float p0 = H.p[0][o], p1 = H.p[1][o], p2 = H.p[2][o], p3 = H.p[3][o];
+            const float q0 = H.q[0][o], q1 = H.q[1][o], q2 = H.q[2][o], q3 = H.q[3][o];
+            r_old += AB1 * r0 + AB2 * r1 + AB3 * r2 + AB4 * r3;
+            p_old += AB1 * p0 + AB2 * p1 + AB3 * p2 + AB4 * p3;
+            q_old += AB1 * q0 + AB2 * q1 + AB3 * q2 + AB4 * q3;
+            const float rn = bip * (r0 * cip - ctemp * AB2 * (r0 + r1) - ctemp * AB3 * (r1 + r2) -
+                                    ctemp * AB4 * (r2 + r3) - (dip * (sumxy + sumyx) * dhi));
+            const float pn = bip * (p0 * cip - ctemp * AB2 * (p0 + p1) - ctemp * AB3 * (p1 + p2) -
+                                    ctemp * AB4 * (p2 + p3) - (ee * (sumxx + sumyy) * dhi) + (2.0f * dd * sumyy * dhi));
+            const float qn = bip * (q0 * cip - ctemp * AB2 * (q0 + q1) - ctemp * AB3 * (q1 + q2) -
+                                    ctemp * AB4 * (q2 + q3) - (ee * (sumxx + sumyy) * dhi) + (2.0f * dd * sumxx * dhi));
+            H.r[3][o] = rn;
+            H.p[3][o] = pn;
+            H.q[3][o] = qn;
+            r_new += AB1 * rn + AB2 * r0 + AB3 * r1 + AB4 * r2;
+            p_new += AB1 * pn + AB2 * p0 + AB3 * p1 + AB4 * p2;
+            q_new += AB1 * qn + AB2 * q0 + AB3 * q1 + AB4 * q2;
+        }
+        u1 = (fipjp * (sumxy + sumyx)) * dhi + (g.dthalbe * (r_old + r_new));
+        u2 = (g_ * (sumxx + sumyy) * dhi) - (2.0f * f_ * sumyy * dhi) + (g.dthalbe * (p_old + p_new));
+        u3 = (g_ * (sumxx + sumyy) * dhi) - (2.0f * f_ * sumxx * dhi) + (g.dthalbe * (q_old + q_new));
+    }
+    sxy += u1;
+    sxx += u2;
+    syy += u3;
+    stress_post<ND, VISCO>(g, fo, m, s, fz, nt, i, j, c, sxx, syy, sxy);
+    f.sxx[c] = sxx;
+    f.syy[c] = syy;
+    f.sxy[c] = sxy;
+}
+
 // ------------------------------------------------------------ TTI passes
 
 // v_derivatives (src/v_derivatives.c:27-50)
@@ -1496,6 +1660,9 @@ struct sofi_dev {
     // budget) advanced through a run's whole step range before the next
     // group; the slots of the group being advanced, and where each slot's
     // entries start in the injection and receiver lists
+    // 4th order in time: the rotating histories
+    bool fdt4 = false;
+    Hist4 h4{};
     int res_mode = 0;
     int cur_lo = 0, cur_hi = 1;
     // resident groups on several streams at once: the extra streams (the
@@ -1639,6 +1806,15 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     bool visco = d.weq == SOFI_VEL_ISO || d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI;
     if (visco && d.L < 1) return fail("sofi_open: L>0 required for viscoelastic rheologies");
     if (d.fw < 0 || 2 * d.fw > d.nx || 2 * d.fw > d.ny) return fail("sofi_open: FW=%d does not fit the grid", d.fw);
+    if (d.fdorder_time != 0 && d.fdorder_time != 2 && d.fdorder_time != 4)
+        return fail("sofi_open: FDORDER_TIME=%d (2 or 4)", d.fdorder_time);
+    if (d.fdorder_time == 4) {
+        if (d.weq != SOFI_EL_ISO && d.weq != SOFI_VEL_ISO) return fail("WEQ not yet implemented for 4th order in time.");
+        if (nshots != 1) return fail("sofi_open_batch: 4th order in time runs one shot per handle");
+        if (d.mode == SOFI_MODE_FWI) return fail("sofi_open: MODE=FWI runs 2nd order in time");
+        if (d.fw > 0 && d.abs_type != 1 && d.abs_type != 2)
+            return fail("sofi_open: 4th order in time needs ABS_TYPE 1 or 2 with FW > 0");
+    }
     if (d.free_surf && (d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI))
         return fail("sofi_open: free surface with VEL_VTI/VEL_TTI is not supported (the reference reads unallocated tau arrays there)");
     int ndev = 0;
@@ -1677,6 +1853,7 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     g.dh = d.dh;
     g.dtdh = d.dt / d.dh;
     g.dhi = 1.0f / d.dh;
+    g.dhi4 = (float)(1.0 / (double)d.dh);
     g.dthalbe = (float)(d.dt / 2.0);
     for (int k = 0; k < 7; ++k) g.hc[k] = d.hc[k];
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1699,6 +1876,25 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
         rc |= dalloc_t(h, &f.r, pl * g.L * nb);
         rc |= dalloc_t(h, &f.p, pl * g.L * nb);
         rc |= dalloc_t(h, &f.q, pl * g.L * nb);
+    }
+    if (d.fdorder_time == 4) {
+        h->fdt4 = true;
+        Hist4 &H = h->h4;
+        for (int k = 0; k < 4; ++k) {
+            rc |= dalloc_t(h, &H.sx[k], pl);
+            rc |= dalloc_t(h, &H.sy[k], pl);
+            for (int q = 0; q < 4; ++q) rc |= dalloc_t(h, &H.dv[q][k], pl);
+        }
+        if (visco) {
+            H.r[0] = f.r;
+            H.p[0] = f.p;
+            H.q[0] = f.q;
+            for (int k = 1; k < 4; ++k) {
+                rc |= dalloc_t(h, &H.r[k], pl * g.L);
+                rc |= dalloc_t(h, &H.p[k], pl * g.L);
+                rc |= dalloc_t(h, &H.q[k], pl * g.L);
+            }
+        }
     }
     if (d.weq == SOFI_EL_TTI || d.weq == SOFI_VEL_TTI) {
         rc |= dalloc_t(h, &f.dvxx, pl * nb);
@@ -2155,6 +2351,16 @@ int sofi_reset(sofi_dev *h)
     for (float *p : { h->w.pvxp1, h->w.pvyp1, h->w.ux, h->w.uy, h->w.uxy })
         if (p) HIPCHK(hipMemsetAsync(p, 0, pl, h->stream));
     HIPCHK(hipMemsetAsync(f.unstable, 0, sizeof(int) * nb, h->stream));
+    if (h->fdt4) {   // zero_wavefield clears every history (src/zero_wavefield.c:88-127)
+        Hist4 &H = h->h4;
+        for (int k = 0; k < 4; ++k) {
+            HIPCHK(hipMemsetAsync(H.sx[k], 0, pl, h->stream));
+            HIPCHK(hipMemsetAsync(H.sy[k], 0, pl, h->stream));
+            for (int q = 0; q < 4; ++q) HIPCHK(hipMemsetAsync(H.dv[q][k], 0, pl, h->stream));
+            for (float *m : { H.r[k], H.p[k], H.q[k] })
+                if (m) HIPCHK(hipMemsetAsync(m, 0, pl * h->g.L, h->stream));
+        }
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     h->nsnap = 0;
     h->lsamp = h->d.ndt;
@@ -2214,6 +2420,17 @@ Rec rec_part(const Rec &a, const std::vector<int> &st, int lo, int hi)
     b.sq_pu = a.sq_pu + o;
     b.pfac = a.pfac + o;
     return b;
+}
+
+// the oldest of four history buffers becomes the newest (shift_var2 /
+// shift_var3, src/util.c:89-109)
+void rot4(float **v)
+{
+    float *t = v[3];
+    v[3] = v[2];
+    v[2] = v[1];
+    v[1] = v[0];
+    v[0] = t;
 }
 
 // A launch of the time loop: on the handle's stream, or, while a block of
@@ -2331,6 +2548,43 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         gg.k_lo = lo;
         gg.k_hi = lo + grp < hi0 ? lo + grp : hi0;
         const int nchunk = (gg.k_hi - lo + g.spt - 1) / g.spt;   // groups of g.spt slots (blockIdx.y)
+        if (h->fdt4) {
+            // 4th order in time (one slot, one thread per cell): the
+            // histories rotate after each update, as shift_var2 after the
+            // velocity update and shift_var3 / shift_var2 after the stress
+            // update (src/time_loop.c:89-91,226-244); the memory variables
+            // the rest of the step sees are the ones just written
+            const dim3 t4((g.nx + BX - 1) / BX * nty);
+            launch(h, k_velocity4<ND>, t4, blk, gg, f, m, h->s, fv, nt, h->h4);
+            rot4(h->h4.sx);
+            rot4(h->h4.sy);
+            if (te) {
+                (void)hipEventRecord(te[1], st);
+                (void)hipEventRecord(te[2], st);
+            }
+            Fuse fss = fs;
+            dim3 ts = t4;
+            if (seis_fused) {
+                fss.sblk = (rr.n + BX * BY - 1) / (BX * BY);
+                fss.ntile = (int)t4.x;
+                fss.ins = ins;
+                fss.ndt = ndt;
+                fss.rec = rr;
+                ts.x += fss.sblk;
+            }
+            if (g.L > 0) launch(h, k_stress4<ND, true>, ts, blk, gg, f, m, h->s, fss, nt, h->h4);
+            else launch(h, k_stress4<ND, false>, ts, blk, gg, f, m, h->s, fss, nt, h->h4);
+            for (int q = 0; q < 4; ++q) rot4(h->h4.dv[q]);
+            if (g.L > 0) {
+                rot4(h->h4.r);
+                rot4(h->h4.p);
+                rot4(h->h4.q);
+                f.r = h->h4.r[0];
+                f.p = h->h4.p[0];
+                f.q = h->h4.q[0];
+            }
+            continue;
+        }
         // a launch of one slot runs the one-shot kernels on that slot's fields
         const bool one = gg.k_hi - gg.k_lo == 1;
         const Fields fk = one ? f.at(lo, g) : f;
@@ -2461,6 +2715,7 @@ int graph_len(const sofi_dev *h)
     const sofi_desc &d = h->d;
     if (h->graph_mode == SOFI_GRAPH_OFF || h->time_every > 0) return 0;
     if (h->cur_lo > 0 || h->cur_hi < h->nb) return 0;       // a batch advanced by resident groups
+    if (h->fdt4) return 0;                                  // its histories rotate every step
     if (h->graph_mode == SOFI_GRAPH_AUTO && (size_t)d.nx * d.ny * h->nb > GRAPH_AUTO_CELLS) return 0;
     int unit = d.seismo && d.ndt > 0 ? d.ndt : 1;
     if (d.mode == SOFI_MODE_FWI) {
@@ -2812,6 +3067,10 @@ size_t sofi_estimate_footprint(const sofi_desc *d, int nshots, int ntr, int nsna
     size_t b = 5 * (pl * nb + A) + (nb * sizeof(int) + A);                  // v, s, instability flags
     if (visco) b += 3 * (pl * L * nb + A);                                    // memory variables
     if (tti) b += 4 * (pl * nb + A);                                          // derivative arrays
+    if (d->fdorder_time == 4) {                                               // Adams-Bashforth histories
+        b += 24 * (pl + A);
+        if (visco) b += 9 * (pl * L + A);
+    }
     if (d->abs_type == 1 && d->fw > 0)
         b += 4 * ((size_t)(d->ny + 1) * wx * 4 * nb + A) + 4 * (wx * (size_t)(d->nx + 1) * 4 * nb + A);
     size_t arrays = 4;                                                        // rip rjp pu ppi
