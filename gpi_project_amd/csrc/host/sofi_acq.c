@@ -5,7 +5,7 @@
  *   sources()   src/sources.c:28-300   (SOURCE_FILE lines "x y td fc amp [azimuth] [type]"; plane wave)
  *   splitsrc()  src/splitsrc.c:27-81   (grid index = floor(x/DH)+1)
  *   wavelet()   src/wavelet.c:33-149   (Ricker, Fuchs-Mueller, file, sin^3, Berlage, Klauder)
- *   read_srcsig src/read_srcsig.c:38   (SIGNAL_FILE: one sample per line)
+ *   read_srcsig src/read_srcsig.c:38-185 (SIGNAL_FILE: text, one sample per line, or SU)
  *   initsrc()   src/initsrc.c:30-101
  *   acq_read()  src/acq_read.c:25-80
  * One device holds the whole grid, so the "local" sources and receivers are
@@ -13,8 +13,11 @@
  */
 #define _GNU_SOURCE
 #include <complex.h>
+#include <ctype.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "sofi_host_internal.h"
@@ -215,21 +218,106 @@ void sofi_free_acq(sofi_acq *a)
     memset(a, 0, sizeof(*a));
 }
 
-/* SIGNAL_FILE: one float per line (src/read_srcsig.c) */
+/* SIGNAL_FILE (src/read_srcsig.c:38-185).  The type follows the name: a
+ * name containing ".su" (any case) is an SU file whose first trace is the
+ * signal, with its header's dt (microseconds) as the sampling interval and
+ * delrt required to be 0; anything else is text, one sample per line (the
+ * first number of the line; a line that holds none counts as a 0 sample),
+ * blank lines skipped, lines starting with '#' comments, of which one of the
+ * form "# DT = <value>" gives the sampling interval.  A sampling interval
+ * that differs from DT is an error; none is accepted. */
 static float *read_srcsig(const sofi_params *p, int *nts)
 {
-    FILE *fp = fopen(p->signal_file, "r");
-    if (!fp) return NULL;
-    int cap = 1024, n = 0;
-    float *s = (float *)malloc(cap * sizeof(float));
-    float v;
-    while (fscanf(fp, "%f", &v) == 1) {
-        if (n == cap) { cap *= 2; s = (float *)realloc(s, cap * sizeof(float)); }
-        s[n++] = v;
+    *nts = 0;
+    float dt = 0.0f;
+    int dt_found = 0;
+    float *sig = NULL;
+    if (strcasestr(p->signal_file, ".su")) {
+        FILE *fp = fopen(p->signal_file, "rb");
+        if (!fp) { sofi_fail("Could not open SU file %s for reading.", p->signal_file); return NULL; }
+        unsigned char h[240];
+        if (fread(h, 1, 240, fp) != 240) {
+            fclose(fp);
+            sofi_fail("Short read in SU file %s.", p->signal_file);
+            return NULL;
+        }
+        uint16_t ns, sudt;
+        int16_t delrt;
+        memcpy(&delrt, h + 108, 2);
+        memcpy(&ns, h + 114, 2);
+        memcpy(&sudt, h + 116, 2);
+        if (ns < 1) {
+            fclose(fp);
+            sofi_fail("SU file has no number of samples (ns) in first SU header.");
+            return NULL;
+        }
+        sig = (float *)calloc(ns, sizeof(float));
+        if (fread(sig, sizeof(float), ns, fp) != ns) {
+            fclose(fp);
+            free(sig);
+            sofi_fail("Short read in SU file while reading data.");
+            return NULL;
+        }
+        fclose(fp);
+        if (delrt != 0) {
+            free(sig);
+            sofi_fail("Delay recording time != 0 not supported for source signature.");
+            return NULL;
+        }
+        if (sudt > 0) {
+            dt = (float)(sudt * 1e-6);
+            dt_found = 1;
+        }
+        *nts = ns;
+    } else {
+        FILE *fp = fopen(p->signal_file, "r");
+        if (!fp) { sofi_fail("Could not open file %s for reading.", p->signal_file); return NULL; }
+        char line[256];
+        int n = 0;
+        while (fgets(line, sizeof(line), fp)) {
+            char *c = line;
+            while (isspace((unsigned char)*c)) ++c;
+            if (*c == '\0') continue;
+            if (*c == '#') {
+                ++c;
+                const char *q = strcasestr(c, "DT");
+                if (!q || !(q = strcasestr(c, "="))) continue;
+                ++q;
+                while (isspace((unsigned char)*q)) ++q;
+                float v;
+                if (sscanf(q, "%e", &v) == 1) {
+                    dt = v;
+                    dt_found = 1;
+                }
+                continue;
+            }
+            ++n;
+        }
+        if (n == 0) {
+            fclose(fp);
+            sofi_fail("Source signature file did not contain valid samples.");
+            return NULL;
+        }
+        sig = (float *)calloc(n, sizeof(float));
+        rewind(fp);
+        int k = 0;
+        while (k < n && fgets(line, sizeof(line), fp)) {
+            char *c = line;
+            while (isspace((unsigned char)*c)) ++c;
+            if (*c == '\0' || *c == '#') continue;
+            if (sscanf(c, "%e", &sig[k]) != 1) sig[k] = 0.0f;
+            ++k;
+        }
+        fclose(fp);
+        *nts = n;
     }
-    fclose(fp);
-    *nts = n;
-    return s;
+    if (dt_found && dt != p->dt) {
+        free(sig);
+        sofi_fail("Sampling interval of source signature does not match FD step size! (dt source signature: %fs, "
+                  "FD step size: %fs)", dt, p->dt);
+        return NULL;
+    }
+    return sig;
 }
 
 int sofi_init_shot(const sofi_params *p, const sofi_acq *a, int ishot, int prev_source_type, sofi_shot *s)
@@ -253,7 +341,7 @@ int sofi_init_shot(const sofi_params *p, const sofi_acq *a, int ishot, int prev_
     int nts = 0;
     if (p->source_shape == 3 && n > 0) {
         sig_file = read_srcsig(p, &nts);
-        if (!sig_file) return sofi_fail("Signal file %s could not be read.", p->signal_file);
+        if (!sig_file) return -1;
     }
     int k = 0;
     for (int l = first; l <= last; ++l) {

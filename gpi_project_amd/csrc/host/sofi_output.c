@@ -14,6 +14,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "sofi_host_internal.h"
@@ -152,21 +153,72 @@ int sofi_snapshot_dims(const sofi_params *p, int *nxs, int *nys)
     return 0;
 }
 
-/* Append one snapshot of component comp (n floats, x-major). first != 0
- * truncates the file (the reference opens with "w" for its first snapshot). */
-int sofi_write_snapshot(const sofi_params *p, int comp, int first, const float *data, int n)
+static int append_floats(const char *fn, int first, int format, const float *data, size_t n)
 {
-    char fn[3 * SOFI_STR];
-    const char *ext = p->snap_format == 2 ? ".asc" : ".bin";
-    snprintf(fn, sizeof(fn), "%s%s.%s", p->snap_file, ext, comp_name[comp]);
     FILE *fp = fopen(fn, first ? "w" : "a");
     if (!fp) return sofi_fail("Could not open snapshot file %s.", fn);
-    if (p->snap_format == 2) {
-        for (int k = 0; k < n; ++k) fprintf(fp, "%e\n", data[k]);
-    } else if (fwrite(data, sizeof(float), n, fp) != (size_t)n) {
+    if (format == 2) {
+        for (size_t k = 0; k < n; ++k) fprintf(fp, "%e\n", data[k]);
+    } else if (n > 0 && fwrite(data, sizeof(float), n, fp) != n) {
         fclose(fp);
         return sofi_fail("Short write to %s.", fn);
     }
     fclose(fp);
+    return 0;
+}
+
+/* The decimated snapshot points of one process's subgrid along one axis, as
+ * zero-based indices into the global list 1, 1+id, 1+2id, ...: the points
+ * of [g0, g1] (src/initproc.c:86-114).  Returns the count (0: none). */
+static int rank_points(int ng, int id, int g0, int g1, int *first)
+{
+    int lo = -1, hi = -2;
+    for (int i = 1, k = 0; i <= ng; i += id, ++k)
+        if (i >= g0 && i <= g1) {
+            if (lo < 0) lo = k;
+            hi = k;
+        }
+    *first = lo;
+    return hi - lo + 1 > 0 ? hi - lo + 1 : 0;
+}
+
+/* Append one snapshot of component comp (n floats, x-major). first != 0
+ * truncates the files (the reference opens with "w" for its first snapshot).
+ * Written twice: the merged file snapmerge produces (src/merge.c:85-111),
+ * and the per-process files the reference's NPROCX x NPROCY processes write
+ * themselves, SNAP_FILE<ext>.<comp>.<px>.<py> (src/snap_ssg.c:62-67), each
+ * holding the points of its subgrid x-major, so that the reference's own
+ * snapmerge applies to them.  The per-process files are skipped when the
+ * subgrids do not divide the grid (the reference refuses such runs,
+ * src/initproc.c:41-47). */
+int sofi_write_snapshot(const sofi_params *p, int comp, int first, const float *data, int n)
+{
+    char fn[3 * SOFI_STR + 32];
+    const char *ext = p->snap_format == 2 ? ".asc" : ".bin";
+    snprintf(fn, sizeof(fn), "%s%s.%s", p->snap_file, ext, comp_name[comp]);
+    if (append_floats(fn, first, p->snap_format, data, (size_t)n)) return -1;
+    const int npx = p->nprocx, npy = p->nprocy;
+    if (npx < 1 || npy < 1 || p->nxg % npx || p->nyg % npy) return 0;
+    int nxs, nys;
+    sofi_snapshot_dims(p, &nxs, &nys);
+    if (n != nxs * nys) return sofi_fail("snapshot of %d values, expected %d x %d", n, nxs, nys);
+    const int lx = p->nxg / npx, ly = p->nyg / npy;
+    float *buf = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    if (!buf) return sofi_fail("out of memory");
+    for (int px = 0; px < npx; ++px)
+        for (int py = 0; py < npy; ++py) {
+            int i0, j0;
+            const int ni = rank_points(p->nxg, p->idx, 1 + px * lx, (px + 1) * lx, &i0);
+            const int nj = rank_points(p->nyg, p->idy, 1 + py * ly, (py + 1) * ly, &j0);
+            size_t m = 0;
+            for (int a = 0; a < ni; ++a)
+                for (int b = 0; b < nj; ++b) buf[m++] = data[(size_t)(i0 + a) * nys + (j0 + b)];
+            snprintf(fn, sizeof(fn), "%s%s.%s.%d.%d", p->snap_file, ext, comp_name[comp], px, py);
+            if (append_floats(fn, first, p->snap_format, buf, m)) {
+                free(buf);
+                return -1;
+            }
+        }
+    free(buf);
     return 0;
 }

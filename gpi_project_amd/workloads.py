@@ -87,7 +87,13 @@ def _json_lines(c: Case, d: str) -> str:
     ]
     for l, f in enumerate(c.fl[: c.L], start=1):
         kv.append((f"FL{l}", repr(float(f))))
-    kv += list(c.extra.items())
+    # extra keys replace the defaults above, or are added after them
+    keys = [k for k, _ in kv]
+    for k, v in c.extra.items():
+        if k in keys:
+            kv[keys.index(k)] = (k, v)
+        else:
+            kv.append((k, v))
     body = ",\n".join(f'\t"{k}" : "{v}"' for k, v in kv)
     return "{\n" + body + "\n}\n"
 

@@ -106,6 +106,11 @@ void sofi_holberg(int fdorder, int maxrelerror, float hc[7]);
 int sofi_write_section(const sofi_params *p, const sofi_acq *a, int ishot, int comp, const float *section);
 int sofi_write_signal(const sofi_params *p, const sofi_acq *a, const sofi_shot *s, int ishot);
 int sofi_count_snapshots(const sofi_params *p);
+/* Appends one snapshot (x-major over the decimated grid) to the merged file
+ * SNAP_FILE<ext>.<comp> (what snapmerge makes, src/merge.c:85-111) and to the
+ * per-process files SNAP_FILE<ext>.<comp>.<px>.<py> the reference's
+ * NPROCX x NPROCY processes write (src/snap_ssg.c:62-67); first != 0
+ * truncates them. */
 int sofi_write_snapshot(const sofi_params *p, int comp, int first, const float *data, int n);
 int sofi_snapshot_dims(const sofi_params *p, int *nxs, int *nys);
 

@@ -145,6 +145,65 @@ NT_CASE = 500
 DT_CASE = 4.0e-4
 
 
+# SOURCE_SHAPE 3: the wavelet read from SIGNAL_FILE (src/read_srcsig.c).
+# "txt": comment lines (one "# DT = ..." matching DT), blank lines, a line
+# with text after the number and one holding no number (a 0 sample), 300
+# samples of a Gaussian derivative; "dat": 120 samples with no interval, the
+# rest of the NT steps zero; "su": an SU file of two traces (the first is
+# read) with dt = DT in microseconds.
+SIGNAL_CASES = ("sig_txt", "sig_dat", "sig_su")
+
+
+def _signal_samples(n, fc, dt):
+    t = np.arange(1, n + 1) * dt - 1.2 / fc
+    return (-t * np.exp(-(np.pi * fc * t) ** 2)).astype(np.float32) * 1e3
+
+
+def _write_su_traces(path, traces, dt_us):
+    with open(path, "wb") as f:
+        for k, tr in enumerate(traces):
+            h = bytearray(240)
+            h[0:4] = np.int32(k + 1).tobytes()
+            h[114:116] = np.uint16(len(tr)).tobytes()
+            h[116:118] = np.uint16(dt_us).tobytes()
+            f.write(bytes(h))
+            f.write(np.asarray(tr, np.float32).tobytes())
+
+
+def signal_case(name: str, td: str):
+    """An EL_ISO FD4 CPML case (explosive source, SEISMO 1) driven by a signal file."""
+    base, model = case_def("el_fd4_cpml_eq")
+    s = _signal_samples(300, 25.0, DT_CASE)
+    if name == "sig_txt":
+        fn = "signal.txt"
+        lines = ["# source wavelet for the SOURCE_SHAPE 3 case", "#", f"# DT = {DT_CASE!r}", ""]
+        for k, v in enumerate(s):
+            if k == 40:
+                lines.append("   ")
+            lines.append(f"{float(v)!r} trailing words" if k == 7 else ("x-no-number" if k == 11 else f"  {float(v)!r}"))
+        open(os.path.join(td, fn), "w").write("\n".join(lines) + "\n")
+    elif name == "sig_dat":
+        fn = "signal.dat"
+        open(os.path.join(td, fn), "w").write("".join(f"{float(v)!r}\n" for v in s[:120]))
+    else:
+        fn = "signal.su"
+        _write_su_traces(os.path.join(td, fn), [s, -s], int(round(DT_CASE * 1e6)))
+    srcs = [(x, y, 0.0, 25.0, 2.5) for (x, y, *_r) in base.sources[:1]]
+    c = W.Case(**{**base.__dict__, "name": name, "source_shape": 3, "source_type": 1, "sources": srcs,
+                  "seismo": 1, "snap": 0,
+                  "extra": {"SIGNAL_FILE": "./" + fn, "SIGOUT": 1, "SIGOUT_FILE": "./out/signal_out",
+                            "SIGOUT_FORMAT": 1}})
+    return W.write_case(td, c, model), c
+
+
+def decomposed_case(td: str, npx: int = 2, npy: int = 3):
+    """el_fd2_cpml (snapshots of every component, IDX 2, IDY 3) with the
+    reference's process grid NPROCX x NPROCY in the parameter file."""
+    c, model = case_def("el_fd2_cpml")
+    c = W.Case(**{**c.__dict__, "extra": {"NPROCX": npx, "NPROCY": npy}})
+    return W.write_case(td, c, model), c
+
+
 def case_def(name: str) -> tuple[W.Case, dict]:
     table = CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else UNSTABLE
     weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = table[name]

@@ -17,7 +17,11 @@ has the reference at /root/reference and its build in oracle/_ref).
 3. ref1/fwi_*.npz: measured data and FWI gradients of the reference build
    at DTINV = 1 (tests/golden/cases.py FWI_CASES).
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi]
+4. ref1/sig_*.npz: SOURCE_SHAPE 3 runs (signal files in text and SU form).
+5. ref1/snap_ranks_2x3.npz: the per-process snapshot files of a 2 x 3
+   process run.
+
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks]
 """
 from __future__ import annotations
 
@@ -171,6 +175,38 @@ def ref1():
             print("ref1", name, "aborts at step", steps[0], flush=True)
 
 
+def signals():
+    """SOURCE_SHAPE 3 (tests/golden/cases.py SIGNAL_CASES): the reference
+    build's SIGOUT file and vx / vy seismograms, one process."""
+    out = os.path.join(HERE, "ref1")
+    os.makedirs(out, exist_ok=True)
+    for name in cases.SIGNAL_CASES:
+        with tempfile.TemporaryDirectory() as td:
+            js, c = cases.signal_case(name, td)
+            run_ref(js, td)
+            r = collect(os.path.join(td, "out", ""), None, 1, 0, 0)
+            r["sigout_su"] = np.frombuffer(open(os.path.join(td, "out", "signal_out.shot1.su"), "rb").read(), np.uint8)
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, flush=True)
+
+
+def snapranks():
+    """The per-process snapshot files of a 2 x 3 process run of the
+    reference build (tests/golden/cases.py decomposed_case)."""
+    out = os.path.join(HERE, "ref1")
+    with tempfile.TemporaryDirectory() as td:
+        js, c = cases.decomposed_case(td)
+        run_ref(js, td, nproc=6)
+        r = {}
+        for comp in COMPS:
+            for px in range(2):
+                for py in range(3):
+                    b = open(os.path.join(td, "out", f"snap.bin.{comp}.{px}.{py}"), "rb").read()
+                    r[f"{comp}_{px}_{py}"] = np.frombuffer(b, np.uint8)
+        np.savez_compressed(os.path.join(out, "snap_ranks_2x3.npz"), **r)
+        print("ref1 snap_ranks_2x3", flush=True)
+
+
 def c1():
     """Config 1 at full size on the reference, one process."""
     out = os.path.join(HERE, "ref1")
@@ -210,6 +246,6 @@ def fwi():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi"]
+    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks"]
     for w in what:
-        {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi}[w]()
+        {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks}[w]()

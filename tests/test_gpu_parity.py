@@ -78,6 +78,17 @@ def test_cases_vs_reference_1proc(name, shape):
     print(name, errs)
 
 
+@pytest.mark.parametrize("name", cases.SIGNAL_CASES)
+def test_signal_file_cases_vs_reference_1proc(name):
+    """SOURCE_SHAPE 3: wavelets read from text and SU signal files."""
+    td = tempfile.mkdtemp(prefix="sofi_sig_")
+    js, c = cases.signal_case(name, td)
+    secs, snaps = _run_gpu(js)
+    r = common.ref1(name)
+    errs = _check(secs, snaps, {k[5:]: v for k, v in r.items() if k.startswith("seis_")}, {})
+    print(name, errs)
+
+
 def test_unstable_quirk_aborts():
     """EL_VTI + free surface diverges in the reference (abort at the step in
     the fixture); the device path reports the same instability."""
