@@ -3,12 +3,14 @@
  * adjoint sources (residuals) and the gradient files.
  *
  *   sofi_read_data      inseis      src/inseis.c:28-80
- *   sofi_residual       calc_res    src/calc_res.c:26-324 (LNORM 2)
+ *   sofi_residual       calc_res    src/calc_res.c:26-324 (LNORM 1-5, 7, 8)
  *   sofi_write_gradient output_gradient + writemod
  *                                   src/output_gradient.c:25-68, src/writemod.c:28-45
  */
 #define _GNU_SOURCE
+#include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "sofi_host_internal.h"
@@ -37,35 +39,83 @@ int sofi_read_data(const sofi_params *p, int ishot, int comp, int ntr, float *se
     return 0;
 }
 
-/* L2 residual of calc_res: with VELOCITY = 0 both sections are integrated
- * in time (running sum times DT), then diff = synthetic - measured, stored
- * time-reversed (diff[i][NS - j + 1]).  *l2 receives the sum of diff^2 (the
- * test-shot misfit of the reference, accumulated in float). */
+/* calc_res (src/calc_res.c:26-324), one component, traces 0..ntr-1 of
+ * [ntr][NS], the adjoint source stored time-reversed (diff[i][NS - j + 1]):
+ *   LNORM 1: sign(measured - synthetic) of the raw sections;
+ *   LNORM 2: synthetic - measured, both integrated in time first (running
+ *            sum times DT) when VELOCITY = 0;
+ *   LNORM 3: r / (1 + r^2) and LNORM 4: tanh(r), r = measured - synthetic
+ *            of the raw sections ("not up to date" in the reference);
+ *   LNORM 5, 7: the derivative of the normalised correlation of the
+ *            (integrated) traces, with the trace norms taken as 1 for a
+ *            zero trace;
+ *   LNORM 8: the envelope residual, whose envelope and Hilbert transforms
+ *            the reference leaves at zero (their calls are commented out):
+ *            s / (0 + WATERLEVEL_LNORM8) * (0 - 0) - 0.
+ * Float arithmetic in the reference's order; RMS is 1.  *l2 receives the
+ * misfit calc_res accumulates for a test shot (float): LNORM 2 the sum of
+ * squares, LNORM 5 / 7 minus the sum of the normalised products, LNORM 8
+ * the (zero) envelope differences, else 0. */
 int sofi_residual(const sofi_params *p, int ntr, const float *data, const float *calc, float *diff, double *l2)
 {
-    if (p->lnorm != 2) return sofi_fail("LNORM=%d is not supported by this build (L2 only).", p->lnorm);
-    const int ns = p->ns;
+    const int ns = p->ns, ln = p->lnorm;
+    if (ln < 1 || ln > 8 || ln == 6) return sofi_fail("LNORM=%d has no residual in the reference.", ln);
+    float *is = (float *)malloc(sizeof(float) * (size_t)(ns > 0 ? ns : 1));
+    float *isd = (float *)malloc(sizeof(float) * (size_t)(ns > 0 ? ns : 1));
+    if (!is || !isd) {
+        free(is);
+        free(isd);
+        return sofi_fail("out of memory");
+    }
+    const float RMS = 1.0f, wl = p->waterlevel_lnorm8;
     float misfit = 0.0f;
     for (int r = 0; r < ntr; r++) {
         const float *s = calc + (size_t)r * ns, *sd = data + (size_t)r * ns;
         float *df = diff + (size_t)r * ns;
-        float intseis_s = 0.0f, intseis_sd = 0.0f;
-        for (int j = 1; j <= ns; j++) {
-            float is, isd;
+        float acc_s = 0.0f, acc_sd = 0.0f;
+        for (int j = 0; j < ns; j++) {
             if (p->velocity == 0) {
-                intseis_s += s[j - 1];
-                is = intseis_s * p->dt;
-                intseis_sd += sd[j - 1];
-                isd = intseis_sd * p->dt;
+                acc_s += s[j];
+                is[j] = acc_s * p->dt;
+                acc_sd += sd[j];
+                isd[j] = acc_sd * p->dt;
             } else {
-                is = s[j - 1];
-                isd = sd[j - 1];
+                is[j] = s[j];
+                isd[j] = sd[j];
             }
-            const int invtime = ns - j + 1;
-            df[invtime - 1] = is - isd;
-            misfit += df[invtime - 1] * df[invtime - 1];
+        }
+        float abs_sd = 0.0f, abs_s = 0.0f, mult = 0.0f;
+        if (ln == 5 || ln == 7) {
+            for (int j = 0; j < ns; j++) {
+                abs_sd += isd[j] * isd[j];
+                abs_s += is[j] * is[j];
+                mult += isd[j] * is[j];
+            }
+            abs_sd = abs_sd == 0 ? 1.0f : (float)sqrt(abs_sd);
+            abs_s = abs_s == 0 ? 1.0f : (float)sqrt(abs_s);
+        }
+        for (int j = 1; j <= ns; j++) {
+            float v = 0.0f;
+            const float rr = (sd[j - 1] - s[j - 1]) / RMS;
+            switch (ln) {
+              case 1: v = (rr > 0 ? 1.0f : rr < 0 ? -1.0f : 0.0f) / RMS; break;
+              case 2: v = is[j - 1] - isd[j - 1]; break;
+              case 3: v = (rr / (1 + (rr * rr))) / RMS; break;
+              case 4: v = (float)tanh(rr) / RMS; break;
+              case 5:
+              case 7:
+                  v = ((is[j - 1] * mult) / (abs_s * abs_s * abs_s * abs_sd)) - (isd[j - 1] / (abs_s * abs_sd));
+                  break;
+              case 8: v = is[j - 1] / (0.0f + wl) * (0.0f - 0.0f) - 0.0f; break;
+            }
+            df[ns - j] = v;
+            if (ln == 2) misfit += v * v;
+            if (ln == 5 || ln == 7) misfit -= (isd[j - 1] * is[j - 1]) / (abs_sd * abs_s);
+            if (ln == 8) misfit += (0.0f - 0.0f) * (0.0f - 0.0f);
         }
     }
+    free(is);
+    free(isd);
     if (l2) *l2 = misfit;
     return 0;
 }

@@ -384,6 +384,7 @@ int sofi_read_params(const char *json_file, sofi_params *p)
         OPT_INT("ADJOINT_TYPE", p->adjoint_type, 1);
         OPT_INT("VELOCITY", p->velocity, 0);
         OPT_INT("LNORM", p->lnorm, 2);
+        if (p->lnorm == 8) REQ_FLT("WATERLEVEL_LNORM8", p->waterlevel_lnorm8);
         OPT_INT("DTINV", p->dtinv, 1);
         OPT_STR("GRADIENT", p->gradient_file, "gradient");
         OPT_INT("TIME_FILT", p->time_filt, 0);
@@ -441,7 +442,8 @@ int sofi_check_supported(const sofi_params *p)
         if (p->adjoint_type < 1 || p->adjoint_type > 3)
             return sofi_fail("MODE=FWI: ADJOINT_TYPE=%d is not supported (1: vx and vy, 2: vy, 3: vx).", p->adjoint_type);
         if (p->seismo != 1 && p->seismo != 4) return sofi_fail("MODE=FWI needs SEISMO=1 or 4 (vx and vy sections).");
-        if (p->lnorm != 2) return sofi_fail("MODE=FWI: LNORM=%d is not supported by this build (L2 only).", p->lnorm);
+        if (p->lnorm < 1 || p->lnorm > 8 || p->lnorm == 6)
+            return sofi_fail("MODE=FWI: LNORM=%d has no residual in the reference (src/calc_res.c: 1-5, 7, 8).", p->lnorm);
         if (p->time_filt) return sofi_fail("MODE=FWI: TIME_FILT is not supported by this build.");
         if (p->eprecond) return sofi_fail("MODE=FWI: EPRECOND is not supported by this build.");
     }

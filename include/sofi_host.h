@@ -52,6 +52,7 @@ typedef struct sofi_params {
     float f_ref, fl[SOFI_MAXL + 1];
     /* FWI (src/read_par_json_fwi.c) */
     int itermax, dtinv, adjoint_type, velocity, lnorm, time_filt, eprecond;
+    float waterlevel_lnorm8;
     char data_dir[SOFI_STR], gradient_file[SOFI_STR];
     /* derived */
     int nd;
@@ -150,8 +151,10 @@ int sofi_problem_snap_comps(const sofi_problem *P, int *comps);
 
 /* Low-level writer shared with tests: one SU trace set (outseis_glob, SEIS_FORMAT 1/2/3). */
 /* FWI gradient path (src/inversion.c:27-258): measured data of a shot
- * (inseis, src/inseis.c:28-80) as section[ntr][ns]; the L2 adjoint sources
- * (calc_res, src/calc_res.c:26-324), time-reversed, *l2 = sum of squares;
+ * (inseis, src/inseis.c:28-80) as section[ntr][ns]; the adjoint sources of
+ * LNORM 1-5, 7 and 8 (calc_res, src/calc_res.c:26-324), time-reversed, with
+ * *l2 the misfit calc_res accumulates for a test shot (LNORM 2: sum of
+ * squares; 5 / 7: minus the normalised correlation; otherwise 0);
  * the gradient files (output_gradient, src/output_gradient.c:25-68), per shot
  * (ishot > 0) or summed (ishot = 0), from padded planes. */
 int sofi_read_data(const sofi_params *p, int ishot, int comp, int ntr, float *section);

@@ -46,6 +46,7 @@ def lib(precision: str = "f32"):
     L.oracle_reset.argtypes = [C.c_void_p, C.c_int]
     L.oracle_fwi_gradient.argtypes = [C.c_void_p, _FP, _FP, _FP]
     L.oracle_fwi_gradient_sum.argtypes = [C.c_void_p, _DP, _DP, _DP]
+    L.oracle_residual.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, _FP, _FP, _FP, _DP]
     _libs[precision] = L
     return L
 
@@ -138,28 +139,28 @@ def run_problem(problem, precision: str = "f32", nt: int | None = None):
     return res
 
 
+def residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int, lnorm: int = 2,
+             waterlevel: float = 0.0, precision: str = "f32"):
+    """calc_res (src/calc_res.c:26-324) on the oracle: the time-reversed
+    adjoint source [ntr][ns] of LNORM 1-5, 7 or 8 and the test-shot misfit."""
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    c = np.ascontiguousarray(calc, dtype=np.float32)
+    out = np.zeros_like(c)
+    m = C.c_double(0.0)
+    rc = lib(precision).oracle_residual(lnorm, velocity, dt, waterlevel, c.shape[0], c.shape[1],
+                                        d.ctypes.data_as(_FP), c.ctypes.data_as(_FP), out.ctypes.data_as(_FP),
+                                        C.byref(m))
+    if rc:
+        raise MemoryError("oracle_residual")
+    return out, m.value
+
+
 def l2_residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int) -> np.ndarray:
-    """calc_res for LNORM 2 (src/calc_res.c:127-143, 220-238): integrate in
-    time unless VELOCITY = 1 (float32 running sums times DT), subtract,
-    reverse in time."""
-    d = np.asarray(data, dtype=np.float32)
-    c = np.asarray(calc, dtype=np.float32)
-    if velocity == 0:
-        dt32 = np.float32(dt)
-        ic = np.empty_like(c)
-        idd = np.empty_like(d)
-        sc = np.zeros(c.shape[0], dtype=np.float32)
-        sd = np.zeros(d.shape[0], dtype=np.float32)
-        for j in range(c.shape[1]):
-            sc = (sc + c[:, j]).astype(np.float32)
-            sd = (sd + d[:, j]).astype(np.float32)
-            ic[:, j] = sc * dt32
-            idd[:, j] = sd * dt32
-        c, d = ic, idd
-    return (c - d)[:, ::-1].astype(np.float32).copy()
+    """calc_res for LNORM 2 (src/calc_res.c:127-143, 220-238)."""
+    return residual(data, calc, dt, velocity, 2)[0]
 
 
-def fwi_gradients(problem, data: dict, precision: str = "f32"):
+def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, waterlevel: float = 0.0):
     """The FWI gradient of every shot of a MODE=FWI problem on the CPU oracle;
     data[ishot] = (vx, vy) measured sections [ntr][ns].  The sums add the
     shots' float32 gradients in shot order, as the reference does."""
@@ -174,8 +175,9 @@ def fwi_gradients(problem, data: dict, precision: str = "f32"):
         calc_x, calc_y = o.section(0).astype(np.float32), o.section(1).astype(np.float32)
         ri, rj = problem.receivers()
         zero = np.zeros((len(ri), d.nt), dtype=np.float32)
-        dx = l2_residual(data[ishot][0], calc_x, d.dt, d.velocity) if d.adjoint_type in (1, 3) else zero
-        dy = l2_residual(data[ishot][1], calc_y, d.dt, d.velocity) if d.adjoint_type in (1, 2) else zero
+        res = lambda q, cal: residual(data[ishot][q], cal, d.dt, d.velocity, lnorm, waterlevel, precision)[0]
+        dx = res(0, calc_x) if d.adjoint_type in (1, 3) else zero
+        dy = res(1, calc_y) if d.adjoint_type in (1, 2) else zero
         o.fwi_set_adjoint(ri, rj, dx, dy)
         o.reset(1)
         o.run()

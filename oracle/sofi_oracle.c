@@ -1209,4 +1209,68 @@ void oracle_free(oracle *o)
 }
 
 /* size of the oracle's step in bytes of REAL (lets tests check which build they loaded) */
+/* calc_res (src/calc_res.c:127-301): the adjoint source of one component
+ * for LNORM 1-5, 7 and 8, traces [ntr][ns], time-reversed into diff, and
+ * the misfit the reference accumulates for a test shot (src/calc_res.c:
+ * 276-291).  RMS = 1 (src/calc_res.c:166); the envelope and Hilbert arrays
+ * of LNORM 8 stay at the zeros matrix() gives them (their computation is
+ * commented out, src/calc_res.c:175-187). */
+static void integrate_trace(const float *x, REAL *out, int ns, int velocity, REAL dt)
+{
+    REAL acc = 0;
+    for (int j = 0; j < ns; ++j) {
+        if (velocity == 0) {
+            acc += x[j];
+            out[j] = acc * dt;
+        } else {
+            out[j] = x[j];
+        }
+    }
+}
+
+int oracle_residual(int lnorm, int velocity, float dt, float waterlevel, int ntr, int ns, const float *data,
+                    const float *calc, float *diff, double *misfit)
+{
+    REAL *is = (REAL *)malloc(sizeof(REAL) * (size_t)ns), *id = (REAL *)malloc(sizeof(REAL) * (size_t)ns);
+    if (!is || !id) {
+        free(is);
+        free(id);
+        return -1;
+    }
+    REAL l2 = 0;
+    for (int tr = 0; tr < ntr; ++tr) {
+        const float *syn = calc + (size_t)tr * ns, *obs = data + (size_t)tr * ns;
+        integrate_trace(syn, is, ns, velocity, (REAL)dt);
+        integrate_trace(obs, id, ns, velocity, (REAL)dt);
+        REAL nd = 0, nsyn = 0, dot = 0;
+        if (lnorm == 5 || lnorm == 7) {
+            for (int j = 0; j < ns; ++j) {
+                nd += id[j] * id[j];
+                nsyn += is[j] * is[j];
+                dot += id[j] * is[j];
+            }
+            nd = nd == 0 ? (REAL)1 : (REAL)sqrt(nd);
+            nsyn = nsyn == 0 ? (REAL)1 : (REAL)sqrt(nsyn);
+        }
+        float *out = diff + (size_t)tr * ns;
+        for (int j = 0; j < ns; ++j) {
+            const REAL r = (REAL)obs[j] - (REAL)syn[j];     /* (sectiondata - section) / RMS */
+            REAL v = 0;
+            if (lnorm == 1) v = r > 0 ? (REAL)1 : (r < 0 ? (REAL)-1 : (REAL)0);
+            else if (lnorm == 2) v = is[j] - id[j];
+            else if (lnorm == 3) v = r / (1 + r * r);
+            else if (lnorm == 4) v = (REAL)tanh(r);
+            else if (lnorm == 5 || lnorm == 7) v = is[j] * dot / (nsyn * nsyn * nsyn * nd) - id[j] / (nsyn * nd);
+            else if (lnorm == 8) v = is[j] / ((REAL)0 + (REAL)waterlevel) * ((REAL)0 - (REAL)0) - (REAL)0;
+            out[ns - 1 - j] = (float)v;
+            if (lnorm == 2) l2 += v * v;
+            else if (lnorm == 5 || lnorm == 7) l2 -= id[j] * is[j] / (nd * nsyn);
+        }
+    }
+    free(is);
+    free(id);
+    *misfit = (double)l2;
+    return 0;
+}
+
 int oracle_real_size(void) { return (int)sizeof(REAL); }

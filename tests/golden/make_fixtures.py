@@ -20,8 +20,9 @@ has the reference at /root/reference and its build in oracle/_ref).
 4. ref1/sig_*.npz: SOURCE_SHAPE 3 runs (signal files in text and SU form).
 5. ref1/snap_ranks_2x3.npz: the per-process snapshot files of a 2 x 3
    process run.
+6. ref1/fwi_lnorm*.npz: FWI gradients for LNORM 1, 3, 4, 5, 7, 8.
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm]
 """
 from __future__ import annotations
 
@@ -245,7 +246,31 @@ def fwi():
             print("ref1", name, "time loops %s s" % t, flush=True)
 
 
+def fwi_lnorm():
+    """Gradient files of the reference build for the other LNORM residuals
+    (tests/golden/cases.py FWI_LNORMS)."""
+    out = os.path.join(HERE, "ref1")
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    for ln, (name, amp) in cases.FWI_LNORMS.items():
+        if os.path.exists(os.path.join(out, f"fwi_lnorm{ln}.npz")):
+            continue
+        with tempfile.TemporaryDirectory() as td:
+            run_ref(cases.fwi_case(td, name, "true", amp=amp), td)
+            r = {}
+            for s in (1, 2):
+                for comp in ("vx", "vy"):
+                    r[f"data_{comp}_shot{s}"] = read_su(os.path.join(td, "data", f"obs_{comp}.su.shot{s}"))
+            run_ref(cases.fwi_case(td, name, "init", lnorm=ln, amp=amp), td)
+            for p in ("vp", "vs", "rho"):
+                for s in (1, 2):
+                    r[f"grad_{p}_shot{s}"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}.shot{s}"), np.float32).reshape(nx, ny)
+                r[f"grad_{p}_sum"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(nx, ny)
+            np.savez_compressed(os.path.join(out, f"fwi_lnorm{ln}.npz"), **r)
+            print("ref1", name, "LNORM", ln, flush=True)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks"]
+    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm"]
     for w in what:
-        {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks}[w]()
+        {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
+         "fwi_lnorm": fwi_lnorm}[w]()

@@ -157,6 +157,31 @@ def test_oracle_fwi_gradients_vs_reference(name):
         P.close()
 
 
+@pytest.mark.parametrize("lnorm", list(cases.FWI_LNORMS))
+def test_oracle_fwi_lnorm_gradients_vs_reference(lnorm):
+    """The other residuals of calc_res (LNORM 1 sign, 3 Cauchy, 4 tanh, 5 / 7
+    normalised correlation, 8 the envelope residual with the reference's
+    zero envelopes): FP32 oracle gradients bit for bit the reference
+    build's, per shot and summed."""
+    import pyoracle
+    from gpi_project_amd.sofi import Problem
+    name, amp = cases.FWI_LNORMS[lnorm]
+    z = common.fwi_fixture(f"fwi_lnorm{lnorm}")
+    td = tempfile.mkdtemp(prefix="sofi_fwi_")
+    js = cases.fwi_case(td, name, "init", lnorm=lnorm, amp=amp)
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        r = pyoracle.fwi_gradients(P, common.fwi_measured(z), "f32", lnorm=lnorm, waterlevel=1e-3)
+        for k, p in enumerate(("vp", "vs", "rho")):
+            for s in (1, 2):
+                g = common.xmajor(r["shots"][s]["gradient"][k], P.nx, P.ny).astype(np.float32)
+                assert np.array_equal(g, z[f"grad_{p}_shot{s}"]), (lnorm, p, s)
+            assert np.array_equal(common.xmajor(r["sum"][k], P.nx, P.ny).astype(np.float32), z[f"grad_{p}_sum"])
+        P.close()
+    nonzero = any(np.abs(z[f"grad_{p}_sum"]).max() > 0 for p in ("vp", "vs", "rho"))
+    assert nonzero == (lnorm != 8)
+
+
 def test_oracle_fwi_gradient_is_zero_for_zero_residual_and_linear_in_it():
     """Properties of the adjoint path that hold without the reference: with
     measured data equal to the synthetics the residuals and so the gradient
