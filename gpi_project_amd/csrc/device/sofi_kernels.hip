@@ -341,6 +341,16 @@ __global__ void k_psource(Grid g, Fields f, Src s, Inj inj, int nt, const int *n
     if (k >= inj.n) return;
     nt = step_of(ntb, nt);
     const int c = inj.cell[k];
+    if (f.syy == f.sxx) {     // acoustic: sxx += amp, then syy (the same array) += amp
+        float p = f.sxx[c];
+        for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) {
+            const float amp = psource_amp(g, s, inj.idx[q], nt);
+            p += amp;
+            p += amp;
+        }
+        f.sxx[c] = p;
+        return;
+    }
     float sxx = f.sxx[c], syy = f.syy[c];
     for (int q = inj.first[k]; q < inj.first[k + 1]; ++q) {
         const float amp = psource_amp(g, s, inj.idx[q], nt);
@@ -1288,6 +1298,68 @@ __global__ __launch_bounds__(BX *BY) void k_stress4(Grid g, Fields f, Mat m, Src
     f.sxy[c] = sxy;
 }
 
+// ------------------------------------------------------------ acoustic
+
+// update_s_acoustic_{interior,PML,abs} with wavefield_update_s_ac
+// (src/update_s_acoustic_interior.c:33-53, src/update_s_acoustic_PML.c:33-124,
+// src/update_s_acoustic_abs.c:107-171, src/wavefield_update_s_ac.c:27-33),
+// one thread per cell: vxx, vyy with the x / y CPML of the reference's
+// cpml_update_s_x_ac / _y_ac (src/cpml_update.c:70-80) in the frame, then
+// the pressure (sxx and syy are one array) takes g vxx and then g vyy; the
+// damping frame scales it once (abs_update_s_ac2, src/abs_update.c:28-31);
+// fused explosive sources add each amplitude twice (psource adds it to sxx
+// and to syy).
+template <int ND, bool ONE>
+__global__ __launch_bounds__(BX *BY) void k_stress_ac(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+{
+    if (ONE && fused_seis(g, f0, fz)) return;
+    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
+    if (frame && g.abs_type != 1 && g.abs_type != 2) return;
+    const int c = IDX(g, i, j), P = g.pitch;
+    const float gc = m.g[c];
+    const int k0 = ONE ? 0 : g.k_lo + blockIdx.y * g.spt, k1 = ONE ? 1 : min(k0 + g.spt, g.k_hi);
+    for (int k = k0; k < k1; ++k) {
+        Fields f = ONE ? f0 : f0.at(k, g);
+        float a = 0.f, e = 0.f;
+#pragma unroll
+        for (int mm = 1; mm <= ND; ++mm) {
+            const float h = g.hc[mm];
+            a += h * (f.vx[c + mm - 1] - f.vx[c - mm]);
+            e += h * (f.vy[c + (mm - 1) * P] - f.vy[c - mm * P]);
+        }
+        float vxx = a * g.dhi, vyy = e * g.dhi;
+        if (frame && g.abs_type == 1) {
+            if (xf) {
+                const int hh = hx(g, i), o = j * g.wx + hh;
+                const float px = PRF(g, m, SOFI_P_B_X, hh) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, hh) * vxx;
+                f.psi_vxx[o] = px;
+                vxx = vxx / PRF(g, m, SOFI_P_K_X, hh) + px;
+            }
+            if (yf) {
+                const int hh = hy(g, j), o = hh * g.wy + i;
+                const float py = PRF(g, m, SOFI_P_B_Y, hh) * f.psi_vyy[o] + PRF(g, m, SOFI_P_A_Y, hh) * vyy;
+                f.psi_vyy[o] = py;
+                vyy = vyy / PRF(g, m, SOFI_P_K_Y, hh) + py;
+            }
+        }
+        float p = f.sxx[c];
+        p += gc * vxx;
+        p += gc * vyy;
+        if (frame && g.abs_type == 2) p *= m.absorb[c];
+        for (int q = 0; q < fz.npsrc; ++q)
+            if (c == fz.c[q]) {
+                const float amp = psource_amp(g, s, q, step_of(fz.ntb, nt));
+                p += amp;
+                p += amp;
+            }
+        f.sxx[c] = p;
+    }
+}
+
 // ------------------------------------------------------------ TTI passes
 
 // v_derivatives (src/v_derivatives.c:27-50)
@@ -1473,7 +1545,7 @@ __global__ void k_snap(Grid g, Fields f, Mat m, float *out, int i0, int di, int 
     vxy *= dhi;
     vxx *= dhi;
     vyy *= dhi;
-    out[SOFI_C_CURL * per + k] = (float)((double)(vxy - vyx) * sqrt((double)m.pu[c]));
+    out[SOFI_C_CURL * per + k] = m.pu ? (float)((double)(vxy - vyx) * sqrt((double)m.pu[c])) : 0.0f;   // (none: acoustic)
     out[SOFI_C_DIV * per + k] = (float)((double)(vxx + vyy) * sqrt((double)m.ppi[c]));
 }
 
@@ -1801,7 +1873,10 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
         return fail("sofi_open_batch: %d slots of a %d x %d grid exceed the 2^31-float index range", nshots, d.nx, d.ny);
     if (d.nx < 1 || d.ny < 1) return fail("sofi_open: bad grid %d x %d", d.nx, d.ny);
     if (d.fdorder < 2 || d.fdorder > 12 || (d.fdorder & 1)) return fail("sofi_open: unsupported FDORDER %d", d.fdorder);
-    if (d.weq < SOFI_EL_ISO || d.weq > SOFI_VEL_TTI) return fail("sofi_open: unsupported WEQ %d", d.weq);
+    if ((d.weq < SOFI_EL_ISO || d.weq > SOFI_VEL_TTI) && d.weq != SOFI_AC_ISO)
+        return fail("sofi_open: unsupported WEQ %d", d.weq);
+    if (d.weq == SOFI_AC_ISO && (d.free_surf || d.mode == SOFI_MODE_FWI || d.fdorder_time == 4))
+        return fail("sofi_open: AC_ISO runs without free surface, FWI or 4th order in time (as the reference)");
     if (d.L < 0 || d.L > SOFI_MAXL) return fail("sofi_open: L=%d out of range", d.L);
     bool visco = d.weq == SOFI_VEL_ISO || d.weq == SOFI_VEL_VTI || d.weq == SOFI_VEL_TTI;
     if (visco && d.L < 1) return fail("sofi_open: L>0 required for viscoelastic rheologies");
@@ -1868,7 +1943,10 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     rc |= dalloc_t(h, &f.vx, pl * nb);
     rc |= dalloc_t(h, &f.vy, pl * nb);
     rc |= dalloc_t(h, &f.sxx, pl * nb);
-    rc |= dalloc_t(h, &f.syy, pl * nb);
+    // the acoustic path keeps one pressure array: syy is sxx, as psyy is
+    // psxx in the reference (src/initmem_wavefield.c:31-33); sxy stays zero
+    if (d.weq == SOFI_AC_ISO) f.syy = f.sxx;
+    else rc |= dalloc_t(h, &f.syy, pl * nb);
     rc |= dalloc_t(h, &f.sxy, pl * nb);
     rc |= dalloc_t(h, &f.unstable, nb);
     rc |= dalloc_t(h, &h->ntb, 1);
@@ -1958,10 +2036,13 @@ int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const 
     int rc = 0;
     rc |= need(SOFI_A_RIP);
     rc |= need(SOFI_A_RJP);
-    rc |= need(SOFI_A_PU);
+    if (d.weq != SOFI_AC_ISO) rc |= need(SOFI_A_PU);
     rc |= need(SOFI_A_PPI);
     if (rc) return -1;
     switch (d.weq) {
+      case SOFI_AC_ISO:
+          rc |= need(SOFI_A_G);
+          break;
       case SOFI_EL_ISO:
           rc |= need(SOFI_A_F) | need(SOFI_A_G) | need(SOFI_A_FIPJP);
           break;
@@ -2093,7 +2174,8 @@ int sofi_set_medium(sofi_dev *h, const float *const *arrays, const float *const 
         }
     }
     h->h_ppi.assign(arrays[SOFI_A_PPI], arrays[SOFI_A_PPI] + pl);
-    h->h_pu.assign(arrays[SOFI_A_PU], arrays[SOFI_A_PU] + pl);
+    if (arrays[SOFI_A_PU]) h->h_pu.assign(arrays[SOFI_A_PU], arrays[SOFI_A_PU] + pl);
+    else h->h_pu.assign(pl, 0.0f);     // the acoustic path has no shear modulus
     h->medium_set = true;
     h->gen++;
     return 0;
@@ -2622,6 +2704,14 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
             else launch(h, k_stress<ND, W, false>, tiles, blk, gg, fk, m, h->s, fss, nt);            \
         } while (0)
         switch (d.weq) {
+          case SOFI_AC_ISO: {     // one thread per cell whatever the stress shape
+              dim3 ta((g.nx + BX - 1) / BX * nty, nchunk);
+              if (seis_fused && one) ta.x += fss.sblk;
+              if (seis_fused && one) fss.ntile = (int)((g.nx + BX - 1) / BX * nty);
+              if (one) launch(h, k_stress_ac<ND, true>, ta, blk, gg, fk, m, h->s, fss, nt);
+              else launch(h, k_stress_ac<ND, false>, ta, blk, gg, fk, m, h->s, fss, nt);
+              break;
+          }
           case SOFI_EL_ISO: SOFI_STRESS(SOFI_EL_ISO); break;
           case SOFI_VEL_ISO: SOFI_STRESS(SOFI_VEL_ISO); break;
           case SOFI_EL_VTI: SOFI_STRESS(SOFI_EL_VTI); break;

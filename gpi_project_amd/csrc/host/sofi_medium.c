@@ -168,6 +168,32 @@ static void visco_taus(const sofi_params *p, sofi_model *m, float *pts, float *w
     *ws = (float)(2.0 * M_PI * p->f_ref);
 }
 
+/* readmod_acoustic (src/readmod_acoustic.c:31-129) and prepmod for AC_ISO
+ * (src/prepmod.c:40-43, matcopy_elastic + av_rho; prepare_update_s_ac,
+ * src/prepare_update_s_ac.c:26-33: g = pi * DT).  The S velocities are 0,
+ * which checkfd takes as the minimum velocity. */
+static int build_ac_iso(const sofi_params *p, sofi_model *m, float **para)
+{
+    int pitch_ = m->pitch;
+    float *ppi = NEW(m, SOFI_A_PPI, 1), *rho = m->prho;
+    minmax_init(m);
+    for (int i = 1; i <= p->nxg; i++)
+        for (int j = 1; j <= p->nyg; j++) {
+            size_t k = (size_t)(i - 1) * p->nyg + (j - 1);
+            float vp = para[0][k], r = para[1][k];
+            minmax(m, vp, 0.0f, vp, vp, 0.0f, 0.0f);
+            AT(rho, i, j) = r;
+            AT(ppi, i, j) = vp * vp * r;
+        }
+    m->vsmin = 0.0f;
+    m->vsmax = 0.0f;
+    av_rho(rho, NEW(m, SOFI_A_RIP, 1), NEW(m, SOFI_A_RJP, 1), m->nx, m->ny, pitch_);
+    float *g = NEW(m, SOFI_A_G, 1);
+    for (int j = 1; j <= m->ny; j++)
+        for (int i = 1; i <= m->nx; i++) AT(g, i, j) = AT(ppi, i, j) * p->dt;
+    return 0;
+}
+
 static int build_el_iso(const sofi_params *p, sofi_model *m, float **para)
 {
     int pitch_ = m->pitch;
@@ -624,6 +650,12 @@ int sofi_build_model(const sofi_params *p, sofi_model *m)
     float *para[8] = { 0 };
     int rc = 0;
     switch (p->weq) {
+      case SOFI_AC_ISO: {
+          const char *sfx[] = { "vp", "rho" };
+          rc = read_model_files(p, 2, sfx, para);
+          if (!rc) rc = build_ac_iso(p, m, para);
+          break;
+      }
       case SOFI_EL_ISO: {
           const char *sfx[] = { "vp", "vs", "rho" };
           rc = read_model_files(p, 3, sfx, para);

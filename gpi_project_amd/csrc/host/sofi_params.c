@@ -410,8 +410,19 @@ fail:
  * than run wrongly. */
 int sofi_check_supported(const sofi_params *p)
 {
-    if (p->weq < SOFI_EL_ISO || p->weq > SOFI_VEL_TTI)
-        return sofi_fail("WEQ %s is not supported by this build (elastic and viscoelastic ISO/VTI/TTI only).", weq_names[p->weq]);
+    if ((p->weq < SOFI_EL_ISO || p->weq > SOFI_VEL_TTI) && p->weq != SOFI_AC_ISO)
+        return sofi_fail("WEQ %s is not supported by the reference's time loop (src/time_loop.c:123-212).", weq_names[p->weq]);
+    if (p->weq == SOFI_AC_ISO) {
+        /* the acoustic path keeps one pressure array (psyy aliases psxx,
+         * src/initmem_wavefield.c:31-33) and allocates no sxy */
+        if (p->fdorder_time == 4) return sofi_fail("WEQ not yet implemented for 4th order in time.");
+        if (p->free_surf)
+            return sofi_fail("FREE_SURF with AC_ISO: the reference's surface_elastic reads the unallocated pu "
+                             "(src/surface_elastic.c:80, src/initmem_model.c:32-39); refused.");
+        if (p->source_type == 5)
+            return sofi_fail("SOURCE_TYPE=5 with AC_ISO: eqsource writes the unallocated sxy (src/eqsource.c); refused.");
+        if (p->mode == SOFI_MODE_FWI) return sofi_fail("MODE=FWI: only WEQ=EL_ISO is supported by this build.");
+    }
     if (p->fdorder_time == 4) {
         /* the reference's 4th-order-in-time loop exists for the isotropic
          * rheologies only (src/prepmod.c:162-187, src/time_loop.c:215-245) */

@@ -160,10 +160,13 @@ def l2_residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int) ->
     return residual(data, calc, dt, velocity, 2)[0]
 
 
-def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, waterlevel: float = 0.0):
+def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, waterlevel: float = 0.0,
+                  synthetic: dict | None = None):
     """The FWI gradient of every shot of a MODE=FWI problem on the CPU oracle;
     data[ishot] = (vx, vy) measured sections [ntr][ns].  The sums add the
-    shots' float32 gradients in shot order, as the reference does."""
+    shots' float32 gradients in shot order, as the reference does.  With
+    synthetic[ishot] = (vx, vy) the residuals use those sections instead of
+    the oracle's own forward pass."""
     d = problem.desc
     shots = {}
     for ishot in range(1, problem.nshots + 1):
@@ -173,6 +176,8 @@ def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, w
         o.reset(0)
         o.run()
         calc_x, calc_y = o.section(0).astype(np.float32), o.section(1).astype(np.float32)
+        if synthetic is not None:     # residuals from given synthetic sections (another build's forward pass)
+            calc_x, calc_y = (np.asarray(a, dtype=np.float32) for a in synthetic[ishot])
         ri, rj = problem.receivers()
         zero = np.zeros((len(ri), d.nt), dtype=np.float32)
         res = lambda q, cal: residual(data[ishot][q], cal, d.dt, d.velocity, lnorm, waterlevel, precision)[0]

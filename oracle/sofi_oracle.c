@@ -122,6 +122,10 @@ oracle *oracle_new(const sofi_desc *d, const float *const *arrays, const float *
     o->dhi = (REAL)1 / (REAL)d->dh; /* src/initfd.c:366: 1.f / DH */
     o->vx = zalloc(o->plane); o->vy = zalloc(o->plane);
     o->sxx = zalloc(o->plane); o->syy = zalloc(o->plane); o->sxy = zalloc(o->plane);
+    if (d->weq == SOFI_AC_ISO) {   /* one pressure array: psyy is psxx (src/initmem_wavefield.c:31-33) */
+        free(o->syy);
+        o->syy = o->sxx;
+    }
     o->vxx = zalloc(o->plane); o->vyy = zalloc(o->plane); o->vyx = zalloc(o->plane); o->vxy = zalloc(o->plane);
     int L = o->L > 0 ? o->L : 1;
     o->r = zalloc(o->plane * L); o->p = zalloc(o->plane * L); o->q = zalloc(o->plane * L);
@@ -381,6 +385,14 @@ static void store_u(oracle *o, int i, int j, REAL u1, REAL u2, REAL u3)
     }
 }
 
+/* wavefield_update_s_ac (src/wavefield_update_s_ac.c:27-33): sxx and syy
+ * are one array, so the pressure takes g vxx, then g vyy */
+static void s_ac(oracle *o, int i, int j)
+{
+    F(o->sxx, i, j) += (M(SOFI_A_G, i, j) * F(o->vxx, i, j));
+    F(o->syy, i, j) += (M(SOFI_A_G, i, j) * F(o->vyy, i, j));
+}
+
 /* wavefield_update_s_el (src/wavefield_update_s_el.c:28-48) */
 static void s_el(oracle *o, int i, int j)
 {
@@ -522,6 +534,7 @@ static void s_visc_tti(oracle *o, int i, int j)
 static void s_cell(oracle *o, int i, int j)
 {
     switch (o->d.weq) {
+      case SOFI_AC_ISO: s_ac(o, i, j); break;
       case SOFI_EL_ISO: s_el(o, i, j); break;
       case SOFI_VEL_ISO: s_visc(o, i, j); break;
       case SOFI_EL_VTI: s_el_vti(o, i, j); break;
@@ -534,6 +547,10 @@ static void s_cell(oracle *o, int i, int j)
 static void abs_s(oracle *o, int i, int j)
 {
     REAL c = M(SOFI_A_ABSORB, i, j);
+    if (o->d.weq == SOFI_AC_ISO) {   /* abs_update_s_ac2 (src/abs_update.c:28-31): the pressure once */
+        F(o->sxx, i, j) *= c;
+        return;
+    }
     F(o->sxy, i, j) *= c;
     F(o->syy, i, j) *= c;
     F(o->sxx, i, j) *= c;
@@ -700,7 +717,8 @@ static void seismo(oracle *o, int lsamp)
     for (int k = 0; k < o->ntr; k++) {
         int i = o->ri[k], j = o->rj[k];
         size_t s = (size_t)k * d->ns + (ins - 1);
-        REAL ppi = M(SOFI_A_PPI, i, j), pu = M(SOFI_A_PU, i, j);
+        /* (the acoustic path has no pu; it records SEISMO 1 and 2 only) */
+        REAL ppi = M(SOFI_A_PPI, i, j), pu = o->a[SOFI_A_PU] ? M(SOFI_A_PU, i, j) : 0;
         switch (d->seismo) {
           case 1:
               o->sec[SOFI_C_VX][s] = F(o->vx, i, j);
@@ -745,7 +763,7 @@ static void snap(oracle *o)
             }
             vyx *= dhi;
             vxy *= dhi;
-            base[SOFI_C_CURL * per + k] = (vxy - vyx) * sqrt(M(SOFI_A_PU, i, j));
+            base[SOFI_C_CURL * per + k] = o->a[SOFI_A_PU] ? (vxy - vyx) * sqrt(M(SOFI_A_PU, i, j)) : 0;
             for (int m = 1; m <= o->nd; m++) {
                 vxx += o->hc[m] * (F(o->vx, i + m - 1, j) - F(o->vx, i - m, j));
                 vyy += o->hc[m] * (F(o->vy, i, j + m - 1) - F(o->vy, i, j - m));
@@ -1185,6 +1203,7 @@ int oracle_get_snapshot(const oracle *o, int k, int comp, double *out)
 void oracle_free(oracle *o)
 {
     if (!o) return;
+    if (o->syy == o->sxx) o->syy = NULL;
     REAL *bufs[] = { o->vx, o->vy, o->sxx, o->syy, o->sxy, o->vxx, o->vyy, o->vyx, o->vxy, o->r, o->p, o->q,
                      o->psi_sxx_x, o->psi_sxy_x, o->psi_syy_y, o->psi_sxy_y, o->psi_vxx, o->psi_vyx, o->psi_vyy,
                      o->psi_vxy, o->sazi, o->samp, o->sig, o->snapbuf, o->ux, o->uy, o->uxy, o->pvxp1, o->pvyp1,

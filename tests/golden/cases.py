@@ -131,6 +131,17 @@ FDT4_CASES = {
     "el_fdt4_fd2_abs_eq": ("EL_ISO", 2, 0, 0, 2, 0, 5, (25.0, 5), 4, 55),
 }
 
+# The acoustic path (AC_ISO): one pressure array, SEISMO / SNAP 1 (vx, vy)
+# or 2 (p); no free surface and no moment sources (the reference touches
+# unallocated arrays there).  Explosive sources, forces in x and y.
+AC_CASES = {
+    "ac_fd4_cpml": ("AC_ISO", 4, 0, 0, 1, 0, 1, (), 2, 61),
+    "ac_fd8_h2_abs": ("AC_ISO", 8, 2, 0, 2, 0, 1, (), 1, 62),
+    "ac_fd2_cpml_fx": ("AC_ISO", 2, 0, 0, 1, 0, 2, (), 2, 63),
+    "ac_fd12_h1_abs_fy": ("AC_ISO", 12, 1, 0, 2, 0, 3, (), 1, 64),
+    "ac_fd6_cpml_fx": ("AC_ISO", 6, 3, 0, 1, 0, 2, (), 1, 65),
+}
+
 # EL_VTI/EL_TTI with a free surface: surface_elastic() reads pu (= c55) and
 # ppi (= c11 already multiplied by DT in prepmod), so the sxx correction is
 # off by ~1/DT and the run diverges; the reference aborts with "Simulation is
@@ -205,14 +216,15 @@ def decomposed_case(td: str, npx: int = 2, npy: int = 3):
 
 
 def case_def(name: str) -> tuple[W.Case, dict]:
-    table = CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else UNSTABLE
+    table = (CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else AC_CASES if name in AC_CASES
+             else UNSTABLE)
     weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = table[name]
     model = _model(weq, seed, L=L)
     c = W.Case(name=name, nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdo,
                maxrelerror=mre, L=L, fl=(4.0, 12.0, 30.0, 60.0, 90.0)[:max(L, 1)], f_ref=20.0,
                abs_type=absn, fw=12, free_surf=fs, damping=8.0, fpml=20.0, vppml=3500.0,
                source_type=st, sources=_src(NX, NY, DH, extra=extra), receivers=_rec_grid(NX, NY, DH),
-               seismo=seismo, ndt=2, snap=4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
+               seismo=seismo, ndt=2, snap=seismo if weq == "AC_ISO" else 4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
                idx=2, idy=3, fdorder_time=4 if name in FDT4_CASES else 2)
     return c, model
 
@@ -342,7 +354,7 @@ def write_measured_data(td: str, data: dict):
 
 
 def multishot_case(td: str, weq: str, nshots: int, source_type: int = 1, free_surf: int = 0, abs_type: int = 1,
-                   fdorder: int = 6) -> str:
+                   fdorder: int = 6, seismo: int = 4) -> str:
     """nshots shots (RUN_MULTIPLE_SHOTS) on one 160 x 120 medium of the given
     rheology, sources spread along x at depth 6 cells (one on row 1 when the
     free surface is on), the common receiver set of the coverage cases."""
@@ -356,5 +368,5 @@ def multishot_case(td: str, weq: str, nshots: int, source_type: int = 1, free_su
     c = W.Case(name=f"ms_{weq.lower()}_{nshots}", nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=300 * DT_CASE, weq=weq,
                fdorder=fdorder, maxrelerror=1, L=L, fl=(4.0, 30.0), f_ref=20.0, abs_type=abs_type, fw=12,
                free_surf=free_surf, damping=8.0, fpml=20.0, vppml=3500.0, source_type=source_type, sources=srcs,
-               receivers=_rec_grid(NX, NY, DH), seismo=4, ndt=2, run_multiple_shots=1)
+               receivers=_rec_grid(NX, NY, DH), seismo=seismo, ndt=2, run_multiple_shots=1)
     return W.write_case(td, c, _model(weq, 60 + nshots, L=L))

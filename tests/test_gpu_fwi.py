@@ -57,11 +57,17 @@ def test_fwi_gradients_vs_reference(name):
 
 @pytest.mark.parametrize("lnorm", list(cases.FWI_LNORMS))
 def test_fwi_lnorm_gradients_vs_reference(lnorm):
-    """The other calc_res residuals through the device path against the
-    reference build's gradient files (LNORM 8: zero, as the reference's
-    envelopes are).  LNORM 1 takes the sign of measured - synthetic, which
-    flips where the two agree to rounding, so it is held to cosine 0.999."""
+    """The other calc_res residuals through the device path.  Against the
+    FP32 oracle fed the device's own synthetic sections (the same adjoint
+    sources): cosine >= 0.99999.  Against the reference build's gradient
+    files: cosine >= 0.9999, except LNORM 1, whose adjoint source is the
+    sign of measured - synthetic and flips wherever the two agree to
+    rounding: there the FP64 oracle itself is at cosine 0.986 / 0.78 / 0.84
+    (vp / vs / rho) from the FP32 reference, and the device is held to
+    0.75.  LNORM 8 is zero, as the reference's envelopes are."""
+    import pyoracle
     from gpi_project_amd import fwi
+    from gpi_project_amd.sofi import Problem
     name, amp = cases.FWI_LNORMS[lnorm]
     z = common.fwi_fixture(f"fwi_lnorm{lnorm}")
     td = tempfile.mkdtemp(prefix="sofi_gfwi_")
@@ -70,15 +76,22 @@ def test_fwi_lnorm_gradients_vs_reference(lnorm):
     nx, ny = cases.FWI_NX, cases.FWI_NY
     with chdir(td):
         r = fwi.gradient_json(os.path.basename(js), verbose=False, write=False)
+        P = Problem(os.path.basename(js))
+        syn = {s: (r["shots"][s]["sections"][0], r["shots"][s]["sections"][1]) for s in (1, 2)}
+        o = pyoracle.fwi_gradients(P, common.fwi_measured(z), "f32", lnorm=lnorm, waterlevel=1e-3, synthetic=syn)
+        P.close()
     for k, p in enumerate(("vp", "vs", "rho")):
         g = common.xmajor(r["sum"][k], nx, ny)
         ref = z[f"grad_{p}_sum"]
         if lnorm == 8:
             assert np.abs(g).max() == 0 and np.abs(ref).max() == 0
             continue
+        go = common.xmajor(o["sum"][k], nx, ny)
+        c_or = common.cosine(g, go)
         c, e = common.cosine(g, ref), rel_l2(g.ravel(), ref.ravel())
-        print(f"LNORM {lnorm} {p}: cosine {c:.7f} rel-L2 {e:.2e}")
-        assert c >= (0.999 if lnorm == 1 else COS_REF), (p, c, e)
+        print(f"LNORM {lnorm} {p}: cosine vs oracle {c_or:.7f}; vs reference {c:.7f} rel-L2 {e:.2e}")
+        assert c_or >= COS_ORACLE, (p, c_or)
+        assert c >= (0.75 if lnorm == 1 else COS_REF), (p, c, e)
 
 
 def test_fwi_gradient_vs_oracle_with_sparse_store():
