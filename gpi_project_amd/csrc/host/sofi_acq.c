@@ -7,7 +7,7 @@
  *   wavelet()   src/wavelet.c:33-149   (Ricker, Fuchs-Mueller, file, sin^3, Berlage, Klauder)
  *   read_srcsig src/read_srcsig.c:38-185 (SIGNAL_FILE: text, one sample per line, or SU)
  *   initsrc()   src/initsrc.c:30-101
- *   acq_read()  src/acq_read.c:25-80
+ *   acq_read()  src/acq_read.c:25-80   (scan_topo, src/scan_topo.c:32-85, for SOURCE_TOPO / REC_TOPO)
  * One device holds the whole grid, so the "local" sources and receivers are
  * all of them (GGRID = 1..NX x 1..NY).
  */
@@ -19,13 +19,58 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "sofi_host_internal.h"
 
 #define SRC(row, l) (a->srcpos[(size_t)(row) * (size_t)(a->nsrc + 1) + (size_t)(l)])
 #define REC(row, k) (a->recpos[(size_t)(row) * (size_t)(a->ntr + 1) + (size_t)(k)])
 
-static int read_receivers(sofi_params *p, sofi_acq *a)
+/* scan_topo (src/scan_topo.c:32-85): for every x position the first grid
+ * row whose P velocity differs from the top cell's (1 when the trace is
+ * uniform), from MFILE.vp.su (traces of NY samples after a 240-byte header)
+ * or MFILE.vp.  topo[1..NX]; NULL on error. */
+static int *scan_topo(const sofi_params *p)
+{
+    char fn[SOFI_STR + 16];
+    snprintf(fn, sizeof(fn), "%s.vp.su", p->mfile);
+    int su = access(fn, R_OK) == 0;
+    if (!su) {
+        snprintf(fn, sizeof(fn), "%s.vp", p->mfile);
+        if (access(fn, R_OK) != 0) { sofi_fail("Could not open P-wave velocity model to scan topography."); return NULL; }
+    }
+    FILE *fp = fopen(fn, "rb");
+    if (!fp) { sofi_fail("Could not open P-wave velocity model %s.", fn); return NULL; }
+    int *topo = (int *)calloc((size_t)p->nxg + 1, sizeof(int));
+    float *tr = (float *)malloc(sizeof(float) * (size_t)p->nyg);
+    for (int i = 1; i <= p->nxg; i++) {
+        if ((su && fseek(fp, 240, SEEK_CUR) != 0) || fread(tr, sizeof(float), (size_t)p->nyg, fp) != (size_t)p->nyg) {
+            fclose(fp);
+            free(tr);
+            free(topo);
+            sofi_fail("Short read in %s while scanning the topography.", fn);
+            return NULL;
+        }
+        topo[i] = 1;
+        for (int j = 2; j <= p->nyg; j++)
+            if (tr[j - 1] != tr[0]) {
+                topo[i] = j;
+                break;
+            }
+    }
+    free(tr);
+    fclose(fp);
+    return topo;
+}
+
+static int topo_at(const sofi_params *p, const int *topo, int ix, int *out)
+{
+    if (ix < 1 || ix > p->nxg) return sofi_fail("Topography: grid column %d is outside 1..%d.", ix, p->nxg);
+    *out = topo[ix];
+    return 0;
+}
+
+static int read_receivers(sofi_params *p, sofi_acq *a, const int *topo)
 {
     if (p->readrec) {
         char buf[SOFI_STR], tok[SOFI_STR];
@@ -42,7 +87,13 @@ static int read_receivers(sofi_params *p, sofi_acq *a)
             float xrec = 0, yrec = 0;
             if (fscanf(fp, "%f%f\n", &xrec, &yrec) != 2) { fclose(fp); free(tmp); return sofi_fail("Receiver file %s: bad line %d.", p->rec_file, itr); }
             tmp[1 * (n + 1) + itr] = (int)floor((xrec + p->refrec[1]) / p->dh) + 1;
-            tmp[2 * (n + 1) + itr] = (int)floor((yrec + p->refrec[2]) / p->dh) + 1;
+            if (p->rec_topo) {   /* below the topography: no REFREC offset, no +1 (src/receiver.c:70-72) */
+                int t = 1;
+                if (topo_at(p, topo, tmp[1 * (n + 1) + itr], &t)) { fclose(fp); free(tmp); return -1; }
+                tmp[2 * (n + 1) + itr] = t + (int)floor(yrec / p->dh);
+            } else {
+                tmp[2 * (n + 1) + itr] = (int)floor((yrec + p->refrec[2]) / p->dh) + 1;
+            }
             tmp[3 * (n + 1) + itr] = itr;
         }
         fclose(fp);
@@ -123,7 +174,7 @@ static int read_receivers(sofi_params *p, sofi_acq *a)
     return 0;
 }
 
-static int read_sources(sofi_params *p, sofi_acq *a)
+static int read_sources(sofi_params *p, sofi_acq *a, const int *topo)
 {
     if (p->srcrec == 1) {
         char buf[SOFI_STR], tok[SOFI_STR];
@@ -163,6 +214,11 @@ static int read_sources(sofi_params *p, sofi_acq *a)
             }
             SRC(1, l) = xsrc;
             SRC(2, l) = ysrc;
+            if (p->source_topo) {   /* depth below the topography (src/sources.c:122-126) */
+                int t = 1;
+                if (topo_at(p, topo, (int)floor(xsrc / p->dh) + 1, &t)) { fclose(fp); return -1; }
+                SRC(2, l) = (t - 1) * p->dh + ysrc;
+            }
             SRC(3, l) = 0.0f;
             SRC(4, l) = tshift;
             for (int r = 5; r <= SOFI_NSPAR; ++r) SRC(r, l) = v[r];
@@ -200,11 +256,17 @@ static int read_sources(sofi_params *p, sofi_acq *a)
 int sofi_read_acq(sofi_params *p, sofi_acq *a)
 {
     memset(a, 0, sizeof(*a));
-    if (p->seismo) {
-        int rc = read_receivers(p, a);
-        if (rc) return rc;
+    int *topo = NULL;
+    if (p->source_topo || p->rec_topo) {   /* acq_read (src/acq_read.c:33-36) */
+        topo = scan_topo(p);
+        if (!topo) return -1;
     }
-    int rc = read_sources(p, a);
+    if (p->seismo) {
+        int rc = read_receivers(p, a, topo);
+        if (rc) { free(topo); return rc; }
+    }
+    int rc = read_sources(p, a, topo);
+    free(topo);
     if (rc) return rc;
     a->nshots = p->run_multiple_shots ? a->nsrc : 1;
     if (p->snapshot_end == -9999 || p->snapshot_end > a->nshots) p->snapshot_end = a->nshots;

@@ -142,6 +142,14 @@ AC_CASES = {
     "ac_fd6_cpml_fx": ("AC_ISO", 6, 3, 0, 1, 0, 2, (), 1, 65),
 }
 
+# Sources and receivers placed below a topography (SOURCE_TOPO = REC_TOPO =
+# 1, src/scan_topo.c): a top layer of water velocity whose thickness varies
+# along x.  The second case reads its model from SU files.
+TOPO_CASES = {
+    "topo_el_fd4_cpml": ("EL_ISO", 4, 0, 0, 1, 0, 1, (), 4, 71),
+    "topo_vel_l1_fd6_abs_su": ("VEL_ISO", 6, 1, 1, 2, 0, 1, (), 4, 72),
+}
+
 # EL_VTI/EL_TTI with a free surface: surface_elastic() reads pu (= c55) and
 # ppi (= c11 already multiplied by DT in prepmod), so the sxx correction is
 # off by ~1/DT and the run diverges; the reference aborts with "Simulation is
@@ -217,7 +225,7 @@ def decomposed_case(td: str, npx: int = 2, npy: int = 3):
 
 def case_def(name: str) -> tuple[W.Case, dict]:
     table = (CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else AC_CASES if name in AC_CASES
-             else UNSTABLE)
+             else TOPO_CASES if name in TOPO_CASES else UNSTABLE)
     weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = table[name]
     model = _model(weq, seed, L=L)
     c = W.Case(name=name, nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdo,
@@ -230,8 +238,46 @@ def case_def(name: str) -> tuple[W.Case, dict]:
 
 
 def build(name: str, td: str):
+    if name in TOPO_CASES:
+        return topo_case(name, td)
     c, model = case_def(name)
     return W.write_case(td, c, model), c
+
+
+def topo_rows(nx=NX):
+    """First row below the water layer at every x (1-based), 4..10."""
+    return (7 + np.round(3.0 * np.sin(2.0 * np.pi * np.arange(nx) / nx))).astype(int)
+
+
+def write_su_model(path, a):
+    """[nx][ny] float32 as nx SU traces of ny samples (headers: tracl, ns)."""
+    a = np.asarray(a, dtype=np.float32)
+    with open(path, "wb") as f:
+        for i in range(a.shape[0]):
+            h = bytearray(240)
+            h[0:4] = np.int32(i + 1).tobytes()
+            h[114:116] = np.uint16(a.shape[1]).tobytes()
+            f.write(bytes(h))
+            f.write(a[i].tobytes())
+
+
+def topo_case(name: str, td: str):
+    c, model = case_def(name)
+    t = topo_rows()
+    for i in range(NX):
+        model["vp"][i, : t[i] - 1] = 1500.0
+        model["vs"][i, : t[i] - 1] = 0.0
+        model["rho"][i, : t[i] - 1] = 1000.0
+    srcs = [(37.0 * DH, 2.0 * DH, 0.0, 20.0, 1.0), (101.0 * DH + 1.0, 1.0 * DH, 0.004, 25.0, 0.7)]
+    recs = [((i - 1) * DH, 1.0 * DH) for i in range(15, 146, 6)] + [(80.0 * DH, 12.0 * DH)]
+    c = W.Case(**{**c.__dict__, "sources": srcs, "receivers": recs,
+                  "extra": {"SOURCE_TOPO": 1, "REC_TOPO": 1}})
+    js = W.write_case(td, c, model)
+    if name.endswith("_su"):
+        for k, a in model.items():
+            write_su_model(os.path.join(td, "model", f"m.{k}.su"), a)
+            os.remove(os.path.join(td, "model", f"m.{k}"))
+    return js, c
 
 
 def nsnap(c: W.Case) -> int:

@@ -129,8 +129,8 @@ def ref1():
             arrs = {k: (v[:, ::SUB, ::SUB] if k.startswith("snap") else v) for k, v in r.items()}
             np.savez_compressed(os.path.join(out, f"{t}.npz"), **arrs)
             print("ref1", t, flush=True)
-    # coverage cases (FDORDER_TIME 2, then 4, then the acoustic path)
-    for name in list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES):
+    # coverage cases (FDORDER_TIME 2, then 4, the acoustic path, topography)
+    for name in list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES) + list(cases.TOPO_CASES):
         if os.path.exists(os.path.join(out, f"{name}.npz")):
             continue
         with tempfile.TemporaryDirectory() as td:

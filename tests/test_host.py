@@ -318,6 +318,31 @@ def test_snapshot_process_files_vs_reference_build():
         assert open(os.path.join(td, "out", f"snap.bin.{comp}"), "rb").read() == merged[comp], comp
 
 
+@pytest.mark.parametrize("name", list(cases.TOPO_CASES))
+def test_topography_placement(name):
+    """SOURCE_TOPO / REC_TOPO: the topography row of every column is the
+    first whose vp differs from the top cell's (binary and SU model files);
+    receivers sit floor(y / DH) rows below it, sources y metres below its
+    depth (src/scan_topo.c:60-72, src/receiver.c:70-72, src/sources.c:122-126)."""
+    import math
+    from gpi_project_amd.sofi import Problem
+    td = tempfile.mkdtemp(prefix="sofi_topo_")
+    js, c = cases.build(name, td)
+    t = cases.topo_rows()
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        ri, rj = P.receivers()
+        s = P.sources()
+        P.close()
+    for (x, y), i, j in zip(c.receivers, ri, rj):
+        ix = int(math.floor(x / c.dh)) + 1
+        assert i == ix and j == t[ix - 1] + int(math.floor(y / c.dh)), (x, y, i, j)
+    for k, (x, y, *_r) in enumerate(c.sources[: s["n"]]):
+        ix = int(math.floor(x / c.dh)) + 1
+        depth = np.float32((t[ix - 1] - 1) * np.float32(c.dh)) + np.float32(y)
+        assert s["i"][k] == ix and s["j"][k] == int(math.floor(depth / np.float32(c.dh))) + 1, (k, s["i"][k], s["j"][k])
+
+
 # ------------------------------------------------------------------ C-ABI
 
 def _declared(header):

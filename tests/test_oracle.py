@@ -64,7 +64,8 @@ def test_tti_golden_depends_on_decomposition(examples_dir):
     assert rel_l2(r["seis_vx"], g["vx"]) > 1e-3
 
 
-@pytest.mark.parametrize("name", list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES))
+@pytest.mark.parametrize("name", list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES) +
+                         list(cases.TOPO_CASES))
 def test_oracle_vs_reference_cases(name):
     td = tempfile.mkdtemp(prefix="sofi_case_")
     js, c = cases.build(name, td)
