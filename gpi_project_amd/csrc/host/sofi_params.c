@@ -455,7 +455,14 @@ int sofi_check_supported(const sofi_params *p)
         if (p->lnorm < 1 || p->lnorm > 8 || p->lnorm == 6)
             return sofi_fail("MODE=FWI: LNORM=%d has no residual in the reference (src/calc_res.c: 1-5, 7, 8).", p->lnorm);
         if (p->time_filt) return sofi_fail("MODE=FWI: TIME_FILT is not supported by this build.");
-        if (p->eprecond) return sofi_fail("MODE=FWI: EPRECOND is not supported by this build.");
+        /* EPRECOND 1 / 3 accumulate the source and receiver wavefield
+         * energies Ws, Wr (src/eprecond.c:9-18, src/time_loop.c:297-301),
+         * which nothing in the reference reads back: the outputs do not
+         * depend on them, so the key is accepted and nothing is accumulated.
+         * EPRECOND 2 zeroes We_sum (src/init_inv.c:28-33), which only 1 and
+         * 3 allocate (src/initmem_fwi.c:57-62). */
+        if (p->eprecond == 2 || p->eprecond < 0 || p->eprecond > 3)
+            return sofi_fail("MODE=FWI: EPRECOND=%d writes an unallocated array in the reference (src/init_inv.c:28-33); refused.", p->eprecond);
     }
     return 0;
 }

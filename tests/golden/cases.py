@@ -346,7 +346,8 @@ def _fwi_models(nx=FWI_NX, ny=FWI_NY, dh=FWI_DH):
     return out
 
 
-def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp: float = 1.0) -> str:
+def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp: float = 1.0,
+             eprecond: int = 0) -> str:
     """EL_ISO, FD4 (Holberg 0.1 %), CPML 12, two explosive 20 Hz Ricker shots
     at (40, 5) and (120, 5) cells, 31 receivers on row 4 every 4 cells.
     which = "true": MODE=FW on the true model writing the measured data
@@ -371,6 +372,8 @@ def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp
                  "MISFIT_LOG_FILE": "./grad/L2_LOG.dat"}
         if lnorm == 8:
             extra["WATERLEVEL_LNORM8"] = "0.001"
+        if eprecond:
+            extra.update({"EPRECOND": eprecond, "EPRECOND_ITER": 0, "EPRECOND_PER_SHOT": 0, "EPSILON_WE": "0.005"})
     c = W.Case(name=f"{name}_{which}", nx=FWI_NX, ny=FWI_NY, dh=dh, dt=FWI_DT, time=FWI_NT * FWI_DT, weq="EL_ISO",
                fdorder=4, maxrelerror=1, abs_type=1, fw=12, free_surf=0, fpml=20.0, vppml=3000.0, source_type=1,
                sources=srcs, receivers=recs, seismo=1, ndt=1, run_multiple_shots=1,

@@ -21,8 +21,9 @@ has the reference at /root/reference and its build in oracle/_ref).
 5. ref1/snap_ranks_2x3.npz: the per-process snapshot files of a 2 x 3
    process run.
 6. ref1/fwi_lnorm*.npz: FWI gradients for LNORM 1, 3, 4, 5, 7, 8.
+7. ref1/fwi_el_v0_eprecond1.npz: the summed gradients with EPRECOND = 1.
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond]
 """
 from __future__ import annotations
 
@@ -269,8 +270,23 @@ def fwi_lnorm():
             print("ref1", name, "LNORM", ln, flush=True)
 
 
+def fwi_eprecond():
+    """fwi_el_v0 with EPRECOND = 1 on the reference build: its gradient
+    files (the energies it accumulates are never read back)."""
+    out = os.path.join(HERE, "ref1")
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    with tempfile.TemporaryDirectory() as td:
+        run_ref(cases.fwi_case(td, "fwi_el_v0", "true"), td)
+        run_ref(cases.fwi_case(td, "fwi_el_v0", "init", eprecond=1), td)
+        r = {}
+        for p in ("vp", "vs", "rho"):
+            r[f"grad_{p}_sum"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(nx, ny)
+        np.savez_compressed(os.path.join(out, "fwi_el_v0_eprecond1.npz"), **r)
+        print("ref1 fwi_el_v0 EPRECOND 1", flush=True)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm"]
+    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond"]
     for w in what:
         {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
-         "fwi_lnorm": fwi_lnorm}[w]()
+         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond}[w]()

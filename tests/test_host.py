@@ -471,6 +471,24 @@ def test_fwi_gradient_files_layout():
         P.close()
 
 
+def test_eprecond_does_not_change_the_reference_gradient():
+    """EPRECOND = 1 accumulates wavefield energies the reference never reads
+    back: its gradient files equal the EPRECOND = 0 ones bit for bit, so the
+    key is accepted (and nothing accumulated); EPRECOND = 2 is refused."""
+    from gpi_project_amd.sofi import Problem, SofiError
+    a, b = common.fwi_fixture("fwi_el_v0_eprecond1"), common.fwi_fixture("fwi_el_v0")
+    for p in ("vp", "vs", "rho"):
+        assert np.array_equal(a[f"grad_{p}_sum"], b[f"grad_{p}_sum"]), p
+    td = tempfile.mkdtemp(prefix="sofi_fwi_")
+    js = cases.fwi_case(td, "fwi_el_v0", "init", eprecond=1)
+    with chdir(td):
+        Problem(os.path.basename(js)).close()
+        txt = open(js).read()
+        open(js, "w").write(txt.replace('"EPRECOND" : "1"', '"EPRECOND" : "2"'))
+        with pytest.raises(SofiError, match="EPRECOND=2"):
+            Problem(os.path.basename(js))
+
+
 def test_fwi_parameter_checks():
     """The gradient path refuses what it does not implement."""
     from gpi_project_amd.sofi import Problem, SofiError
