@@ -107,6 +107,7 @@ def host():
     L.sofi_holberg.argtypes = [C.c_int, C.c_int, _FP]
     L.sofi_read_data.argtypes = [_VP, C.c_int, C.c_int, C.c_int, _FP]
     L.sofi_residual.argtypes = [_VP, C.c_int, _FP, _FP, _FP, _DP]
+    L.sofi_time_filter.argtypes = [_VP, C.c_int, _FP]
     L.sofi_write_gradient.argtypes = [_VP, C.c_int, _FP, _FP, _FP]
     L.sofi_outseis.argtypes = [C.c_char_p, _FP, _IP, C.c_int, _FP, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_float, C.c_float, C.c_int, _FP]

@@ -4,6 +4,7 @@
  *
  *   sofi_read_data      inseis      src/inseis.c:28-80
  *   sofi_residual       calc_res    src/calc_res.c:26-324 (LNORM 1-5, 7, 8)
+ *   sofi_time_filter    timedomain_filt src/timedomain_filt.c:28-113 (TIME_FILT)
  *   sofi_write_gradient output_gradient + writemod
  *                                   src/output_gradient.c:25-68, src/writemod.c:28-45
  */
@@ -39,6 +40,112 @@ int sofi_read_data(const sofi_params *p, int ishot, int comp, int ntr, float *se
     return 0;
 }
 
+/* ---------------------------------------------------- TIME_FILT filters */
+
+/* Recursive (IIR) section y[k] = f0 x[k] + f1 x[k-1] + f2 x[k-2] + g1 y[k-1]
+ * + g2 y[k-2] over x[0..n-1] in place, zero initial state: the recursion of
+ * libseife's rekfl, which timedomain_filt uses (src/timedomain_filt.c:62-99,
+ * contrib/Seitosh/libseife/cseife_rekfl.c). */
+typedef struct { double f0, f1, f2, g1, g2; } iir2;
+
+static void iir_run(double *x, int n, iir2 c)
+{
+    double x1 = 0., x2 = 0., y1 = 0., y2 = 0.;
+    for (int k = 0; k < n; ++k) {
+        const double xn = x[k];
+        const double y = c.f0 * xn + c.f1 * x1 + c.f2 * x2 + c.g1 * y1 + c.g2 * y2;
+        x[k] = y;
+        x2 = x1;
+        x1 = xn;
+        y2 = y1;
+        y1 = y;
+    }
+}
+
+/* Bilinear-transform sections of period t0 (in samples): first order
+ * (h < 0) or second order with damping h; low- or high-pass (the
+ * coefficients of libseife's rfk for its lp1 / hp1 / lp2 / hp2 types) */
+static iir2 iir_section(double t0, double h, int highpass)
+{
+    iir2 c = { 0., 0., 0., 0., 0. };
+    const double eps = (8. * atan(1.)) / t0;
+    if (h < 0) {
+        c.g1 = (2. - eps) / (2. + eps);
+        if (highpass) {
+            c.f0 = 2. / (2. + eps);
+            c.f1 = -(c.f0);
+        } else {
+            c.f0 = eps / (2. + eps);
+            c.f1 = c.f0;
+        }
+        return c;
+    }
+    const double e2 = eps * eps;
+    const double a = 1. - eps * h + e2 / 4., b = -2. + e2 / 2., cc = 1. + eps * h + e2 / 4.;
+    c.g1 = -b / cc;
+    c.g2 = -a / cc;
+    if (highpass) {
+        c.f0 = 1. / cc;
+        c.f1 = -(c.f0) - c.f0;
+    } else {
+        c.f0 = e2 / 4. / cc;
+        c.f1 = c.f0 + c.f0;
+    }
+    c.f2 = c.f0;
+    return c;
+}
+
+/* Butterworth of order m as libseife cascades it (seife_butterworth,
+ * cseife.c:160-190): a high-pass first removes x[0] from every sample; an
+ * odd order starts with a first-order section; then m/2 second-order
+ * sections with damping sin(pi/2 (2j - 1) / m). */
+static void butterworth(double *x, int n, double dt, double period, int m, int highpass)
+{
+    if (highpass) {
+        const double x0 = x[0];
+        for (int k = 0; k < n; ++k) x[k] -= x0;
+    }
+    if (m % 2) iir_run(x, n, iir_section(period / dt, -1., highpass));
+    const double pih = 2. * atan(1.);
+    for (int j = 1; j <= m / 2; ++j) iir_run(x, n, iir_section(period / dt, sin(pih * (2 * j - 1) / m), highpass));
+}
+
+/* One pass of timedomain_filt over a trace: samples 1..NS at x[1..NS] with
+ * x[0] = 0 (the reference filters NS + 1 values, its buffer's element 0
+ * included), filtered forward, rounded to float, reversed, filtered again,
+ * reversed and rounded back. */
+static void zero_phase(float *tr, int ns, double dt, double period, int m, int highpass, double *a, double *b)
+{
+    a[0] = 0.;
+    for (int j = 1; j <= ns; ++j) a[j] = (double)tr[j - 1];
+    butterworth(a, ns + 1, dt, period, m, highpass);
+    b[0] = 0.;
+    for (int j = 1; j <= ns; ++j) b[ns - j + 1] = (double)(float)a[j];
+    butterworth(b, ns + 1, dt, period, m, highpass);
+    for (int j = 1; j <= ns; ++j) tr[j - 1] = (float)b[ns - j + 1];
+}
+
+int sofi_time_filter(const sofi_params *p, int ntr, float *traces)
+{
+    if (!p->time_filt) return 0;
+    const int ns = p->ns;
+    double *a = (double *)malloc(sizeof(double) * (size_t)(ns + 1)), *b = (double *)malloc(sizeof(double) * (size_t)(ns + 1));
+    if (!a || !b) {
+        free(a);
+        free(b);
+        return sofi_fail("out of memory");
+    }
+    const double t_lp = 1.0 / (double)p->f_low_pass;
+    for (int r = 0; r < ntr; ++r) zero_phase(traces + (size_t)r * ns, ns, p->dt, t_lp, p->filt_order, 0, a, b);
+    if (p->f_high_pass) {
+        const double t_hp = 1.0 / (double)p->f_high_pass;
+        for (int r = 0; r < ntr; ++r) zero_phase(traces + (size_t)r * ns, ns, p->dt, t_hp, p->filt_order, 1, a, b);
+    }
+    free(a);
+    free(b);
+    return 0;
+}
+
 /* calc_res (src/calc_res.c:26-324), one component, traces 0..ntr-1 of
  * [ntr][NS], the adjoint source stored time-reversed (diff[i][NS - j + 1]):
  *   LNORM 1: sign(measured - synthetic) of the raw sections;
@@ -60,6 +167,24 @@ int sofi_residual(const sofi_params *p, int ntr, const float *data, const float 
 {
     const int ns = p->ns, ln = p->lnorm;
     if (ln < 1 || ln > 8 || ln == 6) return sofi_fail("LNORM=%d has no residual in the reference.", ln);
+    if (p->time_filt) {   /* both sections filtered first (src/inversion.c:68-84) */
+        const size_t n = (size_t)ntr * (size_t)ns;
+        float *fd = (float *)malloc(sizeof(float) * (n ? n : 1)), *fc = (float *)malloc(sizeof(float) * (n ? n : 1));
+        if (!fd || !fc) {
+            free(fd);
+            free(fc);
+            return sofi_fail("out of memory");
+        }
+        memcpy(fd, data, sizeof(float) * n);
+        memcpy(fc, calc, sizeof(float) * n);
+        sofi_params q = *p;
+        int rc = sofi_time_filter(&q, ntr, fd) || sofi_time_filter(&q, ntr, fc);
+        q.time_filt = 0;
+        if (!rc) rc = sofi_residual(&q, ntr, fd, fc, diff, l2);
+        free(fd);
+        free(fc);
+        return rc;
+    }
     float *is = (float *)malloc(sizeof(float) * (size_t)(ns > 0 ? ns : 1));
     float *isd = (float *)malloc(sizeof(float) * (size_t)(ns > 0 ? ns : 1));
     if (!is || !isd) {

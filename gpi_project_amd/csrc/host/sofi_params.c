@@ -388,6 +388,33 @@ int sofi_read_params(const char *json_file, sofi_params *p)
         OPT_INT("DTINV", p->dtinv, 1);
         OPT_STR("GRADIENT", p->gradient_file, "gradient");
         OPT_INT("TIME_FILT", p->time_filt, 0);
+        if (p->time_filt) {   /* src/read_par_json_fwi.c:497-545, src/sofi2D.c:156-164 */
+            OPT_FLT("F_HIGH_PASS", p->f_high_pass, 0.0f);
+            REQ_INT("ORDER", p->filt_order);
+            if (p->time_filt == 1) {
+                float f_end, f_incr;   /* required by the reference, unused by its loop */
+                REQ_FLT("F_LOW_PASS_START", p->f_low_pass);
+                REQ_FLT("F_LOW_PASS_END", f_end);
+                REQ_FLT("F_LOW_PASS_INCR", f_incr);
+                (void)f_end;
+                (void)f_incr;
+            } else if (p->time_filt == 2) {
+                char ff[SOFI_STR];
+                REQ_STR("FREQ_FILE", ff);
+                /* filter_frequencies (src/filter_frequencies.c:25-60): the
+                 * first nonzero number opening a line; FREQ_NR = 1 */
+                FILE *fq = fopen(ff, "r");
+                if (!fq) { sofi_fail("Freqency file could not be opened!"); goto fail; }
+                char line[SOFI_STR];
+                float v = 0.0f;
+                int found = 0;
+                while (!found && fgets(line, sizeof(line), fq))
+                    if (sscanf(line, "%f", &v) == 1 && v != 0) found = 1;
+                fclose(fq);
+                if (!found) { sofi_fail("FREQ_FILE %s holds no frequency.", ff); goto fail; }
+                p->f_low_pass = v;
+            }
+        }
         OPT_INT("EPRECOND", p->eprecond, 0);
         if (p->dtinv <= 0) { sofi_fail("Variable DTINV must be a positive integer!"); goto fail; }
     }
@@ -454,7 +481,8 @@ int sofi_check_supported(const sofi_params *p)
         if (p->seismo != 1 && p->seismo != 4) return sofi_fail("MODE=FWI needs SEISMO=1 or 4 (vx and vy sections).");
         if (p->lnorm < 1 || p->lnorm > 8 || p->lnorm == 6)
             return sofi_fail("MODE=FWI: LNORM=%d has no residual in the reference (src/calc_res.c: 1-5, 7, 8).", p->lnorm);
-        if (p->time_filt) return sofi_fail("MODE=FWI: TIME_FILT is not supported by this build.");
+        if (p->time_filt < 0 || p->time_filt > 2) return sofi_fail("MODE=FWI: TIME_FILT=%d (0, 1 or 2).", p->time_filt);
+        if (p->time_filt && p->filt_order < 1) return sofi_fail("MODE=FWI: ORDER=%d must be positive.", p->filt_order);
         /* EPRECOND 1 / 3 accumulate the source and receiver wavefield
          * energies Ws, Wr (src/eprecond.c:9-18, src/time_loop.c:297-301),
          * which nothing in the reference reads back: the outputs do not

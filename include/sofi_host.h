@@ -53,6 +53,8 @@ typedef struct sofi_params {
     /* FWI (src/read_par_json_fwi.c) */
     int itermax, dtinv, adjoint_type, velocity, lnorm, time_filt, eprecond;
     float waterlevel_lnorm8;
+    float f_low_pass, f_high_pass;   /* TIME_FILT corner frequencies (F_LOW_PASS_START or FREQ_FILE's first) */
+    int filt_order;
     char data_dir[SOFI_STR], gradient_file[SOFI_STR];
     /* derived */
     int nd;
@@ -154,11 +156,18 @@ int sofi_problem_snap_comps(const sofi_problem *P, int *comps);
  * (inseis, src/inseis.c:28-80) as section[ntr][ns]; the adjoint sources of
  * LNORM 1-5, 7 and 8 (calc_res, src/calc_res.c:26-324), time-reversed, with
  * *l2 the misfit calc_res accumulates for a test shot (LNORM 2: sum of
- * squares; 5 / 7: minus the normalised correlation; otherwise 0);
+ * squares; 5 / 7: minus the normalised correlation; otherwise 0), both
+ * sections filtered by sofi_time_filter first when TIME_FILT is set;
  * the gradient files (output_gradient, src/output_gradient.c:25-68), per shot
  * (ishot > 0) or summed (ishot = 0), from padded planes. */
 int sofi_read_data(const sofi_params *p, int ishot, int comp, int ntr, float *section);
 int sofi_residual(const sofi_params *p, int ntr, const float *data, const float *calc, float *diff, double *l2);
+/* timedomain_filt (src/timedomain_filt.c:28-113): with TIME_FILT the
+ * measured and synthetic traces [ntr][NS] are filtered in place before the
+ * residual, a zero-phase (forward, then reversed) Butterworth low-pass of
+ * order ORDER at F_LOW_PASS, then the same high-pass at F_HIGH_PASS when it
+ * is set; no-op without TIME_FILT. */
+int sofi_time_filter(const sofi_params *p, int ntr, float *traces);
 int sofi_write_gradient(const sofi_params *p, int ishot, const float *gvp, const float *gvs, const float *grho);
 
 int sofi_outseis(const char *path, const float *section, const int *recpos, int ntr,

@@ -47,6 +47,7 @@ def lib(precision: str = "f32"):
     L.oracle_fwi_gradient.argtypes = [C.c_void_p, _FP, _FP, _FP]
     L.oracle_fwi_gradient_sum.argtypes = [C.c_void_p, _DP, _DP, _DP]
     L.oracle_residual.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, _FP, _FP, _FP, _DP]
+    L.oracle_time_filter.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, _FP]
     _libs[precision] = L
     return L
 
@@ -139,12 +140,23 @@ def run_problem(problem, precision: str = "f32", nt: int | None = None):
     return res
 
 
+def time_filter(traces: np.ndarray, dt: float, f_low: float, f_high: float, order: int, precision: str = "f32"):
+    """timedomain_filt (src/timedomain_filt.c:28-113) on a copy of [ntr][ns]."""
+    t = np.array(traces, dtype=np.float32, copy=True, order="C")
+    lib(precision).oracle_time_filter(t.shape[0], t.shape[1], dt, f_low, f_high, order, t.ctypes.data_as(_FP))
+    return t
+
+
 def residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int, lnorm: int = 2,
-             waterlevel: float = 0.0, precision: str = "f32"):
+             waterlevel: float = 0.0, precision: str = "f32", tfilt=None):
     """calc_res (src/calc_res.c:26-324) on the oracle: the time-reversed
-    adjoint source [ntr][ns] of LNORM 1-5, 7 or 8 and the test-shot misfit."""
+    adjoint source [ntr][ns] of LNORM 1-5, 7 or 8 and the test-shot misfit;
+    tfilt = (f_low, f_high, order) filters both sections first (TIME_FILT)."""
     d = np.ascontiguousarray(data, dtype=np.float32)
     c = np.ascontiguousarray(calc, dtype=np.float32)
+    if tfilt is not None:
+        d = time_filter(d, dt, *tfilt, precision=precision)
+        c = time_filter(c, dt, *tfilt, precision=precision)
     out = np.zeros_like(c)
     m = C.c_double(0.0)
     rc = lib(precision).oracle_residual(lnorm, velocity, dt, waterlevel, c.shape[0], c.shape[1],
@@ -161,7 +173,7 @@ def l2_residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int) ->
 
 
 def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, waterlevel: float = 0.0,
-                  synthetic: dict | None = None):
+                  synthetic: dict | None = None, tfilt=None):
     """The FWI gradient of every shot of a MODE=FWI problem on the CPU oracle;
     data[ishot] = (vx, vy) measured sections [ntr][ns].  The sums add the
     shots' float32 gradients in shot order, as the reference does.  With
@@ -180,7 +192,7 @@ def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, w
             calc_x, calc_y = (np.asarray(a, dtype=np.float32) for a in synthetic[ishot])
         ri, rj = problem.receivers()
         zero = np.zeros((len(ri), d.nt), dtype=np.float32)
-        res = lambda q, cal: residual(data[ishot][q], cal, d.dt, d.velocity, lnorm, waterlevel, precision)[0]
+        res = lambda q, cal: residual(data[ishot][q], cal, d.dt, d.velocity, lnorm, waterlevel, precision, tfilt)[0]
         dx = res(0, calc_x) if d.adjoint_type in (1, 3) else zero
         dy = res(1, calc_y) if d.adjoint_type in (1, 2) else zero
         o.fwi_set_adjoint(ri, rj, dx, dy)

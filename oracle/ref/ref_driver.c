@@ -60,6 +60,14 @@ int main(int argc, char **argv)
         read_par_json(argv[1], &gv, &vinv);
     }
     exchange_par(&gv, &vinv);
+    if (gv.MODE == FWI) {   /* workflow and the low-pass start frequency (src/sofi2D.c:150-164) */
+        if (vinv.USE_WORKFLOW) read_workflow(&gv, &vinv);
+        if (vinv.TIME_FILT == 1) vinv.F_LOW_PASS = vinv.F_LOW_PASS_START;
+        if (vinv.TIME_FILT == 2) {
+            filter_frequencies(&vinv);
+            vinv.F_LOW_PASS = vinv.F_LOW_PASS_EXT[vinv.FREQ_NR];
+        }
+    }
     check_fs(&gv, &vinv);
     log_set_output(NULL);
     MPI_Barrier(MPI_COMM_WORLD);

@@ -1228,6 +1228,69 @@ void oracle_free(oracle *o)
 }
 
 /* size of the oracle's step in bytes of REAL (lets tests check which build they loaded) */
+/* timedomain_filt (src/timedomain_filt.c:28-113) with libseife's
+ * Butterworth cascade (contrib/Seitosh/libseife/cseife.c:160-190,
+ * cseife_rfk.c, cseife_rekfl.c), in double as the reference: each trace is
+ * filtered as NS + 1 samples (a leading 0), forward, rounded to float,
+ * reversed, filtered again and reversed back; low-pass at period tlp, then,
+ * when thp > 0, high-pass at thp. */
+static void rec_filter(double *x, int n, double f0, double f1, double f2, double g1, double g2)
+{
+    double xm1 = 0, xm2 = 0, ym1 = 0, ym2 = 0;
+    for (int k = 0; k < n; ++k) {
+        double in = x[k];
+        x[k] = f0 * in + f1 * xm1 + f2 * xm2 + g1 * ym1 + g2 * ym2;
+        xm2 = xm1;
+        xm1 = in;
+        ym2 = ym1;
+        ym1 = x[k];
+    }
+}
+
+static void bw_cascade(double *x, int n, double t0s, int order, int hp)
+{
+    /* t0s: the corner period in samples */
+    const double w = 8.0 * atan(1.0) / t0s;
+    if (hp) {
+        double first = x[0];
+        for (int k = 0; k < n; ++k) x[k] -= first;
+    }
+    if (order % 2 == 1) {
+        double g1 = (2.0 - w) / (2.0 + w);
+        if (hp) rec_filter(x, n, 2.0 / (2.0 + w), -(2.0 / (2.0 + w)), 0, g1, 0);
+        else rec_filter(x, n, w / (2.0 + w), w / (2.0 + w), 0, g1, 0);
+    }
+    for (int s = 1; s <= order / 2; ++s) {
+        const double h = sin(2.0 * atan(1.0) * (2 * s - 1) / order);
+        const double w2 = w * w;
+        const double A = 1.0 - w * h + w2 / 4.0, B = -2.0 + w2 / 2.0, Cc = 1.0 + w * h + w2 / 4.0;
+        double f0 = hp ? 1.0 / Cc : w2 / 4.0 / Cc;
+        double f1 = hp ? -(f0) - f0 : f0 + f0;
+        rec_filter(x, n, f0, f1, f0, -B / Cc, -A / Cc);
+    }
+}
+
+void oracle_time_filter(int ntr, int ns, float dt, float f_low, float f_high, int order, float *traces)
+{
+    double *u = (double *)calloc((size_t)ns + 1, sizeof(double)), *v = (double *)calloc((size_t)ns + 1, sizeof(double));
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1 && !f_high) break;
+        const double period = 1.0 / (double)(pass == 0 ? f_low : f_high);
+        for (int tr = 0; tr < ntr; ++tr) {
+            float *t = traces + (size_t)tr * ns;
+            u[0] = 0;
+            for (int k = 1; k <= ns; ++k) u[k] = t[k - 1];
+            bw_cascade(u, ns + 1, period / (double)dt, order, pass);
+            v[0] = 0;
+            for (int k = 1; k <= ns; ++k) v[ns + 1 - k] = (double)(float)u[k];
+            bw_cascade(v, ns + 1, period / (double)dt, order, pass);
+            for (int k = 1; k <= ns; ++k) t[k - 1] = (float)v[ns + 1 - k];
+        }
+    }
+    free(u);
+    free(v);
+}
+
 /* calc_res (src/calc_res.c:127-301): the adjoint source of one component
  * for LNORM 1-5, 7 and 8, traces [ntr][ns], time-reversed into diff, and
  * the misfit the reference accumulates for a test shot (src/calc_res.c:

@@ -346,8 +346,19 @@ def _fwi_models(nx=FWI_NX, ny=FWI_NY, dh=FWI_DH):
     return out
 
 
+# TIME_FILT variants: name -> (TIME_FILT, F_LOW_PASS_START or the FREQ_FILE
+# frequencies, F_HIGH_PASS, ORDER)
+FWI_TFILT = {"tf1_lp30_o4": (1, 30.0, 0.0, 4), "tf2_lp25_hp5_o3": (2, (25.0, 40.0), 5.0, 3)}
+
+
+def fwi_tfilt(name: str):
+    """(f_low, f_high, order) of a TIME_FILT variant, as the oracle takes them."""
+    tf, flo, fhi, order = FWI_TFILT[name]
+    return (flo if tf == 1 else flo[0]), fhi, order
+
+
 def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp: float = 1.0,
-             eprecond: int = 0) -> str:
+             eprecond: int = 0, tfilt: str | None = None) -> str:
     """EL_ISO, FD4 (Holberg 0.1 %), CPML 12, two explosive 20 Hz Ricker shots
     at (40, 5) and (120, 5) cells, 31 receivers on row 4 every 4 cells.
     which = "true": MODE=FW on the true model writing the measured data
@@ -374,6 +385,15 @@ def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp
             extra["WATERLEVEL_LNORM8"] = "0.001"
         if eprecond:
             extra.update({"EPRECOND": eprecond, "EPRECOND_ITER": 0, "EPRECOND_PER_SHOT": 0, "EPSILON_WE": "0.005"})
+        if tfilt:
+            tf, flo, fhi, order = FWI_TFILT[tfilt]
+            extra.update({"TIME_FILT": tf, "F_HIGH_PASS": repr(float(fhi)), "ORDER": order})
+            if tf == 1:
+                extra["F_LOW_PASS_START"] = repr(float(flo))
+            else:
+                extra["FREQ_FILE"] = "./frequencies.dat"
+                open(os.path.join(td, "frequencies.dat"), "w").write(
+                    "# low-pass corner frequencies\n" + "".join(f"{f!r}\n" for f in flo))
     c = W.Case(name=f"{name}_{which}", nx=FWI_NX, ny=FWI_NY, dh=dh, dt=FWI_DT, time=FWI_NT * FWI_DT, weq="EL_ISO",
                fdorder=4, maxrelerror=1, abs_type=1, fw=12, free_surf=0, fpml=20.0, vppml=3000.0, source_type=1,
                sources=srcs, receivers=recs, seismo=1, ndt=1, run_multiple_shots=1,

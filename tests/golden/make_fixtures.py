@@ -22,8 +22,9 @@ has the reference at /root/reference and its build in oracle/_ref).
    process run.
 6. ref1/fwi_lnorm*.npz: FWI gradients for LNORM 1, 3, 4, 5, 7, 8.
 7. ref1/fwi_el_v0_eprecond1.npz: the summed gradients with EPRECOND = 1.
+8. ref1/fwi_tf*.npz: FWI gradients with TIME_FILT 1 and 2.
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt]
 """
 from __future__ import annotations
 
@@ -285,8 +286,27 @@ def fwi_eprecond():
         print("ref1 fwi_el_v0 EPRECOND 1", flush=True)
 
 
+def fwi_tfilt():
+    """Summed gradients of the reference build with TIME_FILT (measured and
+    synthetic sections filtered before the residual) on fwi_el_v0's data."""
+    out = os.path.join(HERE, "ref1")
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    for name in cases.FWI_TFILT:
+        with tempfile.TemporaryDirectory() as td:
+            run_ref(cases.fwi_case(td, "fwi_el_v0", "true"), td)
+            run_ref(cases.fwi_case(td, "fwi_el_v0", "init", tfilt=name), td)
+            r = {}
+            for p in ("vp", "vs", "rho"):
+                for s in (1, 2):
+                    r[f"grad_{p}_shot{s}"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}.shot{s}"), np.float32).reshape(nx, ny)
+                r[f"grad_{p}_sum"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(nx, ny)
+            np.savez_compressed(os.path.join(out, f"fwi_{name}.npz"), **r)
+            print("ref1 fwi_el_v0", name, flush=True)
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond"]
+    what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond",
+                            "fwi_tfilt"]
     for w in what:
         {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
-         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond}[w]()
+         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt}[w]()

@@ -452,6 +452,41 @@ def test_residual_every_lnorm_vs_oracle(lnorm, velocity):
         assert np.abs(prop).max() <= 1e-5 * np.abs(mine).max()     # cancellation in float
 
 
+@pytest.mark.parametrize("tf", ["tf1_lp30_o4", "tf2_lp25_hp5_o3"])
+def test_time_filter_vs_oracle_and_response(tf):
+    """sofi_time_filter (the product's timedomain_filt) equals the oracle's
+    restatement bit for bit; as a zero-phase Butterworth it passes a tone
+    well inside the band with its phase and amplitude kept, and suppresses
+    one an octave past the low-pass corner by the forward-backward order's
+    attenuation (|H|^2 = 1 / (1 + (f / fc)^(2 n)))."""
+    import pyoracle
+    from gpi_project_amd.sofi import Problem
+    td = tempfile.mkdtemp(prefix="sofi_fwi_")
+    js = cases.fwi_case(td, "fwi_el_v0", "init", tfilt=tf)
+    flo, fhi, order = cases.fwi_tfilt(tf)
+    dt = cases.FWI_DT
+    ns = cases.FWI_NT
+    t = np.arange(1, ns + 1) * dt
+    win = np.sin(np.pi * np.arange(ns) / (ns - 1)) ** 2
+    tones = np.stack([np.sin(2 * np.pi * 12.0 * t) * win, np.sin(2 * np.pi * 2 * flo * t) * win,
+                      np.random.default_rng(1).standard_normal(ns)]).astype(np.float32)
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        from gpi_project_amd import _native as N
+        mine = tones.copy()
+        assert N.host().sofi_time_filter(N.host().sofi_problem_params(P._p), 3, mine.ctypes.data_as(C.POINTER(C.c_float))) == 0
+        P.close()
+    ref = pyoracle.time_filter(tones, dt, flo, fhi, order)
+    assert np.array_equal(mine, ref)
+    mid = slice(ns // 4, 3 * ns // 4)
+    gain_hp = 1.0 if not fhi else 1.0 / np.sqrt(1.0 + (fhi / 12.0) ** (4 * order))
+    gain_lp = 1.0 / np.sqrt(1.0 + (12.0 / flo) ** (4 * order))
+    passed = np.abs(mine[0, mid]).max() / np.abs(tones[0, mid]).max()
+    assert abs(passed - gain_lp * gain_hp) < 0.05, passed
+    assert np.abs(mine[0, mid] / (gain_lp * gain_hp) - tones[0, mid]).max() < 0.05    # zero phase
+    assert np.abs(mine[1, mid]).max() < 2.5 / 2.0 ** (2 * order) * np.abs(tones[1, mid]).max()
+
+
 def test_fwi_gradient_files_layout():
     """GRADIENT_gradient.<p>.shot<n> and GRADIENT_gradient.<p>: NX*NY float32
     written x-major (writemod, src/writemod.c:40-42)."""

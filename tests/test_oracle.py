@@ -183,6 +183,29 @@ def test_oracle_fwi_lnorm_gradients_vs_reference(lnorm):
     assert nonzero == (lnorm != 8)
 
 
+@pytest.mark.parametrize("tf", list(cases.FWI_TFILT))
+def test_oracle_fwi_time_filter_gradients_vs_reference(tf):
+    """TIME_FILT 1 (low-pass at F_LOW_PASS_START, even order) and 2 (the
+    first FREQ_FILE frequency, odd order, plus a high-pass at F_HIGH_PASS):
+    the FP32 oracle's gradients, with both sections filtered before the
+    residual, are the reference build's bit for bit."""
+    import pyoracle
+    from gpi_project_amd.sofi import Problem
+    z0 = common.fwi_fixture("fwi_el_v0")
+    z = common.fwi_fixture(f"fwi_{tf}")
+    td = tempfile.mkdtemp(prefix="sofi_fwi_")
+    js = cases.fwi_case(td, "fwi_el_v0", "init", tfilt=tf)
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        r = pyoracle.fwi_gradients(P, common.fwi_measured(z0), "f32", tfilt=cases.fwi_tfilt(tf))
+        for k, p in enumerate(("vp", "vs", "rho")):
+            for s in (1, 2):
+                g = common.xmajor(r["shots"][s]["gradient"][k], P.nx, P.ny).astype(np.float32)
+                assert np.array_equal(g, z[f"grad_{p}_shot{s}"]), (tf, p, s)
+            assert not np.array_equal(z[f"grad_{p}_sum"], z0[f"grad_{p}_sum"])
+        P.close()
+
+
 def test_oracle_fwi_gradient_is_zero_for_zero_residual_and_linear_in_it():
     """Properties of the adjoint path that hold without the reference: with
     measured data equal to the synthetics the residuals and so the gradient
