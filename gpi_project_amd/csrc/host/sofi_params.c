@@ -210,6 +210,8 @@ static int get_str(objlist *o, const char *key, char *v)
 static const char *weq_names[8] = { "AC_ISO", "VAC_ISO", "EL_ISO", "VEL_ISO", "EL_VTI", "VEL_VTI", "EL_TTI", "VEL_TTI" };
 static const char *mode_names[3] = { "FWI", "FW", "RTM" };
 
+static int apply_workflow_stage1(sofi_params *p);
+
 int sofi_read_params(const char *json_file, sofi_params *p)
 {
     objlist *o = (objlist *)calloc(1, sizeof(objlist));
@@ -416,6 +418,9 @@ int sofi_read_params(const char *json_file, sofi_params *p)
             }
         }
         OPT_INT("EPRECOND", p->eprecond, 0);
+        OPT_INT("USE_WORKFLOW", p->use_workflow, 0);
+        if (p->use_workflow) REQ_STR("FILE_WORKFLOW", p->file_workflow);
+        if (p->use_workflow && apply_workflow_stage1(p)) goto fail;
         if (p->dtinv <= 0) { sofi_fail("Variable DTINV must be a positive integer!"); goto fail; }
     }
     /* NS = iround(NT / NDT) with integer division (src/read_par_json.c:516) */
@@ -431,6 +436,55 @@ int sofi_read_params(const char *json_file, sofi_params *p)
 fail:
     free(o);
     return -1;
+}
+
+/* read_workflow + apply_workflow (src/read_workflow.c:32-113,
+ * src/apply_workflow.c:28-110): the first line of FILE_WORKFLOW is a header;
+ * every later line holding no '#' and something to read is a stage of 14
+ * numbers.  The reference's loop never leaves stage 1 (nothing advances
+ * WORKFLOW_STAGE), so every iteration takes TIME_FILT (column 5), when it
+ * is positive F_HIGH_PASS and F_LOW_PASS (6, 7), and EPRECOND (8) from the
+ * first stage; the other columns steer the model update, which the
+ * reference does not run.  EPRECOND_MAX, the largest EPRECOND of the file
+ * and the parameters, decides whether the energy arrays exist. */
+static int apply_workflow_stage1(sofi_params *p)
+{
+    FILE *fp = fopen(p->file_workflow, "r");
+    if (!fp) return sofi_fail("Workflow file  %s could not be opened.", p->file_workflow);
+    char line[SOFI_STR], tok[SOFI_STR];
+    float w[15] = { 0 }, emax = (float)p->eprecond;
+    int stages = 0, lines = 0;
+    /* the reference sizes its table by the readable lines less one, the
+     * header it expects among them (src/read_workflow.c:48-63) */
+    while (fgets(line, sizeof(line), fp))
+        if (!strchr(line, '#') && sscanf(line, "%255s", tok) == 1) ++lines;
+    rewind(fp);
+    if (!fgets(line, sizeof(line), fp)) line[0] = '\0';        /* header */
+    while (fgets(line, sizeof(line), fp)) {
+        if (strchr(line, '#') || sscanf(line, "%255s", tok) != 1) continue;
+        float v[15] = { 0 };
+        sscanf(line, "%f%f%f%f%f%f%f%f%f%f%f%f%f%f", &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9],
+               &v[10], &v[11], &v[12], &v[13], &v[14]);
+        if (++stages > lines - 1) {
+            fclose(fp);
+            return sofi_fail("read_workflow.c: buffer not large enough to store all workflow parameters - "
+                             "programming error (the first line of %s is its header).", p->file_workflow);
+        }
+        if (stages == 1) memcpy(w, v, sizeof(w));
+        if (v[8] > emax) emax = v[8];
+    }
+    fclose(fp);
+    if (!stages) return sofi_fail("Workflow file %s holds no stage.", p->file_workflow);
+    p->time_filt = (int)w[5];
+    if (p->time_filt > 0) {
+        p->f_high_pass = w[6];
+        p->f_low_pass = w[7];
+    }
+    p->eprecond = (int)w[8];
+    if (p->eprecond > 0 && emax != 1.0f && emax != 3.0f)
+        return sofi_fail("MODE=FWI: the workflow's EPRECOND writes an array the reference does not allocate "
+                         "(src/initmem_fwi.c:57-62, src/init_inv.c:28-33); refused.");
+    return 0;
 }
 
 /* What this build runs; everything else is refused with a message rather
@@ -482,7 +536,6 @@ int sofi_check_supported(const sofi_params *p)
         if (p->lnorm < 1 || p->lnorm > 8 || p->lnorm == 6)
             return sofi_fail("MODE=FWI: LNORM=%d has no residual in the reference (src/calc_res.c: 1-5, 7, 8).", p->lnorm);
         if (p->time_filt < 0 || p->time_filt > 2) return sofi_fail("MODE=FWI: TIME_FILT=%d (0, 1 or 2).", p->time_filt);
-        if (p->time_filt && p->filt_order < 1) return sofi_fail("MODE=FWI: ORDER=%d must be positive.", p->filt_order);
         /* EPRECOND 1 / 3 accumulate the source and receiver wavefield
          * energies Ws, Wr (src/eprecond.c:9-18, src/time_loop.c:297-301),
          * which nothing in the reference reads back: the outputs do not

@@ -55,6 +55,8 @@ typedef struct sofi_params {
     float waterlevel_lnorm8;
     float f_low_pass, f_high_pass;   /* TIME_FILT corner frequencies (F_LOW_PASS_START or FREQ_FILE's first) */
     int filt_order;
+    int use_workflow;                /* USE_WORKFLOW: stage 1 of FILE_WORKFLOW sets TIME_FILT, the corners, EPRECOND */
+    char file_workflow[SOFI_STR];
     char data_dir[SOFI_STR], gradient_file[SOFI_STR];
     /* derived */
     int nd;

@@ -84,7 +84,10 @@ int main(int argc, char **argv)
     if (gv.ABS_TYPE == 2) absorb(mpm.absorb_coeff, &gv);
     MPI_Barrier(MPI_COMM_WORLD);
     for (iter = 1; iter <= vinv.ITERMAX; iter++) {
-        if (gv.MODE == FWI) init_grad(iter, &minv, &gv, &vinv);
+        if (gv.MODE == FWI) {   /* src/sofi2D.c:293-299 */
+            if (vinv.USE_WORKFLOW) apply_workflow(iter, &gv, &vinv);
+            init_grad(iter, &minv, &gv, &vinv);
+        }
         prepmod(&mpm, &gv);
         MPI_Barrier(MPI_COMM_WORLD);
         if (gv.MODE == FWI) init_inv(iter, &minv, &gv, &vinv);

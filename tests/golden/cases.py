@@ -349,10 +349,22 @@ def _fwi_models(nx=FWI_NX, ny=FWI_NY, dh=FWI_DH):
 # TIME_FILT variants: name -> (TIME_FILT, F_LOW_PASS_START or the FREQ_FILE
 # frequencies, F_HIGH_PASS, ORDER)
 FWI_TFILT = {"tf1_lp30_o4": (1, 30.0, 0.0, 4), "tf2_lp25_hp5_o3": (2, (25.0, 40.0), 5.0, 3)}
+# USE_WORKFLOW on top of tf1_lp30_o4: stage 1 of the workflow file sets the
+# filter to a 35 Hz low-pass and a 2 Hz high-pass (the ORDER of the parameters)
+FWI_WORKFLOW = ("INV_VS INV_VP INV_RHO PRO TIME_FILT F_HIGH_PASS F_LOW_PASS EPRECOND EPSILON_WE TIMEWIN "
+                "GAMMA TWLENGTH_PLUS TWLENGTH_MINUS LAMBDA\n"
+                "1 1 1 0.01 1 2.0 35.0 0 0.005 0 20 0 0 0\n"
+                "# a later stage the reference's loop never reaches\n"
+                "1 1 1 0.01 1 4.0 60.0 0 0.005 0 20 0 0 0\n")
+
+
+FWI_TFILT_RUNS = list(FWI_TFILT) + ["workflow"]
 
 
 def fwi_tfilt(name: str):
     """(f_low, f_high, order) of a TIME_FILT variant, as the oracle takes them."""
+    if name == "workflow":
+        return 35.0, 2.0, FWI_TFILT["tf1_lp30_o4"][3]
     tf, flo, fhi, order = FWI_TFILT[name]
     return (flo if tf == 1 else flo[0]), fhi, order
 
@@ -385,6 +397,10 @@ def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp
             extra["WATERLEVEL_LNORM8"] = "0.001"
         if eprecond:
             extra.update({"EPRECOND": eprecond, "EPRECOND_ITER": 0, "EPRECOND_PER_SHOT": 0, "EPSILON_WE": "0.005"})
+        if tfilt == "workflow":
+            extra.update({"USE_WORKFLOW": 1, "FILE_WORKFLOW": "./workflow.txt"})
+            open(os.path.join(td, "workflow.txt"), "w").write(FWI_WORKFLOW)
+            tfilt = "tf1_lp30_o4"
         if tfilt:
             tf, flo, fhi, order = FWI_TFILT[tfilt]
             extra.update({"TIME_FILT": tf, "F_HIGH_PASS": repr(float(fhi)), "ORDER": order})

@@ -291,7 +291,7 @@ def fwi_tfilt():
     synthetic sections filtered before the residual) on fwi_el_v0's data."""
     out = os.path.join(HERE, "ref1")
     nx, ny = cases.FWI_NX, cases.FWI_NY
-    for name in cases.FWI_TFILT:
+    for name in list(cases.FWI_TFILT) + ["workflow"]:
         with tempfile.TemporaryDirectory() as td:
             run_ref(cases.fwi_case(td, "fwi_el_v0", "true"), td)
             run_ref(cases.fwi_case(td, "fwi_el_v0", "init", tfilt=name), td)

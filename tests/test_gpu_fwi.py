@@ -94,7 +94,7 @@ def test_fwi_lnorm_gradients_vs_reference(lnorm):
         assert c >= (0.75 if lnorm == 1 else COS_REF), (p, c, e)
 
 
-@pytest.mark.parametrize("tf", list(cases.FWI_TFILT))
+@pytest.mark.parametrize("tf", cases.FWI_TFILT_RUNS)
 def test_fwi_time_filter_gradients_vs_reference(tf):
     """TIME_FILT: measured and synthetic sections Butterworth-filtered before
     the residual; the device path against the reference build's gradient

@@ -183,10 +183,11 @@ def test_oracle_fwi_lnorm_gradients_vs_reference(lnorm):
     assert nonzero == (lnorm != 8)
 
 
-@pytest.mark.parametrize("tf", list(cases.FWI_TFILT))
+@pytest.mark.parametrize("tf", cases.FWI_TFILT_RUNS)
 def test_oracle_fwi_time_filter_gradients_vs_reference(tf):
-    """TIME_FILT 1 (low-pass at F_LOW_PASS_START, even order) and 2 (the
-    first FREQ_FILE frequency, odd order, plus a high-pass at F_HIGH_PASS):
+    """TIME_FILT 1 (low-pass at F_LOW_PASS_START, even order), 2 (the
+    first FREQ_FILE frequency, odd order, plus a high-pass at F_HIGH_PASS)
+    and USE_WORKFLOW (stage 1 of the workflow file replaces the corners):
     the FP32 oracle's gradients, with both sections filtered before the
     residual, are the reference build's bit for bit."""
     import pyoracle
