@@ -418,6 +418,8 @@ int sofi_read_params(const char *json_file, sofi_params *p)
             }
         }
         OPT_INT("EPRECOND", p->eprecond, 0);
+        OPT_INT("TRKILL", p->trkill, 0);
+        OPT_INT("TIMEWIN", p->timewin, 0);
         OPT_INT("USE_WORKFLOW", p->use_workflow, 0);
         if (p->use_workflow) REQ_STR("FILE_WORKFLOW", p->file_workflow);
         if (p->use_workflow && apply_workflow_stage1(p)) goto fail;
@@ -536,6 +538,10 @@ int sofi_check_supported(const sofi_params *p)
         if (p->lnorm < 1 || p->lnorm > 8 || p->lnorm == 6)
             return sofi_fail("MODE=FWI: LNORM=%d has no residual in the reference (src/calc_res.c: 1-5, 7, 8).", p->lnorm);
         if (p->time_filt < 0 || p->time_filt > 2) return sofi_fail("MODE=FWI: TIME_FILT=%d (0, 1 or 2).", p->time_filt);
+        /* trace killing from a file: calc_res skips killed traces and leaves
+         * their residual buffer as the previous shot left it
+         * (src/calc_res.c:62-125,196-197); not built */
+        if (p->trkill == 1) return sofi_fail("MODE=FWI: TRKILL=1 (trace killing) is not supported by this build.");
         /* EPRECOND 1 / 3 accumulate the source and receiver wavefield
          * energies Ws, Wr (src/eprecond.c:9-18, src/time_loop.c:297-301),
          * which nothing in the reference reads back: the outputs do not

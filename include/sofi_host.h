@@ -56,6 +56,7 @@ typedef struct sofi_params {
     float f_low_pass, f_high_pass;   /* TIME_FILT corner frequencies (F_LOW_PASS_START or FREQ_FILE's first) */
     int filt_order;
     int use_workflow;                /* USE_WORKFLOW: stage 1 of FILE_WORKFLOW sets TIME_FILT, the corners, EPRECOND */
+    int trkill, timewin;             /* TRKILL, TIMEWIN: read to refuse them */
     char file_workflow[SOFI_STR];
     char data_dir[SOFI_STR], gradient_file[SOFI_STR];
     /* derived */
