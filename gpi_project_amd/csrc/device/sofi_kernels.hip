@@ -265,6 +265,7 @@ struct Fuse {
     int npsrc;          // sources whose explosive term the stress kernel adds (0: none)
     int surf;           // the stress kernel applies the free surface
     int c[FUSE_MAX];    // cell offsets of the sources, in source order
+    int soff;           // index of the first of these sources in the handle's source list
     // FWI (src/wavefield_update_v.c:31-43, src/wavefield_update_s_el.c:43-47):
     // velocity kernel: 0 nothing, 1 forward rho-weighted accelerations,
     // 2 adjoint time integral of v, 3 adjoint copy of v (into pvxp1/pvyp1);
@@ -583,12 +584,12 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
         // a frame that is never updated still receives its forces (the
         // reference adds them to every source cell first)
         for (int k = 0; k < fz.nforce; ++k)
-            if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
+            if (c == fz.c[k]) force_add(g, s, fz.soff + k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
         return;
     }
     if (g.abs_type == 1 && frame) cpml_v(g, f, m, i, j, xf, yf, sxx_x, sxy_x, sxy_y, syy_y);
     for (int k = 0; k < fz.nforce; ++k)
-        if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
+        if (c == fz.c[k]) force_add(g, s, fz.soff + k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
     if (fz.fwi_acc == 1) {
         fz.pvxp1[c] = rip * (sxx_x + sxy_y) / g.dh;
         fz.pvyp1[c] = rjp * (sxy_x + syy_y) / g.dh;
@@ -848,7 +849,7 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
     }
     for (int k = 0; k < fz.npsrc; ++k)
         if (c == fz.c[k]) {
-            const float amp = psource_amp(g, s, k, step_of(fz.ntb, nt));
+            const float amp = psource_amp(g, s, fz.soff + k, step_of(fz.ntb, nt));
             sxx += amp;
             syy += amp;
         }
@@ -1183,7 +1184,7 @@ __global__ __launch_bounds__(BX *BY) void k_velocity4(Grid g, Fields f, Mat m, S
     float vx = f.vx[c], vy = f.vy[c];
     if (i == g.nx / 2 && j == g.ny / 2 && isnan(vy) && *f.unstable == 0) *f.unstable = step_of(fz.ntb, nt);
     for (int k = 0; k < fz.nforce; ++k)
-        if (c == fz.c[k]) force_add(g, s, k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
+        if (c == fz.c[k]) force_add(g, s, fz.soff + k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
     bool upd = !frame || g.abs_type == 2;
     if (frame && g.abs_type == 1) {
@@ -1352,7 +1353,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress_ac(Grid g, Fields f0, Mat m, 
         if (frame && g.abs_type == 2) p *= m.absorb[c];
         for (int q = 0; q < fz.npsrc; ++q)
             if (c == fz.c[q]) {
-                const float amp = psource_amp(g, s, q, step_of(fz.ntb, nt));
+                const float amp = psource_amp(g, s, fz.soff + q, step_of(fz.ntb, nt));
                 p += amp;
                 p += amp;
             }
@@ -1684,6 +1685,10 @@ struct sofi_dev {
     bool any_force = false;                // a source of type 2, 3 or 4 (force)
     Fuse fz{};                             // cells of the sources when nsrc <= FUSE_MAX (fz.nforce, npsrc unset)
     bool fusable = false;
+    // a batch whose slots each have at most FUSE_MAX sources and no moment
+    // sources: launches of one slot carry that slot's sources (slot_fz[k])
+    bool slot_fusable = false;
+    std::vector<Fuse> slot_fz;
     int shape_v = 0, shape_s = 0;          // velocity / stress kernel shapes: SOFI_LAUNCH_*
     int spt_mode = 0, grp_mode = 0;        // a batch's slots per thread and per launch (0: tuned in the first run)
     int grp = 0;                           // slots per launch of the update kernels (0: all)
@@ -2321,6 +2326,23 @@ int sync_acq(sofi_dev *h)
         h->fz = Fuse{};
         if (h->fusable)
             for (int l = 0; l < n; ++l) h->fz.c[l] = (int)sofi_idx(g.pitch, ci[l], cj[l]);
+        // per slot: the list holds the slots' sources slot after slot
+        h->slot_fusable = h->nb > 1;
+        h->slot_fz.assign(h->nb, Fuse{});
+        for (int k = 0, l0 = 0; k < h->nb; ++k) {
+            const int nk = (int)h->slots[k].si.size();
+            Fuse &fk = h->slot_fz[k];
+            fk.soff = l0;
+            if (nk > FUSE_MAX || h->slots[k].source_type == 5) h->slot_fusable = false;
+            bool force = false;
+            for (int l = l0; l < l0 + nk && l - l0 < FUSE_MAX; ++l) {
+                fk.c[l - l0] = (int)sofi_idx(g.pitch, ci[l], cj[l]);
+                if (ct[l] >= 2 && ct[l] <= 4) force = true;
+            }
+            fk.nforce = force ? nk : 0;
+            fk.npsrc = h->slots[k].source_type == 1 ? nk : 0;
+            l0 += nk;
+        }
         h->src_dirty = false;
         h->gen++;
     }
@@ -2589,13 +2611,17 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     fv.npsrc = fs.nforce = 0;
     fv.surf = 0;
     fv.nforce = h->fusable && h->any_force ? nsrc : 0;
-    // sources fused into the update kernels (one-shot handles with few sources)
+    // sources fused into the update kernels (one-shot handles with few
+    // sources; batches launched one slot at a time, each launch with its
+    // slot's sources)
     const bool src_fused = !tti && h->fusable && h->source_type != 5;
+    const int span = hi0 - lo0, grp = h->grp > 0 && h->grp < span ? h->grp : span;
+    const bool slot_fused = !tti && !h->fdt4 && h->slot_fusable && grp == 1 && nsrc > 0;
     fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
     // the free surface follows the explosive / moment sources, so it is fused
     // when they are, or when there are none to inject this step
     const bool no_stress_src = nsrc == 0 || (h->psrc_inj.n == 0 && h->n_eq == 0);
-    fs.surf = (src_fused || (!tti && no_stress_src)) && d.free_surf && !sample_dv &&
+    fs.surf = (src_fused || slot_fused || (!tti && no_stress_src)) && d.free_surf && !sample_dv &&
               (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
     fs.fwi_u = store && d.velocity == 1;
@@ -2609,7 +2635,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // vx / vy sampling rides on the stress launch of a one-shot handle
     // (SEISMO = 1, no TTI; the sampler addresses slot 0 of the handle's fields)
     const bool seis_fused = sample && d.seismo == 1 && !tti && h->nb == 1;
-    if (nsrc > 0 && h->any_force && !fv.nforce && src_inj.n > 0)
+    if (nsrc > 0 && h->any_force && !fv.nforce && !slot_fused && src_inj.n > 0)
         launch(h, k_force, dim3((src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, src_inj, nt, ntb);
     if (adj && h->nadj > 0)
         launch(h, k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), g, f, h->adj_inj, h->adj_sx,
@@ -2623,7 +2649,6 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     const dim3 blk(BX, BY);
     // the update kernels, launched per group of h->grp slots (all slots of a
     // one-shot handle or of an untuned batch: one group)
-    const int span = hi0 - lo0, grp = h->grp > 0 && h->grp < span ? h->grp : span;
     if (te) (void)hipEventRecord(te[0], st);
     for (int lo = lo0; lo < hi0; lo += grp) {
         Grid gg = g;
@@ -2670,18 +2695,31 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         // a launch of one slot runs the one-shot kernels on that slot's fields
         const bool one = gg.k_hi - gg.k_lo == 1;
         const Fields fk = one ? f.at(lo, g) : f;
+        Fuse fv_l = fv;
+        if (slot_fused) {
+            const Fuse &sf = h->slot_fz[lo];
+            memcpy(fv_l.c, sf.c, sizeof(sf.c));
+            fv_l.soff = sf.soff;
+            fv_l.nforce = sf.nforce;
+        }
         int vw = vec_width(g, h->shape_v);
         dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
-        if (vw == 4 && one) launch(h, k_velocity_v<ND, 4, true>, tiles, blk, gg, fk, m, h->s, fv, nt);
-        else if (vw == 4) launch(h, k_velocity_v<ND, 4, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
-        else if (vw == 2 && one) launch(h, k_velocity_v<ND, 2, true>, tiles, blk, gg, fk, m, h->s, fv, nt);
-        else if (vw == 2) launch(h, k_velocity_v<ND, 2, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
-        else if (one) launch(h, k_velocity<ND, true>, tiles, blk, gg, fk, m, h->s, fv, nt);
-        else launch(h, k_velocity<ND, false>, tiles, blk, gg, fk, m, h->s, fv, nt);
+        if (vw == 4 && one) launch(h, k_velocity_v<ND, 4, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        else if (vw == 4) launch(h, k_velocity_v<ND, 4, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        else if (vw == 2 && one) launch(h, k_velocity_v<ND, 2, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        else if (vw == 2) launch(h, k_velocity_v<ND, 2, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        else if (one) launch(h, k_velocity<ND, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        else launch(h, k_velocity<ND, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         vw = vec_width(g, h->shape_s);
         tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
         // vx / vy sampling in extra blocks of a one-slot stress launch
         Fuse fss = fs;
+        if (slot_fused) {
+            const Fuse &sf = h->slot_fz[lo];
+            memcpy(fss.c, sf.c, sizeof(sf.c));
+            fss.soff = sf.soff;
+            fss.npsrc = sf.npsrc;
+        }
         if (seis_fused && one) {
             fss.sblk = (rr.n + BX * BY - 1) / (BX * BY);
             fss.ntile = (int)tiles.x;
@@ -2747,7 +2785,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         if (tti) launch(h, k_seis_dv<ND, true>, rg, rb, g, f, m, rr, ins, ntb, ndt);
         else launch(h, k_seis_dv<ND, false>, rg, rb, g, f, m, rr, ins, ntb, ndt);
     }
-    if (nsrc > 0 && !src_fused) {
+    if (nsrc > 0 && !src_fused && !slot_fused) {
         if (psrc_inj.n > 0)
             launch(h, k_psource, dim3((psrc_inj.n + 63) / 64), dim3(64), g, f, h->s, psrc_inj, nt, ntb);
         if (h->n_eq > 0)
