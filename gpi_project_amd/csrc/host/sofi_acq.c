@@ -12,7 +12,6 @@
  * all of them (GGRID = 1..NX x 1..NY).
  */
 #define _GNU_SOURCE
-#include <complex.h>
 #include <ctype.h>
 #include <math.h>
 #include <stdint.h>
@@ -452,8 +451,13 @@ int sofi_init_shot(const sofi_params *p, const sofi_acq *a, int ishot, int prev_
                   if ((t < tshift) || (t > (tau * 2) + tshift)) {
                       amp = 0.0f;
                   } else {
-                      double x = t - tau - tshift;
-                      amp = (float)creal(sin(M_PI * f2 * x * (T - x)) / (M_PI * f2 * x) * cexp(2 * M_PI * I * fc * x));
+                      /* The reference writes the carrier as exp(2 PI I fc x) with
+                       * <complex.h> but without <tgmath.h>, so the real exp()
+                       * receives the argument's real part, 0, and the carrier is
+                       * 1: its Klauder wavelet is the sweep's sinc envelope alone
+                       * (src/wavelet.c:109-123), kept here as it computes it. */
+                      float x = t - tau - tshift;   /* T - x in float, as there */
+                      amp = (float)(sin(M_PI * f2 * x * (T - x)) / (M_PI * f2 * x));
                       if (t < (tau * 0.2) + tshift) {
                           float taper = (float)((1 - cos(((t - tshift) / tau / 0.2) * M_PI)) / 2);
                           amp = amp * taper;

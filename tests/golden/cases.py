@@ -215,6 +215,46 @@ def signal_case(name: str, td: str):
     return W.write_case(td, c, model), c
 
 
+# Wavelet shapes and acquisition geometries beyond the coverage cases, on an
+# EL_ISO FD4 CPML grid: Fuchs-Mueller, sin^3, Berlage and Klauder wavelets
+# (with time shifts), a plane wave (SRCREC 2), a receiver array, a vertical
+# receiver line, and ASCII seismograms and snapshots.  The plane wave
+# (acq_plane_wave) is built for the device and the oracle only: the reference
+# build ends in a segmentation fault for SRCREC 2 (its local source count is
+# read uninitialised, "nsrc_loc: 1610539588" in its log).
+ACQ_CASES = ("acq_fuchs_mueller", "acq_sin3", "acq_berlage", "acq_klauder", "acq_rec_array", "acq_rec_line",
+             "acq_formats_ascii")
+
+
+def acq_case(name: str, td: str):
+    base, model = case_def("el_fd4_cpml_eq")
+    x0, y0 = 70.0 * DH + 1.0, 40.0 * DH + 1.0
+    kw = {"name": name, "source_type": 1, "source_shape": 1, "seismo": 1, "snap": 0,
+          "sources": [(x0, y0, 0.0, 25.0, 1.0)], "extra": {"SIGOUT": 1, "SIGOUT_FILE": "./out/signal_out",
+                                                            "SIGOUT_FORMAT": 1}}
+    ex = kw["extra"]
+    if name == "acq_fuchs_mueller":
+        kw.update(source_shape=2, sources=[(x0, y0, 0.003, 30.0, 1.5)])
+    elif name == "acq_sin3":
+        kw.update(source_shape=4, sources=[(x0, y0, 0.002, 20.0, 0.8)])
+    elif name == "acq_berlage":
+        kw.update(source_shape=5, sources=[(x0, y0, 0.002, 25.0, 1.0, 2.0, 150.0, 30.0)])
+    elif name == "acq_klauder":
+        kw.update(source_shape=6, sources=[(x0, y0, 0.0002, 10.0, 40.0, 1.0, 0.1, 1.0)])
+    elif name == "acq_plane_wave":
+        ex.update({"SRCREC": 2, "PLANE_WAVE_DEPTH": repr(60.0 * DH), "PLANE_WAVE_ANGLE": "10.0", "TS": "0.05"})
+    elif name == "acq_rec_array":
+        kw["receivers"] = None
+        ex.update({"REC_ARRAY": 2, "REC_ARRAY_DEPTH": repr(30.0 * DH), "REC_ARRAY_DIST": repr(10.0 * DH), "DRX": 7})
+    elif name == "acq_rec_line":
+        kw.update(receivers=None, rec_line=(60.0 * DH, 20.0 * DH, 60.0 * DH, 100.0 * DH, 4.0))
+    elif name == "acq_formats_ascii":
+        kw.update(snap=1, seismo=1)
+        ex.update({"SEIS_FORMAT": 2, "SNAP_FORMAT": 2})
+    c = W.Case(**{**base.__dict__, **kw})
+    return W.write_case(td, c, model), c
+
+
 def decomposed_case(td: str, npx: int = 2, npy: int = 3):
     """el_fd2_cpml (snapshots of every component, IDX 2, IDY 3) with the
     reference's process grid NPROCX x NPROCY in the parameter file."""

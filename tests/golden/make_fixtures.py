@@ -22,9 +22,10 @@ has the reference at /root/reference and its build in oracle/_ref).
    process run.
 6. ref1/fwi_lnorm*.npz: FWI gradients for LNORM 1, 3, 4, 5, 7, 8.
 7. ref1/fwi_el_v0_eprecond1.npz: the summed gradients with EPRECOND = 1.
-8. ref1/fwi_tf*.npz: FWI gradients with TIME_FILT 1 and 2.
+8. ref1/fwi_tf*.npz, fwi_workflow.npz: FWI gradients with TIME_FILT 1, 2 and a workflow.
+9. ref1/acq_*.npz: wavelet shapes, plane wave, receiver array and line, ASCII formats.
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq]
 """
 from __future__ import annotations
 
@@ -248,6 +249,25 @@ def fwi():
             print("ref1", name, "time loops %s s" % t, flush=True)
 
 
+def acq():
+    """tests/golden/cases.py ACQ_CASES on the reference build: SIGOUT bytes,
+    seismograms (SU, or the ASCII files' bytes) and ASCII snapshot bytes."""
+    out = os.path.join(HERE, "ref1")
+    for name in os.environ.get("ACQ_ONLY", "").split() or cases.ACQ_CASES:
+        with tempfile.TemporaryDirectory() as td:
+            js, c = cases.acq_case(name, td)
+            run_ref(js, td)
+            r = {"sigout_su": np.frombuffer(open(os.path.join(td, "out", "signal_out.shot1.su"), "rb").read(), np.uint8)}
+            for comp in ("vx", "vy"):
+                if name == "acq_formats_ascii":
+                    r[f"seis_{comp}_txt"] = np.frombuffer(open(os.path.join(td, "out", f"seis_{comp}.txt"), "rb").read(), np.uint8)
+                    r[f"snap_{comp}_asc"] = np.frombuffer(open(os.path.join(td, "out", f"snap.asc.{comp}.0.0"), "rb").read(), np.uint8)
+                else:
+                    r[f"seis_{comp}"] = read_su(os.path.join(td, "out", f"seis_{comp}.su"))
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, flush=True)
+
+
 def fwi_lnorm():
     """Gradient files of the reference build for the other LNORM residuals
     (tests/golden/cases.py FWI_LNORMS)."""
@@ -306,7 +326,7 @@ def fwi_tfilt():
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond",
-                            "fwi_tfilt"]
+                            "fwi_tfilt", "acq"]
     for w in what:
         {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
-         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt}[w]()
+         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt, "acq": acq}[w]()

@@ -156,3 +156,35 @@ def test_fields_vs_oracle(name, shape):
     js, c = cases.build(name, td)
     _fields_vs_oracle(js, td, name, shape)
 
+
+
+def _ascii_section(buf, ntr):
+    return np.array(bytes(buf).split(), dtype=np.float64).astype(np.float32).reshape(ntr, -1)
+
+
+@pytest.mark.parametrize("name", cases.ACQ_CASES)
+def test_acquisition_cases_vs_reference_1proc(name):
+    """Fuchs-Mueller, sin^3, Berlage and Klauder wavelets, a receiver array,
+    a vertical receiver line; for the ASCII case the reference's text
+    seismograms and snapshots, parsed."""
+    td = tempfile.mkdtemp(prefix="sofi_acq_")
+    js, c = cases.acq_case(name, td)
+    secs, snaps = _run_gpu(js)
+    r = common.ref1(name)
+    if name == "acq_formats_ascii":
+        ntr = secs["vx"].shape[0]
+        rs = {nm: _ascii_section(r[f"seis_{nm}_txt"], ntr) for nm in ("vx", "vy")}
+        rn = {nm: _ascii_section(r[f"snap_{nm}_asc"], 1).reshape(snaps[nm].shape) for nm in ("vx", "vy")}
+        errs = _check(secs, snaps, rs, rn)
+    else:
+        errs = _check(secs, snaps, {k[5:]: v for k, v in r.items() if k.startswith("seis_")}, {})
+    print(name, errs)
+
+
+def test_plane_wave_vs_oracle():
+    """SRCREC 2: a plane wave of one source per grid point along a tilted line
+    (the reference build itself crashes for it), whole wavefields against the
+    FP32 / FP64 oracle."""
+    td = tempfile.mkdtemp(prefix="sofi_acq_")
+    js, c = cases.acq_case("acq_plane_wave", td)
+    _fields_vs_oracle(js, td, "acq_plane_wave", "auto")
