@@ -31,9 +31,10 @@
  *                         src/wavefield_update_s_{el,visc}_4.c,
  *                         src/prepare_update_s_{el,visc}_4.c, src/util.c:89-109
  *
- * Parity status of each piece is listed in DESIGN.md; the pinned ones are
- * checked in tests/test_oracle.py against the reference's golden files, the
- * reference build in oracle/_ref and closed forms.
+ * Every piece is checked in tests/test_oracle.py and tests/test_host.py
+ * against the reference's golden files, the reference build in oracle/_ref
+ * and closed forms.  This file is test infrastructure: only the tests, the
+ * smoke check and the benchmark's CPU baseline use it, as the checker.
  */
 #include <math.h>
 #include <stdio.h>

@@ -24,8 +24,8 @@ def _oracle(json_path, precision="f32", nt=None):
     return ({COMPS[c]: v for c, v in r["sections"].items()}, {COMPS[c]: v for c, v in r["snapshots"].items()})
 
 
-# the goldens come from a different compiler build of the reference (the
-# survey measured <= 2.3e-5 between today's code and them), so their bound is
+# the goldens come from a different compiler build of the reference (up to
+# 2.3e-5 rel-L2 separates today's code built here from them), so their bound is
 # 3e-5 rel-L2 except for cancellation-limited curl; against the one-process
 # reference built here the oracle is bit-identical
 GOLDEN_TOL = 3e-5
