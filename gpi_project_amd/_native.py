@@ -153,6 +153,8 @@ def hip():
     L.sofi_select_shot.argtypes = [_VP, C.c_int]
     L.sofi_set_batch_launch.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_get_batch_launch.argtypes = [_VP, _IP, _IP]
+    L.sofi_set_batch_block.argtypes = [_VP, C.c_int]
+    L.sofi_get_batch_block.argtypes = [_VP]
     L.sofi_fwi_set_model.argtypes = [_VP, _FP, _FP, _FP]
     L.sofi_fwi_set_pass.argtypes = [_VP, C.c_int]
     L.sofi_fwi_set_adjoint.argtypes = [_VP, C.c_int, _IP, _IP, _FP, _FP]
@@ -182,5 +184,6 @@ def exported_symbols():
                            "sofi_fwi_gradient", "sofi_fwi_gradient_sum", "sofi_open_batch", "sofi_select_shot",
                            "sofi_set_batch_launch", "sofi_get_batch_launch", "sofi_estimate_footprint",
                            "sofi_set_graph", "sofi_graph_blocks", "sofi_set_batch_resident",
-                           "sofi_get_batch_resident", "sofi_set_batch_streams"],
+                           "sofi_get_batch_resident", "sofi_set_batch_streams", "sofi_set_batch_block",
+                           "sofi_get_batch_block"],
     }

@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -60,6 +61,11 @@ struct Grid {
     int plane;      // floats per padded plane
     int wx, wy;     // row lengths of x-frame and y-frame CPML memory
     int ntx;        // tiles along x of the one-cell-per-thread update kernels
+    int nty;        // tiles along y of the same
+    // tiles (BX x BY) that hold a cell of the absorbing frame or one next to
+    // it: tile rows at the top / bottom, tile columns at the left / right
+    // (the TTI frame passes run over these tiles only)
+    int ftop, fbot, fl, fr;
     int nb, spt;    // shot slots of a batch; slots per thread of the update kernels
     int k_lo, k_hi; // the slots this launch updates (a group of the batch)
     int nxp, nyp;   // floats per slot of the x-frame and y-frame CPML memory
@@ -748,8 +754,23 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f0, Mat m,
 // stresses on entry and the updated ones on exit; dv are the (possibly
 // CPML-modified) derivatives.
 
-// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48); in FWI forward
-// store steps with VELOCITY=1 also the increments over DT (ux, uy, uxy)
+// MODE=FWI: the stress increments over DT (ux, uy, uxy) that every
+// rheology's update writes (src/wavefield_update_s_el.c:43-47,
+// src/wavefield_update_s_visc.c:61-66, src/wavefield_update_s_el_vti.c:42-46,
+// src/wavefield_update_s_visc_VTI.c:61-65, src/wavefield_update_s_el_tti.c:47-51,
+// src/wavefield_update_s_visc_TTI.c:79-83); only the forward store steps
+// with VELOCITY = 1 read them (src/snap_store.c:35-43), so they are written
+// in those steps only
+__device__ __forceinline__ void fwi_u_store(const Fuse &fz, float dt, int c, float u1, float u2, float u3)
+{
+    if (fz.fwi_u) {
+        fz.uxy[c] = u1 / dt;
+        fz.ux[c] = u2 / dt;
+        fz.uy[c] = u3 / dt;
+    }
+}
+
+// wavefield_update_s_el (src/wavefield_update_s_el.c:28-48)
 __device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vxx, float vyx, float vxy, float vyy,
                                        float &sxx, float &syy, float &sxy, const Fuse &fz, float dt, int c)
 {
@@ -760,16 +781,13 @@ __device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vx
     sxy += u1;
     sxx += u2;
     syy += u3;
-    if (fz.fwi_u) {
-        fz.uxy[c] = u1 / dt;
-        fz.ux[c] = u2 / dt;
-        fz.uy[c] = u3 / dt;
-    }
+    fwi_u_store(fz, dt, c, u1, u2, u3);
 }
 
 // wavefield_update_s_el_vti (src/wavefield_update_s_el_vti.c:27-47)
 __device__ __forceinline__ void upd_el_vti(float c11, float c13, float c33, float c55i, float vxx, float vyx,
-                                           float vxy, float vyy, float &sxx, float &syy, float &sxy)
+                                           float vxy, float vyy, float &sxx, float &syy, float &sxy, const Fuse &fz,
+                                           float dt, int c)
 {
     const float u1 = c55i * (vxy + vyx);
     const float u2 = (c11 * vxx) + (c13 * vyy);
@@ -777,6 +795,7 @@ __device__ __forceinline__ void upd_el_vti(float c11, float c13, float c33, floa
     sxy += u1;
     sxx += u2;
     syy += u3;
+    fwi_u_store(fz, dt, c, u1, u2, u3);
 }
 
 // Memory-variable sums of the viscoelastic updates: old (r0, p0, q0) and new
@@ -803,7 +822,8 @@ __device__ __forceinline__ void visc_level(float eta, float bip, float cip, floa
 }
 
 __device__ __forceinline__ void visc_final(float dthalbe, float f_, float g_, float fipjp, float sxyv, float sdiag,
-                                           float vxx, float vyy, const VSum &u, float &sxx, float &syy, float &sxy)
+                                           float vxx, float vyy, const VSum &u, float &sxx, float &syy, float &sxy,
+                                           const Fuse &fz, float dt, int c)
 {
     const float ff = 2.0f * f_;
     const float u1 = (fipjp * sxyv) + (dthalbe * (u.r0 + u.r));
@@ -812,6 +832,7 @@ __device__ __forceinline__ void visc_final(float dthalbe, float f_, float g_, fl
     sxy += u1;
     sxx += u2;
     syy += u3;
+    fwi_u_store(fz, dt, c, u1, u2, u3);
 }
 
 // mechanism l of wavefield_update_s_visc_VTI (src/wavefield_update_s_visc_VTI.c:27-66)
@@ -832,7 +853,7 @@ __device__ __forceinline__ void visc_vti_level(float bip, float cip, float d11, 
 
 __device__ __forceinline__ void visc_vti_final(float dthalbe, float u11, float u13, float u33, float u55i, float sxyv,
                                                float vxx, float vyy, const VSum &u, float &sxx, float &syy,
-                                               float &sxy)
+                                               float &sxy, const Fuse &fz, float dt, int c)
 {
     const float u1 = (u55i * sxyv) + (dthalbe * (u.r0 + u.r));
     const float u2 = (u11 * vxx) + (u13 * vyy) + (dthalbe * (u.p0 + u.p));
@@ -840,6 +861,7 @@ __device__ __forceinline__ void visc_vti_final(float dthalbe, float u11, float u
     sxy += u1;
     sxx += u2;
     syy += u3;
+    fwi_u_store(fz, dt, c, u1, u2, u3);
 }
 
 // Derivatives of a cell from the raw op_s sums (no 1/DH): scaled, and CPML in
@@ -947,7 +969,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
         float sxx = ld1<ONE>(f.sxx, c), syy = ld1<ONE>(f.syy, c),
               sxy = ld1<ONE>(f.sxy, c);
         if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
-        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sxx, syy, sxy);
+        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
         if (WEQ == SOFI_VEL_ISO) {
             const float sxyv = vxy + vyx, sdiag = vxx + vyy;
             VSum u;
@@ -962,7 +984,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
                 st1<ONE>(f.p, o, p);
                 st1<ONE>(f.q, o, q);
             }
-            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy);
+            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy, fz, g.dt, c);
         }
         if (WEQ == SOFI_VEL_VTI) {
             const float sxyv = vxy + vyx;
@@ -977,7 +999,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
                 f.p[o] = p;
                 f.q[o] = q;
             }
-            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sxx, syy, sxy);
+            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sxx, syy, sxy, fz, g.dt, c);
         }
         stress_post<ND, VISCO>(g, f, m, s, fz, nt, i, j, c, sxx, syy, sxy);
         st1<ONE>(f.sxx, c, sxx);
@@ -1066,8 +1088,12 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
         const V c1 = PRE ? pre[0] : ldv<VW>(m.c11, c0), c2 = PRE ? pre[1] : ldv<VW>(m.c13, c0),
                 c3 = PRE ? pre[2] : ldv<VW>(m.c33, c0), c4 = PRE ? pre[3] : ldv<VW>(m.c55i, c0);
 #pragma unroll
-        for (int k = 0; k < VW; ++k)
-            upd_el_vti(c1[k], c2[k], c3[k], c4[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k]);
+        for (int k = 0; k < VW; ++k) {
+            Fuse fk = fz;
+            fk.fwi_u = fz.fwi_u && act[k];
+            upd_el_vti(c1[k], c2[k], c3[k], c4[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fk, g.dt,
+                       c0 + k);
+        }
     }
     if (VISCO) {
         VSum u[VW];
@@ -1117,15 +1143,21 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
         if (WEQ == SOFI_VEL_ISO) {
             const V c1 = ldv<VW>(m.f, c0), c2 = ldv<VW>(m.g, c0), c3 = ldv<VW>(m.fipjp, c0);
 #pragma unroll
-            for (int k = 0; k < VW; ++k)
+            for (int k = 0; k < VW; ++k) {
+                Fuse fk = fz;
+                fk.fwi_u = fz.fwi_u && act[k];
                 visc_final(g.dthalbe, c1[k], c2[k], c3[k], sxyv[k], sdiag[k], vxx[k], vyy[k], u[k], sxx[k], syy[k],
-                           sxy[k]);
+                           sxy[k], fk, g.dt, c0 + k);
+            }
         } else {
             const V c1 = ldv<VW>(m.u11, c0), c2 = ldv<VW>(m.u13, c0), c3 = ldv<VW>(m.u33, c0), c4 = ldv<VW>(m.u55i, c0);
 #pragma unroll
-            for (int k = 0; k < VW; ++k)
+            for (int k = 0; k < VW; ++k) {
+                Fuse fk = fz;
+                fk.fwi_u = fz.fwi_u && act[k];
                 visc_vti_final(g.dthalbe, c1[k], c2[k], c3[k], c4[k], sxyv[k], vxx[k], vyy[k], u[k], sxx[k], syy[k],
-                               sxy[k]);
+                               sxy[k], fk, g.dt, c0 + k);
+            }
         }
     }
     V sxxn = sxx0, syyn = syy0, sxyn = sxy0;
@@ -1391,22 +1423,39 @@ __global__ __launch_bounds__(BX *BY) void k_stress_ac(Grid g, Fields f0, Mat m, 
 
 // ------------------------------------------------------------ TTI passes
 
-// v_derivatives (src/v_derivatives.c:27-50)
-template <int ND>
-__global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f0)
+// The TTI frame tiles: tile q of the list (frame_tile_count of them) ->
+// tile column tx and row ty.  First the tile rows at the top and at the
+// bottom, whole; then, row by row in between, the tile columns at the left
+// and at the right.
+__host__ __device__ __forceinline__ int frame_tile_count(const Grid &g)
 {
-    const Fields f = f0.at(g.k_lo + blockIdx.z, g);   // shot slot of a batch
-    const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
-    if (i > g.nx || j > g.ny) return;
-    const int c = IDX(g, i, j);
-    float vxx, vyx, vxy, vyy;
-    op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
-    f.dvxx[c] = vxx;
-    f.dvyy[c] = vyy;
-    f.dvyx[c] = vyx;
-    f.dvxy[c] = vxy;
+    return (g.ftop + g.fbot) * g.ntx + (g.nty - g.ftop - g.fbot) * (g.fl + g.fr);
 }
+__device__ __forceinline__ void frame_tile(const Grid &g, int q, int &tx, int &ty)
+{
+    const int top = g.ftop * g.ntx, bot = g.fbot * g.ntx;
+    if (q < top) {
+        ty = q / g.ntx;
+        tx = q - ty * g.ntx;
+        return;
+    }
+    q -= top;
+    if (q < bot) {
+        const int r = q / g.ntx;
+        ty = g.nty - g.fbot + r;
+        tx = q - r * g.ntx;
+        return;
+    }
+    q -= bot;
+    const int w = g.fl + g.fr, r = q / w, col = q - r * w;
+    ty = g.ftop + r;
+    tx = col < g.fl ? col : g.ntx - g.fr + (col - g.fl);
+}
+__device__ __forceinline__ bool is_frame_tile(const Grid &g, int tx, int ty)
+{
+    return ty < g.ftop || ty >= g.nty - g.fbot || tx < g.fl || tx >= g.ntx - g.fr;
+}
+
 
 // frame CPML of update_s_*_tti_PML, in place on the derivative arrays.  The
 // reference treats the frame region by region (left, right, top, bottom, then
@@ -1417,9 +1466,11 @@ __global__ __launch_bounds__(BX *BY) void k_dv(Grid g, Fields f0)
 template <bool CORNERS>
 __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f0, Mat m)
 {
-    Fields f = f0.at(g.k_lo + blockIdx.z, g);   // shot slot of a batch
-    const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    Fields f = f0.at(g.k_lo + blockIdx.y, g);   // shot slot of a batch
+    int tx, ty;
+    frame_tile(g, blockIdx.x, tx, ty);          // one of the frame tiles
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
     if (CORNERS ? !(xf && yf) : (xf == yf)) return;
@@ -1433,32 +1484,30 @@ __global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f0, Mat
     f.dvyy[c] = vyy;
 }
 
-// TTI stress update (src/wavefield_update_s_el_tti.c:27-52,
-// src/wavefield_update_s_visc_TTI.c:27-84) reading the derivative arrays.
-// PASS 0: interior cells; PASS 1 / 3: CPML frame sides / corners on the
-// ranges shrunk by two cells at the outer edges
-// (src/update_s_elastic_TTI_PML.c:37-143); PASS 2: damping frame, every frame
-// cell (src/update_s_elastic_TTI_abs.c).
+// TTI stress update of the CPML frame (src/wavefield_update_s_el_tti.c:27-52,
+// src/wavefield_update_s_visc_TTI.c:27-84) reading the derivative arrays
+// after the frame CPML: PASS 1 the sides, PASS 3 the corners, on the ranges
+// shrunk by two cells at the outer edges (src/update_s_elastic_TTI_PML.c:37-143).
+// Launched over the frame tiles (blockIdx.x) and slots (blockIdx.y); the
+// interior and the damping frame are updated by k_stress_tti_f.
 template <int WEQ, int PASS>
-__global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m)
+__global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m, Fuse fz)
 {
-    const Fields f = f0.at(g.k_lo + blockIdx.z, g);   // shot slot of a batch
-    const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    static_assert(PASS == 1 || PASS == 3, "frame passes only");
+    const Fields f = f0.at(g.k_lo + blockIdx.y, g);   // shot slot of a batch
+    int tx, ty;
+    frame_tile(g, blockIdx.x, tx, ty);
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
-    if (PASS == 0 && frame) return;
-    if (PASS != 0) {
-        if (!frame) return;
-        if (PASS == 1 && xf && yf) return;
-        if (PASS == 3 && !(xf && yf)) return;
-        if (PASS == 1 || PASS == 3) {
-            if (i <= g.gx2 && i < 1 + 2) return;
-            if (i > g.gx3 && i > g.nx - 2) return;
-            if (j <= g.gy2 && j < 1 + 2) return;
-            if (j > g.gy3 && j > g.ny - 2) return;
-        }
-    }
+    if (!frame) return;
+    if (PASS == 1 && xf && yf) return;
+    if (PASS == 3 && !(xf && yf)) return;
+    if (i <= g.gx2 && i < 1 + 2) return;
+    if (i > g.gx3 && i > g.nx - 2) return;
+    if (j <= g.gy2 && j < 1 + 2) return;
+    if (j > g.gy3 && j > g.ny - 2) return;
     const int c = IDX(g, i, j), P = g.pitch;
     const float *X = f.dvxx, *Y = f.dvyy, *YX = f.dvyx, *XY = f.dvxy;
     const float vxx = X[c], vyy = Y[c];
@@ -1498,16 +1547,271 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m)
         u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (m.u15[c] * vij) + (g.dthalbe * (sp0 + sp));
         u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (m.u35[c] * vij) + (g.dthalbe * (sq0 + sq));
     }
+    fwi_u_store(fz, g.dt, c, u1, u2, u3);
     float sxy = f.sxy[c] + u1, sxx = f.sxx[c] + u2, syy = f.syy[c] + u3;
-    if (PASS == 2) {
-        const float a = m.absorb[c];
-        sxy *= a;
-        syy *= a;
-        sxx *= a;
-    }
     f.sxy[c] = sxy;
     f.sxx[c] = sxx;
     f.syy[c] = syy;
+}
+
+
+// ------------------------------------------ batches: S slots per thread
+//
+// The update kernels of a batch in its "slot-block" shape: one thread per
+// cell for S consecutive shot slots (S fixed at compile time), the cell's
+// material coefficients read once for them, every slot's stencil and state
+// loaded before any slot is updated (the slots' planes are disjoint parts of
+// the same allocations, so the compiler cannot move one slot's loads above
+// another's stores by itself), then the slots updated and stored in turn.
+// blockIdx.x is the tile, blockIdx.y the group of S slots, so consecutive
+// workgroups read one tile's coefficients.  Sources, the free surface and
+// sampling run as separate launches; the per-cell arithmetic of a slot is
+// that of the one-thread-per-cell kernels.
+
+template <int ND, int S>
+__global__ __launch_bounds__(BX *BY) void k_velocity_b(Grid g, Fields f0, Mat m, int nt, const int *ntb)
+{
+    const int t = blockIdx.x, ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j);
+    const float rip = m.rip[c], rjp = m.rjp[c];
+    const int k0 = g.k_lo + blockIdx.y * S;
+    float a[S], b[S], d[S], e[S], vx[S], vy[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const Fields f = f0.at(k0 + q, g);
+        op_v<ND>(g, f.sxx, f.syy, f.sxy, c, a[q], b[q], d[q], e[q]);
+        vx[q] = f.vx[c];
+        vy[q] = f.vy[c];
+    }
+    const Src s0{};
+    Fuse fz{};
+    fz.ntb = ntb;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        Fields f = f0.at(k0 + q, g);
+        velocity_core(g, f, m, s0, fz, nt, i, j, c, a[q], b[q], d[q], e[q], rip, rjp, vx[q], vy[q]);
+        f.vx[c] = vx[q];
+        f.vy[c] = vy[q];
+    }
+}
+
+template <int ND, int WEQ, int S>
+__global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, int nt, const int *ntb)
+{
+    constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
+    const int t = blockIdx.x, ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int c = IDX(g, i, j), P = g.pitch;
+    float c1 = 0.f, c2 = 0.f, c3 = 0.f, c4 = 0.f, br = 0.f, bd = 0.f, be = 0.f;
+    if (WEQ == SOFI_EL_ISO || WEQ == SOFI_VEL_ISO) {
+        c1 = m.f[c];
+        c2 = m.g[c];
+        c3 = m.fipjp[c];
+    }
+    if (WEQ == SOFI_VEL_ISO) {
+        br = m.br[c];
+        bd = m.bd[c];
+        be = m.be[c];
+    }
+    if (WEQ == SOFI_EL_VTI) {
+        c1 = m.c11[c];
+        c2 = m.c13[c];
+        c3 = m.c33[c];
+        c4 = m.c55i[c];
+    }
+    if (WEQ == SOFI_VEL_VTI) {
+        c1 = m.u11[c];
+        c2 = m.u13[c];
+        c3 = m.u33[c];
+        c4 = m.u55i[c];
+    }
+    const int k0 = g.k_lo + blockIdx.y * S;
+    float a[S], b[S], d[S], e[S], sxx[S], syy[S], sxy[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        const Fields f = f0.at(k0 + q, g);
+        float a_ = 0.f, b_ = 0.f, d_ = 0.f, e_ = 0.f;
+#pragma unroll
+        for (int mm = 1; mm <= ND; ++mm) {
+            const float h = g.hc[mm];
+            a_ += h * (f.vx[c + mm - 1] - f.vx[c - mm]);
+            b_ += h * (f.vy[c + mm] - f.vy[c - mm + 1]);
+            d_ += h * (f.vx[c + mm * P] - f.vx[c - (mm - 1) * P]);
+            e_ += h * (f.vy[c + (mm - 1) * P] - f.vy[c - mm * P]);
+        }
+        a[q] = a_;
+        b[q] = b_;
+        d[q] = d_;
+        e[q] = e_;
+        sxx[q] = f.sxx[c];
+        syy[q] = f.syy[c];
+        sxy[q] = f.sxy[c];
+    }
+    const Src s0{};
+    Fuse fz{};
+    fz.ntb = ntb;
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+        Fields f = f0.at(k0 + q, g);
+        float vxx, vyx, vxy, vyy;
+        if (!stress_dv(g, f, m, i, j, a[q], b[q], d[q], e[q], vxx, vyx, vxy, vyy)) continue;
+        float sx = sxx[q], sy = syy[q], sz = sxy[q];
+        if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sx, sy, sz, fz, g.dt, c);
+        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sx, sy, sz, fz, g.dt, c);
+        if (WEQ == SOFI_VEL_ISO) {
+            const float sxyv = vxy + vyx, sdiag = vxx + vyy;
+            VSum u;
+#pragma unroll
+            for (int l = 1; l <= SOFI_MAXL; ++l) {
+                if (l > g.L) break;
+                const int o = (l - 1) * g.plane + c;
+                float r = f.r[o], p = f.p[o], qq = f.q[o];
+                visc_level(m.eta[l], m.bip[l], m.cip[l], br, bd, be, sxyv, sdiag, vxx, vyy, r, p, qq, u);
+                f.r[o] = r;
+                f.p[o] = p;
+                f.q[o] = qq;
+            }
+            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sx, sy, sz, fz, g.dt, c);
+        }
+        if (WEQ == SOFI_VEL_VTI) {
+            const float sxyv = vxy + vyx;
+            VSum u;
+#pragma unroll
+            for (int l = 1; l <= SOFI_MAXL; ++l) {
+                if (l > g.L) break;
+                const int o = (l - 1) * g.plane + c;
+                float r = f.r[o], p = f.p[o], qq = f.q[o];
+                visc_vti_level(m.bip[l], m.cip[l], m.d11[o], m.d13[o], m.d33[o], m.d55i[o], sxyv, vxx, vyy, r, p, qq, u);
+                f.r[o] = r;
+                f.p[o] = p;
+                f.q[o] = qq;
+            }
+            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sx, sy, sz, fz, g.dt, c);
+        }
+        stress_post<ND, VISCO>(g, f, m, s0, fz, nt, i, j, c, sx, sy, sz);
+        f.sxx[c] = sx;
+        f.syy[c] = sy;
+        f.sxy[c] = sz;
+    }
+}
+
+// TTI stress update with the velocity derivatives computed in the kernel:
+// each workgroup first computes the four derivatives (op_s, raw: no CPML) of
+// its BX x BY tile and a one-cell ring into LDS (zero outside 1..NX x 1..NY,
+// the reference's never-written ghosts of the derivative arrays), then
+// updates its cells from them (src/wavefield_update_s_el_tti.c:27-52,
+// src/wavefield_update_s_visc_TTI.c:27-84).
+//   * interior cells: updated here; they read the raw derivatives of their
+//     neighbours, frame cells included, as the reference's interior update
+//     runs before the frame CPML (src/time_loop.c:180-186);
+//   * with the damping frame (ABS_TYPE 2) every frame cell is updated here
+//     too, then damped (src/update_s_elastic_TTI_abs.c:35-99);
+//   * with the CPML frame, tiles that hold a frame cell or one next to it
+//     store their raw derivatives into the derivative arrays, where the
+//     frame passes (k_cpml_dv_frame, k_stress_tti PASS 1 / 3, over these
+//     tiles only) apply the CPML region by region and update the frame.
+// S slots per thread (blockIdx.y: group of S slots of a batch; 1 for a
+// one-shot handle).
+template <int ND, int WEQ, int S>
+__global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat m, Fuse fz)
+{
+    constexpr int LX = BX + 2, LY = BY + 2;
+    __shared__ float sd[S][4][LY][LX];   // dvxx, dvyy, dvyx, dvxy of rows j0-1..j0+BY, columns i0-1..i0+BX
+    const int t = blockIdx.x, ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i0 = 1 + tx * BX, j0 = 1 + ty * BY;
+    const int k0 = g.k_lo + blockIdx.y * S;
+    const bool cpml = g.abs_type == 1 && g.fw > 0;
+    const bool store_dv = cpml && is_frame_tile(g, tx, ty);
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    for (int q = tid; q < LX * LY; q += BX * BY) {
+        const int lj = q / LX, li = q - lj * LX;
+        const int ii = i0 - 1 + li, jj = j0 - 1 + lj;
+        const bool in = ii >= 1 && ii <= g.nx && jj >= 1 && jj <= g.ny;
+        const int cc = IDX(g, ii, jj);
+        const bool own = store_dv && in && li >= 1 && li <= BX && lj >= 1 && lj <= BY;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            float vxx = 0.f, vyx = 0.f, vxy = 0.f, vyy = 0.f;
+            const Fields f = f0.at(k0 + s, g);
+            if (in) op_s<ND>(g, f.vx, f.vy, cc, vxx, vyx, vxy, vyy);
+            sd[s][0][lj][li] = vxx;
+            sd[s][1][lj][li] = vyy;
+            sd[s][2][lj][li] = vyx;
+            sd[s][3][lj][li] = vxy;
+            if (own) {
+                f.dvxx[cc] = vxx;
+                f.dvyy[cc] = vyy;
+                f.dvyx[cc] = vyx;
+                f.dvxy[cc] = vxy;
+            }
+        }
+    }
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const bool frame = in_xframe(g, i) || in_yframe(g, j);
+    const bool damp = frame && g.abs_type == 2 && g.fw > 0;
+    if (frame && !damp) return;     // CPML frame: the frame passes; no absorbing frame: never updated
+    const int c = IDX(g, i, j);
+    const int li = threadIdx.x + 1, lj = threadIdx.y + 1;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const Fields f = f0.at(k0 + s, g);
+        const float(*X)[LX] = sd[s][0], (*Y)[LX] = sd[s][1], (*YX)[LX] = sd[s][2], (*XY)[LX] = sd[s][3];
+        const float vxx = X[lj][li], vyy = Y[lj][li];
+        const float vxxipjp = 0.25f * (X[lj][li] + X[lj + 1][li] + X[lj][li + 1] + X[lj + 1][li + 1]);
+        const float vyyipjp = 0.25f * (Y[lj][li] + Y[lj + 1][li] + Y[lj][li + 1] + Y[lj + 1][li + 1]);
+        const float vyxij = 0.25f * (YX[lj][li] + YX[lj - 1][li] + YX[lj][li - 1] + YX[lj - 1][li - 1]);
+        const float vxyij = 0.25f * (XY[lj][li] + XY[lj - 1][li] + XY[lj][li - 1] + XY[lj - 1][li - 1]);
+        const float vij = vyxij + vxyij;
+        const float v = XY[lj][li] + YX[lj][li];
+        float u1, u2, u3;
+        if (WEQ == SOFI_EL_TTI) {
+            u1 = (m.c55i[c] * v) + (m.c15i[c] * vxxipjp) + (m.c35i[c] * vyyipjp);
+            u2 = (m.c11[c] * vxx) + (m.c13[c] * vyy) + (m.c15[c] * vij);
+            u3 = (m.c13[c] * vxx) + (m.c33[c] * vyy) + (m.c35[c] * vij);
+        } else {
+            float sr0 = 0.f, sp0 = 0.f, sq0 = 0.f, sr = 0.f, sp = 0.f, sq = 0.f;
+#pragma unroll
+            for (int l = 1; l <= SOFI_MAXL; ++l) {
+                if (l > g.L) break;
+                const int o = (l - 1) * g.plane + c;
+                const float r0 = f.r[o], p0 = f.p[o], q0 = f.q[o];
+                sr0 += r0;
+                sp0 += p0;
+                sq0 += q0;
+                const float bip = m.bip[l], cip = m.cip[l];
+                const float r1 = bip * (r0 * cip - (m.d55i[o] * v) - (m.d15i[o] * vxxipjp) - (m.d35i[o] * vyyipjp));
+                const float p1 = bip * (p0 * cip - (m.d11[o] * vxx) - (m.d13[o] * vyy) - (m.d15[o] * vij));
+                const float q1 = bip * (q0 * cip - (m.d13[o] * vxx) - (m.d33[o] * vyy) - (m.d35[o] * vij));
+                f.r[o] = r1;
+                f.p[o] = p1;
+                f.q[o] = q1;
+                sr += r1;
+                sp += p1;
+                sq += q1;
+            }
+            u1 = (m.u55i[c] * v) + (m.u15i[c] * vxxipjp) + (m.u35i[c] * vyyipjp) + (g.dthalbe * (sr0 + sr));
+            u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (m.u15[c] * vij) + (g.dthalbe * (sp0 + sp));
+            u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (m.u35[c] * vij) + (g.dthalbe * (sq0 + sq));
+        }
+        if (S == 1) fwi_u_store(fz, g.dt, c, u1, u2, u3);    // FWI handles hold one slot
+        float sxy = f.sxy[c] + u1, sxx = f.sxx[c] + u2, syy = f.syy[c] + u3;
+        if (damp) {
+            const float a = m.absorb[c];
+            sxy *= a;
+            syy *= a;
+            sxx *= a;
+        }
+        f.sxy[c] = sxy;
+        f.sxx[c] = sxx;
+        f.syy[c] = syy;
+    }
 }
 
 // ------------------------------------------------------------- samplers
@@ -1524,10 +1828,17 @@ __global__ void k_seis_dv(Grid g, Fields f0, Mat m, Rec r, int ins, const int *n
     const int i = r.i[k], j = r.j[k], c = IDX(g, i, j);
     float vxx, vyx, vxy, vyy;
     if (TTI) {
-        vxx = f.dvxx[c];
-        vyy = f.dvyy[c];
-        vyx = f.dvyx[c];
-        vxy = f.dvxy[c];
+        // the derivative arrays hold the post-CPML values of the CPML frame;
+        // elsewhere the derivatives are the raw ones, recomputed here
+        const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
+        if (g.abs_type == 1 && g.fw > 0 && (xf || yf)) {
+            vxx = f.dvxx[c];
+            vyy = f.dvyy[c];
+            vyx = f.dvyx[c];
+            vxy = f.dvxy[c];
+        } else {
+            op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
+        }
     } else {
         op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
         const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
@@ -1718,6 +2029,7 @@ struct sofi_dev {
     bool slot_fusable = false;
     std::vector<Fuse> slot_fz;
     int shape_v = 0, shape_s = 0;          // velocity / stress kernel shapes: SOFI_LAUNCH_*
+    int bs = 0;                            // a batch's slot blocks: slots per thread of k_*_b (0: slot kernels)
     int spt_mode = 0, grp_mode = 0;        // a batch's slots per thread and per launch (0: tuned in the first run)
     int grp = 0;                           // slots per launch of the update kernels (0: all)
     bool spt_tuned = false;
@@ -1757,6 +2069,15 @@ struct sofi_dev {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t gbuild = nullptr;           // while a block is recorded: the graph its launches go to
     hipGraphNode_t glast = nullptr;        // the last node recorded (each node follows it)
+    // argument values and pointer arrays of the recorded kernel nodes, held
+    // until the graph is destroyed (nothing that a node's parameters point
+    // to may die before the node: a runtime or tool that reads them after
+    // hipGraphAddKernelNode returns finds them alive)
+    struct NodeArgs {
+        std::shared_ptr<void> vals;
+        std::vector<void *> ptrs;
+    };
+    std::vector<NodeArgs> gargs;
     hipError_t gerr = hipSuccess;
     int graph_k = 0;
     unsigned gen = 1, graph_gen = 0;       // launch configuration: current / of the captured graph
@@ -1788,6 +2109,22 @@ namespace {
 // a 256-B boundary and a wavefront's 64 cells of a row are two whole 128-B
 // cache lines.
 constexpr size_t ALIGN_SHIFT = 224;
+
+// the tiles of the TTI frame passes: tiles holding a cell of the absorbing
+// frame or one next to it (the frame passes read the derivatives of the
+// frame cells' neighbours); rows 1..GY[2]+1 and GY[3]..NY, columns
+// 1..GX[2]+1 and GX[3]..NX
+void frame_tiles(Grid &g)
+{
+    g.ftop = g.gy2 > 0 ? g.gy2 / BY + 1 : 0;
+    g.fbot = g.gy3 < g.ny ? g.nty - (g.gy3 - 1) / BY : 0;
+    g.fl = g.gx2 > 0 ? g.gx2 / BX + 1 : 0;
+    g.fr = g.gx3 < g.nx ? g.ntx - (g.gx3 - 1) / BX : 0;
+    if (g.ftop > g.nty) g.ftop = g.nty;
+    if (g.ftop + g.fbot > g.nty) g.fbot = g.nty - g.ftop;
+    if (g.fl > g.ntx) g.fl = g.ntx;
+    if (g.fl + g.fr > g.ntx) g.fr = g.ntx - g.fl;
+}
 
 int dalloc(sofi_dev *h, void **p, size_t bytes)
 {
@@ -1955,6 +2292,8 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     g.wx = 2 * d.fw + 1;
     g.wy = d.nx + 1;
     g.ntx = (d.nx + BX - 1) / BX;
+    g.nty = (d.ny + BY - 1) / BY;
+    frame_tiles(g);
     g.nxp = (d.ny + 1) * g.wx;
     g.nyp = g.wx * g.wy;
     g.dt = d.dt;
@@ -2572,24 +2911,36 @@ void rot4(float **v)
 template <typename... P, typename... A>
 void launch(sofi_dev *h, void (*k)(P...), dim3 grid, dim3 block, A &&...a)
 {
-    std::tuple<std::decay_t<P>...> v(std::forward<A>(a)...);
-    void *args[sizeof...(P) > 0 ? sizeof...(P) : 1];
-    std::apply([&](auto &...x) {
-        int q = 0;
-        ((args[q++] = (void *)&x), ...);
-    }, v);
+    using Tup = std::tuple<std::decay_t<P>...>;
     if (!h->gbuild) {
+        Tup v(std::forward<A>(a)...);
+        void *args[sizeof...(P) > 0 ? sizeof...(P) : 1];
+        std::apply([&](auto &...x) {
+            int q = 0;
+            ((args[q++] = (void *)&x), ...);
+        }, v);
         const hipError_t e =
             hipLaunchKernel(reinterpret_cast<const void *>(k), grid, block, args, 0, h->cur ? h->cur : h->stream);
         if (e != hipSuccess && h->gerr == hipSuccess) h->gerr = e;
         return;
     }
+    // a node of the block being recorded: its argument values live in the
+    // handle until the graph is dropped
+    sofi_dev::NodeArgs na;
+    auto v = std::make_shared<Tup>(std::forward<A>(a)...);
+    na.ptrs.resize(sizeof...(P) > 0 ? sizeof...(P) : 1);
+    std::apply([&](auto &...x) {
+        int q = 0;
+        ((na.ptrs[q++] = (void *)&x), ...);
+    }, *v);
+    na.vals = v;
+    h->gargs.push_back(std::move(na));
     hipKernelNodeParams p{};
     p.func = reinterpret_cast<void *>(k);
     p.gridDim = grid;
     p.blockDim = block;
     p.sharedMemBytes = 0;
-    p.kernelParams = args;
+    p.kernelParams = h->gargs.back().ptrs.data();
     p.extra = nullptr;
     hipGraphNode_t n = nullptr;
     const hipError_t e = hipGraphAddKernelNode(&n, h->gbuild, h->glast ? &h->glast : nullptr, h->glast ? 1 : 0, &p);
@@ -2599,6 +2950,71 @@ void launch(sofi_dev *h, void (*k)(P...), dim3 grid, dim3 block, A &&...a)
         h->glast = n;
     }
 }
+
+// the TTI CPML frame of slots lo..hi-1, region by region over the frame
+// tiles: CPML of the sides, their update, CPML of the corners, their update
+// (src/update_s_elastic_TTI_PML.c:32-145)
+template <int ND>
+void tti_frame(sofi_dev *h, Grid gg, int lo, int hi, const Fields &f, const Mat &m, const Fuse &fz)
+{
+    const int nft = frame_tile_count(gg);
+    if (gg.fw <= 0 || gg.abs_type != 1 || nft <= 0 || hi <= lo) return;
+    gg.k_lo = lo;
+    gg.k_hi = hi;
+    const dim3 tb(BX, BY), tg((unsigned)nft, (unsigned)(hi - lo));
+    const bool el = gg.weq == SOFI_EL_TTI;
+    launch(h, k_cpml_dv_frame<false>, tg, tb, gg, f, m);
+    if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 1>, tg, tb, gg, f, m, fz);
+    else launch(h, k_stress_tti<SOFI_VEL_TTI, 1>, tg, tb, gg, f, m, fz);
+    launch(h, k_cpml_dv_frame<true>, tg, tb, gg, f, m);
+    if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 3>, tg, tb, gg, f, m, fz);
+    else launch(h, k_stress_tti<SOFI_VEL_TTI, 3>, tg, tb, gg, f, m, fz);
+}
+
+// Launches of a batch's update kernels in slot blocks of S slots per thread
+// (k_velocity_b, k_stress_b, k_stress_tti_f) over slots lo..hi-1; the slots
+// that do not fill a block of S go in blocks of one.
+template <int ND, template <int> class K, typename... A>
+void launch_blocks(sofi_dev *h, Grid gg, int lo, int hi, int S, A &&...a)
+{
+    const dim3 blk(BX, BY);
+    const unsigned ntile = (unsigned)(gg.ntx * gg.nty);
+    const int n = (hi - lo) / S;
+    if (n > 0) {
+        gg.k_lo = lo;
+        gg.k_hi = lo + n * S;
+        if (S == 4) launch(h, K<4>::fn(), dim3(ntile, n), blk, gg, a...);
+        else if (S == 2) launch(h, K<2>::fn(), dim3(ntile, n), blk, gg, a...);
+        else launch(h, K<1>::fn(), dim3(ntile, n), blk, gg, a...);
+    }
+    const int r0 = lo + n * S;
+    if (r0 < hi && S != 1) {
+        gg.k_lo = r0;
+        gg.k_hi = hi;
+        launch(h, K<1>::fn(), dim3(ntile, hi - r0), blk, gg, a...);
+    }
+}
+template <int ND>
+struct VelB {
+    template <int S>
+    struct K {
+        static auto fn() { return k_velocity_b<ND, S>; }
+    };
+};
+template <int ND, int WEQ>
+struct StrB {
+    template <int S>
+    struct K {
+        static auto fn() { return k_stress_b<ND, WEQ, S>; }
+    };
+};
+template <int ND, int WEQ>
+struct TtiF {
+    template <int S>
+    struct K {
+        static auto fn() { return k_stress_tti_f<ND, WEQ, S>; }
+    };
+};
 
 template <int ND>
 int step_nd(sofi_dev *h, int nt, const int *ntb)
@@ -2643,13 +3059,17 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // sources; batches launched one slot at a time, each launch with its
     // slot's sources)
     const bool src_fused = !tti && h->fusable && h->source_type != 5;
-    const int span = hi0 - lo0, grp = h->grp > 0 && h->grp < span ? h->grp : span;
-    const bool slot_fused = !tti && !h->fdt4 && h->slot_fusable && grp == 1 && nsrc > 0;
+    // a batch in slot blocks (h->bs slots per thread, all slots of the group
+    // in one launch per kernel); sources, free surface and sampling then run
+    // as launches of their own
+    const bool bm = h->nb > 1 && h->bs > 0 && !h->fdt4 && d.weq != SOFI_AC_ISO && d.mode == SOFI_MODE_FW;
+    const int span = hi0 - lo0, grp = !bm && h->grp > 0 && h->grp < span ? h->grp : span;
+    const bool slot_fused = !bm && !tti && !h->fdt4 && h->slot_fusable && grp == 1 && nsrc > 0;
     fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
     // the free surface follows the explosive / moment sources, so it is fused
     // when they are, or when there are none to inject this step
     const bool no_stress_src = nsrc == 0 || (h->psrc_inj.n == 0 && h->n_eq == 0);
-    fs.surf = (src_fused || slot_fused || (!tti && no_stress_src)) && d.free_surf && !sample_dv &&
+    fs.surf = !bm && (src_fused || slot_fused || (!tti && no_stress_src)) && d.free_surf && !sample_dv &&
               (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
     fs.fwi_u = store && d.velocity == 1;
@@ -2720,6 +3140,27 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
             }
             continue;
         }
+        if (bm) {
+            const int S = h->bs;
+            launch_blocks<ND, VelB<ND>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb);
+            if (te && lo == lo0 && grp == span) {
+                (void)hipEventRecord(te[1], st);
+                (void)hipEventRecord(te[2], st);
+            }
+            switch (d.weq) {
+              case SOFI_EL_ISO: launch_blocks<ND, StrB<ND, SOFI_EL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
+              case SOFI_VEL_ISO: launch_blocks<ND, StrB<ND, SOFI_VEL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
+              case SOFI_EL_VTI: launch_blocks<ND, StrB<ND, SOFI_EL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
+              case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
+              default: {
+                  const Fuse f0{};
+                  if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0);
+                  else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0);
+                  tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, f0);
+              }
+            }
+            continue;
+        }
         // a launch of one slot runs the one-shot kernels on that slot's fields
         const bool one = gg.k_hi - gg.k_lo == 1;
         const Fields fk = one ? f.at(lo, g) : f;
@@ -2784,22 +3225,13 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
           case SOFI_EL_VTI: SOFI_STRESS(SOFI_EL_VTI); break;
           case SOFI_VEL_VTI: SOFI_STRESS(SOFI_VEL_VTI); break;
           default: {
-              dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, gg.k_hi - lo);
-              launch(h, k_dv<ND>, tg, tb, gg, f);
-              const bool el = d.weq == SOFI_EL_TTI;
-              if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 0>, tg, tb, gg, f, m);
-              else launch(h, k_stress_tti<SOFI_VEL_TTI, 0>, tg, tb, gg, f, m);
-              if (d.fw > 0 && d.abs_type == 1) {
-                  launch(h, k_cpml_dv_frame<false>, tg, tb, gg, f, m);
-                  if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 1>, tg, tb, gg, f, m);
-                  else launch(h, k_stress_tti<SOFI_VEL_TTI, 1>, tg, tb, gg, f, m);
-                  launch(h, k_cpml_dv_frame<true>, tg, tb, gg, f, m);
-                  if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 3>, tg, tb, gg, f, m);
-                  else launch(h, k_stress_tti<SOFI_VEL_TTI, 3>, tg, tb, gg, f, m);
-              } else if (d.fw > 0 && d.abs_type == 2) {
-                  if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 2>, tg, tb, gg, f, m);
-                  else launch(h, k_stress_tti<SOFI_VEL_TTI, 2>, tg, tb, gg, f, m);
-              }
+              // the interior (and a damping frame) with the derivatives
+              // computed in the kernel, one slot per thread; then the CPML
+              // frame region by region over the frame tiles
+              // (src/time_loop.c:180-186)
+              if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, 1, f, m, fss);
+              else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, 1, f, m, fss);
+              tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, fss);
           }
         }
     }
@@ -2911,6 +3343,7 @@ void graph_drop(sofi_dev *h)
     h->gexec = nullptr;
     h->graph = nullptr;
     h->graph_k = 0;
+    h->gargs.clear();
 }
 
 // steps nt .. nt + k - 1 from the captured block (captured first when the
@@ -2931,6 +3364,7 @@ int run_block(sofi_dev *h, int nt, int k)
         h->gerr = hipSuccess;
         if (rc || e != hipSuccess) {
             (void)hipGraphDestroy(gr);
+            h->gargs.clear();
             return rc ? rc : fail("recording a block of steps failed: %s", hipGetErrorString(e));
         }
         h->graph = gr;
@@ -2997,7 +3431,7 @@ int run_range(sofi_dev *h, int nt_first, int nt_last)
 {
     int nt = nt_first;
     const int span = h->cur_hi - h->cur_lo;
-    if (h->nb > 1 && (h->spt_mode == 0 || h->grp_mode == 0) && !h->spt_tuned) {
+    if (h->nb > 1 && h->bs == 0 && (h->spt_mode == 0 || h->grp_mode == 0) && !h->spt_tuned) {
         std::vector<std::pair<int, int>> cand;   // (slots per launch, slots per thread)
         for (int gr : { span, 16, 8, 4, 1 }) {
             if (gr > span || (gr != span && gr >= span)) continue;
@@ -3139,6 +3573,18 @@ int sofi_set_batch_launch(sofi_dev *h, int group, int spt)
     h->gen++;
     return 0;
 }
+
+int sofi_set_batch_block(sofi_dev *h, int slots)
+{
+    if (!h) return fail("sofi_set_batch_block: null handle");
+    if (slots != 0 && slots != 1 && slots != 2 && slots != 4)
+        return fail("sofi_set_batch_block: slots per thread must be 0, 1, 2 or 4 (got %d)", slots);
+    h->bs = slots;
+    h->gen++;
+    return 0;
+}
+
+int sofi_get_batch_block(const sofi_dev *h) { return h ? h->bs : 0; }
 
 int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt)
 {

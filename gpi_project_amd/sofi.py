@@ -363,6 +363,15 @@ class Batch(Simulation):
         group (0: the largest group that fits the memory-side cache)."""
         _chk_hip(self._L.sofi_set_batch_resident(self._h, slots))
 
+    def set_block(self, slots: int = 0):
+        """Slot blocks: all slots in one launch per kernel, `slots` (1, 2 or
+        4) slots per thread; 0 for the slot-loop kernels."""
+        _chk_hip(self._L.sofi_set_batch_block(self._h, slots))
+
+    @property
+    def block(self) -> int:
+        return int(self._L.sofi_get_batch_block(self._h))
+
     def set_streams(self, streams: int = 0):
         """Streams the resident groups run on at once (0: 2)."""
         _chk_hip(self._L.sofi_set_batch_streams(self._h, streams))

@@ -55,6 +55,18 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out);
 int sofi_select_shot(sofi_dev *h, int k);
 int sofi_set_batch_launch(sofi_dev *h, int group, int spt);
 int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
+/* Slot blocks of a batch: with slots > 0 the update kernels of every step
+ * cover all slots of the (resident) group in one launch each, one thread per
+ * cell for `slots` consecutive slots (1, 2 or 4), the cell's material
+ * coefficients read once for them and every slot's stencil loaded before any
+ * slot is updated; sources, free surface and sampling run as launches of
+ * their own.  TTI stress updates then compute the velocity derivatives in
+ * the kernel for the same slot blocks.  0 (the default) keeps the slot-loop
+ * kernels of sofi_set_batch_launch.  Not for AC_ISO, 4th order in time or
+ * MODE=FWI (the slot-loop kernels run).  The results do not depend on it.
+ * No counterpart in the reference. */
+int sofi_set_batch_block(sofi_dev *h, int slots);
+int sofi_get_batch_block(const sofi_dev *h);
 /* Resident groups of a batch: sofi_run advances `slots` slots through its
  * whole step range before the next group, so that a group's wavefields and
  * the shared medium stay in the 256 MB memory-side cache from one kernel to

@@ -24,8 +24,10 @@ has the reference at /root/reference and its build in oracle/_ref).
 7. ref1/fwi_el_v0_eprecond1.npz: the summed gradients with EPRECOND = 1.
 8. ref1/fwi_tf*.npz, fwi_workflow.npz: FWI gradients with TIME_FILT 1, 2 and a workflow.
 9. ref1/acq_*.npz: wavelet shapes, plane wave, receiver array and line, ASCII formats.
+10. ref1/c1.npz, c3_*.npz, c2_800.npz, c5_small.npz: the benchmark workloads
+    (config 1 in full; the others at the sizes given in configs()).
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq] [configs]
 """
 from __future__ import annotations
 
@@ -226,6 +228,70 @@ def c1():
         print("ref1 c1 %.2fs" % t[0])
 
 
+C3_SMALL = dict(nshots=8, nx=500, ny=250, nt=600)
+C2_STEPS, C2_SNAP_DEC = 800, 8
+C5_SMALL = dict(nshots=2, nt=1000)
+C5_SUB = 3   # gradient subsampling of the C5 fixture (x and y)
+
+
+def configs():
+    """The benchmark workloads on the reference build, one process each:
+    * c3_{elvti,eltti}: workloads.config3 at 500 x 250, 8 shots, NT 600
+      (RUN_MULTIPLE_SHOTS; every shot's vx / vy section);
+    * c2_800: workloads.config2 at its full 4000 x 2000 size for 800 steps
+      (sections, and a snapshot of vx / vy at step 800 decimated 8 x 8);
+    * c5_small: workloads.config5 at 1500 x 500 with 2 shots, NT 1000,
+      DTINV 1 (the summed gradients, every 3rd cell in x and y)."""
+    out = os.path.join(HERE, "ref1")
+    from gpi_project_amd import workloads
+    only = os.environ.get("CFG_ONLY", "").split()
+    for weq in ("EL_VTI", "EL_TTI"):
+        name = f"c3_{weq.lower().replace('_', '')}"
+        if (only and name not in only) or os.path.exists(os.path.join(out, f"{name}.npz")):
+            continue
+        with tempfile.TemporaryDirectory() as td:
+            js = workloads.config3(td, weq, **C3_SMALL)
+            t = run_ref(js, td)
+            r = {}
+            for s in range(1, C3_SMALL["nshots"] + 1):
+                for comp in ("vx", "vy"):
+                    r[f"shot{s}_seis_{comp}"] = read_su(os.path.join(td, "out", f"seis_{comp}.su.shot{s}"))
+            r["time_loop_s_1rank"] = np.array(t)
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, "time loops %s s" % t, flush=True)
+    name = f"c2_{C2_STEPS}"
+    if not (only and name not in only) and not os.path.exists(os.path.join(out, f"{name}.npz")):
+        with tempfile.TemporaryDirectory() as td:
+            js = workloads.config2(td, nt=C2_STEPS)
+            tsnap = repr(C2_STEPS * 2.5e-4)
+            txt = open(js).read()
+            for k, v in (("SNAP", "1"), ("TSNAP1", tsnap), ("TSNAP2", tsnap), ("TSNAPINC", "1.0"),
+                         ("IDX", str(C2_SNAP_DEC)), ("IDY", str(C2_SNAP_DEC))):
+                a = txt.index(f'"{k}" : "')
+                b = txt.index('"', a + len(k) + 6)
+                txt = txt[:a] + f'"{k}" : "{v}' + txt[b:]
+            open(js, "w").write(txt)
+            t = run_ref(js, td)
+            nxs, nys = (4000 - 1) // C2_SNAP_DEC + 1, (2000 - 1) // C2_SNAP_DEC + 1
+            r = collect(os.path.join(td, "out", ""), (nxs, nys), 1, 1, 1)
+            r["time_loop_s_1rank"] = np.array(t)
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, "time loop %s s" % t, flush=True)
+    name = "c5_small"
+    if not (only and name not in only) and not os.path.exists(os.path.join(out, f"{name}.npz")):
+        with tempfile.TemporaryDirectory() as td:
+            run_ref(workloads.config5(td, "true", **C5_SMALL), td)
+            t = run_ref(workloads.config5(td, "init", dtinv=1, **C5_SMALL), td)
+            r = {}
+            for p in ("vp", "vs", "rho"):
+                g = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(1500, 500)
+                r[f"grad_{p}_sum_sub"] = g[::C5_SUB, ::C5_SUB].copy()
+                r[f"grad_{p}_sum_norm"] = np.array(np.linalg.norm(g.astype(np.float64)))
+            r["time_loops_s_1rank"] = np.array(t)
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, "time loops %s s" % t, flush=True)
+
+
 def fwi():
     """FWI gradients of the reference build (DTINV = 1, one process): the
     measured data from a MODE=FW run on the true model, then MODE=FWI on the
@@ -326,7 +392,7 @@ def fwi_tfilt():
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond",
-                            "fwi_tfilt", "acq"]
+                            "fwi_tfilt", "acq", "configs"]
     for w in what:
         {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
-         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt, "acq": acq}[w]()
+         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt, "acq": acq, "configs": configs}[w]()
