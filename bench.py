@@ -421,10 +421,11 @@ def bench_fwi(args):
     sim.close()
     # one shot after the other on one handle, then args.fwi_workers shots at once
     t0 = time.perf_counter()
-    rs = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=1)
+    rs = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=1, sections=False)
     t_seq = time.perf_counter() - t0
     t0 = time.perf_counter()
-    rc = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=args.fwi_workers)
+    rc = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, workers=args.fwi_workers,
+                           sections=False)
     t_it = time.perf_counter() - t0
     same = all(np.array_equal(a, b) for a, b in zip(rs["sum"], rc["sum"]))
     dev_ms = sum(r["ms_forward"] + r["ms_adjoint"] for r in rs["shots"].values())

@@ -41,6 +41,12 @@ class SofiDesc(C.Structure):
         return d
 
 
+class ResidualOpts(C.Structure):
+    """Mirror of ``sofi_residual_opts`` (include/sofi_hip.h)."""
+    _fields_ = [("lnorm", C.c_int), ("time_filt", C.c_int), ("f_low_pass", C.c_float), ("f_high_pass", C.c_float),
+                ("order", C.c_int), ("waterlevel_lnorm8", C.c_float)]
+
+
 def pitch(nx: int) -> int:
     return ((nx + 2 * PAD + 63) // 64) * 64
 
@@ -81,6 +87,7 @@ def host():
     L.sofi_problem_set_shot.argtypes = [_VP, C.c_int]
     L.sofi_problem_desc.restype = C.POINTER(SofiDesc)
     L.sofi_problem_desc.argtypes = [_VP]
+    L.sofi_problem_residual_opts.argtypes = [_VP, _IP, _FP]
     L.sofi_problem_params.restype = _VP
     L.sofi_problem_params.argtypes = [_VP]
     for fn in ("array", "profile", "model"):
@@ -160,6 +167,9 @@ def hip():
     L.sofi_fwi_set_adjoint.argtypes = [_VP, C.c_int, _IP, _IP, _FP, _FP]
     L.sofi_fwi_gradient.argtypes = [_VP, _FP, _FP, _FP]
     L.sofi_fwi_gradient_sum.argtypes = [_VP, _FP, _FP, _FP, C.c_int]
+    L.sofi_fwi_residual.argtypes = [_VP, C.POINTER(ResidualOpts), _FP, _FP, _DP]
+    L.sofi_fwi_get_adjoint.argtypes = [_VP, C.c_int, _FP]
+    L.sofi_fwi_adjoint_count.argtypes = [_VP]
     _hip = L
     return L
 
@@ -185,5 +195,6 @@ def exported_symbols():
                            "sofi_set_batch_launch", "sofi_get_batch_launch", "sofi_estimate_footprint",
                            "sofi_set_graph", "sofi_graph_blocks", "sofi_set_batch_resident",
                            "sofi_get_batch_resident", "sofi_set_batch_streams", "sofi_set_batch_block",
-                           "sofi_get_batch_block"],
+                           "sofi_get_batch_block", "sofi_fwi_residual", "sofi_fwi_get_adjoint",
+                           "sofi_fwi_adjoint_count"],
     }

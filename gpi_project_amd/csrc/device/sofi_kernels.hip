@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "sofi_hip.h"
+#include "../common/sofi_iir.h"
 
 namespace {
 
@@ -1980,6 +1981,131 @@ __global__ __launch_bounds__(BX *BY) void k_fwi_grad(Grid g, Fwi w, int idx, int
 
 } // namespace
 
+
+// ------------------------------------------------ FWI residuals on the device
+//
+// calc_res (src/calc_res.c:127-301) on the shot's resident sections, one
+// thread per trace walking its samples in order, so that the running sums
+// (the time integral when VELOCITY = 0, the trace norms of LNORM 5 / 7) and
+// every sample are the host restatement's float operations in the same order
+// (no contraction into fused multiply-adds, which the host build does not
+// form either).
+
+// one TIME_FILT pass (src/timedomain_filt.c:28-113) over each trace of in:
+// the Butterworth sections in cascade, in double, from a zero state, the
+// result rounded to float and written time-reversed into out (the reference
+// filters its buffer, rounds to float and reverses it, twice)
+struct TFilt {
+    int n;
+    double f0[SOFI_IIR_MAXSEC], f1[SOFI_IIR_MAXSEC], f2[SOFI_IIR_MAXSEC], g1[SOFI_IIR_MAXSEC], g2[SOFI_IIR_MAXSEC];
+};
+
+__global__ void k_tfilt(const float *in, float *out, int ntr, int ns, TFilt f)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ntr) return;
+    double x1[SOFI_IIR_MAXSEC], x2[SOFI_IIR_MAXSEC], y1[SOFI_IIR_MAXSEC], y2[SOFI_IIR_MAXSEC];
+#pragma unroll
+    for (int q = 0; q < SOFI_IIR_MAXSEC; ++q) x1[q] = x2[q] = y1[q] = y2[q] = 0.0;
+    const float *a = in + (size_t)r * ns;
+    float *b = out + (size_t)r * ns;
+    for (int k = 0; k < ns; ++k) {
+        double x = (double)a[k];
+#pragma unroll
+        for (int q = 0; q < SOFI_IIR_MAXSEC; ++q) {
+            if (q >= f.n) break;
+            const double y = f.f0[q] * x + f.f1[q] * x1[q] + f.f2[q] * x2[q] + f.g1[q] * y1[q] + f.g2[q] * y2[q];
+            x2[q] = x1[q];
+            x1[q] = x;
+            y2[q] = y1[q];
+            y1[q] = y;
+            x = y;
+        }
+        b[ns - 1 - k] = (float)x;
+    }
+}
+
+// the adjoint source of trace r, time-reversed, into adj[r][nt] (the first
+// ns samples), and the trace's share of the misfit calc_res accumulates:
+// LNORM 1 sign(measured - synthetic); 2 synthetic - measured, integrated
+// (running sum times DT) when VELOCITY = 0; 3 Cauchy r / (1 + r^2); 4
+// tanh(r); 5 / 7 the normalised correlation's derivative of the
+// (integrated) traces, a zero trace norm taken as 1; 8 the envelope residual
+// with the envelopes the reference never fills, s / (0 + WATERLEVEL) * (0 - 0) - 0
+__global__ void k_residual(const float *syn, const float *obs, float *adj, float *partial, int ntr, int ns, int nt,
+                           int lnorm, int velocity, float dt, float wl)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ntr) return;
+    const float *s = syn + (size_t)r * ns, *sd = obs + (size_t)r * ns;
+    float *df = adj + (size_t)r * nt;
+    const float RMS = 1.0f;
+    float abs_sd = 0.0f, abs_s = 0.0f, mult = 0.0f;
+    if (lnorm == 5 || lnorm == 7) {
+        float acc_s = 0.0f, acc_sd = 0.0f;
+        for (int j = 0; j < ns; ++j) {
+            float is = s[j], isd = sd[j];
+            if (velocity == 0) {
+                acc_s += s[j];
+                is = acc_s * dt;
+                acc_sd += sd[j];
+                isd = acc_sd * dt;
+            }
+            abs_sd += isd * isd;
+            abs_s += is * is;
+            mult += isd * is;
+        }
+        abs_sd = abs_sd == 0 ? 1.0f : (float)sqrt((double)abs_sd);
+        abs_s = abs_s == 0 ? 1.0f : (float)sqrt((double)abs_s);
+    }
+    float misfit = 0.0f, acc_s = 0.0f, acc_sd = 0.0f;
+    for (int j = 1; j <= ns; ++j) {
+        float is = s[j - 1], isd = sd[j - 1];
+        if (velocity == 0) {
+            acc_s += s[j - 1];
+            is = acc_s * dt;
+            acc_sd += sd[j - 1];
+            isd = acc_sd * dt;
+        }
+        float v = 0.0f;
+        const float rr = (sd[j - 1] - s[j - 1]) / RMS;
+        switch (lnorm) {
+          case 1: v = (rr > 0 ? 1.0f : rr < 0 ? -1.0f : 0.0f) / RMS; break;
+          case 2: v = is - isd; break;
+          case 3: v = (rr / (1 + (rr * rr))) / RMS; break;
+          case 4: v = (float)tanh((double)rr) / RMS; break;
+          case 5:
+          case 7: v = ((is * mult) / (abs_s * abs_s * abs_s * abs_sd)) - (isd / (abs_s * abs_sd)); break;
+          case 8: v = is / (0.0f + wl) * (0.0f - 0.0f) - 0.0f; break;
+        }
+        df[ns - j] = v;
+        if (lnorm == 2) misfit += v * v;
+        if (lnorm == 5 || lnorm == 7) misfit -= (isd * is) / (abs_sd * abs_s);
+        if (lnorm == 8) misfit += (0.0f - 0.0f) * (0.0f - 0.0f);
+    }
+    partial[r] = misfit;
+}
+
+// the traces' misfit shares summed in trace order into *out
+__global__ void k_misfit_sum(const float *partial, int ntr, float *out)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float m = 0.0f;
+    for (int r = 0; r < ntr; ++r) m += partial[r];
+    *out = m;
+}
+
+// n floats set to v (a fill kernel rather than hipMemsetAsync, which must
+// not be recorded where a caller captures the stream into a graph)
+__global__ void k_fill(float *p, size_t n, float v)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
 // ====================================================================== handle
 
 struct sofi_dev {
@@ -2061,6 +2187,10 @@ struct sofi_dev {
     int *adj_inj_buf = nullptr;
     float *adj_sx = nullptr, *adj_sy = nullptr;
     int nadj = 0;
+    bool adj_at_rec = false;               // the adjoint arrays and plan are those of slot 0's receivers
+    float *rbuf = nullptr;                 // workspace of sofi_fwi_residual (measured data, filtered copies, shares)
+    size_t rbuf_len = 0;
+    int dev = 0;                           // the device the handle was opened on
     // captured blocks of time steps: one graph of graph_k steps whose kernels
     // read the block's base step from ntb, replayed for every aligned block
     int *ntb = nullptr;
@@ -2303,6 +2433,10 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     g.dhi4 = (float)(1.0 / (double)d.dh);
     g.dthalbe = (float)(d.dt / 2.0);
     for (int k = 0; k < 7; ++k) g.hc[k] = d.hc[k];
+    if (hipGetDevice(&h->dev) != hipSuccess) {
+        sofi_close(h);
+        return fail("sofi_open: hipGetDevice failed");
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
         hipEventCreate(&h->tune0) != hipSuccess || hipEventCreate(&h->tune1) != hipSuccess) {
@@ -2364,9 +2498,13 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
         rc |= dalloc_t(h, &f.psi_vxy, nyp);
     }
     if (d.mode == SOFI_MODE_FWI) {
-        if (d.weq != SOFI_EL_ISO) {
+        if (d.weq == SOFI_AC_ISO) {
             sofi_close(h);
-            return fail("sofi_open: MODE=FWI is supported for WEQ=EL_ISO only");
+            return fail("sofi_open: MODE=FWI needs an elastic or viscoelastic WEQ (calc_conv reads sxy)");
+        }
+        if (nshots != 1) {
+            sofi_close(h);
+            return fail("sofi_open_batch: MODE=FWI runs one shot per handle");
         }
         if (d.adjoint_type < 1 || d.adjoint_type > 3) {
             sofi_close(h);
@@ -2588,6 +2726,7 @@ int sofi_set_receivers(sofi_dev *h, int ntr, const int *rec_i, const int *rec_j)
     sl.ri.assign(rec_i, rec_i + ntr);
     sl.rj.assign(rec_j, rec_j + ntr);
     h->rec_dirty = true;
+    h->adj_at_rec = false;
     return 0;
 }
 
@@ -3484,6 +3623,158 @@ int run_range(sofi_dev *h, int nt_first, int nt_last)
 
 } // namespace
 
+namespace {
+
+// the adjoint-source arrays [ntr][NT] and their injection plan at the
+// receivers of the handle's shot, zero (what the device residuals fill)
+int adj_at_receivers(sofi_dev *h)
+{
+    const sofi_dev::Slot &sl = h->slots[0];
+    const int n = (int)sl.ri.size();
+    if (h->adj_at_rec && h->nadj == n) return 0;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h, &h->adj_sx);
+    dfree(h, &h->adj_sy);
+    const size_t len = (size_t)(n > 0 ? n : 1) * h->d.nt;
+    if (dalloc_t(h, &h->adj_sx, len) || dalloc_t(h, &h->adj_sy, len)) return -1;
+    HIPCHK(hipMemset(h->adj_sx, 0, len * sizeof(float)));
+    HIPCHK(hipMemset(h->adj_sy, 0, len * sizeof(float)));
+    if (build_inj(h, n, sl.ri.data(), sl.rj.data(), nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
+    h->nadj = n;
+    h->adj_at_rec = true;
+    h->gen++;
+    return 0;
+}
+
+// the Butterworth sections of one TIME_FILT pass (sofi_iir.h)
+TFilt tfilt(double dt, double period, int order, int highpass)
+{
+    sofi_iir2 sec[SOFI_IIR_MAXSEC];
+    TFilt f{};
+    f.n = sofi_iir_butterworth(dt, period, order, highpass, sec);
+    for (int q = 0; q < f.n; ++q) {
+        f.f0[q] = sec[q].f0;
+        f.f1[q] = sec[q].f1;
+        f.f2[q] = sec[q].f2;
+        f.g1[q] = sec[q].g1;
+        f.g2[q] = sec[q].g2;
+    }
+    return f;
+}
+
+// floats of the residual workspace for ntr traces of ns samples: a filtered
+// synthetic section, a filter buffer, a filtered measured section, the
+// traces' misfit shares and the two components' misfits
+size_t residual_ws_floats(int ntr, int ns) { return 3 * (size_t)ntr * ns + (size_t)ntr + 2; }
+
+const char *residual_check(const sofi_dev *h, const sofi_residual_opts *o)
+{
+    if (!h) return "null handle";
+    if (!o) return "null options";
+    if (h->d.mode != SOFI_MODE_FWI || h->nb != 1) return "the handle was not opened with MODE=FWI";
+    if (o->lnorm < 1 || o->lnorm > 8 || o->lnorm == 6) return "LNORM has no residual in the reference (1-5, 7, 8)";
+    if (o->time_filt && (o->order < 1 || o->order > 2 * SOFI_IIR_MAXSEC - 1 || !(o->f_low_pass > 0.0f) ||
+                         o->f_high_pass < 0.0f))
+        return "time filter: ORDER must be 1..17 and the corner frequencies positive";
+    if (h->rec_dirty || !h->sec || h->slots[0].ri.empty()) return "no synthetic sections: run the forward pass first";
+    if (h->d.ndt != 1) return "MODE=FWI needs NDT = 1";
+    return nullptr;
+}
+
+// calc_res of the components ADJOINT_TYPE uses, on the handle's stream:
+// obs[c] the measured section of component c (device, [ntr][ns]), the
+// synthetic one the handle's section; TIME_FILT filters both first
+// (src/inversion.c:68-84).  The residuals go time-reversed into the
+// adjoint arrays, the components' misfits into the last two floats of ws.
+int residual_core(sofi_dev *h, const sofi_residual_opts &o, const float *const obs[2], float *ws)
+{
+    const sofi_desc &d = h->d;
+    const int ntr = h->nadj, ns = d.ns;
+    const size_t nn = (size_t)ntr * ns;
+    float *syncopy = ws, *tmp = ws + nn, *obsf = ws + 2 * nn, *partial = ws + 3 * nn, *mis = partial + ntr;
+    hipStream_t st = h->stream;
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, mis, (size_t)2, 0.0f);
+    const dim3 tb(64), tg((unsigned)((ntr + 63) / 64));
+    const double dt = (double)d.dt;
+    for (int c = 0; c < 2 && ntr > 0; ++c) {
+        const bool used = c == 0 ? (d.adjoint_type == 1 || d.adjoint_type == 3)
+                                 : (d.adjoint_type == 1 || d.adjoint_type == 2);
+        if (!used) continue;
+        float *adj = c == 0 ? h->adj_sx : h->adj_sy;
+        const float *sy = h->sec + h->sec_base[0] + (size_t)c * nn, *ob = obs[c];
+        if (o.time_filt) {
+            // timedomain_filt: the zero-phase low-pass, then the high-pass (src/timedomain_filt.c:57-111)
+            const TFilt lp = tfilt(dt, 1.0 / (double)o.f_low_pass, o.order, 0);
+            hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, sy, tmp, ntr, ns, lp);
+            hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, (const float *)tmp, syncopy, ntr, ns, lp);
+            hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, ob, tmp, ntr, ns, lp);
+            hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, (const float *)tmp, obsf, ntr, ns, lp);
+            if (o.f_high_pass > 0.0f) {
+                const TFilt hp = tfilt(dt, 1.0 / (double)o.f_high_pass, o.order, 1);
+                hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, (const float *)syncopy, tmp, ntr, ns, hp);
+                hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, (const float *)tmp, syncopy, ntr, ns, hp);
+                hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, (const float *)obsf, tmp, ntr, ns, hp);
+                hipLaunchKernelGGL(k_tfilt, tg, tb, 0, st, (const float *)tmp, obsf, ntr, ns, hp);
+            }
+            sy = syncopy;
+            ob = obsf;
+        }
+        hipLaunchKernelGGL(k_residual, tg, tb, 0, st, sy, ob, adj, partial, ntr, ns, d.nt, o.lnorm, d.velocity, d.dt,
+                           o.waterlevel_lnorm8);
+        hipLaunchKernelGGL(k_misfit_sum, dim3(1), dim3(64), 0, st, (const float *)partial, ntr, mis + c);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sofi_fwi_residual(sofi_dev *h, const sofi_residual_opts *o, const float *obs_x, const float *obs_y, double *misfit)
+{
+    if (const char *e = residual_check(h, o)) return fail("sofi_fwi_residual: %s", e);
+    const sofi_desc &d = h->d;
+    const bool ux = d.adjoint_type == 1 || d.adjoint_type == 3, uy = d.adjoint_type == 1 || d.adjoint_type == 2;
+    if ((ux && !obs_x) || (uy && !obs_y)) return fail("sofi_fwi_residual: measured section missing for ADJOINT_TYPE=%d", d.adjoint_type);
+    if (adj_at_receivers(h)) return -1;
+    const int ntr = h->nadj;
+    const size_t nn = (size_t)ntr * d.ns, need = 2 * nn + residual_ws_floats(ntr, d.ns);
+    if (h->rbuf_len < need) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dfree(h, &h->rbuf);
+        h->rbuf_len = 0;
+        if (dalloc_t(h, &h->rbuf, need)) return -1;
+        h->rbuf_len = need;
+    }
+    float *ox = h->rbuf, *oy = h->rbuf + nn, *ws = h->rbuf + 2 * nn;
+    if (ux && nn) HIPCHK(hipMemcpyAsync(ox, obs_x, nn * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    if (uy && nn) HIPCHK(hipMemcpyAsync(oy, obs_y, nn * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    const float *obs[2] = { ox, oy };
+    if (residual_core(h, *o, obs, ws)) return -1;
+    float mis[2] = { 0.f, 0.f };
+    HIPCHK(hipMemcpyAsync(mis, ws + 3 * nn + ntr, sizeof(mis), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (misfit) *misfit = (double)mis[0] + (double)mis[1];
+    return 0;
+}
+
+int sofi_fwi_get_adjoint(sofi_dev *h, int comp, float *out)
+{
+    if (!h) return fail("sofi_fwi_get_adjoint: null handle");
+    if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_get_adjoint: the handle was not opened with MODE=FWI");
+    if (comp != SOFI_C_VX && comp != SOFI_C_VY) return fail("sofi_fwi_get_adjoint: component %d (0 vx, 1 vy)", comp);
+    if (!out) return fail("sofi_fwi_get_adjoint: null output");
+    const float *a = comp == SOFI_C_VX ? h->adj_sx : h->adj_sy;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (a && h->nadj > 0) HIPCHK(hipMemcpy(out, a, (size_t)h->nadj * h->d.nt * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sofi_fwi_adjoint_count(const sofi_dev *h) { return h ? h->nadj : 0; }
+
+} // extern "C"
+
 extern "C" {
 
 int sofi_run(sofi_dev *h, int nt_first, int nt_last)
@@ -3763,6 +4054,7 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const
     }
     if (build_inj(h, n, ai, aj, nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
+    h->adj_at_rec = false;
     h->gen++;
     return 0;
 }

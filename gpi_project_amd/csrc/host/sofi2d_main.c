@@ -48,7 +48,6 @@ typedef struct driver {
     sofi_dev *h;
     int ntr, nsnap, nxs, nys, snaps_written;
     float *sec, *snapb;
-    float *calc[2];          /* FWI: the shot's vx, vy sections */
 } driver;
 
 /* one forward pass of the current shot: sources, time loop, seismograms and
@@ -76,8 +75,6 @@ static int forward_shot(driver *D, int ishot, double *loop_s)
         for (int c = 0; c < nc; ++c) {
             if (sofi_get_section(h, comps[c], D->sec)) return die_dev();
             if (sofi_problem_write_section(P, comps[c], D->sec)) return die_host();
-            if (p->mode == SOFI_MODE_FWI && comps[c] <= SOFI_C_VY)
-                memcpy(D->calc[comps[c]], D->sec, sizeof(float) * (size_t)D->ntr * p->ns);
         }
     }
     int ns_taken = sofi_snapshot_count(h);
@@ -96,26 +93,28 @@ static int forward_shot(driver *D, int ishot, double *loop_s)
     return 0;
 }
 
-/* residuals, adjoint pass and gradient of the shot just run forward */
+/* residuals (on the device, from the synthetic sections the forward pass
+ * left there), adjoint pass and gradient of the shot just run forward */
 static int adjoint_shot(driver *D, int ishot, float *g[3], double *loop_s)
 {
     const sofi_params *p = D->p;
     const size_t n = (size_t)D->ntr * p->nt;
-    float *data = (float *)malloc(sizeof(float) * n);
-    float *diff[2] = { (float *)calloc(n, sizeof(float)), (float *)calloc(n, sizeof(float)) };
+    float *data[2] = { NULL, NULL };
     int rc = 0;
     double l2 = 0.0;
     for (int c = SOFI_C_VX; c <= SOFI_C_VY && !rc; ++c) {
         if (!((c == SOFI_C_VX && (p->adjoint_type == 1 || p->adjoint_type == 3)) ||
               (c == SOFI_C_VY && (p->adjoint_type == 1 || p->adjoint_type == 2))))
             continue;
-        double e = 0.0;
-        rc = sofi_read_data(p, ishot, c, D->ntr, data) || sofi_residual(p, D->ntr, data, D->calc[c], diff[c], &e);
-        l2 += e;
-        if (rc) rc = die_host();
+        data[c] = (float *)malloc(sizeof(float) * (n ? n : 1));
+        if (sofi_read_data(p, ishot, c, D->ntr, data[c])) rc = die_host();
     }
     if (!rc) {
-        if (sofi_fwi_set_adjoint(D->h, D->ntr, sofi_problem_rec_i(D->P), sofi_problem_rec_j(D->P), diff[0], diff[1]) ||
+        int iv[3];
+        float fv[3];
+        sofi_problem_residual_opts(D->P, iv, fv);
+        const sofi_residual_opts o = { iv[0], iv[1], fv[0], fv[1], iv[2], fv[2] };
+        if (sofi_fwi_residual(D->h, &o, data[0], data[1], &l2) ||
             sofi_fwi_set_pass(D->h, 1) || sofi_reset(D->h) || sofi_run(D->h, 1, p->nt) ||
             sofi_fwi_gradient(D->h, g[0], g[1], g[2]) || sofi_fwi_set_pass(D->h, 0))
             rc = die_dev();
@@ -126,9 +125,8 @@ static int adjoint_shot(driver *D, int ishot, float *g[3], double *loop_s)
         printf("shot %d: adjoint time loop %.3f s, L2 %.6e\n", ishot, ms * 1e-3, l2);
         if (sofi_write_gradient(p, ishot, g[0], g[1], g[2])) rc = die_host();
     }
-    free(data);
-    free(diff[0]);
-    free(diff[1]);
+    free(data[0]);
+    free(data[1]);
     return rc;
 }
 
@@ -159,8 +157,6 @@ int main(int argc, char **argv)
     D.snapb = (float *)malloc(sizeof(float) * (size_t)D.nxs * D.nys);
     float *g[3] = { NULL, NULL, NULL };
     if (fwi) {
-        D.calc[0] = (float *)calloc(nsec, sizeof(float));
-        D.calc[1] = (float *)calloc(nsec, sizeof(float));
         for (int k = 0; k < 3; ++k) g[k] = (float *)calloc(sofi_plane(p->nxg, p->nyg), sizeof(float));
         if (sofi_fwi_set_model(h, sofi_problem_model(P, 0), sofi_problem_model(P, 1), sofi_problem_model(P, 2)))
             return die_dev();
@@ -184,8 +180,6 @@ int main(int argc, char **argv)
     printf("Total time loop %.3f s, program %.3f s\n", loop_s, now_s() - t0);
     free(D.sec);
     free(D.snapb);
-    free(D.calc[0]);
-    free(D.calc[1]);
     for (int k = 0; k < 3; ++k) free(g[k]);
     sofi_close(h);
     sofi_problem_free(P);

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include "sofi_host_internal.h"
+#include "../common/sofi_iir.h"
 
 /* Traces 1..ntr of DATA_DIR_<vx|vy|p>.su.shot<ishot>: each a 240-byte SU
  * header (skipped) and NS float32 samples, into section[ntr][ns]. */
@@ -46,9 +47,7 @@ int sofi_read_data(const sofi_params *p, int ishot, int comp, int ntr, float *se
  * + g2 y[k-2] over x[0..n-1] in place, zero initial state: the recursion of
  * libseife's rekfl, which timedomain_filt uses (src/timedomain_filt.c:62-99,
  * contrib/Seitosh/libseife/cseife_rekfl.c). */
-typedef struct { double f0, f1, f2, g1, g2; } iir2;
-
-static void iir_run(double *x, int n, iir2 c)
+static void iir_run(double *x, int n, sofi_iir2 c)
 {
     double x1 = 0., x2 = 0., y1 = 0., y2 = 0.;
     for (int k = 0; k < n; ++k) {
@@ -62,52 +61,17 @@ static void iir_run(double *x, int n, iir2 c)
     }
 }
 
-/* Bilinear-transform sections of period t0 (in samples): first order
- * (h < 0) or second order with damping h; low- or high-pass (the
- * coefficients of libseife's rfk for its lp1 / hp1 / lp2 / hp2 types) */
-static iir2 iir_section(double t0, double h, int highpass)
-{
-    iir2 c = { 0., 0., 0., 0., 0. };
-    const double eps = (8. * atan(1.)) / t0;
-    if (h < 0) {
-        c.g1 = (2. - eps) / (2. + eps);
-        if (highpass) {
-            c.f0 = 2. / (2. + eps);
-            c.f1 = -(c.f0);
-        } else {
-            c.f0 = eps / (2. + eps);
-            c.f1 = c.f0;
-        }
-        return c;
-    }
-    const double e2 = eps * eps;
-    const double a = 1. - eps * h + e2 / 4., b = -2. + e2 / 2., cc = 1. + eps * h + e2 / 4.;
-    c.g1 = -b / cc;
-    c.g2 = -a / cc;
-    if (highpass) {
-        c.f0 = 1. / cc;
-        c.f1 = -(c.f0) - c.f0;
-    } else {
-        c.f0 = e2 / 4. / cc;
-        c.f1 = c.f0 + c.f0;
-    }
-    c.f2 = c.f0;
-    return c;
-}
-
-/* Butterworth of order m as libseife cascades it (seife_butterworth,
- * cseife.c:160-190): a high-pass first removes x[0] from every sample; an
- * odd order starts with a first-order section; then m/2 second-order
- * sections with damping sin(pi/2 (2j - 1) / m). */
+/* Butterworth of order m (sofi_iir.h): a high-pass first removes x[0]
+ * from every sample, then the sections run one after the other. */
 static void butterworth(double *x, int n, double dt, double period, int m, int highpass)
 {
     if (highpass) {
         const double x0 = x[0];
         for (int k = 0; k < n; ++k) x[k] -= x0;
     }
-    if (m % 2) iir_run(x, n, iir_section(period / dt, -1., highpass));
-    const double pih = 2. * atan(1.);
-    for (int j = 1; j <= m / 2; ++j) iir_run(x, n, iir_section(period / dt, sin(pih * (2 * j - 1) / m), highpass));
+    sofi_iir2 sec[SOFI_IIR_MAXSEC];
+    const int ns = sofi_iir_butterworth(dt, period, m, highpass, sec);
+    for (int k = 0; k < ns; ++k) iir_run(x, n, sec[k]);
 }
 
 /* One pass of timedomain_filt over a trace: samples 1..NS at x[1..NS] with

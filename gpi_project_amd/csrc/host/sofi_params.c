@@ -504,7 +504,9 @@ int sofi_check_supported(const sofi_params *p)
                              "(src/surface_elastic.c:80, src/initmem_model.c:32-39); refused.");
         if (p->source_type == 5)
             return sofi_fail("SOURCE_TYPE=5 with AC_ISO: eqsource writes the unallocated sxy (src/eqsource.c); refused.");
-        if (p->mode == SOFI_MODE_FWI) return sofi_fail("MODE=FWI: only WEQ=EL_ISO is supported by this build.");
+        if (p->mode == SOFI_MODE_FWI)
+            return sofi_fail("MODE=FWI with AC_ISO: calc_conv reads the unallocated psxy (src/calc_conv.c:58-60, "
+                             "src/initmem_wavefield.c:31-33); refused.");
     }
     if (p->fdorder_time == 4) {
         /* the reference's 4th-order-in-time loop exists for the isotropic
@@ -527,9 +529,9 @@ int sofi_check_supported(const sofi_params *p)
         return sofi_fail("SNAP_FORMAT=%d is not supported (the reference supports 2 and 3, src/writedsk.c).", p->snap_format);
     if (p->source_shape == 7) return sofi_fail("SOURCE_SHAPE=7 (STF inversion) is not supported.");
     if (p->mode == SOFI_MODE_FWI) {
-        /* the gradient path of this build: elastic isotropic, L2 residuals of vx
-         * and/or vy, no filtering or energy preconditioning */
-        if (p->weq != SOFI_EL_ISO) return sofi_fail("MODE=FWI: only WEQ=EL_ISO is supported by this build.");
+        /* the gradient path: every elastic and viscoelastic rheology (the
+         * stress updates write ux, uy, uxy for the store, calc_conv is the
+         * same for all), the residuals of every LNORM calc_res has */
         if (p->ndt != 1)
             return sofi_fail("MODE=FWI needs NDT=1: the adjoint sources are read at every time step (src/update_v_interior.c:79-86).");
         if (p->adjoint_type < 1 || p->adjoint_type > 3)
@@ -538,6 +540,8 @@ int sofi_check_supported(const sofi_params *p)
         if (p->lnorm < 1 || p->lnorm > 8 || p->lnorm == 6)
             return sofi_fail("MODE=FWI: LNORM=%d has no residual in the reference (src/calc_res.c: 1-5, 7, 8).", p->lnorm);
         if (p->time_filt < 0 || p->time_filt > 2) return sofi_fail("MODE=FWI: TIME_FILT=%d (0, 1 or 2).", p->time_filt);
+        if (p->time_filt && (p->filt_order < 1 || p->filt_order > 17))
+            return sofi_fail("MODE=FWI: ORDER=%d of the time filter is not supported (1..17).", p->filt_order);
         /* trace killing from a file: calc_res skips killed traces and leaves
          * their residual buffer as the previous shot left it
          * (src/calc_res.c:62-125,196-197); not built */

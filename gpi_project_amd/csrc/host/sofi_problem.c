@@ -102,6 +102,22 @@ const int *sofi_problem_rec_i(const sofi_problem *P) { return P->rec_i; }
 const int *sofi_problem_rec_j(const sofi_problem *P) { return P->rec_j; }
 int sofi_problem_itermax(const sofi_problem *P) { return P->p.itermax; }
 
+/* the residual options calc_res and timedomain_filt read (LNORM, TIME_FILT
+ * and its corners and ORDER, WATERLEVEL_LNORM8; src/calc_res.c,
+ * src/timedomain_filt.c), after the workflow's stage 1 applied them:
+ * ivals = {lnorm, time_filt, order}, fvals = {f_low_pass, f_high_pass,
+ * waterlevel_lnorm8} */
+int sofi_problem_residual_opts(const sofi_problem *P, int *ivals, float *fvals)
+{
+    ivals[0] = P->p.lnorm;
+    ivals[1] = P->p.time_filt;
+    ivals[2] = P->p.filt_order;
+    fvals[0] = P->p.f_low_pass;
+    fvals[1] = P->p.f_high_pass;
+    fvals[2] = P->p.waterlevel_lnorm8;
+    return 0;
+}
+
 int sofi_problem_nsnap(const sofi_problem *P) { return P->p.snap ? sofi_count_snapshots(&P->p) : 0; }
 
 int sofi_problem_write_section(const sofi_problem *P, int comp, const float *section)

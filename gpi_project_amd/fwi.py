@@ -6,8 +6,11 @@ src/init_inv.c:20-25); per shot:
   forward pass  time loop with the shot's sources, storing the forward
                 fields every DTINV steps; the vx/vy seismograms are written
                 as in MODE=FW (saveseis)
-  residuals     measured data DATA_DIR_<vx|vy>.su.shot<n> (inseis), L2
-                adjoint sources, time-reversed (calc_res)
+  residuals     measured data DATA_DIR_<vx|vy>.su.shot<n> (inseis), the
+                adjoint sources of LNORM, time-reversed (calc_res), formed on
+                the device from the resident synthetic sections
+                (sofi_fwi_residual; host_residual=True forms them on the
+                host from sections read back, as the reference's loop does)
   adjoint pass  the residuals injected at the receivers (ADJOINT_TYPE),
                 zero-lag correlation with the stored forward fields
   gradient      vp / vs / rho' of the shot, written as
@@ -26,8 +29,11 @@ from .sofi import Problem, Simulation
 VX, VY = 0, 1
 
 
-def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True) -> dict:
-    """Forward pass, residuals, adjoint pass and gradient of one shot."""
+def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True, sections: bool = True,
+                  host_residual: bool = False) -> dict:
+    """Forward pass, residuals, adjoint pass and gradient of one shot.
+    sections: also return the synthetic sections (read back from the
+    device; they are read back anyway when written or for host residuals)."""
     d = prob.desc
     if ishot != prob.ishot:
         prob.set_shot(ishot)
@@ -35,22 +41,29 @@ def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True
     sim.fwi_set_pass(0)
     sim.load_shot(nsnap=prob.nsnap if prob.snapcheck else 0)
     ms_fwd = sim.run()
-    secs = {c: sim.section(c) for c in prob.seismo_comps()}
+    secs = {}
+    if write or sections or host_residual:
+        secs = {c: sim.section(c) for c in prob.seismo_comps()}
     if write:
         for c, a in secs.items():
             prob.write_section(c, a)
     ri, rj = prob.receivers()
     ntr = len(ri)
-    zero = np.zeros((ntr, d.ns), dtype=np.float32)
-    diff_x, diff_y, l2 = zero, zero, 0.0
-    if d.adjoint_type in (1, 3):
-        diff_x, e = prob.residual(prob.read_data(VX, ntr), secs[VX])
-        l2 += e
-    if d.adjoint_type in (1, 2):
-        diff_y, e = prob.residual(prob.read_data(VY, ntr), secs[VY])
-        l2 += e
-    # NS == NT (NDT = 1): the adjoint signal of step nt is sample nt
-    sim.fwi_set_adjoint(ri, rj, diff_x, diff_y)
+    obs_x = prob.read_data(VX, ntr) if d.adjoint_type in (1, 3) else None
+    obs_y = prob.read_data(VY, ntr) if d.adjoint_type in (1, 2) else None
+    if host_residual:
+        zero = np.zeros((ntr, d.ns), dtype=np.float32)
+        diff_x, diff_y, l2 = zero, zero, 0.0
+        if obs_x is not None:
+            diff_x, e = prob.residual(obs_x, secs[VX])
+            l2 += e
+        if obs_y is not None:
+            diff_y, e = prob.residual(obs_y, secs[VY])
+            l2 += e
+        # NS == NT (NDT = 1): the adjoint signal of step nt is sample nt
+        sim.fwi_set_adjoint(ri, rj, diff_x, diff_y)
+    else:
+        l2 = sim.fwi_residual(obs_x, obs_y)
     sim.fwi_set_pass(1)
     sim.reset()
     ms_adj = sim.run()
@@ -61,7 +74,8 @@ def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True
             "s": time.perf_counter() - t0}
 
 
-def gradient_json(json_path: str, verbose: bool = True, write: bool = True, workers: int = 1) -> dict:
+def gradient_json(json_path: str, verbose: bool = True, write: bool = True, workers: int = 1,
+                  sections: bool = True, host_residual: bool = False) -> dict:
     """Every iteration and shot of a MODE=FWI parameter file; returns the
     per-shot results of the last iteration and the summed gradients.
 
@@ -71,7 +85,7 @@ def gradient_json(json_path: str, verbose: bool = True, write: bool = True, work
     then summed in shot order, the order of the one-handle path, which gives
     the same sums bit for bit."""
     if workers > 1:
-        return _gradient_json_concurrent(json_path, verbose, write, workers)
+        return _gradient_json_concurrent(json_path, verbose, write, workers, sections, host_residual)
     prob = Problem(json_path)
     if prob.desc.mode != 0:
         raise ValueError(f"{json_path}: MODE is not FWI")
@@ -81,7 +95,7 @@ def gradient_json(json_path: str, verbose: bool = True, write: bool = True, work
     for it in range(1, prob.itermax + 1):
         sim.fwi_gradient_sum(reset=True)
         for ishot in range(1, prob.nshots + 1):
-            r = shot_gradient(prob, sim, ishot, write)
+            r = shot_gradient(prob, sim, ishot, write, sections, host_residual)
             shots[ishot] = r
             if verbose:
                 print(f"iteration {it} shot {ishot}: forward {r['ms_forward']:.1f} ms, adjoint {r['ms_adjoint']:.1f} ms, "
@@ -94,7 +108,8 @@ def gradient_json(json_path: str, verbose: bool = True, write: bool = True, work
     return {"shots": shots, "sum": gsum}
 
 
-def _gradient_json_concurrent(json_path: str, verbose: bool, write: bool, workers: int) -> dict:
+def _gradient_json_concurrent(json_path: str, verbose: bool, write: bool, workers: int, sections: bool,
+                              host_residual: bool) -> dict:
     import threading
     probe = Problem(json_path)
     if probe.desc.mode != 0:
@@ -119,7 +134,7 @@ def _gradient_json_concurrent(json_path: str, verbose: bool, write: bool, worker
                         if not queue or errors:
                             return
                         ishot = queue.pop(0)
-                    r = shot_gradient(prob, sim, ishot, write)
+                    r = shot_gradient(prob, sim, ishot, write, sections, host_residual)
                     with lock:
                         shots[ishot] = r
                     if verbose:

@@ -165,6 +165,15 @@ class Problem:
                                         _as_float_ptr(c), _as_float_ptr(out), C.byref(l2)))
         return out, l2.value
 
+    def residual_opts(self) -> N.ResidualOpts:
+        """LNORM, TIME_FILT (corners, ORDER) and WATERLEVEL_LNORM8 as the
+        device residual takes them."""
+        iv = (C.c_int * 3)()
+        fv = (C.c_float * 3)()
+        _chk_host(self._L.sofi_problem_residual_opts(self._p, iv, fv))
+        return N.ResidualOpts(lnorm=iv[0], time_filt=iv[1], f_low_pass=fv[0], f_high_pass=fv[1], order=iv[2],
+                              waterlevel_lnorm8=fv[2])
+
     def write_gradient(self, ishot: int, gvp: np.ndarray, gvs: np.ndarray, grho: np.ndarray):
         a = [np.ascontiguousarray(x, dtype=np.float32).ravel() for x in (gvp, gvs, grho)]
         _chk_host(self._L.sofi_write_gradient(self._L.sofi_problem_params(self._p), ishot, *[_as_float_ptr(x) for x in a]))
@@ -248,6 +257,28 @@ class Simulation:
         assert sx.shape == sy.shape == (len(ri), self.desc.nt), (sx.shape, len(ri), self.desc.nt)
         _chk_hip(self._L.sofi_fwi_set_adjoint(self._h, len(ri), _as_int_ptr(ri), _as_int_ptr(rj), _as_float_ptr(sx),
                                               _as_float_ptr(sy)))
+
+    def fwi_residual(self, obs_x, obs_y, opts: N.ResidualOpts | None = None) -> float:
+        """calc_res on the device: the adjoint sources at the receivers from
+        the resident synthetic sections of the last forward pass and the
+        measured sections ([ntr][ns], None for a component ADJOINT_TYPE does
+        not use); returns the misfit."""
+        opts = opts or self.problem.residual_opts()
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (obs_x, obs_y)]
+        for a in keep:
+            assert a is None or a.shape == (self.ntr, self.desc.ns), (None if a is None else a.shape)
+        ptr = [C.cast(None, C.POINTER(C.c_float)) if a is None else _as_float_ptr(a) for a in keep]
+        mis = C.c_double()
+        _chk_hip(self._L.sofi_fwi_residual(self._h, C.byref(opts), ptr[0], ptr[1], C.byref(mis)))
+        return mis.value
+
+    def fwi_adjoint(self, comp: int) -> np.ndarray:
+        """The adjoint sources the next adjoint pass injects, [n][NT]."""
+        n = int(self._L.sofi_fwi_adjoint_count(self._h))
+        out = np.zeros((n, self.desc.nt), dtype=np.float32)
+        if out.size:
+            _chk_hip(self._L.sofi_fwi_get_adjoint(self._h, comp, _as_float_ptr(out)))
+        return out
 
     def fwi_gradient(self):
         """The shot's (vp, vs, rho') gradient planes; adds them to the sums."""

@@ -164,6 +164,36 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *i, const int *j, const f
 int sofi_fwi_gradient(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho);
 int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho, int reset);
 
+/* The residuals of calc_res on the device (src/calc_res.c:127-301, every
+ * LNORM it has: 1-5, 7, 8), from the shot's synthetic vx / vy sections as the
+ * forward pass left them in device memory and the measured sections obs_x /
+ * obs_y ([ntr][ns] host arrays in receiver order; NULL for a component that
+ * ADJOINT_TYPE does not use), time-reversed into the adjoint sources at the
+ * handle's receivers: what inversion.c hands the adjoint pass
+ * (src/inversion.c:86-110).  With time_filt set both sections are first
+ * filtered as timedomain_filt does (src/timedomain_filt.c:28-113: zero-phase
+ * Butterworth low-pass, then high-pass when f_high_pass > 0).  *misfit (may
+ * be NULL) receives the misfit calc_res accumulates for a test shot, the
+ * components' float sums added in double; the device sums each trace's share
+ * and then the traces in order, so it agrees with the reference's single
+ * float accumulator to float rounding.  The adjoint sources set by
+ * sofi_fwi_set_adjoint are replaced.  No counterpart as a separate call in
+ * the reference, whose calc_res runs on the host sections. */
+typedef struct sofi_residual_opts {
+    int lnorm;               /* LNORM */
+    int time_filt;           /* TIME_FILT: 0 none, else filter both sections first */
+    float f_low_pass;        /* low-pass corner [Hz] (F_LOW_PASS_START or FREQ_FILE's frequency) */
+    float f_high_pass;       /* high-pass corner [Hz], 0: none */
+    int order;               /* ORDER of the Butterworth filter (1..17) */
+    float waterlevel_lnorm8; /* WATERLEVEL_LNORM8 */
+} sofi_residual_opts;
+int sofi_fwi_residual(sofi_dev *h, const sofi_residual_opts *opts, const float *obs_x, const float *obs_y,
+                      double *misfit);
+/* The adjoint sources the next adjoint pass injects: component comp (0 vx,
+ * 1 vy) as [n][NT] floats, n = sofi_fwi_adjoint_count. */
+int sofi_fwi_get_adjoint(sofi_dev *h, int comp, float *out);
+int sofi_fwi_adjoint_count(const sofi_dev *h);
+
 /* Shapes of the isotropic/VTI velocity and stress update kernels.  No counterpart in the
  * reference (a launch-tuning knob): SOFI_LAUNCH_CELL runs one thread per
  * cell in 64 x 4 tiles, SOFI_LAUNCH_VEC2 / SOFI_LAUNCH_VEC4 one thread per

@@ -148,6 +148,9 @@ const int *sofi_problem_rec_i(const sofi_problem *P);
 const int *sofi_problem_rec_j(const sofi_problem *P);
 int sofi_problem_nsnap(const sofi_problem *P);
 int sofi_problem_itermax(const sofi_problem *P);     /* FWI iterations (ITERMAX) */
+/* FWI residual options: ivals {LNORM, TIME_FILT, ORDER}, fvals {F_LOW_PASS,
+ * F_HIGH_PASS, WATERLEVEL_LNORM8} (the values calc_res / timedomain_filt use) */
+int sofi_problem_residual_opts(const sofi_problem *P, int *ivals, float *fvals);
 int sofi_problem_write_section(const sofi_problem *P, int comp, const float *section);
 int sofi_problem_write_signal(const sofi_problem *P);
 int sofi_problem_write_snapshot(const sofi_problem *P, int comp, int first, const float *data, int n);

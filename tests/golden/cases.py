@@ -365,6 +365,11 @@ FWI_NX, FWI_NY, FWI_DH, FWI_DT, FWI_NT = 160, 100, 5.0, 4.0e-4, 700
 # The normalised correlation (5, 7) needs traces well above 1e-10: there the
 # cube of a trace norm underflows in float and the reference's adjoint pass
 # turns unstable, so its cases scale the sources by 1e6.
+# FWI on the other rheologies (VELOCITY 1, ADJOINT_TYPE 1): name -> WEQ.  The
+# models are the EL_ISO ones plus Q (L = 2) and seeded smooth Thomsen / tilt
+# fields shared by the true and the initial model.
+FWI_WEQ = {"fwi_vel_l2": "VEL_ISO", "fwi_elvti": "EL_VTI", "fwi_eltti": "EL_TTI", "fwi_velvti_l2": "VEL_VTI",
+           "fwi_veltti_l2": "VEL_TTI"}
 FWI_LNORMS = {1: ("fwi_el_v0", 1.0), 3: ("fwi_el_v0", 1.0), 4: ("fwi_el_v1_y", 1.0), 5: ("fwi_el_v1_y", 1e6),
               7: ("fwi_el_v0", 1e6), 8: ("fwi_el_v1_y", 1.0)}
 
@@ -383,6 +388,23 @@ def _fwi_models(nx=FWI_NX, ny=FWI_NY, dh=FWI_DH):
     for tag, f in (("init", 1.0), ("true", bump)):
         v = (vp * f).astype(np.float32)
         out[tag] = {"vp": v, "vs": (v / 1.73).astype(np.float32), "rho": (310.0 * (vp ** 0.25)).astype(np.float32)}
+    return out
+
+
+def _fwi_extra_fields(weq: str, L: int, nx=FWI_NX, ny=FWI_NY):
+    """Q files (tau 0.05) and smooth epsilon / delta / theta fields (seed 41)
+    of the non-isotropic-elastic FWI cases, the same for both models."""
+    rng = np.random.Generator(np.random.PCG64(41))
+    out = {}
+    if weq.startswith("VEL"):
+        q = np.full((nx, ny), W.q_from_tau(0.05, L), dtype=np.float32)
+        out["qp"] = q
+        out["qs"] = (q * 0.8).astype(np.float32)
+    if "VTI" in weq or "TTI" in weq:
+        out["epsilon"] = (0.2 * (0.5 + 0.5 * np.tanh(W.smooth_noise(rng, nx, ny, 6.0)))).astype(np.float32)
+        out["delta"] = (-0.05 + 0.15 * (0.5 + 0.5 * np.tanh(W.smooth_noise(rng, nx, ny, 6.0)))).astype(np.float32)
+    if "TTI" in weq:
+        out["theta"] = (40.0 * (0.5 + 0.5 * np.tanh(W.smooth_noise(rng, nx, ny, 8.0)))).astype(np.float32)
     return out
 
 
@@ -416,7 +438,9 @@ def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp
     which = "true": MODE=FW on the true model writing the measured data
     ./data/obs_{vx,vy}.su.shot<n>; "init": MODE=FWI on the background model
     reading them, gradients to ./grad/g_gradient.*."""
-    velocity, adjoint_type = FWI_CASES[name]
+    weq = FWI_WEQ.get(name, "EL_ISO")
+    velocity, adjoint_type = FWI_CASES.get(name, (1, 1))
+    L = 2 if weq.startswith("VEL") else 0
     dh = FWI_DH
     srcs = [((40 - 1) * dh + 1.0, (5 - 1) * dh + 1.0, 0.0, 20.0, amp), ((120 - 1) * dh + 1.0, (5 - 1) * dh + 1.0, 0.0, 20.0, amp)]
     recs = [((i - 1) * dh + 1.0, (4 - 1) * dh + 1.0) for i in range(20, 141, 4)]
@@ -450,11 +474,14 @@ def fwi_case(td: str, name: str, which: str, dtinv: int = 1, lnorm: int = 2, amp
                 extra["FREQ_FILE"] = "./frequencies.dat"
                 open(os.path.join(td, "frequencies.dat"), "w").write(
                     "# low-pass corner frequencies\n" + "".join(f"{f!r}\n" for f in flo))
-    c = W.Case(name=f"{name}_{which}", nx=FWI_NX, ny=FWI_NY, dh=dh, dt=FWI_DT, time=FWI_NT * FWI_DT, weq="EL_ISO",
-               fdorder=4, maxrelerror=1, abs_type=1, fw=12, free_surf=0, fpml=20.0, vppml=3000.0, source_type=1,
-               sources=srcs, receivers=recs, seismo=1, ndt=1, run_multiple_shots=1,
+    c = W.Case(name=f"{name}_{which}", nx=FWI_NX, ny=FWI_NY, dh=dh, dt=FWI_DT, time=FWI_NT * FWI_DT, weq=weq,
+               fdorder=4, maxrelerror=1, L=L, fl=(4.0, 30.0), f_ref=20.0, abs_type=1, fw=12, free_surf=0, fpml=20.0,
+               vppml=3000.0, source_type=1, sources=srcs, receivers=recs, seismo=1, ndt=1, run_multiple_shots=1,
                mode="FWI" if fwi else "FW", extra=extra)
-    js = W.write_case(td, c, _fwi_models()[which])
+    model = dict(_fwi_models()[which])
+    if weq != "EL_ISO":
+        model.update(_fwi_extra_fields(weq, L))
+    js = W.write_case(td, c, model)
     os.makedirs(os.path.join(td, "data"), exist_ok=True)
     os.makedirs(os.path.join(td, "grad"), exist_ok=True)
     if not fwi:   # the measured data are this run's seismograms

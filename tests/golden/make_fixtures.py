@@ -24,10 +24,11 @@ has the reference at /root/reference and its build in oracle/_ref).
 7. ref1/fwi_el_v0_eprecond1.npz: the summed gradients with EPRECOND = 1.
 8. ref1/fwi_tf*.npz, fwi_workflow.npz: FWI gradients with TIME_FILT 1, 2 and a workflow.
 9. ref1/acq_*.npz: wavelet shapes, plane wave, receiver array and line, ASCII formats.
+11. ref1/fwi_vel_l2.npz ... fwi_veltti_l2.npz: FWI gradients on the other rheologies.
 10. ref1/c1.npz, c3_*.npz, c2_800.npz, c5_small.npz: the benchmark workloads
     (config 1 in full; the others at the sizes given in configs()).
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq] [configs]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq] [configs] [fwi_weq]
 """
 from __future__ import annotations
 
@@ -230,7 +231,7 @@ def c1():
 
 C3_SMALL = dict(nshots=8, nx=500, ny=250, nt=600)
 C2_STEPS, C2_SNAP_DEC = 800, 8
-C5_SMALL = dict(nshots=2, nt=1000)
+C5_SMALL = dict(nshots=2, nt=2000)
 C5_SUB = 3   # gradient subsampling of the C5 fixture (x and y)
 
 
@@ -240,7 +241,7 @@ def configs():
       (RUN_MULTIPLE_SHOTS; every shot's vx / vy section);
     * c2_800: workloads.config2 at its full 4000 x 2000 size for 800 steps
       (sections, and a snapshot of vx / vy at step 800 decimated 8 x 8);
-    * c5_small: workloads.config5 at 1500 x 500 with 2 shots, NT 1000,
+    * c5_small: workloads.config5 at 1500 x 500 with 2 shots, NT 2000,
       DTINV 1 (the summed gradients, every 3rd cell in x and y)."""
     out = os.path.join(HERE, "ref1")
     from gpi_project_amd import workloads
@@ -357,6 +358,27 @@ def fwi_lnorm():
             print("ref1", name, "LNORM", ln, flush=True)
 
 
+def fwi_weq():
+    """FWI gradients of the reference build on the other rheologies
+    (tests/golden/cases.py FWI_WEQ): measured data and the summed gradients."""
+    out = os.path.join(HERE, "ref1")
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    for name in cases.FWI_WEQ:
+        if os.path.exists(os.path.join(out, f"{name}.npz")):
+            continue
+        with tempfile.TemporaryDirectory() as td:
+            run_ref(cases.fwi_case(td, name, "true"), td)
+            r = {}
+            for s in (1, 2):
+                for comp in ("vx", "vy"):
+                    r[f"data_{comp}_shot{s}"] = read_su(os.path.join(td, "data", f"obs_{comp}.su.shot{s}"))
+            run_ref(cases.fwi_case(td, name, "init"), td)
+            for p in ("vp", "vs", "rho"):
+                r[f"grad_{p}_sum"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(nx, ny)
+            np.savez_compressed(os.path.join(out, f"{name}.npz"), **r)
+            print("ref1", name, flush=True)
+
+
 def fwi_eprecond():
     """fwi_el_v0 with EPRECOND = 1 on the reference build: its gradient
     files (the energies it accumulates are never read back)."""
@@ -392,7 +414,8 @@ def fwi_tfilt():
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond",
-                            "fwi_tfilt", "acq", "configs"]
+                            "fwi_tfilt", "acq", "configs", "fwi_weq"]
     for w in what:
         {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
-         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt, "acq": acq, "configs": configs}[w]()
+         "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt, "acq": acq, "configs": configs,
+         "fwi_weq": fwi_weq}[w]()

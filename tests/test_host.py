@@ -525,7 +525,9 @@ def test_eprecond_does_not_change_the_reference_gradient():
 
 
 def test_fwi_parameter_checks():
-    """The gradient path refuses what it does not implement."""
+    """The gradient path refuses what the reference cannot run or has no
+    residual for (AC_ISO reads an unallocated sxy in calc_conv); every other
+    rheology is accepted."""
     from gpi_project_amd.sofi import Problem, SofiError
     td = tempfile.mkdtemp(prefix="sofi_fwi_")
     js = cases.fwi_case(td, "fwi_el_v0", "init")
@@ -533,9 +535,17 @@ def test_fwi_parameter_checks():
     for old, new, msg in ((('"NDT" : "1"', '"NDT" : "2"', "NDT=1")),
                           ('"LNORM" : "2"', '"LNORM" : "6"', "LNORM"),
                           ('"ADJOINT_TYPE" : "1"', '"ADJOINT_TYPE" : "4"', "ADJOINT_TYPE"),
-                          ('"WEQ" : "EL_ISO"', '"WEQ" : "EL_VTI"', "EL_ISO")):
-        open(js, "w").write(base.replace(old, new))
+                          ('"WEQ" : "EL_ISO"', '"WEQ" : "AC_ISO"', "psxy"),
+                          ('"ORDER" : "8"', '"ORDER" : "18"', "ORDER"),
+                          ('"TIME_FILT" : "0"', '"TIME_FILT" : "1"', None)):
+        txt = base.replace(old, new)
+        if msg == "ORDER":      # the order is only checked with the filter on
+            txt = txt.replace('"TIME_FILT" : "0"', '"TIME_FILT" : "1"')
+        open(js, "w").write(txt)
         with chdir(td):
+            if msg is None:     # accepted
+                Problem(os.path.basename(js)).close()
+                continue
             with pytest.raises(SofiError, match=msg):
                 Problem(os.path.basename(js))
 

@@ -158,6 +158,29 @@ def test_oracle_fwi_gradients_vs_reference(name):
         P.close()
 
 
+@pytest.mark.parametrize("name", list(cases.FWI_WEQ))
+def test_oracle_fwi_other_rheologies_vs_reference(name):
+    """MODE=FWI on VEL_ISO, EL_VTI, EL_TTI, VEL_VTI and VEL_TTI: each stress
+    update writes its increments over DT (ux, uy, uxy) for the store and
+    calc_conv is the isotropic one on the Vp0 / Vs0 / Rho0 model; the FP32
+    oracle's summed gradients are the reference build's bit for bit."""
+    import pyoracle
+    from gpi_project_amd.sofi import Problem
+    z = common.fwi_fixture(name)
+    td = tempfile.mkdtemp(prefix="sofi_fwiw_")
+    js = cases.fwi_case(td, name, "init")
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        assert P.desc.weq == ["AC_ISO", "VAC_ISO", "EL_ISO", "VEL_ISO", "EL_VTI", "VEL_VTI", "EL_TTI",
+                              "VEL_TTI"].index(cases.FWI_WEQ[name])
+        r32 = pyoracle.fwi_gradients(P, common.fwi_measured(z), "f32")
+        for k, p in enumerate(("vp", "vs", "rho")):
+            g = common.xmajor(r32["sum"][k], P.nx, P.ny).astype(np.float32)
+            assert np.abs(z[f"grad_{p}_sum"]).max() > 0
+            assert np.array_equal(g, z[f"grad_{p}_sum"]), (name, p)
+        P.close()
+
+
 @pytest.mark.parametrize("lnorm", list(cases.FWI_LNORMS))
 def test_oracle_fwi_lnorm_gradients_vs_reference(lnorm):
     """The other residuals of calc_res (LNORM 1 sign, 3 Cauchy, 4 tanh, 5 / 7
