@@ -2188,6 +2188,7 @@ struct sofi_dev {
     float *adj_sx = nullptr, *adj_sy = nullptr;
     int nadj = 0;
     bool adj_at_rec = false;               // the adjoint arrays and plan are those of slot 0's receivers
+    bool adj_custom = false;               // the adjoint sources were set by sofi_fwi_set_adjoint
     float *rbuf = nullptr;                 // workspace of sofi_fwi_residual (measured data, filtered copies, shares)
     size_t rbuf_len = 0;
     int dev = 0;                           // the device the handle was opened on
@@ -2744,6 +2745,8 @@ namespace {
 
 // The device lists of sources and receivers of all slots (rebuilt when a
 // slot's acquisition changed).
+int adj_at_receivers(sofi_dev *h);
+
 int sync_acq(sofi_dev *h)
 {
     if (!h->src_dirty && !h->rec_dirty) return 0;
@@ -2916,6 +2919,9 @@ int sync_acq(sofi_dev *h)
         r.ns = d.ns;
         h->rec_dirty = false;
         h->gen++;
+        // an FWI handle's adjoint sources sit at its receivers unless set
+        // elsewhere: their arrays exist from the first forward pass on
+        if (d.mode == SOFI_MODE_FWI && h->nb == 1 && !h->adj_custom && adj_at_receivers(h)) return -1;
     }
     return 0;
 }
@@ -3637,11 +3643,12 @@ int adj_at_receivers(sofi_dev *h)
     dfree(h, &h->adj_sy);
     const size_t len = (size_t)(n > 0 ? n : 1) * h->d.nt;
     if (dalloc_t(h, &h->adj_sx, len) || dalloc_t(h, &h->adj_sy, len)) return -1;
-    HIPCHK(hipMemset(h->adj_sx, 0, len * sizeof(float)));
-    HIPCHK(hipMemset(h->adj_sy, 0, len * sizeof(float)));
+    HIPCHK(hipMemsetAsync(h->adj_sx, 0, len * sizeof(float), h->stream));
+    HIPCHK(hipMemsetAsync(h->adj_sy, 0, len * sizeof(float), h->stream));
     if (build_inj(h, n, sl.ri.data(), sl.rj.data(), nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
     h->adj_at_rec = true;
+    h->adj_custom = false;
     h->gen++;
     return 0;
 }
@@ -3774,6 +3781,8 @@ int sofi_fwi_get_adjoint(sofi_dev *h, int comp, float *out)
 int sofi_fwi_adjoint_count(const sofi_dev *h) { return h ? h->nadj : 0; }
 
 } // extern "C"
+
+
 
 extern "C" {
 
@@ -4055,6 +4064,7 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const
     if (build_inj(h, n, ai, aj, nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
     h->adj_at_rec = false;
+    h->adj_custom = true;
     h->gen++;
     return 0;
 }

@@ -194,6 +194,7 @@ int sofi_fwi_residual(sofi_dev *h, const sofi_residual_opts *opts, const float *
 int sofi_fwi_get_adjoint(sofi_dev *h, int comp, float *out);
 int sofi_fwi_adjoint_count(const sofi_dev *h);
 
+
 /* Shapes of the isotropic/VTI velocity and stress update kernels.  No counterpart in the
  * reference (a launch-tuning knob): SOFI_LAUNCH_CELL runs one thread per
  * cell in 64 x 4 tiles, SOFI_LAUNCH_VEC2 / SOFI_LAUNCH_VEC4 one thread per
