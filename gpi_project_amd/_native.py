@@ -32,7 +32,7 @@ class SofiDesc(C.Structure):
         ("snap_idx", C.c_int * 5), ("idx", C.c_int), ("idy", C.c_int),
         ("lsnap", C.c_int), ("isnap", C.c_int), ("esnap", C.c_int), ("mode", C.c_int),
         ("dtinv", C.c_int), ("velocity", C.c_int), ("adjoint_type", C.c_int),
-        ("fdorder_time", C.c_int),
+        ("fdorder_time", C.c_int), ("boundary", C.c_int),
     ]
 
     def copy(self) -> "SofiDesc":

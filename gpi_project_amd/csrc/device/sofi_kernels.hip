@@ -67,6 +67,7 @@ struct Grid {
     // it: tile rows at the top / bottom, tile columns at the left / right
     // (the TTI frame passes run over these tiles only)
     int ftop, fbot, fl, fr;
+    int periodic;   // BOUNDARY = 1: the grid wraps around in x
     int nb, spt;    // shot slots of a batch; slots per thread of the update kernels
     int k_lo, k_hi; // the slots this launch updates (a group of the batch)
     int nxp, nyp;   // floats per slot of the x-frame and y-frame CPML memory
@@ -441,7 +442,7 @@ __device__ __forceinline__ float surface_sxx(const Grid &g, Fields &f, const Mat
     }
     vxx *= dhi;
     vyy *= dhi;
-    if (g.abs_type == 1) {
+    if (g.abs_type == 1 && !g.periodic) {   // no x-CPML at a periodic edge (src/surface_elastic.c:64-78)
         if (i <= g.fw) {
             const int o = j * g.wx + i;
             const float ps = PRF(g, m, SOFI_P_B_X, i) * f.psi_vxx[o] + PRF(g, m, SOFI_P_A_X, i) * vxx;
@@ -1556,6 +1557,31 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m,
 }
 
 
+// ------------------------------------------------- periodic boundary in x
+//
+// exchange_v / exchange_s with BOUNDARY = 1 on one process: the process is
+// its own left and right neighbour, so the ghost columns of rows 1..NY wrap
+// around the grid (src/exchange_v.c:91-148, src/exchange_s.c:92-146,193-236).
+// The reference fills the array in the vx / sxy position with ND - 1
+// columns on the right and ND on the left, the one in the vy / sxx position
+// with ND on the right and ND - 1 on the left; the same here.  a may be null
+// (the acoustic path has no sxy).  One thread per row, blockIdx.y the slot.
+__global__ void k_wrap_x(Grid g, float *a, float *b)
+{
+    const int j = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > g.ny) return;
+    const size_t off = (size_t)(g.k_lo + blockIdx.y) * g.plane;
+    const int r0 = IDX(g, 1, j), nx = g.nx, nd = g.nd;
+    if (a) {
+        float *p = a + off + r0;                 // p[i - 1] = a[j][i]
+        for (int l = 1; l <= nd - 1; ++l) p[nx + l - 1] = p[l - 1];
+        for (int l = 1; l <= nd; ++l) p[-l] = p[nx - l];
+    }
+    float *q = b + off + r0;
+    for (int l = 1; l <= nd; ++l) q[nx + l - 1] = q[l - 1];
+    for (int l = 1; l <= nd - 1; ++l) q[-l] = q[nx - l];
+}
+
 // ------------------------------------------ batches: S slots per thread
 //
 // The update kernels of a batch in its "slot-block" shape: one thread per
@@ -1733,18 +1759,28 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat 
         const int lj = q / LX, li = q - lj * LX;
         const int ii = i0 - 1 + li, jj = j0 - 1 + lj;
         const bool in = ii >= 1 && ii <= g.nx && jj >= 1 && jj <= g.ny;
-        const int cc = IDX(g, ii, jj);
+        // periodic in x: the ghost columns 0 and NX + 1 of rows 1..NY hold
+        // the wrapped derivatives as exchange_v leaves them in the derivative
+        // arrays (src/time_loop.c:107-110): vxx and vyx (the vx position)
+        // on the left, vyy and vxy (the vy position) on the right, the other
+        // pair only when ND >= 2
+        const bool wl = g.periodic && ii == 0 && jj >= 1 && jj <= g.ny;
+        const bool wr = g.periodic && ii == g.nx + 1 && jj >= 1 && jj <= g.ny;
+        const int cc = IDX(g, ii, jj), cw = wl ? IDX(g, g.nx, jj) : wr ? IDX(g, 1, jj) : cc;
         const bool own = store_dv && in && li >= 1 && li <= BX && lj >= 1 && lj <= BY;
+        const bool ghost = store_dv && (wl || wr) && lj >= 1 && lj <= BY;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             float vxx = 0.f, vyx = 0.f, vxy = 0.f, vyy = 0.f;
             const Fields f = f0.at(k0 + s, g);
-            if (in) op_s<ND>(g, f.vx, f.vy, cc, vxx, vyx, vxy, vyy);
+            if (in || wl || wr) op_s<ND>(g, f.vx, f.vy, cw, vxx, vyx, vxy, vyy);
+            if (wl && ND < 2) vyy = vxy = 0.f;
+            if (wr && ND < 2) vxx = vyx = 0.f;
             sd[s][0][lj][li] = vxx;
             sd[s][1][lj][li] = vyy;
             sd[s][2][lj][li] = vyx;
             sd[s][3][lj][li] = vxy;
-            if (own) {
+            if (own || ghost) {
                 f.dvxx[cc] = vxx;
                 f.dvyy[cc] = vyy;
                 f.dvyx[cc] = vyx;
@@ -2424,6 +2460,7 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     g.wy = d.nx + 1;
     g.ntx = (d.nx + BX - 1) / BX;
     g.nty = (d.ny + BY - 1) / BY;
+    g.periodic = d.boundary == 1;
     frame_tiles(g);
     g.nxp = (d.ny + 1) * g.wx;
     g.nyp = g.wx * g.wy;
@@ -3096,6 +3133,15 @@ void launch(sofi_dev *h, void (*k)(P...), dim3 grid, dim3 block, A &&...a)
     }
 }
 
+// the periodic wrap of the ghost columns of a (vx / sxy position) and b (vy
+// / sxx position) for slots lo..hi-1 (BOUNDARY = 1)
+void wrap_x(sofi_dev *h, Grid gg, int lo, int hi, float *a, float *b)
+{
+    if (!gg.periodic || hi <= lo) return;
+    gg.k_lo = lo;
+    launch(h, k_wrap_x, dim3((unsigned)((gg.ny + 255) / 256), (unsigned)(hi - lo)), dim3(256), gg, a, b);
+}
+
 // the TTI CPML frame of slots lo..hi-1, region by region over the frame
 // tiles: CPML of the sides, their update, CPML of the corners, their update
 // (src/update_s_elastic_TTI_PML.c:32-145)
@@ -3256,6 +3302,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
             // the rest of the step sees are the ones just written
             const dim3 t4((g.nx + BX - 1) / BX * nty);
             launch(h, k_velocity4<ND>, t4, blk, gg, f, m, h->s, fv, nt, h->h4);
+            wrap_x(h, gg, lo, gg.k_hi, f.vx, f.vy);
             rot4(h->h4.sx);
             rot4(h->h4.sy);
             if (te) {
@@ -3288,6 +3335,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         if (bm) {
             const int S = h->bs;
             launch_blocks<ND, VelB<ND>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb);
+            wrap_x(h, gg, lo, gg.k_hi, f.vx, f.vy);
             if (te && lo == lo0 && grp == span) {
                 (void)hipEventRecord(te[1], st);
                 (void)hipEventRecord(te[2], st);
@@ -3325,6 +3373,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         else if (one && tti) launch(h, k_velocity<ND, true, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (one) launch(h, k_velocity<ND, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else launch(h, k_velocity<ND, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        wrap_x(h, gg, lo, gg.k_hi, f.vx, f.vy);
         vw = vec_width(g, h->shape_s);
         tiles = dim3((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
         // vx / vy sampling in extra blocks of a one-slot stress launch
@@ -3404,6 +3453,9 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         if (g.L > 0) launch(h, k_surface<ND, true>, sg, sb, gs, f, m);
         else launch(h, k_surface<ND, false>, sg, sb, gs, f, m);
     }
+    // the stresses' ghost columns wrap once the step's stresses are final
+    // (exchange_s after the sources and the free surface, src/time_loop.c:273)
+    wrap_x(h, g, lo0, hi0, d.weq == SOFI_AC_ISO ? nullptr : f.sxy, f.sxx);
     if (sample && d.seismo != 3 && !seis_fused) {
         dim3 rb(64), rg((rr.n + 63) / 64);
         launch(h, k_seis, rg, rb, g, f, rr, ins, d.seismo, ntb, ndt);

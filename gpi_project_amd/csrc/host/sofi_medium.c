@@ -117,6 +117,17 @@ static void halo_periodic(float *a, int nx, int ny, int pitch_)
     }
 }
 
+/* matcopy_elastic's left / right exchange with BOUNDARY = 1 on one
+ * process: columns 0 and NX + 1 of rows 0..NY + 1 wrap around
+ * (src/matcopy_elastic.c:84-119); the zero rows 0 and NY + 1 stay zero */
+static void halo_periodic_x(float *a, int nx, int ny, int pitch_)
+{
+    for (int j = 0; j <= ny + 1; j++) {
+        AT(a, nx + 1, j) = AT(a, 1, j);
+        AT(a, 0, j) = AT(a, nx, j);
+    }
+}
+
 static void dt_mult(float *a, float dt, int nx, int ny, int pitch_)
 {
     for (int i = 1; i <= nx; i++)
@@ -187,6 +198,10 @@ static int build_ac_iso(const sofi_params *p, sofi_model *m, float **para)
         }
     m->vsmin = 0.0f;
     m->vsmax = 0.0f;
+    if (p->boundary == 1) {
+        halo_periodic_x(rho, m->nx, m->ny, pitch_);
+        halo_periodic_x(ppi, m->nx, m->ny, pitch_);
+    }
     av_rho(rho, NEW(m, SOFI_A_RIP, 1), NEW(m, SOFI_A_RJP, 1), m->nx, m->ny, pitch_);
     float *g = NEW(m, SOFI_A_G, 1);
     for (int j = 1; j <= m->ny; j++)
@@ -209,7 +224,12 @@ static int build_el_iso(const sofi_params *p, sofi_model *m, float **para)
             AT(rho, i, j) = r;
             AT(ppi, i, j) = vp * vp * r;
         }
-    /* prepmod EL_ISO */
+    /* prepmod EL_ISO (matcopy_elastic: the x halo wraps with BOUNDARY = 1) */
+    if (p->boundary == 1) {
+        halo_periodic_x(rho, m->nx, m->ny, pitch_);
+        halo_periodic_x(ppi, m->nx, m->ny, pitch_);
+        halo_periodic_x(pu, m->nx, m->ny, pitch_);
+    }
     float *puipjp = NEW(m, SOFI_A_PUIPJP, 1);
     av_mue(pu, puipjp, m->nx, m->ny, pitch_);
     av_rho(rho, NEW(m, SOFI_A_RIP, 1), NEW(m, SOFI_A_RJP, 1), m->nx, m->ny, pitch_);
@@ -432,8 +452,16 @@ static int build_aniso(const sofi_params *p, sofi_model *m, float **para)
                 }
             }
         }
-    /* prepmod: averages with a zero halo (matcopy_elastic at the global edge) */
+    /* prepmod: averages with a zero halo (matcopy_elastic at the global edge;
+     * with BOUNDARY = 1 its x halo wraps: rho and c55, c15 / c35 (TTI), the
+     * tau arrays that are averaged (visco), src/prepmod.c:56-95) */
     int nx = m->nx, ny = m->ny;
+    if (p->boundary == 1) {
+        float *wrap[] = { rho, c55, visco ? NULL : c11, tti ? c15 : NULL, tti ? c35 : NULL, visco ? t55 : NULL,
+                          visco && tti ? t15 : NULL, visco && tti ? t35 : NULL };
+        for (size_t k = 0; k < sizeof(wrap) / sizeof(wrap[0]); ++k)
+            if (wrap[k]) halo_periodic_x(wrap[k], nx, ny, pitch_);
+    }
     float *c55ipjp = NEW(m, SOFI_A_PUIPJP, 1);
     av_mue(c55, c55ipjp, nx, ny, pitch_);
     float *c15ipjp = NULL, *c35ipjp = NULL;
@@ -599,15 +627,18 @@ static void build_absorb(const sofi_params *p, sofi_model *m)
     for (int i = 1; i <= fw; i++) coeff[i] = (float)exp(-(a * a * (fw - i) * (fw - i)));
     for (int j = 1; j <= ny; j++)
         for (int i = 1; i <= nx; i++) AT(ac, i, j) = 1.0f;
+    /* periodic in x (BOUNDARY = 1): no left / right frame, and the top and
+     * bottom frames run over the whole width (src/absorb.c:61-112) */
+    const int per = p->boundary == 1;
     int yb = 1, ye = ny;
-    for (int i = 1; i <= fw; i++) {
+    for (int i = 1; i <= fw && !per; i++) {
         if (!p->free_surf) yb = i;
         ye = ny - i + 1;
         for (int j = yb; j <= ye; j++) AT(ac, i, j) = coeff[i];
     }
     yb = 1;
     ye = ny;
-    for (int i = 1; i <= fw; i++) {
+    for (int i = 1; i <= fw && !per; i++) {
         int ii = nx - i + 1;
         if (!p->free_surf) yb = i;
         ye = ny - i + 1;
@@ -615,12 +646,12 @@ static void build_absorb(const sofi_params *p, sofi_model *m)
     }
     if (!p->free_surf) {
         for (int j = 1; j <= fw; j++) {
-            int xb = j, xe = nx - j + 1;
+            int xb = per ? 1 : j, xe = per ? nx : nx - j + 1;
             for (int i = xb; i <= xe; i++) AT(ac, i, j) = coeff[j];
         }
     }
     for (int j = 1; j <= fw; j++) {
-        int jj = ny - j + 1, xb = j, xe = nx - j + 1;
+        int jj = ny - j + 1, xb = per ? 1 : j, xe = per ? nx : nx - j + 1;
         for (int i = xb; i <= xe; i++) AT(ac, i, jj) = coeff[j];
     }
     free(coeff);

@@ -521,7 +521,6 @@ int sofi_check_supported(const sofi_params *p)
         return sofi_fail("Error in selection of FD coefficients: wrong FDORDER!");
     if (p->maxrelerror < 0 || p->maxrelerror > 4)
         return sofi_fail("Error in selection of FD coefficients: wrong choice of maximum relative error!");
-    if (p->boundary) return sofi_fail("BOUNDARY=1 (periodic) is not supported by this build.");
     if (p->readmod != 1) return sofi_fail("READMOD=0 (internal hh_* models) is not supported by this build.");
     if (p->free_surf && (p->weq == SOFI_VEL_VTI || p->weq == SOFI_VEL_TTI))
         return sofi_fail("FREE_SURF with VEL_VTI/VEL_TTI dereferences unallocated tau arrays in the reference (src/surface.c:94-95); refused.");

@@ -154,6 +154,16 @@ void sofi_fill_desc(const sofi_params *p, const sofi_shot *s, sofi_desc *d)
     d->gx[1] = 1; d->gx[2] = p->fw; d->gx[3] = p->nxg - p->fw; d->gx[4] = p->nxg;
     d->gy[1] = 1; d->gy[2] = p->fw; d->gy[3] = p->nyg - p->fw; d->gy[4] = p->nyg;
     if (p->free_surf == 1) d->gy[2] = 0;
+    /* periodic in x: no left / right frame (src/subgrid_bounds.c:265-270 on
+     * one process: GX[2] = 0, GX[3] = NX).  The reference also sets GX[4] =
+     * NX - 1, which bounds only the loops over the right frame
+     * (GX[3] + 1 .. GX[4]), empty either way; GX[4] stays NX here so that
+     * loops over the whole grid keep the last column. */
+    if (p->boundary == 1) {
+        d->gx[2] = 0;
+        d->gx[3] = p->nxg;
+    }
+    d->boundary = p->boundary == 1;
     d->source_type = s ? s->source_type : p->source_type;
     d->seismo = p->seismo;
     d->ndt = p->ndt;

@@ -91,6 +91,7 @@ typedef struct sofi_desc {
     int velocity;           /* VELOCITY switch of the adjoint (src/wavefield_update_v.c:37-44) */
     int adjoint_type;       /* ADJOINT_TYPE (src/update_v_interior.c:72-90) */
     int fdorder_time;       /* FDORDER_TIME: 2 (leapfrog) or 4 (Adams-Bashforth, src/time_loop.c:78-92,215-245); 0 = 2 */
+    int boundary;           /* BOUNDARY: 1 = periodic in x (exchange_v / exchange_s wrap the grid, src/exchange_v.c:94-148) */
 } sofi_desc;
 
 /*

@@ -675,7 +675,7 @@ static void free_surface(oracle *o)
         }
         vxx *= dhi;
         vyy *= dhi;
-        if (d->abs_type == 1) {
+        if (d->abs_type == 1 && !d->boundary) {   /* no x-CPML at a periodic edge (src/surface_elastic.c:64-78) */
             if (i <= d->fw) {
                 PX(o->psi_vxx, j, i) = PR(SOFI_P_B_X, i) * PX(o->psi_vxx, j, i) + PR(SOFI_P_A_X, i) * vxx;
                 vxx = vxx / PR(SOFI_P_K_X, i) + PX(o->psi_vxx, j, i);
@@ -1091,20 +1091,56 @@ static void update_s4(oracle *o)
     for (int a = 0; a < 4; ++a) ab_shift(o->dvh[a]);
 }
 
+/* ------------------------------------------- periodic boundary in x */
+
+/* exchange_v with BOUNDARY = 1 on one process (src/exchange_v.c:91-148):
+ * the process is its own left and right neighbour, so the ghost columns
+ * wrap around; the array in the vx position gets ND - 1 columns on the
+ * right and ND on the left, the one in the vy position ND on the right and
+ * ND - 1 on the left (rows 1..NY) */
+static void exch_v(oracle *o, REAL *a, REAL *b)
+{
+    const int nx = o->nx, nd = o->nd;
+    for (int j = 1; j <= o->ny; j++) {
+        for (int l = 1; l <= nd; l++) F(b, nx + l, j) = F(b, l, j);
+        for (int l = 1; l <= nd - 1; l++) F(a, nx + l, j) = F(a, l, j);
+        for (int l = 1; l <= nd - 1; l++) F(b, 1 - l, j) = F(b, nx - l + 1, j);
+        for (int l = 1; l <= nd; l++) F(a, 1 - l, j) = F(a, nx - l + 1, j);
+    }
+}
+
+/* exchange_s with BOUNDARY = 1 on one process (src/exchange_s.c:92-146,
+ * 193-236): sxy gets ND - 1 columns on the right and ND on the left, sxx ND
+ * on the right and ND - 1 on the left; the acoustic path has sxx only */
+static void exch_s(oracle *o)
+{
+    const int nx = o->nx, nd = o->nd, ac = o->d.weq == SOFI_AC_ISO;
+    for (int j = 1; j <= o->ny; j++) {
+        if (!ac) {
+            for (int l = 1; l <= nd - 1; l++) F(o->sxy, nx + l, j) = F(o->sxy, l, j);
+            for (int l = 1; l <= nd; l++) F(o->sxy, 1 - l, j) = F(o->sxy, nx - l + 1, j);
+        }
+        for (int l = 1; l <= nd; l++) F(o->sxx, nx + l, j) = F(o->sxx, l, j);
+        for (int l = 1; l <= nd - 1; l++) F(o->sxx, 1 - l, j) = F(o->sxx, nx - l + 1, j);
+    }
+}
+
 /* --------------------------------------------------------- one step */
 
 void oracle_step(oracle *o, int nt)
 {
     const sofi_desc *d = &o->d;
-    if (o->fdt4) {
-        update_v4(o, nt);
-        v_derivatives(o);
-        update_s4(o);
-    } else {
-        update_v(o, nt);
-        v_derivatives(o);
-        update_s(o);
+    const int per = d->boundary;
+    if (o->fdt4) update_v4(o, nt);
+    else update_v(o, nt);
+    if (per) exch_v(o, o->vx, o->vy);
+    v_derivatives(o);
+    if (per) {   /* the derivative arrays too (src/time_loop.c:107-110) */
+        exch_v(o, o->vxx, o->vyy);
+        if (d->weq >= SOFI_EL_ISO) exch_v(o, o->vyx, o->vxy);
     }
+    if (o->fdt4) update_s4(o);
+    else update_s(o);
     int stype = d->source_type;
     /* (ADJOINT_TYPE 4, a pressure adjoint source through psource, is refused
      * by the host library: the reference injects sectionvxdiff there while
@@ -1112,6 +1148,7 @@ void oracle_step(oracle *o, int nt)
     if (o->sw == 0 && stype == 1) psource(o, nt, o->sig);
     if (o->sw == 0 && stype == 5) eqsource(o, nt);
     if (d->free_surf) free_surface(o);
+    if (per) exch_s(o);
     if (d->seismo && nt == o->lsamp && nt < d->nt) {
         seismo(o, o->lsamp);
         o->lsamp += d->ndt;

@@ -150,6 +150,22 @@ TOPO_CASES = {
     "topo_vel_l1_fd6_abs_su": ("VEL_ISO", 6, 1, 1, 2, 0, 1, (), 4, 72),
 }
 
+# Periodic boundary in x (BOUNDARY = 1): the grid wraps around through the
+# halo exchanges, with no left / right absorbing frame; the source sits 10
+# cells from the left edge so that the wave crosses it early.  FD2 TTI shows
+# the derivative arrays' ghost pattern of ND = 1 (one column fewer on one
+# side), the free-surface case the x-CPML the surface then skips.
+PERIODIC_CASES = {
+    "per_el_fd4_cpml": ("EL_ISO", 4, 0, 0, 1, 0, 1, (), 4, 81),
+    "per_el_fd6_cpml_fs": ("EL_ISO", 6, 1, 0, 1, 1, 1, (), 4, 82),
+    "per_vel_l2_fd6_abs": ("VEL_ISO", 6, 1, 2, 2, 0, 1, (), 4, 83),
+    "per_elvti_fd8_cpml_fx": ("EL_VTI", 8, 1, 0, 1, 0, 2, (), 4, 84),
+    "per_eltti_fd2_cpml": ("EL_TTI", 2, 0, 0, 1, 0, 1, (), 4, 85),
+    "per_eltti_fd6_abs": ("EL_TTI", 6, 1, 0, 2, 0, 1, (), 4, 86),
+    "per_veltti_l1_fd4_cpml": ("VEL_TTI", 4, 0, 1, 1, 0, 1, (), 4, 87),
+    "per_ac_fd4_cpml": ("AC_ISO", 4, 0, 0, 1, 0, 1, (), 2, 88),
+}
+
 # EL_VTI/EL_TTI with a free surface: surface_elastic() reads pu (= c55) and
 # ppi (= c11 already multiplied by DT in prepmod), so the sxx correction is
 # off by ~1/DT and the run diverges; the reference aborts with "Simulation is
@@ -265,7 +281,7 @@ def decomposed_case(td: str, npx: int = 2, npy: int = 3):
 
 def case_def(name: str) -> tuple[W.Case, dict]:
     table = (CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else AC_CASES if name in AC_CASES
-             else TOPO_CASES if name in TOPO_CASES else UNSTABLE)
+             else TOPO_CASES if name in TOPO_CASES else PERIODIC_CASES if name in PERIODIC_CASES else UNSTABLE)
     weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = table[name]
     model = _model(weq, seed, L=L)
     c = W.Case(name=name, nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdo,
@@ -274,6 +290,9 @@ def case_def(name: str) -> tuple[W.Case, dict]:
                source_type=st, sources=_src(NX, NY, DH, extra=extra), receivers=_rec_grid(NX, NY, DH),
                seismo=seismo, ndt=2, snap=seismo if weq == "AC_ISO" else 4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
                idx=2, idy=3, fdorder_time=4 if name in FDT4_CASES else 2)
+    if name in PERIODIC_CASES:
+        src = [(10 * DH + 1.0, (NY // 2) * DH + 1.0, 0.0, 20.0, 1.0, *extra)]
+        c = W.Case(**{**c.__dict__, "sources": src, "extra": {"BOUNDARY": 1}})
     return c, model
 
 
