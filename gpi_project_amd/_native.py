@@ -196,5 +196,5 @@ def exported_symbols():
                            "sofi_set_graph", "sofi_graph_blocks", "sofi_set_batch_resident",
                            "sofi_get_batch_resident", "sofi_set_batch_streams", "sofi_set_batch_block",
                            "sofi_get_batch_block", "sofi_fwi_residual", "sofi_fwi_get_adjoint",
-                           "sofi_fwi_adjoint_count"],
+                           "sofi_fwi_adjoint_count", "sofi_fwi_residual_noalloc"],
     }

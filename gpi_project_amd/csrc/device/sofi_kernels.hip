@@ -3834,6 +3834,91 @@ int sofi_fwi_adjoint_count(const sofi_dev *h) { return h ? h->nadj : 0; }
 
 } // extern "C"
 
+// ------------------------------------------- residuals that allocate nothing
+//
+// sofi_fwi_residual_noalloc runs sofi_fwi_residual's kernels on a workspace
+// in this library's static device array `scratch` instead of one it
+// allocates, for callers that cannot allocate device memory.  Each GPU holds
+// its own copy of the array (SCRATCH_FLOATS floats, 16 MiB) from the first
+// use until the process ends; its device address is looked up once per
+// device with hipGetSymbolAddress and used from then on (host code cannot
+// take the array's address by name).  Calls on one GPU share the array and
+// are serialised by a per-device lock held until the call's work has
+// finished on the device.
+
+constexpr size_t SCRATCH_FLOATS = size_t(4) << 20;
+constexpr int SCRATCH_MAXDEV = 64;
+static __device__ __align__(256) float scratch[SCRATCH_FLOATS];
+
+namespace {
+
+std::mutex g_scratch_mu[SCRATCH_MAXDEV];
+std::mutex g_scratch_lookup;
+float *g_scratch_ptr[SCRATCH_MAXDEV] = {};
+
+// the device address of `scratch` on device dev (current), looked up once
+int scratch_of(int dev, float **out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAXDEV) return fail("device %d: no scratch slot (at most %d devices)", dev, SCRATCH_MAXDEV);
+    std::lock_guard<std::mutex> lk(g_scratch_lookup);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// true when p is device memory (of any device) or memory the device can address
+bool device_pointer(const void *p)
+{
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+} // namespace
+
+extern "C" {
+
+int sofi_fwi_residual_noalloc(sofi_dev *h, const sofi_residual_opts *o, const float *obs_x, const float *obs_y,
+                              float *misfit)
+{
+    if (const char *e = residual_check(h, o)) return fail("sofi_fwi_residual_noalloc: %s", e);
+    const sofi_desc &d = h->d;
+    const bool ux = d.adjoint_type == 1 || d.adjoint_type == 3, uy = d.adjoint_type == 1 || d.adjoint_type == 2;
+    if ((ux && !obs_x) || (uy && !obs_y))
+        return fail("sofi_fwi_residual_noalloc: measured section missing for ADJOINT_TYPE=%d", d.adjoint_type);
+    if ((ux && !device_pointer(obs_x)) || (uy && !device_pointer(obs_y)) || (misfit && !device_pointer(misfit)))
+        return fail("sofi_fwi_residual_noalloc: the measured sections and the misfit must be device memory");
+    if (!h->adj_at_rec || h->nadj != (int)h->slots[0].ri.size())
+        return fail("sofi_fwi_residual_noalloc: the adjoint sources are not those of the receivers (set by "
+                    "sofi_fwi_set_adjoint or receivers changed): call sofi_fwi_residual once first");
+    const int ntr = h->nadj;
+    const size_t need = residual_ws_floats(ntr, d.ns);
+    if (need > SCRATCH_FLOATS)
+        return fail("sofi_fwi_residual_noalloc: %d traces of %d samples need %zu floats of scratch, more than its "
+                    "%zu; use sofi_fwi_residual", ntr, d.ns, need, SCRATCH_FLOATS);
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != h->dev) return fail("sofi_fwi_residual_noalloc: device %d is current, the handle's is %d", cur, h->dev);
+    float *ws = nullptr;
+    if (scratch_of(h->dev, &ws)) return -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu[h->dev]);
+    const float *obs[2] = { obs_x, obs_y };
+    if (residual_core(h, *o, obs, ws)) return -1;
+    if (misfit) HIPCHK(hipMemcpyAsync(misfit, ws + 3 * (size_t)ntr * d.ns + ntr, 2 * sizeof(float),
+                                      hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+} // extern "C"
 
 
 extern "C" {

@@ -194,6 +194,20 @@ int sofi_fwi_residual(sofi_dev *h, const sofi_residual_opts *opts, const float *
 int sofi_fwi_get_adjoint(sofi_dev *h, int comp, float *out);
 int sofi_fwi_adjoint_count(const sofi_dev *h);
 
+/* sofi_fwi_residual without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * measured sections obs_x / obs_y are device arrays [ntr][ns] (NULL for a
+ * component ADJOINT_TYPE does not use); misfit (may be NULL) is a device
+ * array of 2 floats that receives the vx and vy misfits.  The workspace is
+ * a static device array of 4 Mi floats (16 MiB) per GPU, held until the
+ * process ends: 3 ntr ns + ntr + 2 floats must fit, else the call fails.
+ * The adjoint arrays must be those of the handle's receivers (set up by the
+ * first forward pass, or by sofi_fwi_residual).  Calls on one GPU share the
+ * workspace and are serialised; each returns when its work is done.  The
+ * results are those of sofi_fwi_residual bit for bit. */
+int sofi_fwi_residual_noalloc(sofi_dev *h, const sofi_residual_opts *opts, const float *obs_x, const float *obs_y,
+                              float *misfit);
 
 /* Shapes of the isotropic/VTI velocity and stress update kernels.  No counterpart in the
  * reference (a launch-tuning knob): SOFI_LAUNCH_CELL runs one thread per
