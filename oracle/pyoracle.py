@@ -46,6 +46,7 @@ def lib(precision: str = "f32"):
     L.oracle_reset.argtypes = [C.c_void_p, C.c_int]
     L.oracle_fwi_gradient.argtypes = [C.c_void_p, _FP, _FP, _FP]
     L.oracle_fwi_gradient_sum.argtypes = [C.c_void_p, _DP, _DP, _DP]
+    L.oracle_fwi_set_conv_shift.argtypes = [C.c_void_p, C.c_int]
     L.oracle_residual.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, _FP, _FP, _FP, _DP]
     L.oracle_time_filter.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int, _FP]
     _libs[precision] = L
@@ -173,7 +174,7 @@ def l2_residual(data: np.ndarray, calc: np.ndarray, dt: float, velocity: int) ->
 
 
 def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, waterlevel: float = 0.0,
-                  synthetic: dict | None = None, tfilt=None):
+                  synthetic: dict | None = None, tfilt=None, conv_shift: int = 0):
     """The FWI gradient of every shot of a MODE=FWI problem on the CPU oracle;
     data[ishot] = (vx, vy) measured sections [ntr][ns].  The sums add the
     shots' float32 gradients in shot order, as the reference does.  With
@@ -196,6 +197,7 @@ def fwi_gradients(problem, data: dict, precision: str = "f32", lnorm: int = 2, w
         dx = res(0, calc_x) if d.adjoint_type in (1, 3) else zero
         dy = res(1, calc_y) if d.adjoint_type in (1, 2) else zero
         o.fwi_set_adjoint(ri, rj, dx, dy)
+        o.L.oracle_fwi_set_conv_shift(o.o, conv_shift)   # a planted schedule error, for tests only
         o.reset(1)
         o.run()
         shots[ishot] = {"gradient": o.fwi_gradient(), "sections": (calc_x, calc_y)}

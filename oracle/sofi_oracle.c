@@ -82,6 +82,7 @@ typedef struct oracle {
     int sw;                            /* 0 forward, 1 adjoint */
     REAL *ux, *uy, *uxy, *pvxp1, *pvyp1;
     int nstore, hin, hin1;
+    int conv_shift;                    /* test knob: the adjoint pass correlates with slot + conv_shift (0 = the schedule) */
     REAL *store;                       /* forward store [nstore][5][ny][nx]: pvxp1 pvyp1 sxx syy sxy (or ux uy uxy) */
     REAL *grad_lam, *grad_mu, *grad_rho;   /* gradLam_shot, gradMu_shot, gradRhos_shot */
     REAL *gsum[3];                     /* gradVp, gradVs, gradRho summed over shots */
@@ -851,6 +852,11 @@ void oracle_fwi_set_adjoint(oracle *o, int n, const int *ai, const int *aj, cons
 
 /* zero_wavefield (src/zero_wavefield.c:28-130), including the FWI fields, and
  * the per-call counters of time_loop; sw selects the pass that follows */
+/* Test knob, for showing that a wrong correlation slot is detected: the
+ * adjoint pass correlates its fields with store slot + shift (slots outside
+ * the store are skipped).  0 restores the schedule. */
+void oracle_fwi_set_conv_shift(oracle *o, int shift) { o->conv_shift = shift; }
+
 void oracle_reset(oracle *o, int sw)
 {
     REAL *planes[] = { o->vx, o->vy, o->sxx, o->syy, o->sxy, o->ux, o->uy, o->uxy, o->pvxp1, o->pvyp1 };
@@ -1163,7 +1169,10 @@ void oracle_step(oracle *o, int nt)
             o->hin1 += dtinv;
         }
         int f = d->nt - nt + 1;
-        if (o->sw == 1 && (f - 1) % dtinv == 0) fwi_conv(o, (f - 1) / dtinv);
+        if (o->sw == 1 && (f - 1) % dtinv == 0) {
+            const int slot = (f - 1) / dtinv + o->conv_shift;
+            if (slot >= 0 && slot < o->nstore) fwi_conv(o, slot);
+        }
         o->hin++;
     }
     if (d->snap && nt == o->lsnap && nt <= d->esnap) {

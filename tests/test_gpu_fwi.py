@@ -153,8 +153,10 @@ def test_fwi_time_filter_gradients_vs_reference(tf):
 def test_fwi_gradient_vs_oracle_with_sparse_store():
     """DTINV = 5: the device stores every 5th forward step and correlates at
     the matching adjoint steps; the FP32 oracle restates the same schedule.
-    Against the full-rate (DTINV = 1) reference the direction still agrees
-    (cosine >= 0.99: the correlation is sampled 25 times per wavelet period)."""
+    Against the full-rate (DTINV = 1) reference the direction agrees to
+    cosine >= 0.9995 (sampling and FP32 rounding together); a schedule
+    shifted by one slot falls below 0.99
+    (tests/test_oracle.py, test_sparse_store_schedule_is_pinned_by_the_full_rate_gradient)."""
     import pyoracle
     from gpi_project_amd import fwi
     from gpi_project_amd.sofi import Problem
@@ -170,7 +172,7 @@ def test_fwi_gradient_vs_oracle_with_sparse_store():
         go = common.xmajor(o["sum"][k], nx, ny)
         c_or, c_ref = common.cosine(g, go), common.cosine(g, z[f"grad_{p}_sum"])
         print(f"DTINV=5 {p}: cosine vs oracle {c_or:.7f}, vs DTINV=1 reference {c_ref:.5f}")
-        assert c_or >= COS_ORACLE and c_ref >= 0.99
+        assert c_or >= COS_ORACLE and c_ref >= 0.9995
 
 
 def test_fwi_cli_writes_gradient_files():
