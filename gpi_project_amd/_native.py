@@ -79,6 +79,8 @@ def host():
     L.sofi_host_last_error.restype = C.c_char_p
     L.sofi_problem_load.restype = _VP
     L.sofi_problem_load.argtypes = [C.c_char_p]
+    L.sofi_problem_load_in.restype = _VP
+    L.sofi_problem_load_in.argtypes = [C.c_char_p, C.c_char_p]
     L.sofi_problem_free.argtypes = [_VP]
     for fn in ("nshots", "nsrc", "source_type", "snapcheck", "ntr", "nsnap", "itermax"):
         f = getattr(L, "sofi_problem_" + fn)
@@ -161,6 +163,8 @@ def hip():
     L.sofi_set_batch_launch.argtypes = [_VP, C.c_int, C.c_int]
     L.sofi_get_batch_launch.argtypes = [_VP, _IP, _IP]
     L.sofi_set_batch_block.argtypes = [_VP, C.c_int]
+    L.sofi_set_batch_tuning.argtypes = [_VP, C.c_int]
+    L.sofi_get_batch_config.argtypes = [_VP, _IP]
     L.sofi_get_batch_block.argtypes = [_VP]
     L.sofi_fwi_set_model.argtypes = [_VP, _FP, _FP, _FP]
     L.sofi_fwi_set_pass.argtypes = [_VP, C.c_int]
@@ -196,5 +200,6 @@ def exported_symbols():
                            "sofi_set_graph", "sofi_graph_blocks", "sofi_set_batch_resident",
                            "sofi_get_batch_resident", "sofi_set_batch_streams", "sofi_set_batch_block",
                            "sofi_get_batch_block", "sofi_fwi_residual", "sofi_fwi_get_adjoint",
-                           "sofi_fwi_adjoint_count", "sofi_fwi_residual_noalloc"],
+                           "sofi_fwi_adjoint_count", "sofi_fwi_residual_noalloc", "sofi_set_batch_tuning",
+                           "sofi_get_batch_config"],
     }

@@ -24,6 +24,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <map>
 #include <tuple>
 #include <type_traits>
 #include <unordered_map>
@@ -2192,6 +2193,12 @@ struct sofi_dev {
     std::vector<Fuse> slot_fz;
     int shape_v = 0, shape_s = 0;          // velocity / stress kernel shapes: SOFI_LAUNCH_*
     int bs = 0;                            // a batch's slot blocks: slots per thread of k_*_b (0: slot kernels)
+    // a batch's launch configuration as the autotuner chooses it
+    struct BatchCfg {
+        int bs, res, streams, grp, spt;    // slot blocks, resident group, streams, slots per launch / thread
+    };
+    bool tune_on = true, tuned = false;    // measured on the first run unless fixed through the API
+    BatchCfg tune_cfg{};
     int spt_mode = 0, grp_mode = 0;        // a batch's slots per thread and per launch (0: tuned in the first run)
     int grp = 0;                           // slots per launch of the update kernels (0: all)
     bool spt_tuned = false;
@@ -3921,22 +3928,16 @@ int sofi_fwi_residual_noalloc(sofi_dev *h, const sofi_residual_opts *o, const fl
 } // extern "C"
 
 
-extern "C" {
+namespace {
 
-int sofi_run(sofi_dev *h, int nt_first, int nt_last)
+// steps nt_first..nt_last of every slot: a batch advances its slots in
+// resident groups (all slots together when one group holds them), each group
+// through the whole step range with the sampling state of the range's
+// start; groups go to S streams at once (the extra streams wait for the work
+// enqueued so far on the handle's stream, which waits for them at the end)
+int run_groups(sofi_dev *h, int nt_first, int nt_last)
 {
-    if (!h) return fail("sofi_run: null handle");
-    if (!h->medium_set) return fail("sofi_run: medium not set");
-    if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
-    if (sync_acq(h)) return -1;
-    h->gerr = hipSuccess;
-    HIPCHK(hipEventRecord(h->ev0, h->stream));
-    // a batch advances its slots in resident groups (all slots together when
-    // one group holds them): each group through the whole step range, with
-    // the sampling state of the range's start
     const int G = resident_slots(h), lsamp0 = h->lsamp, ngroups = (h->nb + G - 1) / G;
-    // groups on S streams at once: the extra streams wait for the work
-    // enqueued so far on the handle's stream, which waits for them at the end
     const int S = batch_streams(h, ngroups);
     if (S > 1) {
         while ((int)h->xs.size() < S - 1) {
@@ -3966,6 +3967,117 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
         HIPCHK(hipEventRecord(h->xev[q], h->xs[q]));
         HIPCHK(hipStreamWaitEvent(h->stream, h->xev[q], 0));
     }
+    return rc ? -1 : 0;
+}
+
+// ------------------------------------------------------------ autotuner
+//
+// A batch's launch configuration (slot blocks and their slots per thread,
+// or the slot-loop kernels one slot or all slots per launch; resident
+// groups; streams) is measured on the first steps of its first run and
+// kept for every later run of the handle and for every later batch of the
+// same rheology, FD order, grid, slot count and boundaries in the process.
+// Each candidate advances all slots by one step untimed (code loading, the
+// caches filled) and TUNE_STEPS timed steps; the results do not depend on
+// the configuration.
+
+struct TuneKey {
+    int weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic) <
+               std::tie(o.weq, o.nd, o.nx, o.ny, o.nb, o.abs_type, o.free_surf, o.L, o.periodic);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, sofi_dev::BatchCfg> g_tune_cache;
+constexpr int TUNE_STEPS = 4;
+
+void apply_cfg(sofi_dev *h, const sofi_dev::BatchCfg &c)
+{
+    h->bs = c.bs;
+    h->res_mode = c.res;
+    h->str_mode = c.streams;
+    h->grp_mode = c.grp;
+    h->spt_mode = c.spt;
+    h->grp = c.grp;
+    h->g.spt = c.spt > 0 ? c.spt : 1;
+    h->spt_tuned = true;
+    h->gen++;
+}
+
+int batch_tune(sofi_dev *h, int *nt, int nt_last)
+{
+    const sofi_desc &d = h->d;
+    if (h->nb == 1 || !h->tune_on || h->tuned || d.mode != SOFI_MODE_FW || h->fdt4) return 0;
+    const TuneKey key{ d.weq, h->g.nd, d.nx, d.ny, h->nb, d.abs_type, d.free_surf, h->g.L, h->g.periodic };
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            apply_cfg(h, it->second);
+            h->tuned = true;
+            h->tune_cfg = it->second;
+            return 0;
+        }
+    }
+    const int nb = h->nb;
+    std::vector<sofi_dev::BatchCfg> cand;
+    // the slot-loop kernels: one slot per launch in cache-sized resident
+    // groups on two streams; all slots per launch, up to four per thread
+    cand.push_back({ 0, 0, 2, 1, 1 });
+    cand.push_back({ 0, nb, 1, nb, nb >= 4 ? 4 : 1 });
+    if (d.weq != SOFI_AC_ISO) {
+        // slot blocks over all slots, or over cache-sized groups on two streams
+        for (int bs : { 1, 2, 4 }) {
+            if (bs > nb) continue;
+            cand.push_back({ bs, nb, 1, 0, 0 });
+            cand.push_back({ bs, 0, 2, 0, 0 });
+        }
+    }
+    const int need = (int)cand.size() * (TUNE_STEPS + 1);
+    if (nt_last - *nt + 1 < 2 * need) return 0;      // too short a run: the defaults
+    float best = 1e30f;
+    sofi_dev::BatchCfg bestc = cand[0];
+    for (const auto &c : cand) {
+        apply_cfg(h, c);
+        if (run_groups(h, *nt, *nt)) return -1;
+        ++*nt;
+        HIPCHK(hipEventRecord(h->tune0, h->stream));
+        if (run_groups(h, *nt, *nt + TUNE_STEPS - 1)) return -1;
+        *nt += TUNE_STEPS;
+        HIPCHK(hipEventRecord(h->tune1, h->stream));
+        HIPCHK(hipEventSynchronize(h->tune1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->tune0, h->tune1));
+        if (ms < best) {
+            best = ms;
+            bestc = c;
+        }
+    }
+    apply_cfg(h, bestc);
+    h->tuned = true;
+    h->tune_cfg = bestc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = bestc;
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sofi_run(sofi_dev *h, int nt_first, int nt_last)
+{
+    if (!h) return fail("sofi_run: null handle");
+    if (!h->medium_set) return fail("sofi_run: medium not set");
+    if (nt_first < 1 || nt_last > h->d.nt || nt_first > nt_last) return fail("sofi_run: bad step range %d..%d (NT=%d)", nt_first, nt_last, h->d.nt);
+    if (sync_acq(h)) return -1;
+    h->gerr = hipSuccess;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    int nt = nt_first;
+    if (batch_tune(h, &nt, nt_last)) return -1;
+    const int rc = nt <= nt_last ? run_groups(h, nt, nt_last) : 0;
     if (rc) return -1;
     if (h->gerr != hipSuccess) {
         const hipError_t e = h->gerr;
@@ -4007,6 +4119,7 @@ int sofi_set_batch_launch(sofi_dev *h, int group, int spt)
     h->spt_tuned = false;
     h->grp = group;
     h->g.spt = spt > 0 ? spt : 1;
+    if (group > 0 || spt > 0) h->tune_on = false;
     h->gen++;
     return 0;
 }
@@ -4017,6 +4130,7 @@ int sofi_set_batch_block(sofi_dev *h, int slots)
     if (slots != 0 && slots != 1 && slots != 2 && slots != 4)
         return fail("sofi_set_batch_block: slots per thread must be 0, 1, 2 or 4 (got %d)", slots);
     h->bs = slots;
+    h->tune_on = false;
     h->gen++;
     return 0;
 }
@@ -4265,6 +4379,7 @@ int sofi_set_batch_resident(sofi_dev *h, int slots)
     if (slots < 0 || slots > h->nb) return fail("sofi_set_batch_resident: %d slots (0..%d)", slots, h->nb);
     h->res_mode = slots;
     h->spt_tuned = false;
+    if (slots > 0) h->tune_on = false;
     h->gen++;
     return 0;
 }
@@ -4276,6 +4391,27 @@ int sofi_set_batch_streams(sofi_dev *h, int streams)
     if (!h) return fail("sofi_set_batch_streams: null handle");
     if (streams < 0 || streams > 8) return fail("sofi_set_batch_streams: %d streams (0..8)", streams);
     h->str_mode = streams;
+    if (streams > 0) h->tune_on = false;
+    return 0;
+}
+
+int sofi_set_batch_tuning(sofi_dev *h, int on)
+{
+    if (!h) return fail("sofi_set_batch_tuning: null handle");
+    h->tune_on = on != 0;
+    h->tuned = false;
+    return 0;
+}
+
+int sofi_get_batch_config(const sofi_dev *h, int *cfg)
+{
+    if (!h || !cfg) return fail("sofi_get_batch_config: null argument");
+    cfg[0] = h->bs;
+    cfg[1] = h->nb > 1 ? resident_slots(h) : 1;
+    cfg[2] = h->nb > 1 ? batch_streams(h, (h->nb + cfg[1] - 1) / cfg[1]) : 1;
+    cfg[3] = h->grp > 0 ? h->grp : h->nb;
+    cfg[4] = h->bs > 0 ? h->bs : h->g.spt;
+    cfg[5] = h->tuned ? 1 : 0;
     return 0;
 }
 

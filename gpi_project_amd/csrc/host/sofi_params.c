@@ -212,11 +212,39 @@ static const char *mode_names[3] = { "FWI", "FW", "RTM" };
 
 static int apply_workflow_stage1(sofi_params *p);
 
-int sofi_read_params(const char *json_file, sofi_params *p)
+/* a relative file name of the parameter file taken relative to base_dir */
+static int rebase(const char *base, char *path)
+{
+    if (!base || !base[0] || !path[0] || path[0] == '/') return 0;
+    char tmp[SOFI_STR];
+    const int n = snprintf(tmp, sizeof(tmp), "%s/%s", base, path);
+    if (n < 0 || n >= (int)sizeof(tmp)) return sofi_fail("path %s/%s is too long", base, path);
+    strcpy(path, tmp);
+    return 0;
+}
+
+int sofi_read_params(const char *json_file, sofi_params *p) { return sofi_read_params_in(json_file, NULL, p); }
+
+int sofi_read_params_in(const char *json_arg, const char *base, sofi_params *p)
 {
     objlist *o = (objlist *)calloc(1, sizeof(objlist));
     if (!o) return sofi_fail("out of memory");
     memset(p, 0, sizeof(*p));
+    if (base && strlen(base) >= SOFI_STR) {
+        free(o);
+        return sofi_fail("base directory %s is too long", base);
+    }
+    if (base) strcpy(p->base_dir, base);
+    char json_file[SOFI_STR];
+    if (strlen(json_arg) >= SOFI_STR) {
+        free(o);
+        return sofi_fail("parameter file name %s is too long", json_arg);
+    }
+    strcpy(json_file, json_arg);
+    if (rebase(p->base_dir, json_file)) {
+        free(o);
+        return -1;
+    }
     /* defaults of the GlobVar / GlobVarInv initialisers in src/sofi2D.c:86-89 */
     p->outntimestepinfo = 100;
     p->ndt = 1;
@@ -403,6 +431,7 @@ int sofi_read_params(const char *json_file, sofi_params *p)
             } else if (p->time_filt == 2) {
                 char ff[SOFI_STR];
                 REQ_STR("FREQ_FILE", ff);
+                if (rebase(p->base_dir, ff)) goto fail;
                 /* filter_frequencies (src/filter_frequencies.c:25-60): the
                  * first nonzero number opening a line; FREQ_NR = 1 */
                 FILE *fq = fopen(ff, "r");
@@ -422,8 +451,16 @@ int sofi_read_params(const char *json_file, sofi_params *p)
         OPT_INT("TIMEWIN", p->timewin, 0);
         OPT_INT("USE_WORKFLOW", p->use_workflow, 0);
         if (p->use_workflow) REQ_STR("FILE_WORKFLOW", p->file_workflow);
+        if (p->use_workflow && rebase(p->base_dir, p->file_workflow)) goto fail;
         if (p->use_workflow && apply_workflow_stage1(p)) goto fail;
         if (p->dtinv <= 0) { sofi_fail("Variable DTINV must be a positive integer!"); goto fail; }
+    }
+    /* the other file names of the parameter file */
+    {
+        char *names[] = { p->sigout_file, p->signal_file, p->source_file, p->snap_file, p->rec_file, p->seis_file,
+                          p->mfile, p->log_file, p->data_dir, p->gradient_file };
+        for (size_t k = 0; k < sizeof(names) / sizeof(names[0]); ++k)
+            if (rebase(p->base_dir, names[k])) goto fail;
     }
     /* NS = iround(NT / NDT) with integer division (src/read_par_json.c:516) */
     p->ns = p->nt / p->ndt;

@@ -23,11 +23,13 @@ typedef struct sofi_problem {
     int *rec_i, *rec_j;  /* 0-based arrays of receiver grid indices */
 } sofi_problem;
 
-sofi_problem *sofi_problem_load(const char *json)
+sofi_problem *sofi_problem_load(const char *json) { return sofi_problem_load_in(json, NULL); }
+
+sofi_problem *sofi_problem_load_in(const char *json, const char *base_dir)
 {
     sofi_problem *P = (sofi_problem *)calloc(1, sizeof(sofi_problem));
     if (!P) { sofi_fail("out of memory"); return NULL; }
-    if (sofi_read_params(json, &P->p) || sofi_check_supported(&P->p) || sofi_read_acq(&P->p, &P->a) ||
+    if (sofi_read_params_in(json, base_dir, &P->p) || sofi_check_supported(&P->p) || sofi_read_acq(&P->p, &P->a) ||
         sofi_build_model(&P->p, &P->m) || sofi_checkfd(&P->p, &P->m, &P->a)) {
         sofi_free_model(&P->m);
         sofi_free_acq(&P->a);

@@ -26,7 +26,6 @@ one after the other, src/sofi2D.c:320-395).
 """
 from __future__ import annotations
 
-import contextlib
 import ctypes as C
 import os
 import threading
@@ -58,23 +57,6 @@ class Request:
         return None if self.end is None else self.end - self.arrival
 
 
-_CWD_LOCK = threading.Lock()
-
-
-@contextlib.contextmanager
-def in_dir(directory: str):
-    """The reference's parameter files name their inputs relative to the run
-    directory; the host library opens them while this holds, so the working
-    directory is switched under a process-wide lock."""
-    with _CWD_LOCK:
-        cwd = os.getcwd()
-        os.chdir(directory)
-        try:
-            yield
-        finally:
-            os.chdir(cwd)
-
-
 def device_free_bytes() -> int:
     """Free HBM of the current device (hipMemGetInfo)."""
     hip = C.CDLL("libamdhip64.so")
@@ -85,16 +67,19 @@ def device_free_bytes() -> int:
 
 
 def open_problem(req: Request):
-    """Read the request's parameter, model and acquisition files."""
+    """Read the request's parameter, model and acquisition files.  The
+    reference's parameter files name their inputs relative to the run
+    directory: the host library resolves them against the request's
+    directory, so setups of several requests run at once without touching
+    the process's working directory."""
     from .sofi import Problem
     d = os.path.dirname(os.path.abspath(req.json_path))
-    with in_dir(d):
-        return Problem(os.path.basename(req.json_path))
+    return Problem(os.path.basename(req.json_path), base_dir=d)
 
 
 def gpu_prepare(req: Request):
     """Setup stage: the problem and its device footprint (one slot, the
-    receivers of the first shot, no snapshots)."""
+    problem's receivers, which every shot shares, no snapshots)."""
     from . import _native as N
     P = open_problem(req)
     try:
@@ -111,15 +96,13 @@ def gpu_execute(req: Request, P, should_stop: Callable[[], bool], launch: str = 
     """Run stage: every shot of the request on one handle, in blocks of
     block_steps time steps; fills req.sections[ishot][comp]."""
     from .sofi import Simulation
-    d = os.path.dirname(os.path.abspath(req.json_path))
     sim = None
     try:
         sim = Simulation(P)
         sim.set_launch(launch)
         for ishot in range(1, P.nshots + 1):
             if ishot > 1:
-                with in_dir(d):
-                    P.set_shot(ishot)
+                P.set_shot(ishot)
             sim.load_shot(nsnap=0)
             nt = P.desc.nt
             for t0 in range(1, nt + 1, block_steps):

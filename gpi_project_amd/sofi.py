@@ -10,6 +10,7 @@ snapshots.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -42,9 +43,14 @@ def _as_float_ptr(a):
 class Problem:
     """A run set up from a JSON parameter file (params, model, acquisition)."""
 
-    def __init__(self, json_path: str):
+    def __init__(self, json_path: str, base_dir: str | None = None):
+        """base_dir: the directory the parameter file's relative file names
+        (its own name included) are taken from; None: the working directory."""
         self._L = N.host()
-        p = self._L.sofi_problem_load(json_path.encode())
+        if base_dir:
+            p = self._L.sofi_problem_load_in(json_path.encode(), os.fspath(base_dir).encode())
+        else:
+            p = self._L.sofi_problem_load(json_path.encode())
         if not p:
             raise SofiError(N.host_error())
         self._p = p
@@ -402,6 +408,18 @@ class Batch(Simulation):
     @property
     def block(self) -> int:
         return int(self._L.sofi_get_batch_block(self._h))
+
+    def set_tuning(self, on: bool = True):
+        """The launch autotuner (on by default; fixing any knob turns it off)."""
+        _chk_hip(self._L.sofi_set_batch_tuning(self._h, int(on)))
+
+    @property
+    def config(self) -> dict:
+        """The launch configuration in use (chosen by the autotuner unless fixed)."""
+        c = (C.c_int * 6)()
+        _chk_hip(self._L.sofi_get_batch_config(self._h, c))
+        return {"slot_block": c[0], "resident": c[1], "streams": c[2], "slots_per_launch": c[3],
+                "slots_per_thread": c[4], "tuned": bool(c[5])}
 
     def set_streams(self, streams: int = 0):
         """Streams the resident groups run on at once (0: 2)."""

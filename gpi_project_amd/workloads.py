@@ -292,14 +292,17 @@ FWI_KEYS = {"ITERMAX": 1, "PARAMETERIZATION": 1, "INV_VP_ITER": 1, "INV_VS_ITER"
 
 
 def config5(d: str, which: str, nshots: int = 32, nx: int = 1500, ny: int = 500, nt: int = 3000,
-            dtinv: int = 5) -> str:
+            dtinv: int = 5, amp: float = 1e9) -> str:
     """Elastic FWI: 1500 x 500, DH 5 m, FD8 (Holberg 0.1 %), DT 5e-4 s,
     NT 3000, 20-point CPML all round.  Background vp 2000 -> 4000 m/s linear
     in depth, vs = vp/1.73, rho = 310 vp^0.25; the true model adds a
     Gaussian anomaly (sigma 200 m, +10 % vp and vs) centred at a point of
     seed 6 (x in the middle half, depth in the middle 40 %).  nshots
     explosive Ricker shots evenly spaced along x at 125 m depth, fc ~ U(8, 20)
-    Hz (seed 7); 300 receivers at 115 m depth.  which = "true": MODE=FW on
+    Hz (seed 7), amplitude amp; 300 receivers at 115 m depth.  The
+    gradient scales with the square of the source amplitude: at amplitude 1
+    its float terms underflow on this grid (the reference's float gradient is
+    zero but for denormals), at 1e9 they are ~1e-25.  which = "true": MODE=FW on
     the true model writing the measured data ./data/obs_{vx,vy}.su.shot<n>;
     "init": MODE=FWI (LNORM 2, VELOCITY 1, ADJOINT_TYPE 1, DTINV) on the
     background reading them."""
@@ -318,7 +321,7 @@ def config5(d: str, which: str, nshots: int = 32, nx: int = 1500, ny: int = 500,
     fcs = np.random.Generator(np.random.PCG64(7)).uniform(8.0, 20.0, nshots)
     x0, x1 = (fw + 10) * dh, (nx - fw - 11) * dh
     xs = np.linspace(x0, x1, nshots)
-    srcs = [(float(xx) + 1.0, 125.0 + 1.0, 0.0, float(f), 1.0) for xx, f in zip(xs, fcs)]
+    srcs = [(float(xx) + 1.0, 125.0 + 1.0, 0.0, float(f), float(amp)) for xx, f in zip(xs, fcs)]
     recs = [(float(xx) + 1.0, 115.0 + 1.0) for xx in np.linspace((fw + 5) * dh, (nx - fw - 6) * dh, 300).round() // dh * dh]
     fwi = which == "init"
     extra = dict(FWI_KEYS, DATA_DIR="./data/obs", ADJOINT_TYPE=1, VELOCITY=1, DTINV=dtinv) if fwi else {}

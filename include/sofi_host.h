@@ -62,6 +62,9 @@ typedef struct sofi_params {
     /* derived */
     int nd;
     float dtdh;
+    /* the directory relative paths of the parameter file are resolved
+     * against ("" : the working directory, as the reference does) */
+    char base_dir[SOFI_STR];
 } sofi_params;
 
 /* Model: every per-cell array of enum sofi_array that the rheology uses, in
@@ -97,6 +100,11 @@ typedef struct sofi_shot {
 const char *sofi_host_last_error(void);
 
 int sofi_read_params(const char *json_file, sofi_params *p);
+/* sofi_read_params with every relative file name of the parameter file (the
+ * parameter file's own included) taken relative to base_dir instead of the
+ * working directory, so that several runs can be set up at once from
+ * different directories in one process. */
+int sofi_read_params_in(const char *json_file, const char *base_dir, sofi_params *p);
 int sofi_check_supported(const sofi_params *p);
 int sofi_build_model(const sofi_params *p, sofi_model *m);
 void sofi_free_model(sofi_model *m);
@@ -124,6 +132,9 @@ int sofi_snapshot_dims(const sofi_params *p, int *nxs, int *nys);
  * getters; 0-based source/receiver arrays hold 1-based grid indices. */
 typedef struct sofi_problem sofi_problem;
 sofi_problem *sofi_problem_load(const char *json);
+/* sofi_problem_load with relative file names resolved against base_dir
+ * (sofi_read_params_in); NULL or "" behaves as sofi_problem_load */
+sofi_problem *sofi_problem_load_in(const char *json, const char *base_dir);
 void sofi_problem_free(sofi_problem *P);
 int sofi_problem_nshots(const sofi_problem *P);
 int sofi_problem_set_shot(sofi_problem *P, int ishot);

@@ -52,6 +52,7 @@ def test_batch_matches_one_shot_handles(weq, stype, fs, absn, shape):
         for grp, spt, res, nst in ((4, 1, 0, 0), (4, 3, 0, 0), (2, 2, 0, 0), (1, 1, 0, 0), (0, 0, 0, 0),
                                    (2, 1, 3, 1), (0, 0, 1, 0), (1, 1, 1, 3), (2, 2, 2, 2)):
             B = Batch(P, 4)
+            B.set_tuning(False)         # the slot-loop kernels of these knobs, bit for bit one-shot handles
             B.set_launch(shape)
             B.set_launch_groups(grp, spt)
             B.set_resident(res)
