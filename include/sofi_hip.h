@@ -156,9 +156,13 @@ size_t sofi_footprint(const sofi_dev *h);
  * sofi_footprint after the same calls, by at most 1 MiB of small buffers. */
 size_t sofi_estimate_footprint(const sofi_desc *desc, int nshots, int ntr, int nsnap);
 
-/* FWI gradient path (MODE=FWI, WEQ=EL_ISO): what time_loop does with
- * sw = 0 / 1 and MemInv (src/time_loop.c:27-326 with src/snap_store.c:27-47,
+/* FWI gradient path (MODE=FWI; EL_ISO, VEL_ISO, EL_VTI, VEL_VTI, EL_TTI,
+ * VEL_TTI; one-slot handles): what time_loop does with sw = 0 / 1 and
+ * MemInv (src/time_loop.c:27-326 with src/snap_store.c:27-47,
  * src/calc_conv.c:30-73), driven per shot as src/inversion.c:27-258 does.
+ * The viscoelastic and anisotropic stress updates write the strain rates
+ * ux / uy / uxy the store reads with VELOCITY = 1, as
+ * src/wavefield_update_s_visc.c:60-66 and its siblings do.
  *   sofi_fwi_set_model    the iteration's model Vp0, Vs0, Rho0 (padded planes)
  *   sofi_fwi_set_pass     sw = 0: the next sofi_run is the forward pass and
  *                         stores the forward fields every DTINV steps;

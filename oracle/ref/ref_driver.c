@@ -10,11 +10,22 @@
  * FFTW-only calc_envelope() (needed for LNORM=8 only), which this image
  * cannot build.  It prints the wall time of each time_loop() as
  * "REF_TIME_LOOP <shot> <seconds> <ranks>".
+ *
+ * Built with -DTIME_LOOP_MI355X (sofi2D_ref_hip) it calls
+ * time_loop_mi355x() instead of time_loop() for MODE=FW: the reference's
+ * own setup and output around the GPU time loop, as a maintainer would
+ * wire libsofi_hip.so into src/sofi2D.c:369.
  */
 #include <unistd.h>
 
 #include "fd.h"
 #include "logging.h"
+
+#ifdef TIME_LOOP_MI355X
+/* oracle/ref/time_loop_mi355x.c: the time loop on the GPU through libsofi_hip.so */
+void time_loop_mi355x(int ishot, int snapcheck, float *hc, float **srcpos_loc, float **signalx, int nsrc,
+                      AcqVar *acq, MemModel *mpm, GlobVar *gv);
+#endif
 
 static void write_sections(int ishot, AcqVar *acq, GlobVar *gv)
 {
@@ -98,8 +109,13 @@ int main(int argc, char **argv)
             subgrid_bounds(1, gv.NX, 1, gv.NY, &gv);
             MPI_Barrier(MPI_COMM_WORLD);
             double t0 = MPI_Wtime();
-            time_loop(iter, ishot, snapcheck, hc, acq.srcpos_loc, acq.signals, acq.signals, acq.nsrc_loc, 0, &acq,
-                      &mpm, &mpw, &minv, &gv, &vinv, &perf);
+#ifdef TIME_LOOP_MI355X
+            if (gv.MODE == FW)
+                time_loop_mi355x(ishot, snapcheck, hc, acq.srcpos_loc, acq.signals, acq.nsrc_loc, &acq, &mpm, &gv);
+            else
+#endif
+                time_loop(iter, ishot, snapcheck, hc, acq.srcpos_loc, acq.signals, acq.signals, acq.nsrc_loc, 0,
+                          &acq, &mpm, &mpw, &minv, &gv, &vinv, &perf);
             MPI_Barrier(MPI_COMM_WORLD);
             double t1 = MPI_Wtime();
             if (gv.MPID == 0) {
