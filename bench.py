@@ -128,17 +128,21 @@ def pmc_child(case_dir: str, launch: str, steps: int):
 
 
 def pmc_traffic(case_dir: str, launch: str, steps: int = 20):
-    """HBM bytes per launch of the velocity and stress kernels from two
+    """HBM bytes per step of the velocity and stress kernels from two
     rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
     each scaled by what the same counter reads for the calibration copy
-    (CAL_BYTES read and written).  Returns ({kernel: bytes}, factors) or
-    (None, reason)."""
+    (CAL_BYTES read and written).  "velocity" sums the k_velocity* kernels,
+    "stress" the k_stress* and k_tti_frame* kernels (the TTI stress
+    half-step is a fused interior kernel and frame passes), over the `steps`
+    steps of the one-shot run.  Returns ({kind: bytes per step}, factors,
+    {kernel: {counter: bytes per dispatch, "dispatches": n}}) or
+    (None, reason, None)."""
     import csv
     import glob
     import re
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(prof):
-        return None, "rocprofv3 not found"
+        return None, "rocprofv3 not found", None
     env = dict(os.environ, TMPDIR="/tmp")
     per = {}
     factors = {}
@@ -152,7 +156,7 @@ def pmc_traffic(case_dir: str, launch: str, steps: int = 20):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if r.returncode != 0 or not files:
             shutil.rmtree(d, ignore_errors=True)
-            return None, f"rocprofv3 --pmc {cnt} failed (rc {r.returncode}): {(r.stdout + r.stderr)[-300:]}"
+            return None, f"rocprofv3 --pmc {cnt} failed (rc {r.returncode}): {(r.stdout + r.stderr)[-300:]}", None
         vals = {}
         for row in csv.DictReader(open(files[0])):
             name = re.sub(r"\(anonymous namespace\)::", "", row["Kernel_Name"]).replace("void ", "").split("(")[0]
@@ -160,16 +164,23 @@ def pmc_traffic(case_dir: str, launch: str, steps: int = 20):
         shutil.rmtree(d, ignore_errors=True)
         cal = max(vals.get("__amd_rocclr_copyBuffer", [0.0]))
         if cal <= 0:
-            return None, f"no calibration copy in the {cnt} pass"
+            return None, f"no calibration copy in the {cnt} pass", None
         factors[cnt] = (CAL_BYTES / 1024) / cal
         for name, v in vals.items():
-            if name.startswith("k_velocity") or name.startswith("k_stress"):
-                kind = "velocity" if name.startswith("k_velocity") else "stress"
-                per.setdefault(kind, {})[cnt] = sum(v) / len(v) * factors[cnt] * 1024
+            if name.startswith("k_"):
+                per.setdefault(name, {})[cnt] = sum(v) / len(v) * factors[cnt] * 1024
+                per[name]["dispatches"] = len(v)
                 log(f"  {cnt} {name}: {sum(v) / len(v):.1f} KiB per dispatch ({len(v)} dispatches)")
         log(f"  {cnt} calibration copy of {CAL_BYTES >> 20} MiB: {cal:.1f} KiB -> factor {factors[cnt]:.4f}")
-    out = {k: v["FETCH_SIZE"] + v["WRITE_SIZE"] for k, v in per.items() if len(v) == 2}
-    return out, factors
+    out = {}
+    for name, v in per.items():
+        if "FETCH_SIZE" not in v or "WRITE_SIZE" not in v:
+            continue
+        kind = ("velocity" if name.startswith("k_velocity") else
+                "stress" if name.startswith("k_stress") or name.startswith("k_tti_frame") else None)
+        if kind:
+            out[kind] = out.get(kind, 0.0) + (v["FETCH_SIZE"] + v["WRITE_SIZE"]) * v["dispatches"] / steps
+    return out, factors, per
 
 
 def bench_batch(args, weq: str):
@@ -221,6 +232,11 @@ def bench_batch(args, weq: str):
     B.close()
     P.close()
     os.chdir(cwd)
+    traffic = None
+    if not args.no_traffic:
+        # HBM bytes per step of one shot on a one-shot handle (the first shot of the workload)
+        traffic, info, kern = pmc_traffic(td, args.launch)
+        log(f"one-shot traffic per step {traffic}, counter factors {info}, kernels {kern}")
     shutil.rmtree(td, ignore_errors=True)
     nb = args.shots
     glups_b = nb * cells * d.nt / (ms_b * 1e-3) / 1e9
@@ -246,6 +262,12 @@ def bench_batch(args, weq: str):
                      "traffic": None, "kernel": "whole step (all kernels)",
                      "algorithmic_bytes_per_shot_cell": round(bytes_b, 2)},
         "slot0_bitwise_equal_to_one_shot_run": bool(same), "slot0_rel_l2_vs_one_shot_run": err0,
+        "one_shot": {"glups": round(cells * d.nt / (ms_one * 1e-3) / 1e9, 3),
+                     "algorithmic_bytes_per_cell": 4 * (15 + 2 + p_s),
+                     "achieved_gbs": round(4 * (15 + 2 + p_s) * cells * d.nt / (ms_one * 1e-3) / 1e9, 1),
+                     "traffic_bytes_per_step": ({k: round(v) for k, v in traffic.items()} if traffic else None),
+                     "traffic_ratio": (round(sum(traffic.values()) / (4 * (15 + 2 + p_s) * cells), 3)
+                                       if traffic else None)},
     }
     print(json.dumps(out), flush=True)
 
@@ -602,8 +624,8 @@ def main():
     os.chdir(cwd)
     traffic = None
     if not args.no_traffic:
-        traffic, info = pmc_traffic(td, args.launch)
-        log(f"traffic per launch {traffic}, counter factors {info}")
+        traffic, info, _ = pmc_traffic(td, args.launch)
+        log(f"traffic per step {traffic}, counter factors {info}")
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(td, os.path.basename(js), d.nx, d.ny, args.cpu_steps, args.cpu_cores)

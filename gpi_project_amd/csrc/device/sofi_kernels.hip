@@ -1460,66 +1460,29 @@ __device__ __forceinline__ bool is_frame_tile(const Grid &g, int tx, int ty)
 }
 
 
-// frame CPML of update_s_*_tti_PML, in place on the derivative arrays.  The
-// reference treats the frame region by region (left, right, top, bottom, then
-// the four corners), CPML then update per region, so side cells read the
-// corner derivatives before the corner CPML.  CORNERS = false handles the
-// four sides, true the corners; the elastic right-bottom corner loop stops at
-// GY[4]-1 (src/update_s_elastic_TTI_PML.c:132 vs src/update_s_visc_TTI_PML.c:132)
-template <bool CORNERS>
-__global__ __launch_bounds__(BX *BY) void k_cpml_dv_frame(Grid g, Fields f0, Mat m)
-{
-    Fields f = f0.at(g.k_lo + blockIdx.y, g);   // shot slot of a batch
-    int tx, ty;
-    frame_tile(g, blockIdx.x, tx, ty);          // one of the frame tiles
-    const int i = 1 + tx * BX + threadIdx.x;
-    const int j = 1 + ty * BY + threadIdx.y;
-    if (i > g.nx || j > g.ny) return;
-    const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
-    if (CORNERS ? !(xf && yf) : (xf == yf)) return;
-    if (CORNERS && g.weq == SOFI_EL_TTI && i > g.gx3 && j > g.gy3 && j == g.ny) return;
-    const int c = IDX(g, i, j);
-    float vxx = f.dvxx[c], vyx = f.dvyx[c], vxy = f.dvxy[c], vyy = f.dvyy[c];
-    cpml_dv(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
-    f.dvxx[c] = vxx;
-    f.dvyx[c] = vyx;
-    f.dvxy[c] = vxy;
-    f.dvyy[c] = vyy;
-}
+// The TTI CPML frame (src/update_s_elastic_TTI_PML.c:32-145,
+// src/update_s_visc_TTI_PML.c).  The reference treats the frame region by
+// region (left, right, top, bottom, then the four corners), CPML then update
+// per region, so a side cell reads its neighbours' derivatives after the
+// sides' CPML and before the corners' CPML, and a corner cell reads them
+// after both.  Here k_stress_tti_f applies the CPML of every frame cell as it
+// stores the frame derivatives (the elastic right-bottom corner loop stops
+// at GY[4]-1, src/update_s_elastic_TTI_PML.c:132 vs
+// src/update_s_visc_TTI_PML.c:132, so those cells keep their raw ones), and
+// k_tti_frame then updates sides and corners in one launch over the frame
+// tiles (blockIdx.x) and slots (blockIdx.y): a side cell next to a corner
+// recomputes that corner cell's raw derivatives from vx, vy instead of
+// reading the stored ones.  Both updates only read the derivative arrays, so
+// they do not depend on each other.  The updates cover the ranges shrunk by
+// two cells at the outer edges.
 
-// TTI stress update of the CPML frame (src/wavefield_update_s_el_tti.c:27-52,
-// src/wavefield_update_s_visc_TTI.c:27-84) reading the derivative arrays
-// after the frame CPML: PASS 1 the sides, PASS 3 the corners, on the ranges
-// shrunk by two cells at the outer edges (src/update_s_elastic_TTI_PML.c:37-143).
-// Launched over the frame tiles (blockIdx.x) and slots (blockIdx.y); the
-// interior and the damping frame are updated by k_stress_tti_f.
-template <int WEQ, int PASS>
-__global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m, Fuse fz)
+// the TTI stress update of one frame cell from its derivatives and their
+// 4-point averages (src/wavefield_update_s_el_tti.c:27-52,
+// src/wavefield_update_s_visc_TTI.c:27-84)
+template <int WEQ>
+__device__ __forceinline__ void tti_frame_update(const Grid &g, const Fields &f, const Mat &m, const Fuse &fz, int c,
+                                                 float vxx, float vyy, float vxxipjp, float vyyipjp, float vij, float v)
 {
-    static_assert(PASS == 1 || PASS == 3, "frame passes only");
-    const Fields f = f0.at(g.k_lo + blockIdx.y, g);   // shot slot of a batch
-    int tx, ty;
-    frame_tile(g, blockIdx.x, tx, ty);
-    const int i = 1 + tx * BX + threadIdx.x;
-    const int j = 1 + ty * BY + threadIdx.y;
-    if (i > g.nx || j > g.ny) return;
-    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
-    if (!frame) return;
-    if (PASS == 1 && xf && yf) return;
-    if (PASS == 3 && !(xf && yf)) return;
-    if (i <= g.gx2 && i < 1 + 2) return;
-    if (i > g.gx3 && i > g.nx - 2) return;
-    if (j <= g.gy2 && j < 1 + 2) return;
-    if (j > g.gy3 && j > g.ny - 2) return;
-    const int c = IDX(g, i, j), P = g.pitch;
-    const float *X = f.dvxx, *Y = f.dvyy, *YX = f.dvyx, *XY = f.dvxy;
-    const float vxx = X[c], vyy = Y[c];
-    const float vxxipjp = 0.25f * (X[c] + X[c + P] + X[c + 1] + X[c + 1 + P]);
-    const float vyyipjp = 0.25f * (Y[c] + Y[c + P] + Y[c + 1] + Y[c + 1 + P]);
-    const float vyxij = 0.25f * (YX[c] + YX[c - P] + YX[c - 1] + YX[c - 1 - P]);
-    const float vxyij = 0.25f * (XY[c] + XY[c - P] + XY[c - 1] + XY[c - 1 - P]);
-    const float vij = vyxij + vxyij;
-    const float v = XY[c] + YX[c];
     float u1, u2, u3;
     if (WEQ == SOFI_EL_TTI) {
         u1 = (m.c55i[c] * v) + (m.c15i[c] * vxxipjp) + (m.c35i[c] * vyyipjp);
@@ -1555,6 +1518,89 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti(Grid g, Fields f0, Mat m,
     f.sxy[c] = sxy;
     f.sxx[c] = sxx;
     f.syy[c] = syy;
+}
+
+// a frame cell the frame update updates: not within two cells of the outer
+// edge of its frame side
+__device__ __forceinline__ bool tti_frame_updated(const Grid &g, int i, int j)
+{
+    return (in_xframe(g, i) || in_yframe(g, j)) &&
+           !((i <= g.gx2 && i < 1 + 2) || (i > g.gx3 && i > g.nx - 2) || (j <= g.gy2 && j < 1 + 2) ||
+             (j > g.gy3 && j > g.ny - 2));
+}
+
+// the frame cell whose derivatives keep their raw values (no CPML): the
+// elastic right-bottom corner's last row
+__device__ __forceinline__ bool tti_cpml_skipped(const Grid &g, int i, int j)
+{
+    return g.weq == SOFI_EL_TTI && i > g.gx3 && j > g.gy3 && j == g.ny;
+}
+
+// explosive sources of cell c at step nt, in source order (psource)
+__device__ __forceinline__ void psource_cell(const Grid &g, const Fields &f, const Src &src, const Fuse &fz, int c,
+                                             int nt)
+{
+    for (int k = 0; k < fz.npsrc; ++k)
+        if (c == fz.c[k]) {
+            const float amp = psource_amp(g, src, fz.soff + k, step_of(fz.ntb, nt));
+            f.sxx[c] += amp;
+            f.syy[c] += amp;
+        }
+}
+
+// The frame update, the last stress kernel of a TTI step with a CPML frame.
+// With fz.npsrc (one-shot handles) it also adds the explosive sources after
+// every update (psource, src/time_loop.c:248-250): the thread of an updated
+// frame cell for that cell, the first thread of the launch for all other
+// cells, whose stresses no kernel touches any more.
+template <int ND, int WEQ>
+__global__ __launch_bounds__(BX *BY) void k_tti_frame(Grid g, Fields f0, Mat m, Fuse fz, Src src, int nt)
+{
+    const Fields f = f0.at(g.k_lo + blockIdx.y, g);   // shot slot of a batch
+    if (fz.npsrc > 0 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 && threadIdx.y == 0)
+        for (int k = 0; k < fz.npsrc; ++k) {
+            const int c = fz.c[k], ci = c % g.pitch - SOFI_PAD + 1, cj = c / g.pitch - SOFI_PAD + 1;
+            if (tti_frame_updated(g, ci, cj)) continue;
+            bool first = true;      // a cell's sources are added once, at its first source
+            for (int q = 0; q < k; ++q) first = first && fz.c[q] != c;
+            if (first) psource_cell(g, f, src, fz, c, nt);
+        }
+    int tx, ty;
+    frame_tile(g, blockIdx.x, tx, ty);
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * BY + threadIdx.y;
+    if (i > g.nx || j > g.ny || !tti_frame_updated(g, i, j)) return;
+    const int c = IDX(g, i, j), P = g.pitch;
+    const bool corner = in_xframe(g, i) && in_yframe(g, j);
+    // the derivatives of cell (i + di, j + dj) as this update reads them: for a
+    // side cell, a corner cell's raw ones (its CPML comes after the sides'
+    // update in the reference), else the stored ones
+    auto rd = [&](int di, int dj, float &xx, float &yy, float &yx, float &xy) {
+        const int ii = i + di, jj = j + dj, cc = c + di + dj * P;
+        if (!corner && in_xframe(g, ii) && in_yframe(g, jj) && ii >= 1 && ii <= g.nx && jj >= 1 && jj <= g.ny) {
+            op_s<ND>(g, f.vx, f.vy, cc, xx, yx, xy, yy);
+        } else {
+            xx = f.dvxx[cc];
+            yy = f.dvyy[cc];
+            yx = f.dvyx[cc];
+            xy = f.dvxy[cc];
+        }
+    };
+    float x00, y00, yx00, xy00, x01, y01, d0, d1, x10, y10, x11, y11;
+    float yxm0, xym0, yx0m, xy0m, yxmm, xymm;
+    rd(0, 0, x00, y00, yx00, xy00);
+    rd(0, 1, x01, y01, d0, d1);
+    rd(1, 0, x10, y10, d0, d1);
+    rd(1, 1, x11, y11, d0, d1);
+    rd(0, -1, d0, d1, yx0m, xy0m);
+    rd(-1, 0, d0, d1, yxm0, xym0);
+    rd(-1, -1, d0, d1, yxmm, xymm);
+    const float vxxipjp = 0.25f * (x00 + x01 + x10 + x11);
+    const float vyyipjp = 0.25f * (y00 + y01 + y10 + y11);
+    const float vyxij = 0.25f * (yx00 + yx0m + yxm0 + yxmm);
+    const float vxyij = 0.25f * (xy00 + xy0m + xym0 + xymm);
+    tti_frame_update<WEQ>(g, f, m, fz, c, x00, y00, vxxipjp, vyyipjp, vyxij + vxyij, xy00 + yx00);
+    psource_cell(g, f, src, fz, c, nt);
 }
 
 
@@ -1740,13 +1786,16 @@ __global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, i
 //   * with the damping frame (ABS_TYPE 2) every frame cell is updated here
 //     too, then damped (src/update_s_elastic_TTI_abs.c:35-99);
 //   * with the CPML frame, tiles that hold a frame cell or one next to it
-//     store their raw derivatives into the derivative arrays, where the
-//     frame passes (k_cpml_dv_frame, k_stress_tti PASS 1 / 3, over these
-//     tiles only) apply the CPML region by region and update the frame.
+//     store the derivatives of their cells into the derivative arrays: the
+//     frame cells' after their CPML, the others' raw; k_tti_frame (over
+//     these tiles only) then updates the frame;
+//   * explosive sources (fz.npsrc: one-shot handles with no CPML frame
+//     pass to follow) are added to their cells after the update, as psource
+//     does after the stress update (src/time_loop.c:248-250).
 // S slots per thread (blockIdx.y: group of S slots of a batch; 1 for a
 // one-shot handle).
 template <int ND, int WEQ, int S>
-__global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat m, Fuse fz)
+__global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat m, Fuse fz, Src src, int nt)
 {
     constexpr int LX = BX + 2, LY = BY + 2;
     __shared__ float sd[S][4][LY][LX];   // dvxx, dvyy, dvyx, dvxy of rows j0-1..j0+BY, columns i0-1..i0+BX
@@ -1781,7 +1830,10 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat 
             sd[s][1][lj][li] = vyy;
             sd[s][2][lj][li] = vyx;
             sd[s][3][lj][li] = vxy;
-            if (own || ghost) {
+            // raw derivatives of the tile's cells outside the frame (the
+            // frame cells' follow below, after their CPML) and of periodic
+            // ghost columns
+            if ((own && !in_xframe(g, ii) && !in_yframe(g, jj)) || ghost) {
                 f.dvxx[cc] = vxx;
                 f.dvyy[cc] = vyy;
                 f.dvyx[cc] = vyx;
@@ -1792,11 +1844,28 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat 
     __syncthreads();
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
-    const bool frame = in_xframe(g, i) || in_yframe(g, j);
-    const bool damp = frame && g.abs_type == 2 && g.fw > 0;
-    if (frame && !damp) return;     // CPML frame: the frame passes; no absorbing frame: never updated
     const int c = IDX(g, i, j);
     const int li = threadIdx.x + 1, lj = threadIdx.y + 1;
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
+    const bool damp = frame && g.abs_type == 2 && g.fw > 0;
+    if (frame && !damp) {
+        // CPML frame: updated by k_tti_frame (no absorbing frame: never
+        // updated); a frame cell's derivatives get their CPML here and are
+        // stored for it (the tile's updates read the raw ones from LDS)
+        if (store_dv) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                Fields f = f0.at(k0 + s, g);
+                float vxx = sd[s][0][lj][li], vyy = sd[s][1][lj][li], vyx = sd[s][2][lj][li], vxy = sd[s][3][lj][li];
+                if (!tti_cpml_skipped(g, i, j)) cpml_dv(g, m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
+                f.dvxx[c] = vxx;
+                f.dvyy[c] = vyy;
+                f.dvyx[c] = vyx;
+                f.dvxy[c] = vxy;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const Fields f = f0.at(k0 + s, g);
@@ -1846,6 +1915,13 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat 
             syy *= a;
             sxx *= a;
         }
+        if (S == 1)
+            for (int k = 0; k < fz.npsrc; ++k)
+                if (c == fz.c[k]) {
+                    const float amp = psource_amp(g, src, fz.soff + k, step_of(fz.ntb, nt));
+                    sxx += amp;
+                    syy += amp;
+                }
         f.sxy[c] = sxy;
         f.sxx[c] = sxx;
         f.syy[c] = syy;
@@ -3149,24 +3225,19 @@ void wrap_x(sofi_dev *h, Grid gg, int lo, int hi, float *a, float *b)
     launch(h, k_wrap_x, dim3((unsigned)((gg.ny + 255) / 256), (unsigned)(hi - lo)), dim3(256), gg, a, b);
 }
 
-// the TTI CPML frame of slots lo..hi-1, region by region over the frame
-// tiles: CPML of the sides, their update, CPML of the corners, their update
-// (src/update_s_elastic_TTI_PML.c:32-145)
+// the TTI CPML frame of slots lo..hi-1 over the frame tiles, after
+// k_stress_tti_f has applied the frame's CPML: the sides' and corners'
+// update (src/update_s_elastic_TTI_PML.c:32-145)
 template <int ND>
-void tti_frame(sofi_dev *h, Grid gg, int lo, int hi, const Fields &f, const Mat &m, const Fuse &fz)
+void tti_frame(sofi_dev *h, Grid gg, int lo, int hi, const Fields &f, const Mat &m, const Fuse &fz, int nt)
 {
     const int nft = frame_tile_count(gg);
     if (gg.fw <= 0 || gg.abs_type != 1 || nft <= 0 || hi <= lo) return;
     gg.k_lo = lo;
     gg.k_hi = hi;
     const dim3 tb(BX, BY), tg((unsigned)nft, (unsigned)(hi - lo));
-    const bool el = gg.weq == SOFI_EL_TTI;
-    launch(h, k_cpml_dv_frame<false>, tg, tb, gg, f, m);
-    if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 1>, tg, tb, gg, f, m, fz);
-    else launch(h, k_stress_tti<SOFI_VEL_TTI, 1>, tg, tb, gg, f, m, fz);
-    launch(h, k_cpml_dv_frame<true>, tg, tb, gg, f, m);
-    if (el) launch(h, k_stress_tti<SOFI_EL_TTI, 3>, tg, tb, gg, f, m, fz);
-    else launch(h, k_stress_tti<SOFI_VEL_TTI, 3>, tg, tb, gg, f, m, fz);
+    if (gg.weq == SOFI_EL_TTI) launch(h, k_tti_frame<ND, SOFI_EL_TTI>, tg, tb, gg, f, m, fz, h->s, nt);
+    else launch(h, k_tti_frame<ND, SOFI_VEL_TTI>, tg, tb, gg, f, m, fz, h->s, nt);
 }
 
 // Launches of a batch's update kernels in slot blocks of S slots per thread
@@ -3256,7 +3327,9 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // sources fused into the update kernels (one-shot handles with few
     // sources; batches launched one slot at a time, each launch with its
     // slot's sources)
-    const bool src_fused = !tti && h->fusable && h->source_type != 5;
+    // (TTI: not with a frame that is never updated, ABS_TYPE other than 1 or
+    // 2, where a source cell would be skipped by every update kernel)
+    const bool src_fused = h->fusable && h->source_type != 5 && (!tti || g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     // a batch in slot blocks (h->bs slots per thread, all slots of the group
     // in one launch per kernel); sources, free surface and sampling then run
     // as launches of their own
@@ -3267,7 +3340,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // the free surface follows the explosive / moment sources, so it is fused
     // when they are, or when there are none to inject this step
     const bool no_stress_src = nsrc == 0 || (h->psrc_inj.n == 0 && h->n_eq == 0);
-    fs.surf = !bm && (src_fused || slot_fused || (!tti && no_stress_src)) && d.free_surf && !sample_dv &&
+    fs.surf = !bm && !tti && (src_fused || slot_fused || no_stress_src) && d.free_surf && !sample_dv &&
               (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
     fs.fwi_u = store && d.velocity == 1;
@@ -3354,9 +3427,9 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
               case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
               default: {
                   const Fuse f0{};
-                  if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0);
-                  else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0);
-                  tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, f0);
+                  if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0, h->s, nt);
+                  else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0, h->s, nt);
+                  tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, f0, nt);
               }
             }
             continue;
@@ -3430,9 +3503,14 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
               // computed in the kernel, one slot per thread; then the CPML
               // frame region by region over the frame tiles
               // (src/time_loop.c:180-186)
-              if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, 1, f, m, fss);
-              else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, 1, f, m, fss);
-              tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, fss);
+              // fused explosive sources: added by the last stress kernel,
+              // k_tti_frame when the CPML frame passes run
+              const bool frame_passes = g.abs_type == 1 && g.fw > 0 && frame_tile_count(g) > 0;
+              Fuse fin = fss;
+              if (frame_passes) fin.npsrc = 0;
+              if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, 1, f, m, fin, h->s, nt);
+              else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, 1, f, m, fin, h->s, nt);
+              tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, fss, nt);
           }
         }
     }
