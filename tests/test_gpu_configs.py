@@ -31,7 +31,7 @@ pytestmark = pytest.mark.gpu
 
 C3_SMALL = dict(nshots=8, nx=500, ny=250, nt=600)
 C2_STEPS, C2_SNAP_DEC = 800, 8
-C5_SMALL = dict(nshots=2, nt=2000)
+C5_SMALL = dict(nshots=3, nt=2000)
 C5_SUB = 3
 
 
@@ -181,16 +181,26 @@ def test_config5_gradient_vs_reference(dtinv, bound):
         js = workloads.config5(td, "true", **C5_SMALL)
         P = Problem(os.path.basename(js))
         sim = Simulation(P)
+        measured = {}
         for ishot in range(1, P.nshots + 1):
             P.set_shot(ishot)
             sim.load_shot(nsnap=0)
             sim.run()
             for c in P.seismo_comps():
-                P.write_section(c, sim.section(c))
+                measured[ishot, c] = sim.section(c)
+                P.write_section(c, measured[ishot, c])
         sim.close()
         P.close()
         js = workloads.config5(td, "init", dtinv=dtinv, **C5_SMALL)
         r = fwi.gradient_json(os.path.basename(js), verbose=False, write=False)
+    # the anomaly shows in the middle shot's data: its residual (~3e-5 of
+    # the data) is far above float rounding (~1e-7); the end shots' waves do
+    # not reach the anomaly and back within NT, so theirs are rounding only
+    for ishot, sh in r["shots"].items():
+        d = max(rel_l2(sh["sections"][c], measured[ishot, c]) for c in (0, 1))
+        print(f"config 5, shot {ishot}: synthetic vs measured rel-L2 {d:.2e}")
+        if ishot == 2:
+            assert d > 1e-5, d
     nx, ny = 1500, 500
     from common import xmajor
     cos = []
@@ -202,4 +212,7 @@ def test_config5_gradient_vs_reference(dtinv, bound):
         cos.append(cs)
         print(f"config 5, DTINV {dtinv}, grad {p}: cosine {cs:.6f}, norm ratio x DTINV {ratio:.4f}")
         assert cs >= bound, (p, cs)
-        assert abs(ratio - 1.0) < (0.01 if dtinv == 1 else 0.05), (p, ratio)
+        # the residual is ~3e-5 of the data, so a float forward run's ~1e-6
+        # relative rounding is a few percent of the residual: the norms agree
+        # to that, the direction to cosine >= 0.999
+        assert abs(ratio - 1.0) < (0.03 if dtinv == 1 else 0.05), (p, ratio)
