@@ -2386,14 +2386,27 @@ int dalloc(sofi_dev *h, void **p, size_t bytes)
         (void)hipFree(raw);
         return fail("hipMalloc returned a pointer that is not 256-B aligned");
     }
-    e = hipMemset(raw, 0, bytes + 256);
+    // zeroed on the handle's stream and waited for: the handle's streams are
+    // non-blocking, so a fill on the null stream would not be ordered before
+    // the kernels and copies that follow on them
+    e = hipMemsetAsync(raw, 0, bytes + 256, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
         (void)hipFree(raw);
-        return fail("hipMemset failed: %s", hipGetErrorString(e));
+        return fail("zeroing a new allocation failed: %s", hipGetErrorString(e));
     }
     *p = (char *)raw + ALIGN_SHIFT;
     h->allocs.push_back({ *p, raw, bytes + 256 });
     h->bytes += bytes + 256;
+    return 0;
+}
+
+// host -> device copy ordered on the handle's stream and waited for (a copy
+// on the null stream is not ordered with the handle's non-blocking stream)
+int h2d(sofi_dev *h, void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -2421,7 +2434,7 @@ int dalloc_t(sofi_dev *h, T **p, size_t n)
 int upload(sofi_dev *h, float **dst, const float *src, size_t n)
 {
     if (!*dst && dalloc_t(h, dst, n)) return -1;
-    HIPCHK(hipMemcpy(*dst, src, n * sizeof(float), hipMemcpyHostToDevice));
+    if (h2d(h, *dst, src, n * sizeof(float))) return -1;
     return 0;
 }
 
@@ -2458,7 +2471,7 @@ int build_inj(sofi_dev *h, int n, const int *si, const int *sj, const int *slots
     all.insert(all.end(), first.begin(), first.end());
     all.insert(all.end(), order.begin(), order.end());
     if (dalloc_t(h, buf, all.size())) return -1;
-    HIPCHK(hipMemcpy(*buf, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (h2d(h, *buf, all.data(), all.size() * sizeof(int))) return -1;
     inj->n = nc;
     inj->cell = *buf;
     inj->first = *buf + nc;
@@ -2912,12 +2925,12 @@ int sync_acq(sofi_dev *h)
             ib.insert(ib.end(), cslot.begin(), cslot.end());
             std::vector<float> fb(cazi);
             fb.insert(fb.end(), camp.begin(), camp.end());
-            HIPCHK(hipMemcpy(h->src_ibuf, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->src_fbuf, fb.data(), fb.size() * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->src_sig, csig.data(), csig.size() * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->src_sc, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice));
+            if (h2d(h, h->src_ibuf, ib.data(), ib.size() * sizeof(int))) return -1;
+            if (h2d(h, h->src_fbuf, fb.data(), fb.size() * sizeof(float))) return -1;
+            if (h2d(h, h->src_sig, csig.data(), csig.size() * sizeof(float))) return -1;
+            if (h2d(h, h->src_sc, sc.data(), sc.size() * sizeof(double))) return -1;
         }
-        if (!eq.empty()) HIPCHK(hipMemcpy(h->eq_list, eq.data(), eq.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!eq.empty()) if (h2d(h, h->eq_list, eq.data(), eq.size() * sizeof(int))) return -1;
         h->n_eq = (int)eq.size();
         Src &sv = h->s;
         sv.i = h->src_ibuf;
@@ -3021,9 +3034,9 @@ int sync_acq(sofi_dev *h)
             std::vector<double> db(sq_ppi);
             db.insert(db.end(), sq_pu.begin(), sq_pu.end());
             db.insert(db.end(), pfac.begin(), pfac.end());
-            HIPCHK(hipMemcpy(h->rec_ibuf, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->rec_lbuf, lb.data(), lb.size() * sizeof(long long), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->rec_d, db.data(), db.size() * sizeof(double), hipMemcpyHostToDevice));
+            if (h2d(h, h->rec_ibuf, ib.data(), ib.size() * sizeof(int))) return -1;
+            if (h2d(h, h->rec_lbuf, lb.data(), lb.size() * sizeof(long long))) return -1;
+            if (h2d(h, h->rec_d, db.data(), db.size() * sizeof(double))) return -1;
         }
         Rec &r = h->r;
         r.i = h->rec_ibuf;
@@ -4387,8 +4400,8 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const
     const size_t ns = (size_t)(n > 0 ? n : 1) * h->d.nt;
     if (dalloc_t(h, &h->adj_sx, ns) || dalloc_t(h, &h->adj_sy, ns)) return -1;
     if (n > 0) {
-        HIPCHK(hipMemcpy(h->adj_sx, sig_x, (size_t)n * h->d.nt * sizeof(float), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->adj_sy, sig_y, (size_t)n * h->d.nt * sizeof(float), hipMemcpyHostToDevice));
+        if (h2d(h, h->adj_sx, sig_x, (size_t)n * h->d.nt * sizeof(float))) return -1;
+        if (h2d(h, h->adj_sy, sig_y, (size_t)n * h->d.nt * sizeof(float))) return -1;
     }
     if (build_inj(h, n, ai, aj, nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
@@ -4424,7 +4437,10 @@ int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *gr
     float *out[3] = { grad_vp, grad_vs, grad_rho };
     for (int k = 0; k < 3; ++k)
         if (out[k]) HIPCHK(hipMemcpy(out[k], h->w.gsum + k * pl, pl * sizeof(float), hipMemcpyDeviceToHost));
-    if (reset) HIPCHK(hipMemset(h->w.gsum, 0, 3 * pl * sizeof(float)));
+    if (reset) {
+        HIPCHK(hipMemsetAsync(h->w.gsum, 0, 3 * pl * sizeof(float), h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     return 0;
 }
 

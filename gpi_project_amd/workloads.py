@@ -300,9 +300,10 @@ def config5(d: str, which: str, nshots: int = 32, nx: int = 1500, ny: int = 500,
     seed 6 (x in the middle half, depth in the middle 40 %).  nshots
     explosive Ricker shots evenly spaced along x at 125 m depth, fc ~ U(8, 20)
     Hz (seed 7), amplitude amp; 300 receivers at 115 m depth.  The
-    gradient scales with the square of the source amplitude: at amplitude 1
-    its float terms underflow on this grid (the reference's float gradient is
-    zero but for denormals), at 1e9 they are ~1e-25.  which = "true": MODE=FW on
+    gradient scales with the square of the source amplitude: the norm of the
+    middle shot's gradient (3 shots, NT 2000) is 1.1e-22 at amplitude 1 and
+    1.2e-4 at the default 1e9, which keeps the gradient files' values away
+    from the bottom of the float range.  which = "true": MODE=FW on
     the true model writing the measured data ./data/obs_{vx,vy}.su.shot<n>;
     "init": MODE=FWI (LNORM 2, VELOCITY 1, ADJOINT_TYPE 1, DTINV) on the
     background reading them."""
