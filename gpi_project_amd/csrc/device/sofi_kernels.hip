@@ -641,8 +641,9 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
 // per step, stress kernel 72 -> 77 % of 8 TB/s).  The vector-shape kernels of
 // the elastic rheologies ran slower with the same hints (-17 % for a batch of
 // EL_VTI shots on 2000 x 1000) and keep plain accesses, and so does the
-// velocity kernel of the TTI rheologies (NT false: -7 % with the hints,
-// whose derivative pass reads vx, vy right after).
+// velocity kernel of the TTI rheologies (NT false: the TTI stress kernel
+// reads vx, vy right after for its derivatives; with the hints one EL_TTI
+// shot on 2000 x 1000 ran 11 % slower).
 //
 // one thread per cell; blockIdx.y picks g.spt shot slots of a batch, which
 // the thread updates in turn with the cell's coefficients read once (ONE: the
