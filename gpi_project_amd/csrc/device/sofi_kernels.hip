@@ -87,6 +87,11 @@ struct Fields {
     float *dvxx, *dvyy, *dvyx, *dvxy;           // TTI only
     float *psi_sxx_x, *psi_sxy_x, *psi_syy_y, *psi_sxy_y;
     float *psi_vxx, *psi_vyx, *psi_vyy, *psi_vxy;
+    // MODE=FWI, one plane each per slot: the rho-weighted accelerations of
+    // the forward pass (the adjoint velocities or their time integral in the
+    // adjoint pass), the stress increments over DT, and the shot's lambda /
+    // mu / rho gradient accumulators
+    float *pvxp1, *pvyp1, *ux, *uy, *uxy, *glam, *gmu, *grho;
     int *unstable;                             // per slot: first step whose start finds vy(NX/2, NY/2) = NaN
 
     __host__ __device__ __forceinline__ Fields at(int k, const Grid &g) const
@@ -110,6 +115,16 @@ struct Fields {
             s.dvyy += pl;
             s.dvyx += pl;
             s.dvxy += pl;
+        }
+        if (pvxp1) {
+            s.pvxp1 += pl;
+            s.pvyp1 += pl;
+            s.ux += pl;
+            s.uy += pl;
+            s.uxy += pl;
+            s.glam += pl;
+            s.gmu += pl;
+            s.grho += pl;
         }
         if (psi_sxx_x) {
             s.psi_sxx_x += xp;
@@ -280,7 +295,6 @@ struct Fuse {
     // 2 adjoint time integral of v, 3 adjoint copy of v (into pvxp1/pvyp1);
     // stress kernel: u / DT of the EL_ISO update into ux/uy/uxy
     int fwi_acc, fwi_u;
-    float *pvxp1, *pvyp1, *ux, *uy, *uxy;
     const int *ntb;     // inside a captured block of steps: the block's base step (nt is then local)
     // vx / vy sampling carried by the stress launch of a one-shot handle
     // (SEISMO = 1): blocks ntile.. of the launch sample the receivers, whose
@@ -615,14 +629,14 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
     for (int k = 0; k < fz.nforce; ++k)
         if (c == fz.c[k]) force_add(g, s, fz.soff + k, step_of(fz.ntb, nt), rip, rjp, vx, vy);
     if (fz.fwi_acc == 1) {
-        fz.pvxp1[c] = rip * (sxx_x + sxy_y) / g.dh;
-        fz.pvyp1[c] = rjp * (sxy_x + syy_y) / g.dh;
+        f.pvxp1[c] = rip * (sxx_x + sxy_y) / g.dh;
+        f.pvyp1[c] = rjp * (sxy_x + syy_y) / g.dh;
     } else if (fz.fwi_acc == 2) {
-        fz.pvxp1[c] += vx * g.dt;
-        fz.pvyp1[c] += vy * g.dt;
+        f.pvxp1[c] += vx * g.dt;
+        f.pvyp1[c] += vy * g.dt;
     } else if (fz.fwi_acc == 3) {
-        fz.pvxp1[c] = vx;
-        fz.pvyp1[c] = vy;
+        f.pvxp1[c] = vx;
+        f.pvyp1[c] = vy;
     }
     vx = vx + (sxx_x + sxy_y) * g.dtdh * rip;
     vy = vy + (sxy_x + syy_y) * g.dtdh * rjp;
@@ -765,18 +779,19 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_v(Grid g, Fields f0, Mat m,
 // src/wavefield_update_s_visc_TTI.c:79-83); only the forward store steps
 // with VELOCITY = 1 read them (src/snap_store.c:35-43), so they are written
 // in those steps only
-__device__ __forceinline__ void fwi_u_store(const Fuse &fz, float dt, int c, float u1, float u2, float u3)
+__device__ __forceinline__ void fwi_u_store(const Fuse &fz, const Fields &f, float dt, int c, float u1, float u2,
+                                            float u3)
 {
     if (fz.fwi_u) {
-        fz.uxy[c] = u1 / dt;
-        fz.ux[c] = u2 / dt;
-        fz.uy[c] = u3 / dt;
+        f.uxy[c] = u1 / dt;
+        f.ux[c] = u2 / dt;
+        f.uy[c] = u3 / dt;
     }
 }
 
 // wavefield_update_s_el (src/wavefield_update_s_el.c:28-48)
 __device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vxx, float vyx, float vxy, float vyy,
-                                       float &sxx, float &syy, float &sxy, const Fuse &fz, float dt, int c)
+                                       float &sxx, float &syy, float &sxy, const Fuse &fz, const Fields &f, float dt, int c)
 {
     const float u1 = fipjp * (vyx + vxy);
     const float ff = 2.0f * f_;
@@ -785,13 +800,13 @@ __device__ __forceinline__ void upd_el(float f_, float g_, float fipjp, float vx
     sxy += u1;
     sxx += u2;
     syy += u3;
-    fwi_u_store(fz, dt, c, u1, u2, u3);
+    fwi_u_store(fz, f, dt, c, u1, u2, u3);
 }
 
 // wavefield_update_s_el_vti (src/wavefield_update_s_el_vti.c:27-47)
 __device__ __forceinline__ void upd_el_vti(float c11, float c13, float c33, float c55i, float vxx, float vyx,
                                            float vxy, float vyy, float &sxx, float &syy, float &sxy, const Fuse &fz,
-                                           float dt, int c)
+                                           const Fields &f, float dt, int c)
 {
     const float u1 = c55i * (vxy + vyx);
     const float u2 = (c11 * vxx) + (c13 * vyy);
@@ -799,7 +814,7 @@ __device__ __forceinline__ void upd_el_vti(float c11, float c13, float c33, floa
     sxy += u1;
     sxx += u2;
     syy += u3;
-    fwi_u_store(fz, dt, c, u1, u2, u3);
+    fwi_u_store(fz, f, dt, c, u1, u2, u3);
 }
 
 // Memory-variable sums of the viscoelastic updates: old (r0, p0, q0) and new
@@ -827,7 +842,7 @@ __device__ __forceinline__ void visc_level(float eta, float bip, float cip, floa
 
 __device__ __forceinline__ void visc_final(float dthalbe, float f_, float g_, float fipjp, float sxyv, float sdiag,
                                            float vxx, float vyy, const VSum &u, float &sxx, float &syy, float &sxy,
-                                           const Fuse &fz, float dt, int c)
+                                           const Fuse &fz, const Fields &f, float dt, int c)
 {
     const float ff = 2.0f * f_;
     const float u1 = (fipjp * sxyv) + (dthalbe * (u.r0 + u.r));
@@ -836,7 +851,7 @@ __device__ __forceinline__ void visc_final(float dthalbe, float f_, float g_, fl
     sxy += u1;
     sxx += u2;
     syy += u3;
-    fwi_u_store(fz, dt, c, u1, u2, u3);
+    fwi_u_store(fz, f, dt, c, u1, u2, u3);
 }
 
 // mechanism l of wavefield_update_s_visc_VTI (src/wavefield_update_s_visc_VTI.c:27-66)
@@ -857,7 +872,7 @@ __device__ __forceinline__ void visc_vti_level(float bip, float cip, float d11, 
 
 __device__ __forceinline__ void visc_vti_final(float dthalbe, float u11, float u13, float u33, float u55i, float sxyv,
                                                float vxx, float vyy, const VSum &u, float &sxx, float &syy,
-                                               float &sxy, const Fuse &fz, float dt, int c)
+                                               float &sxy, const Fuse &fz, const Fields &f, float dt, int c)
 {
     const float u1 = (u55i * sxyv) + (dthalbe * (u.r0 + u.r));
     const float u2 = (u11 * vxx) + (u13 * vyy) + (dthalbe * (u.p0 + u.p));
@@ -865,7 +880,7 @@ __device__ __forceinline__ void visc_vti_final(float dthalbe, float u11, float u
     sxy += u1;
     sxx += u2;
     syy += u3;
-    fwi_u_store(fz, dt, c, u1, u2, u3);
+    fwi_u_store(fz, f, dt, c, u1, u2, u3);
 }
 
 // Derivatives of a cell from the raw op_s sums (no 1/DH): scaled, and CPML in
@@ -972,8 +987,8 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
         if (!stress_dv(g, f, m, i, j, a, b, d, e, vxx, vyx, vxy, vyy)) continue;
         float sxx = ld1<ONE>(f.sxx, c), syy = ld1<ONE>(f.syy, c),
               sxy = ld1<ONE>(f.sxy, c);
-        if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
-        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, g.dt, c);
+        if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, f, g.dt, c);
+        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sxx, syy, sxy, fz, f, g.dt, c);
         if (WEQ == SOFI_VEL_ISO) {
             const float sxyv = vxy + vyx, sdiag = vxx + vyy;
             VSum u;
@@ -988,7 +1003,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
                 st1<ONE>(f.p, o, p);
                 st1<ONE>(f.q, o, q);
             }
-            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy, fz, g.dt, c);
+            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sxx, syy, sxy, fz, f, g.dt, c);
         }
         if (WEQ == SOFI_VEL_VTI) {
             const float sxyv = vxy + vyx;
@@ -1003,7 +1018,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
                 f.p[o] = p;
                 f.q[o] = q;
             }
-            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sxx, syy, sxy, fz, g.dt, c);
+            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sxx, syy, sxy, fz, f, g.dt, c);
         }
         stress_post<ND, VISCO>(g, f, m, s, fz, nt, i, j, c, sxx, syy, sxy);
         st1<ONE>(f.sxx, c, sxx);
@@ -1086,7 +1101,7 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
                 c3 = PRE ? pre[2] : ldv<VW>(m.fipjp, c0);
 #pragma unroll
         for (int k = 0; k < VW; ++k)
-            if (act[k]) upd_el(c1[k], c2[k], c3[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fz, g.dt, c0 + k);
+            if (act[k]) upd_el(c1[k], c2[k], c3[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fz, f, g.dt, c0 + k);
     }
     if (WEQ == SOFI_EL_VTI) {
         const V c1 = PRE ? pre[0] : ldv<VW>(m.c11, c0), c2 = PRE ? pre[1] : ldv<VW>(m.c13, c0),
@@ -1095,7 +1110,7 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
         for (int k = 0; k < VW; ++k) {
             Fuse fk = fz;
             fk.fwi_u = fz.fwi_u && act[k];
-            upd_el_vti(c1[k], c2[k], c3[k], c4[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fk, g.dt,
+            upd_el_vti(c1[k], c2[k], c3[k], c4[k], vxx[k], vyx[k], vxy[k], vyy[k], sxx[k], syy[k], sxy[k], fk, f, g.dt,
                        c0 + k);
         }
     }
@@ -1151,7 +1166,7 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
                 Fuse fk = fz;
                 fk.fwi_u = fz.fwi_u && act[k];
                 visc_final(g.dthalbe, c1[k], c2[k], c3[k], sxyv[k], sdiag[k], vxx[k], vyy[k], u[k], sxx[k], syy[k],
-                           sxy[k], fk, g.dt, c0 + k);
+                           sxy[k], fk, f, g.dt, c0 + k);
             }
         } else {
             const V c1 = ldv<VW>(m.u11, c0), c2 = ldv<VW>(m.u13, c0), c3 = ldv<VW>(m.u33, c0), c4 = ldv<VW>(m.u55i, c0);
@@ -1160,7 +1175,7 @@ __device__ __forceinline__ void stress_v_slot(const Grid &g, Fields f, const Mat
                 Fuse fk = fz;
                 fk.fwi_u = fz.fwi_u && act[k];
                 visc_vti_final(g.dthalbe, c1[k], c2[k], c3[k], c4[k], sxyv[k], vxx[k], vyy[k], u[k], sxx[k], syy[k],
-                               sxy[k], fk, g.dt, c0 + k);
+                               sxy[k], fk, f, g.dt, c0 + k);
             }
         }
     }
@@ -1514,7 +1529,7 @@ __device__ __forceinline__ void tti_frame_update(const Grid &g, const Fields &f,
         u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (m.u15[c] * vij) + (g.dthalbe * (sp0 + sp));
         u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (m.u35[c] * vij) + (g.dthalbe * (sq0 + sq));
     }
-    fwi_u_store(fz, g.dt, c, u1, u2, u3);
+    fwi_u_store(fz, f, g.dt, c, u1, u2, u3);
     float sxy = f.sxy[c] + u1, sxx = f.sxx[c] + u2, syy = f.syy[c] + u3;
     f.sxy[c] = sxy;
     f.sxx[c] = sxx;
@@ -1644,7 +1659,7 @@ __global__ void k_wrap_x(Grid g, float *a, float *b)
 // that of the one-thread-per-cell kernels.
 
 template <int ND, int S>
-__global__ __launch_bounds__(BX *BY) void k_velocity_b(Grid g, Fields f0, Mat m, int nt, const int *ntb)
+__global__ __launch_bounds__(BX *BY) void k_velocity_b(Grid g, Fields f0, Mat m, int nt, const int *ntb, int fwi_acc)
 {
     const int t = blockIdx.x, ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
@@ -1664,6 +1679,7 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_b(Grid g, Fields f0, Mat m,
     const Src s0{};
     Fuse fz{};
     fz.ntb = ntb;
+    fz.fwi_acc = fwi_acc;     // MODE=FWI: the accelerations / adjoint velocities of every slot
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         Fields f = f0.at(k0 + q, g);
@@ -1674,7 +1690,7 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_b(Grid g, Fields f0, Mat m,
 }
 
 template <int ND, int WEQ, int S>
-__global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, int nt, const int *ntb)
+__global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, int nt, const int *ntb, int fwi_u)
 {
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
     const int t = blockIdx.x, ty = t / g.ntx, tx = t - ty * g.ntx;
@@ -1730,14 +1746,15 @@ __global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, i
     const Src s0{};
     Fuse fz{};
     fz.ntb = ntb;
+    fz.fwi_u = fwi_u;         // MODE=FWI: the stress increments of every slot
 #pragma unroll
     for (int q = 0; q < S; ++q) {
         Fields f = f0.at(k0 + q, g);
         float vxx, vyx, vxy, vyy;
         if (!stress_dv(g, f, m, i, j, a[q], b[q], d[q], e[q], vxx, vyx, vxy, vyy)) continue;
         float sx = sxx[q], sy = syy[q], sz = sxy[q];
-        if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sx, sy, sz, fz, g.dt, c);
-        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sx, sy, sz, fz, g.dt, c);
+        if (WEQ == SOFI_EL_ISO) upd_el(c1, c2, c3, vxx, vyx, vxy, vyy, sx, sy, sz, fz, f, g.dt, c);
+        if (WEQ == SOFI_EL_VTI) upd_el_vti(c1, c2, c3, c4, vxx, vyx, vxy, vyy, sx, sy, sz, fz, f, g.dt, c);
         if (WEQ == SOFI_VEL_ISO) {
             const float sxyv = vxy + vyx, sdiag = vxx + vyy;
             VSum u;
@@ -1751,7 +1768,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, i
                 f.p[o] = p;
                 f.q[o] = qq;
             }
-            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sx, sy, sz, fz, g.dt, c);
+            visc_final(g.dthalbe, c1, c2, c3, sxyv, sdiag, vxx, vyy, u, sx, sy, sz, fz, f, g.dt, c);
         }
         if (WEQ == SOFI_VEL_VTI) {
             const float sxyv = vxy + vyx;
@@ -1766,7 +1783,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, i
                 f.p[o] = p;
                 f.q[o] = qq;
             }
-            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sx, sy, sz, fz, g.dt, c);
+            visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sx, sy, sz, fz, f, g.dt, c);
         }
         stress_post<ND, VISCO>(g, f, m, s0, fz, nt, i, j, c, sx, sy, sz);
         f.sxx[c] = sx;
@@ -1908,7 +1925,7 @@ __global__ __launch_bounds__(BX *BY) void k_stress_tti_f(Grid g, Fields f0, Mat 
             u2 = (m.u11[c] * vxx) + (m.u13[c] * vyy) + (m.u15[c] * vij) + (g.dthalbe * (sp0 + sp));
             u3 = (m.u13[c] * vxx) + (m.u33[c] * vyy) + (m.u35[c] * vij) + (g.dthalbe * (sq0 + sq));
         }
-        if (S == 1) fwi_u_store(fz, g.dt, c, u1, u2, u3);    // FWI handles hold one slot
+        fwi_u_store(fz, f, g.dt, c, u1, u2, u3);
         float sxy = f.sxy[c] + u1, sxx = f.sxx[c] + u2, syy = f.syy[c] + u3;
         if (damp) {
             const float a = m.absorb[c];
@@ -2007,77 +2024,87 @@ __global__ void k_snap(Grid g, Fields f, Mat m, float *out, int i0, int di, int 
 // ------------------------------------------------------------------ FWI
 
 struct Fwi {
-    float *pvxp1, *pvyp1, *ux, *uy, *uxy;       // rho-weighted accelerations / adjoint v; u over DT
-    float *glam, *gmu, *grho;                  // gradLam_shot, gradMu_shot, gradRhos_shot
     float *gsum;                               // gradVp, gradVs, gradRho summed over shots (3 planes)
-    float *gshot;                              // the shot's gradVp, gradVs, gradRho (3 planes)
+    float *gshot;                              // a shot's gradVp, gradVs, gradRho (3 planes)
     const float *vp0, *vs0, *rho0, *muss, *lamss;
-    float *store;                              // [nstore][5][ny][nx]
+    float *store;                              // per slot [nstore][5][ny][nx]
+    size_t sstride;                            // floats per slot of the store
     int velocity;
 };
 
 // snap_store (src/snap_store.c:27-47): the forward fields of one store step
-// into slot `slot` of the store (compact [5][ny][nx] per slot)
-__global__ __launch_bounds__(BX *BY) void k_fwi_store(Grid g, Fields f, Fwi w, int slot, const int *ntb, int dtinv)
+// into store slot `slot` (compact [5][ny][nx] per store slot) of every shot
+// slot g.k_lo + blockIdx.y
+__global__ __launch_bounds__(BX *BY) void k_fwi_store(Grid g, Fields f0, Fwi w, int slot, const int *ntb, int dtinv)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    const int j = 1 + blockIdx.z * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     if (ntb) slot += *ntb / dtinv;          // in a captured block (its base step is a multiple of DTINV)
+    const int k = g.k_lo + blockIdx.y;
+    const Fields f = f0.at(k, g);
     const int c = IDX(g, i, j);
-    const size_t n = (size_t)g.nx * g.ny, o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
-    w.store[o] = w.pvxp1[c];
-    w.store[o + n] = w.pvyp1[c];
+    const size_t n = (size_t)g.nx * g.ny;
+    float *st = w.store + (size_t)k * w.sstride;
+    const size_t o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
+    st[o] = f.pvxp1[c];
+    st[o + n] = f.pvyp1[c];
     if (w.velocity == 0) {
-        w.store[o + 2 * n] = f.sxx[c];
-        w.store[o + 3 * n] = f.syy[c];
-        w.store[o + 4 * n] = f.sxy[c];
+        st[o + 2 * n] = f.sxx[c];
+        st[o + 3 * n] = f.syy[c];
+        st[o + 4 * n] = f.sxy[c];
     } else {
-        w.store[o + 2 * n] = w.ux[c];
-        w.store[o + 3 * n] = w.uy[c];
-        w.store[o + 4 * n] = w.uxy[c];
+        st[o + 2 * n] = f.ux[c];
+        st[o + 3 * n] = f.uy[c];
+        st[o + 4 * n] = f.uxy[c];
     }
 }
 
 // calc_conv (src/calc_conv.c:30-73): zero-lag correlation of the adjoint
-// fields with the forward fields of slot `slot`, with the reference's double
-// promotions (1.0 / (4.0 ...), 1.0 / 4.0)
-__global__ __launch_bounds__(BX *BY) void k_fwi_conv(Grid g, Fields f, Mat m, Fwi w, int slot, const int *ntb, int dtinv)
+// fields with the forward fields of store slot `slot`, with the reference's
+// double promotions (1.0 / (4.0 ...), 1.0 / 4.0), for every shot slot
+// g.k_lo + blockIdx.y
+__global__ __launch_bounds__(BX *BY) void k_fwi_conv(Grid g, Fields f0, Mat m, Fwi w, int slot, const int *ntb, int dtinv)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
-    const int j = 1 + blockIdx.y * BY + threadIdx.y;
+    const int j = 1 + blockIdx.z * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     if (ntb) slot -= *ntb / dtinv;          // in a captured block: the forward step runs backwards
+    const int k = g.k_lo + blockIdx.y;
+    const Fields f = f0.at(k, g);
     const int c = IDX(g, i, j);
-    const size_t n = (size_t)g.nx * g.ny, o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
-    const float frx = w.store[o], fry = w.store[o + n];
-    const float fx = w.store[o + 2 * n], fy = w.store[o + 3 * n], fu = w.store[o + 4 * n];
+    const size_t n = (size_t)g.nx * g.ny;
+    const float *st = w.store + (size_t)k * w.sstride;
+    const size_t o = (size_t)slot * 5 * n + (size_t)(j - 1) * g.nx + (i - 1);
+    const float frx = st[o], fry = st[o + n];
+    const float fx = st[o + 2 * n], fy = st[o + 3 * n], fu = st[o + 4 * n];
     const float sxx = f.sxx[c], syy = f.syy[c], sxy = f.sxy[c];
     const float muss = w.muss[c], lamss = w.lamss[c], DT = g.dt;
-    w.grho[c] += -DT * ((w.pvxp1[c] * frx) + (w.pvyp1[c] * fry));
-    w.glam[c] = (float)((double)w.glam[c] +
+    f.grho[c] += -DT * ((f.pvxp1[c] * frx) + (f.pvyp1[c] * fry));
+    f.glam[c] = (float)((double)f.glam[c] +
                         (double)(-DT) * (1.0 / (4.0 * (double)(lamss + muss) * (double)(lamss + muss))) *
                             (double)((fx + fy) * (sxx + syy)));
     if (m.pu[c] > 0.0f) {
         const double a = (1.0 / (double)(muss * muss)) * (double)(fu * sxy);
         const double b = (1.0 / 4.0) * (double)((fx + fy) * (sxx + syy)) / (double)((lamss + muss) * (lamss + muss));
         const double e = (1.0 / 4.0) * (double)((fx - fy) * (sxx - syy)) / (double)(muss * muss);
-        w.gmu[c] = (float)((double)w.gmu[c] + (double)(-DT) * (a + b + e));
+        f.gmu[c] = (float)((double)f.gmu[c] + (double)(-DT) * (a + b + e));
     }
 }
 
-// gradient of the shot (src/inversion.c:226-249): vp / vs / rho' from the
+// gradient of shot slot k (src/inversion.c:226-249): vp / vs / rho' from the
 // lambda / mu / rho accumulators, added to the sums; accumulators cleared
-__global__ __launch_bounds__(BX *BY) void k_fwi_grad(Grid g, Fwi w, int idx, int idy)
+__global__ __launch_bounds__(BX *BY) void k_fwi_grad(Grid g, Fields f0, Fwi w, int idx, int idy, int k)
 {
     const int i = 1 + blockIdx.x * BX + threadIdx.x;
     const int j = 1 + blockIdx.y * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
+    const Fields f = f0.at(k, g);
     const int c = IDX(g, i, j);
     float vp = 0.f, vs = 0.f, rho = 0.f;
     if ((i - 1) % idx == 0 && (j - 1) % idy == 0) {
         const double Vp0 = w.vp0[c], Vs0 = w.vs0[c], Rho0 = w.rho0[c];
-        const float gl = w.glam[c], gm = w.gmu[c], gr = w.grho[c];
+        const float gl = f.glam[c], gm = f.gmu[c], gr = f.grho[c];
         vp = (float)(2.0 * Rho0 * Vp0 * gl);
         vs = (float)((-4.0 * Vs0 * Rho0 * gl) + 2.0 * Rho0 * Vp0 * gm);
         rho = (float)((((double)(w.vp0[c] * w.vp0[c]) - (2.0 * Vs0 * Vs0)) * gl) + (double)(w.vs0[c] * w.vs0[c] * gm) + gr);
@@ -2088,9 +2115,9 @@ __global__ __launch_bounds__(BX *BY) void k_fwi_grad(Grid g, Fwi w, int idx, int
     w.gshot[c] = vp;
     w.gshot[g.plane + c] = vs;
     w.gshot[2 * g.plane + c] = rho;
-    w.glam[c] = 0.f;
-    w.gmu[c] = 0.f;
-    w.grho[c] = 0.f;
+    f.glam[c] = 0.f;
+    f.gmu[c] = 0.f;
+    f.grho[c] = 0.f;
 }
 
 } // namespace
@@ -2307,7 +2334,8 @@ struct sofi_dev {
     int *adj_inj_buf = nullptr;
     float *adj_sx = nullptr, *adj_sy = nullptr;
     int nadj = 0;
-    bool adj_at_rec = false;               // the adjoint arrays and plan are those of slot 0's receivers
+    std::vector<int> adj_start;            // first adjoint row of each slot (receivers slot after slot)
+    bool adj_at_rec = false;               // the adjoint arrays and plan are those of the slots' receivers
     bool adj_custom = false;               // the adjoint sources were set by sofi_fwi_set_adjoint
     float *rbuf = nullptr;                 // workspace of sofi_fwi_residual (measured data, filtered copies, shares)
     size_t rbuf_len = 0;
@@ -2502,7 +2530,6 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
     if (!desc || !out) return fail("sofi_open: null argument");
     if (nshots < 1 || nshots > 4096) return fail("sofi_open_batch: %d shot slots (1..4096)", nshots);
     const sofi_desc &d = *desc;
-    if (nshots > 1 && d.mode == SOFI_MODE_FWI) return fail("sofi_open_batch: MODE=FWI runs one shot per handle");
     if ((double)nshots * (double)sofi_plane(d.nx, d.ny) * (d.L > 0 ? d.L : 1) > 2147483647.0)
         return fail("sofi_open_batch: %d slots of a %d x %d grid exceed the 2^31-float index range", nshots, d.nx, d.ny);
     if (d.nx < 1 || d.ny < 1) return fail("sofi_open: bad grid %d x %d", d.nx, d.ny);
@@ -2637,10 +2664,6 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
             sofi_close(h);
             return fail("sofi_open: MODE=FWI needs an elastic or viscoelastic WEQ (calc_conv reads sxy)");
         }
-        if (nshots != 1) {
-            sofi_close(h);
-            return fail("sofi_open_batch: MODE=FWI runs one shot per handle");
-        }
         if (d.adjoint_type < 1 || d.adjoint_type > 3) {
             sofi_close(h);
             return fail("sofi_open: ADJOINT_TYPE=%d is not supported (1, 2 or 3)", d.adjoint_type);
@@ -2649,11 +2672,12 @@ int sofi_open_batch(const sofi_desc *desc, int nshots, sofi_dev **out)
         h->nstore = (d.nt + dtinv - 1) / dtinv;
         Fwi &w = h->w;
         w.velocity = d.velocity;
-        rc |= dalloc_t(h, &w.pvxp1, pl) | dalloc_t(h, &w.pvyp1, pl);
-        rc |= dalloc_t(h, &w.ux, pl) | dalloc_t(h, &w.uy, pl) | dalloc_t(h, &w.uxy, pl);
-        rc |= dalloc_t(h, &w.glam, pl) | dalloc_t(h, &w.gmu, pl) | dalloc_t(h, &w.grho, pl);
+        // per shot slot: the FWI planes of Fields and a store of its own
+        for (float **q : { &f.pvxp1, &f.pvyp1, &f.ux, &f.uy, &f.uxy, &f.glam, &f.gmu, &f.grho })
+            rc |= dalloc_t(h, q, pl * nb);
         rc |= dalloc_t(h, &w.gsum, 3 * pl) | dalloc_t(h, &w.gshot, 3 * pl);
-        rc |= dalloc_t(h, &w.store, (size_t)h->nstore * 5 * d.nx * d.ny);
+        w.sstride = (size_t)h->nstore * 5 * d.nx * d.ny;
+        rc |= dalloc_t(h, &w.store, w.sstride * nb);
     }
     if (rc) {
         std::string e = g_err;
@@ -3055,7 +3079,7 @@ int sync_acq(sofi_dev *h)
         h->gen++;
         // an FWI handle's adjoint sources sit at its receivers unless set
         // elsewhere: their arrays exist from the first forward pass on
-        if (d.mode == SOFI_MODE_FWI && h->nb == 1 && !h->adj_custom && adj_at_receivers(h)) return -1;
+        if (d.mode == SOFI_MODE_FWI && !h->adj_custom && adj_at_receivers(h)) return -1;
     }
     return 0;
 }
@@ -3098,8 +3122,8 @@ int sofi_reset(sofi_dev *h)
     for (float *p : { f.psi_syy_y, f.psi_sxy_y, f.psi_vyy, f.psi_vxy })
         if (p) HIPCHK(hipMemsetAsync(p, 0, nyp, h->stream));
     if (h->sec) HIPCHK(hipMemsetAsync(h->sec, 0, h->sec_len * sizeof(float), h->stream));
-    for (float *p : { h->w.pvxp1, h->w.pvyp1, h->w.ux, h->w.uy, h->w.uxy })
-        if (p) HIPCHK(hipMemsetAsync(p, 0, pl, h->stream));
+    for (float *p : { f.pvxp1, f.pvyp1, f.ux, f.uy, f.uxy })
+        if (p) HIPCHK(hipMemsetAsync(p, 0, pl * nb, h->stream));
     HIPCHK(hipMemsetAsync(f.unstable, 0, sizeof(int) * nb, h->stream));
     if (h->fdt4) {   // zero_wavefield clears every history (src/zero_wavefield.c:88-127)
         Hist4 &H = h->h4;
@@ -3347,7 +3371,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // a batch in slot blocks (h->bs slots per thread, all slots of the group
     // in one launch per kernel); sources, free surface and sampling then run
     // as launches of their own
-    const bool bm = h->nb > 1 && h->bs > 0 && !h->fdt4 && d.weq != SOFI_AC_ISO && d.mode == SOFI_MODE_FW;
+    const bool bm = h->nb > 1 && h->bs > 0 && !h->fdt4 && d.weq != SOFI_AC_ISO;
     const int span = hi0 - lo0, grp = !bm && h->grp > 0 && h->grp < span ? h->grp : span;
     const bool slot_fused = !bm && !tti && !h->fdt4 && h->slot_fusable && grp == 1 && nsrc > 0;
     fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
@@ -3358,11 +3382,6 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
               (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
     fs.fwi_u = store && d.velocity == 1;
-    fv.pvxp1 = fs.pvxp1 = h->w.pvxp1;
-    fv.pvyp1 = fs.pvyp1 = h->w.pvyp1;
-    fv.ux = fs.ux = h->w.ux;
-    fv.uy = fs.uy = h->w.uy;
-    fv.uxy = fs.uxy = h->w.uxy;
     fv.ntb = fs.ntb = ntb;
     fv.sblk = fs.sblk = 0;
     // vx / vy sampling rides on the stress launch of a one-shot handle
@@ -3428,19 +3447,21 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         }
         if (bm) {
             const int S = h->bs;
-            launch_blocks<ND, VelB<ND>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb);
+            launch_blocks<ND, VelB<ND>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fv.fwi_acc);
             wrap_x(h, gg, lo, gg.k_hi, f.vx, f.vy);
             if (te && lo == lo0 && grp == span) {
                 (void)hipEventRecord(te[1], st);
                 (void)hipEventRecord(te[2], st);
             }
             switch (d.weq) {
-              case SOFI_EL_ISO: launch_blocks<ND, StrB<ND, SOFI_EL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
-              case SOFI_VEL_ISO: launch_blocks<ND, StrB<ND, SOFI_VEL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
-              case SOFI_EL_VTI: launch_blocks<ND, StrB<ND, SOFI_EL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
-              case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb); break;
+              case SOFI_EL_ISO: launch_blocks<ND, StrB<ND, SOFI_EL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
+              case SOFI_VEL_ISO: launch_blocks<ND, StrB<ND, SOFI_VEL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
+              case SOFI_EL_VTI: launch_blocks<ND, StrB<ND, SOFI_EL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
+              case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
               default: {
-                  const Fuse f0{};
+                  Fuse f0{};
+                  f0.fwi_u = fs.fwi_u;
+                  f0.ntb = ntb;
                   if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0, h->s, nt);
                   else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0, h->s, nt);
                   tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, f0, nt);
@@ -3561,9 +3582,12 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     }
     if (!inblk && d.seismo && nt == h->lsamp && nt < d.nt) h->lsamp += d.ndt;
     if (store || conv) {
-        dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
-        if (store) launch(h, k_fwi_store, tg, tb, g, f, h->w, (nt - 1) / dtinv, ntb, dtinv);
-        else launch(h, k_fwi_conv, tg, tb, g, f, m, h->w, (fstep - 1) / dtinv, ntb, dtinv);
+        // every shot slot of the step (blockIdx.y), rows of tiles in blockIdx.z
+        Grid gs = g;
+        gs.k_lo = lo0;
+        dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, span, (g.ny + BY - 1) / BY);
+        if (store) launch(h, k_fwi_store, tg, tb, gs, f, h->w, (nt - 1) / dtinv, ntb, dtinv);
+        else launch(h, k_fwi_conv, tg, tb, gs, f, m, h->w, (fstep - 1) / dtinv, ntb, dtinv);
     }
     if (!inblk && !adj && d.snap && nt == h->lsnap && nt <= d.esnap) {
         if (h->nsnap < h->nsnap_max) {
@@ -3688,6 +3712,7 @@ int run_block(sofi_dev *h, int nt, int k)
 int resident_slots(const sofi_dev *h)
 {
     if (h->nb == 1) return 1;
+    if (h->d.mode == SOFI_MODE_FWI) return h->nb;     // the adjoint sources and the store span every slot
     if (h->res_mode > 0) return h->res_mode < h->nb ? h->res_mode : h->nb;
     const sofi_desc &d = h->d;
     const size_t pl = (size_t)h->g.plane * sizeof(float);
@@ -3786,8 +3811,19 @@ namespace {
 // receivers of the handle's shot, zero (what the device residuals fill)
 int adj_at_receivers(sofi_dev *h)
 {
-    const sofi_dev::Slot &sl = h->slots[0];
-    const int n = (int)sl.ri.size();
+    // the receivers of every slot, slot after slot; slot k's rows of the
+    // adjoint arrays start at adj_start[k]
+    std::vector<int> ri, rj, rs;
+    h->adj_start.assign(h->nb + 1, 0);
+    for (int k = 0; k < h->nb; ++k) {
+        const sofi_dev::Slot &sl = h->slots[k];
+        h->adj_start[k] = (int)ri.size();
+        ri.insert(ri.end(), sl.ri.begin(), sl.ri.end());
+        rj.insert(rj.end(), sl.rj.begin(), sl.rj.end());
+        rs.insert(rs.end(), sl.ri.size(), k);
+    }
+    const int n = (int)ri.size();
+    h->adj_start[h->nb] = n;
     if (h->adj_at_rec && h->nadj == n) return 0;
     HIPCHK(hipStreamSynchronize(h->stream));
     dfree(h, &h->adj_sx);
@@ -3796,7 +3832,7 @@ int adj_at_receivers(sofi_dev *h)
     if (dalloc_t(h, &h->adj_sx, len) || dalloc_t(h, &h->adj_sy, len)) return -1;
     HIPCHK(hipMemsetAsync(h->adj_sx, 0, len * sizeof(float), h->stream));
     HIPCHK(hipMemsetAsync(h->adj_sy, 0, len * sizeof(float), h->stream));
-    if (build_inj(h, n, sl.ri.data(), sl.rj.data(), nullptr, nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
+    if (build_inj(h, n, ri.data(), rj.data(), rs.data(), nullptr, &h->adj_inj, &h->adj_inj_buf)) return -1;
     h->nadj = n;
     h->adj_at_rec = true;
     h->adj_custom = false;
@@ -3829,12 +3865,12 @@ const char *residual_check(const sofi_dev *h, const sofi_residual_opts *o)
 {
     if (!h) return "null handle";
     if (!o) return "null options";
-    if (h->d.mode != SOFI_MODE_FWI || h->nb != 1) return "the handle was not opened with MODE=FWI";
+    if (h->d.mode != SOFI_MODE_FWI) return "the handle was not opened with MODE=FWI";
     if (o->lnorm < 1 || o->lnorm > 8 || o->lnorm == 6) return "LNORM has no residual in the reference (1-5, 7, 8)";
     if (o->time_filt && (o->order < 1 || o->order > 2 * SOFI_IIR_MAXSEC - 1 || !(o->f_low_pass > 0.0f) ||
                          o->f_high_pass < 0.0f))
         return "time filter: ORDER must be 1..17 and the corner frequencies positive";
-    if (h->rec_dirty || !h->sec || h->slots[0].ri.empty()) return "no synthetic sections: run the forward pass first";
+    if (h->rec_dirty || !h->sec || h->slots[h->sel].ri.empty()) return "no synthetic sections: run the forward pass first";
     if (h->d.ndt != 1) return "MODE=FWI needs NDT = 1";
     return nullptr;
 }
@@ -3844,10 +3880,25 @@ const char *residual_check(const sofi_dev *h, const sofi_residual_opts *o)
 // synthetic one the handle's section; TIME_FILT filters both first
 // (src/inversion.c:68-84).  The residuals go time-reversed into the
 // adjoint arrays, the components' misfits into the last two floats of ws.
+// rows of the selected slot in the adjoint arrays: its receivers' (first,
+// count), or all of a one-slot handle's custom adjoint sources
+void adj_rows(const sofi_dev *h, int *first, int *count)
+{
+    if (h->adj_at_rec && (int)h->adj_start.size() == h->nb + 1) {
+        *first = h->adj_start[h->sel];
+        *count = h->adj_start[h->sel + 1] - *first;
+    } else {
+        *first = 0;
+        *count = h->nadj;
+    }
+}
+
 int residual_core(sofi_dev *h, const sofi_residual_opts &o, const float *const obs[2], float *ws)
 {
     const sofi_desc &d = h->d;
-    const int ntr = h->nadj, ns = d.ns;
+    int first, ntr;
+    adj_rows(h, &first, &ntr);
+    const int ns = d.ns;
     const size_t nn = (size_t)ntr * ns;
     float *syncopy = ws, *tmp = ws + nn, *obsf = ws + 2 * nn, *partial = ws + 3 * nn, *mis = partial + ntr;
     hipStream_t st = h->stream;
@@ -3858,8 +3909,8 @@ int residual_core(sofi_dev *h, const sofi_residual_opts &o, const float *const o
         const bool used = c == 0 ? (d.adjoint_type == 1 || d.adjoint_type == 3)
                                  : (d.adjoint_type == 1 || d.adjoint_type == 2);
         if (!used) continue;
-        float *adj = c == 0 ? h->adj_sx : h->adj_sy;
-        const float *sy = h->sec + h->sec_base[0] + (size_t)c * nn, *ob = obs[c];
+        float *adj = (c == 0 ? h->adj_sx : h->adj_sy) + (size_t)first * d.nt;
+        const float *sy = h->sec + h->sec_base[h->sel] + (size_t)c * nn, *ob = obs[c];
         if (o.time_filt) {
             // timedomain_filt: the zero-phase low-pass, then the high-pass (src/timedomain_filt.c:57-111)
             const TFilt lp = tfilt(dt, 1.0 / (double)o.f_low_pass, o.order, 0);
@@ -3896,7 +3947,8 @@ int sofi_fwi_residual(sofi_dev *h, const sofi_residual_opts *o, const float *obs
     const bool ux = d.adjoint_type == 1 || d.adjoint_type == 3, uy = d.adjoint_type == 1 || d.adjoint_type == 2;
     if ((ux && !obs_x) || (uy && !obs_y)) return fail("sofi_fwi_residual: measured section missing for ADJOINT_TYPE=%d", d.adjoint_type);
     if (adj_at_receivers(h)) return -1;
-    const int ntr = h->nadj;
+    int first, ntr;
+    adj_rows(h, &first, &ntr);
     const size_t nn = (size_t)ntr * d.ns, need = 2 * nn + residual_ws_floats(ntr, d.ns);
     if (h->rbuf_len < need) {
         HIPCHK(hipStreamSynchronize(h->stream));
@@ -3924,12 +3976,21 @@ int sofi_fwi_get_adjoint(sofi_dev *h, int comp, float *out)
     if (comp != SOFI_C_VX && comp != SOFI_C_VY) return fail("sofi_fwi_get_adjoint: component %d (0 vx, 1 vy)", comp);
     if (!out) return fail("sofi_fwi_get_adjoint: null output");
     const float *a = comp == SOFI_C_VX ? h->adj_sx : h->adj_sy;
+    int first, n;
+    adj_rows(h, &first, &n);
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (a && h->nadj > 0) HIPCHK(hipMemcpy(out, a, (size_t)h->nadj * h->d.nt * sizeof(float), hipMemcpyDeviceToHost));
+    if (a && n > 0)
+        HIPCHK(hipMemcpy(out, a + (size_t)first * h->d.nt, (size_t)n * h->d.nt * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 
-int sofi_fwi_adjoint_count(const sofi_dev *h) { return h ? h->nadj : 0; }
+int sofi_fwi_adjoint_count(const sofi_dev *h)
+{
+    if (!h) return 0;
+    int first, n;
+    adj_rows(h, &first, &n);
+    return n;
+}
 
 } // extern "C"
 
@@ -3995,10 +4056,12 @@ int sofi_fwi_residual_noalloc(sofi_dev *h, const sofi_residual_opts *o, const fl
         return fail("sofi_fwi_residual_noalloc: measured section missing for ADJOINT_TYPE=%d", d.adjoint_type);
     if ((ux && !device_pointer(obs_x)) || (uy && !device_pointer(obs_y)) || (misfit && !device_pointer(misfit)))
         return fail("sofi_fwi_residual_noalloc: the measured sections and the misfit must be device memory");
-    if (!h->adj_at_rec || h->nadj != (int)h->slots[0].ri.size())
+    if (!h->adj_at_rec || (int)h->adj_start.size() != h->nb + 1 ||
+        h->adj_start[h->sel + 1] - h->adj_start[h->sel] != (int)h->slots[h->sel].ri.size())
         return fail("sofi_fwi_residual_noalloc: the adjoint sources are not those of the receivers (set by "
                     "sofi_fwi_set_adjoint or receivers changed): call sofi_fwi_residual once first");
-    const int ntr = h->nadj;
+    int first, ntr;
+    adj_rows(h, &first, &ntr);
     const size_t need = residual_ws_floats(ntr, d.ns);
     if (need > SCRATCH_FLOATS)
         return fail("sofi_fwi_residual_noalloc: %d traces of %d samples need %zu floats of scratch, more than its "
@@ -4074,11 +4137,11 @@ int run_groups(sofi_dev *h, int nt_first, int nt_last)
 // the configuration.
 
 struct TuneKey {
-    int weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic;
+    int weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic, mode;
     bool operator<(const TuneKey &o) const
     {
-        return std::tie(weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic) <
-               std::tie(o.weq, o.nd, o.nx, o.ny, o.nb, o.abs_type, o.free_surf, o.L, o.periodic);
+        return std::tie(weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic, mode) <
+               std::tie(o.weq, o.nd, o.nx, o.ny, o.nb, o.abs_type, o.free_surf, o.L, o.periodic, o.mode);
     }
 };
 std::mutex g_tune_mu;
@@ -4101,8 +4164,8 @@ void apply_cfg(sofi_dev *h, const sofi_dev::BatchCfg &c)
 int batch_tune(sofi_dev *h, int *nt, int nt_last)
 {
     const sofi_desc &d = h->d;
-    if (h->nb == 1 || !h->tune_on || h->tuned || d.mode != SOFI_MODE_FW || h->fdt4) return 0;
-    const TuneKey key{ d.weq, h->g.nd, d.nx, d.ny, h->nb, d.abs_type, d.free_surf, h->g.L, h->g.periodic };
+    if (h->nb == 1 || !h->tune_on || h->tuned || h->fdt4) return 0;
+    const TuneKey key{ d.weq, h->g.nd, d.nx, d.ny, h->nb, d.abs_type, d.free_surf, h->g.L, h->g.periodic, d.mode };
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -4341,9 +4404,9 @@ size_t sofi_estimate_footprint(const sofi_desc *d, int nshots, int ntr, int nsna
     }
     if (d->mode == SOFI_MODE_FWI) {
         const size_t dtinv = d->dtinv > 0 ? (size_t)d->dtinv : 1, nstore = ((size_t)d->nt + dtinv - 1) / dtinv;
-        b += 8 * (pl + A) + 2 * (3 * pl + A) + 5 * (pl + A);
-        b += nstore * 5 * (size_t)d->nx * d->ny * 4 + A;
-        b += 2 * ((size_t)(ntr > 0 ? ntr : 1) * d->nt * 4 + A);
+        b += 8 * (pl * nb + A) + 2 * (3 * pl + A) + 5 * (pl + A);
+        b += nstore * 5 * (size_t)d->nx * d->ny * 4 * nb + A;
+        b += 2 * ((size_t)(ntr > 0 ? ntr : 1) * nb * d->nt * 4 + A);
     }
     b += (size_t)1 << 20;                                                     // sources and small buffers
     return b;
@@ -4391,6 +4454,7 @@ int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *ai, const int *aj, const
 {
     if (!h) return fail("sofi_fwi_set_adjoint: null handle");
     if (h->d.mode != SOFI_MODE_FWI) return fail("sofi_fwi_set_adjoint: the handle was not opened with MODE=FWI");
+    if (h->nb != 1) return fail("sofi_fwi_set_adjoint: one-slot handles only (batches: sofi_fwi_residual per slot)");
     if (n < 0 || (n > 0 && (!ai || !aj || !sig_x || !sig_y))) return fail("sofi_fwi_set_adjoint: bad arguments");
     for (int k = 0; k < n; ++k)
         if (ai[k] < 1 || ai[k] > h->d.nx || aj[k] < 1 || aj[k] > h->d.ny)
@@ -4418,8 +4482,8 @@ int sofi_fwi_gradient(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_r
     if (h->d.mode != SOFI_MODE_FWI || !h->model_set) return fail("sofi_fwi_gradient: no FWI model set");
     const Grid &g = h->g;
     dim3 tb(BX, BY), tg((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY);
-    hipLaunchKernelGGL(k_fwi_grad, tg, tb, 0, h->stream, g, h->w, h->d.idx > 0 ? h->d.idx : 1,
-                       h->d.idy > 0 ? h->d.idy : 1);
+    hipLaunchKernelGGL(k_fwi_grad, tg, tb, 0, h->stream, g, h->f, h->w, h->d.idx > 0 ? h->d.idx : 1,
+                       h->d.idy > 0 ? h->d.idy : 1, h->sel);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     const size_t pl = (size_t)g.plane;

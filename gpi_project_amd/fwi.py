@@ -24,7 +24,7 @@ import time
 
 import numpy as np
 
-from .sofi import Problem, Simulation
+from .sofi import Batch, Problem, Simulation
 
 VX, VY = 0, 1
 
@@ -75,7 +75,7 @@ def shot_gradient(prob: Problem, sim: Simulation, ishot: int, write: bool = True
 
 
 def gradient_json(json_path: str, verbose: bool = True, write: bool = True, workers: int = 1,
-                  sections: bool = True, host_residual: bool = False) -> dict:
+                  sections: bool = True, host_residual: bool = False, batch: int = 1) -> dict:
     """Every iteration and shot of a MODE=FWI parameter file; returns the
     per-shot results of the last iteration and the summed gradients.
 
@@ -83,7 +83,15 @@ def gradient_json(json_path: str, verbose: bool = True, write: bool = True, work
     handle and HIP stream (one host thread each), so that the kernels of
     several shots overlap on the device; the shots' float32 gradients are
     then summed in shot order, the order of the one-handle path, which gives
-    the same sums bit for bit."""
+    the same sums bit for bit.
+
+    batch > 1 advances groups of that many shots as the slots of one batch
+    handle sharing the medium (each update launch covers every slot of the
+    group: forward passes together, residuals slot by slot, adjoint passes
+    together); with workers > 1 as well, that many such handles run at once.
+    The residuals are formed on the device."""
+    if batch > 1:
+        return _gradient_json_batched(json_path, verbose, write, workers, sections, batch)
     if workers > 1:
         return _gradient_json_concurrent(json_path, verbose, write, workers, sections, host_residual)
     prob = Problem(json_path)
@@ -146,6 +154,94 @@ def _gradient_json_concurrent(json_path: str, verbose: bool, write: bool, worker
             finally:
                 if sim is not None:
                     sim.close()
+                if prob is not None:
+                    prob.close()
+
+        threads = [threading.Thread(target=worker) for _ in range(workers)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            probe.close()
+            raise errors[0]
+    gsum = [np.zeros(probe.plane, dtype=np.float32) for _ in range(3)]
+    for ishot in range(1, nshots + 1):
+        for q in range(3):
+            gsum[q] = gsum[q] + shots[ishot]["gradient"][q]
+    if write:
+        probe.write_gradient(0, *gsum)
+    probe.close()
+    return {"shots": shots, "sum": gsum}
+
+
+def _gradient_json_batched(json_path: str, verbose: bool, write: bool, workers: int, sections: bool,
+                           batch: int) -> dict:
+    import threading
+    probe = Problem(json_path)
+    if probe.desc.mode != 0:
+        probe.close()
+        raise ValueError(f"{json_path}: MODE is not FWI")
+    nshots, itermax = probe.nshots, probe.itermax
+    batch = min(batch, nshots)
+    groups_all = [list(range(a, min(a + batch, nshots + 1))) for a in range(1, nshots + 1, batch)]
+    workers = max(1, min(workers, len(groups_all)))
+    lock = threading.Lock()
+    shots: dict = {}
+    errors: list = []
+    for it in range(1, itermax + 1):
+        queue = list(groups_all)
+
+        def worker():
+            prob = B = None
+            try:
+                prob = Problem(json_path)
+                B = Batch(prob, batch)
+                B.fwi_set_model(prob.model(0), prob.model(1), prob.model(2))
+                d = prob.desc
+                while True:
+                    with lock:
+                        if not queue or errors:
+                            return
+                        group = queue.pop(0)
+                    t0 = time.perf_counter()
+                    B.fwi_set_pass(0)
+                    B.load_shots(group)
+                    ms_fwd = B.run()
+                    res = {}
+                    for k, ishot in enumerate(group):
+                        B.select(k)
+                        prob.set_shot(ishot)
+                        secs = {}
+                        if write or sections:
+                            secs = {c: B.section(c) for c in prob.seismo_comps()}
+                        if write:
+                            for c, a in secs.items():
+                                prob.write_section(c, a)
+                        obs_x = prob.read_data(VX, B.ntr) if d.adjoint_type in (1, 3) else None
+                        obs_y = prob.read_data(VY, B.ntr) if d.adjoint_type in (1, 2) else None
+                        res[ishot] = {"sections": secs, "l2": B.fwi_residual(obs_x, obs_y)}
+                    B.fwi_set_pass(1)
+                    B.select(0)
+                    B.reset()
+                    ms_adj = B.run()
+                    for k, ishot in enumerate(group):
+                        B.select(k)
+                        g = B.fwi_gradient()
+                        if write:
+                            prob.write_gradient(ishot, *g)
+                        res[ishot].update({"gradient": tuple(g), "ms_forward": ms_fwd / len(group),
+                                           "ms_adjoint": ms_adj / len(group), "s": time.perf_counter() - t0})
+                    with lock:
+                        shots.update(res)
+                    if verbose:
+                        print(f"iteration {it} shots {group}: forward {ms_fwd:.1f} ms, adjoint {ms_adj:.1f} ms")
+            except Exception as e:  # noqa: BLE001
+                with lock:
+                    errors.append(e)
+            finally:
+                if B is not None:
+                    B.close()
                 if prob is not None:
                     prob.close()
 

@@ -372,6 +372,7 @@ class Batch(Simulation):
         zero wavefield in every slot."""
         assert len(shots) <= self.nslots
         p = self.problem
+        self._keep = []
         for k, ishot in enumerate(shots):
             p.set_shot(ishot)
             _chk_hip(self._L.sofi_select_shot(self._h, k))

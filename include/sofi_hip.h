@@ -46,7 +46,9 @@ int sofi_open(const sofi_desc *desc, sofi_dev **out);
  * src/sofi2D.c:320-395).  sofi_select_shot picks the slot that
  * sofi_set_sources / sofi_set_receivers / sofi_get_section / sofi_get_field
  * address (slot 0 after opening); sofi_reset clears every slot.  Snapshots
- * and MODE=FWI need one-slot handles.  The update kernels run over groups of
+ * need one-slot handles; under MODE=FWI each slot keeps its own forward
+ * store and gradient accumulators, and the residual and gradient calls
+ * address the selected slot.  The update kernels run over groups of
  * `group` slots per launch and `spt` slots per thread (the CELL shape reads
  * a cell's material once for them); sofi_set_batch_launch fixes them (0:
  * measured on the first steps of the next run, the default; the results do
@@ -62,8 +64,8 @@ int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
  * slot is updated; sources, free surface and sampling run as launches of
  * their own.  TTI stress updates then compute the velocity derivatives in
  * the kernel for the same slot blocks.  0 (the default) keeps the slot-loop
- * kernels of sofi_set_batch_launch.  Not for AC_ISO, 4th order in time or
- * MODE=FWI (the slot-loop kernels run).  The results do not depend on it.
+ * kernels of sofi_set_batch_launch.  Not for AC_ISO or 4th order in time
+ * (the slot-loop kernels run).  The results do not depend on it.
  * No counterpart in the reference. */
 int sofi_set_batch_block(sofi_dev *h, int slots);
 int sofi_get_batch_block(const sofi_dev *h);
@@ -157,7 +159,7 @@ size_t sofi_footprint(const sofi_dev *h);
 size_t sofi_estimate_footprint(const sofi_desc *desc, int nshots, int ntr, int nsnap);
 
 /* FWI gradient path (MODE=FWI; EL_ISO, VEL_ISO, EL_VTI, VEL_VTI, EL_TTI,
- * VEL_TTI; one-slot handles): what time_loop does with sw = 0 / 1 and
+ * VEL_TTI; one-slot handles or batches): what time_loop does with sw = 0 / 1 and
  * MemInv (src/time_loop.c:27-326 with src/snap_store.c:27-47,
  * src/calc_conv.c:30-73), driven per shot as src/inversion.c:27-258 does.
  * The viscoelastic and anisotropic stress updates write the strain rates
@@ -168,15 +170,18 @@ size_t sofi_estimate_footprint(const sofi_desc *desc, int nshots, int ntr, int n
  *                         stores the forward fields every DTINV steps;
  *                         sw = 1: the adjoint pass, correlating with them
  *   sofi_fwi_set_adjoint  adjoint sources: the receivers and the time-reversed
- *                         residuals [n][nt] in x and y (ADJOINT_TYPE picks)
- *   sofi_fwi_gradient     the shot's vp / vs / rho' gradients (padded planes,
- *                         NULL to skip), added to the sums; clears the shot's
- *                         accumulators (src/inversion.c:226-249)
+ *                         residuals [n][nt] in x and y (ADJOINT_TYPE picks;
+ *                         one-slot handles)
+ *   sofi_fwi_gradient     the selected slot's vp / vs / rho' gradients (padded
+ *                         planes, NULL to skip), added to the sums; clears the
+ *                         slot's accumulators (src/inversion.c:226-249)
  *   sofi_fwi_gradient_sum the sums over shots; reset != 0 clears them
  *                         (init_inv, src/init_inv.c:20-25)
  * At DTINV = 1 the store and correlation steps are the reference's; with
  * DTINV > 1 the reference indexes its store past its end, and the forward
- * step s is kept in slot (s - 1) / DTINV. */
+ * step s is kept in store slot (s - 1) / DTINV.  In a batch every shot slot
+ * runs both passes in the same launches (the reference runs its shots one
+ * after the other, src/sofi2D.c:320-395). */
 int sofi_fwi_set_model(sofi_dev *h, const float *vp0, const float *vs0, const float *rho0);
 int sofi_fwi_set_pass(sofi_dev *h, int sw);
 int sofi_fwi_set_adjoint(sofi_dev *h, int n, const int *i, const int *j, const float *sig_x, const float *sig_y);
@@ -184,11 +189,11 @@ int sofi_fwi_gradient(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_r
 int sofi_fwi_gradient_sum(sofi_dev *h, float *grad_vp, float *grad_vs, float *grad_rho, int reset);
 
 /* The residuals of calc_res on the device (src/calc_res.c:127-301, every
- * LNORM it has: 1-5, 7, 8), from the shot's synthetic vx / vy sections as the
+ * LNORM it has: 1-5, 7, 8), from the selected slot's synthetic vx / vy sections as the
  * forward pass left them in device memory and the measured sections obs_x /
  * obs_y ([ntr][ns] host arrays in receiver order; NULL for a component that
  * ADJOINT_TYPE does not use), time-reversed into the adjoint sources at the
- * handle's receivers: what inversion.c hands the adjoint pass
+ * slot's receivers: what inversion.c hands the adjoint pass
  * (src/inversion.c:86-110).  With time_filt set both sections are first
  * filtered as timedomain_filt does (src/timedomain_filt.c:28-113: zero-phase
  * Butterworth low-pass, then high-pass when f_high_pass > 0).  *misfit (may

@@ -206,3 +206,31 @@ def test_fwi_concurrent_shots_equal_one_handle():
     for q in range(3):
         assert np.abs(a["sum"][q]).max() > 0
         assert np.array_equal(a["sum"][q], b["sum"][q]), q
+
+
+@pytest.mark.parametrize("name,dtinv", [("fwi_el_v1_y", 5), ("fwi_el_v0", 1), ("fwi_vel_l2", 1), ("fwi_eltti", 1)])
+def test_fwi_batched_shots(name, dtinv):
+    """Shots as the slots of one batch handle (forward and adjoint passes of
+    both shots in the same launches, residuals slot by slot): each shot's
+    gradient equals the one-handle gradient to rounding, and the sums match
+    the reference build's files (at DTINV 1) like the one-handle path's."""
+    from gpi_project_amd import fwi
+    z, td, js = _setup(name, dtinv=dtinv)
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    with chdir(td):
+        a = fwi.gradient_json(os.path.basename(js), verbose=False, write=False)
+        b = fwi.gradient_json(os.path.basename(js), verbose=False, write=False, batch=2)
+    for s in a["shots"]:
+        assert np.array_equal(a["shots"][s]["sections"][0], b["shots"][s]["sections"][0]) or \
+            rel_l2(b["shots"][s]["sections"][0], a["shots"][s]["sections"][0]) <= 1e-5
+        for q in range(3):
+            ga, gb = a["shots"][s]["gradient"][q], b["shots"][s]["gradient"][q]
+            assert np.abs(gb).max() > 0
+            cs = common.cosine(gb, ga)
+            print(f"{name} DTINV {dtinv} shot {s} q {q}: batch vs one handle cosine {cs:.8f}")
+            assert cs >= COS_ORACLE, (s, q, cs)
+    for k, p in enumerate(("vp", "vs", "rho")):
+        if dtinv == 1:
+            g = common.xmajor(b["sum"][k], nx, ny)
+            assert common.cosine(g, z[f"grad_{p}_sum"]) >= COS_REF, p
+        assert common.cosine(b["sum"][k], a["sum"][k]) >= COS_ORACLE, p
