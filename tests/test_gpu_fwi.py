@@ -208,7 +208,7 @@ def test_fwi_concurrent_shots_equal_one_handle():
         assert np.array_equal(a["sum"][q], b["sum"][q]), q
 
 
-@pytest.mark.parametrize("name,dtinv", [("fwi_el_v1_y", 5), ("fwi_el_v0", 1), ("fwi_vel_l2", 1), ("fwi_eltti", 1)])
+@pytest.mark.parametrize("name,dtinv", [("fwi_el_v1_y", 5), ("fwi_el_v0", 1)] + [(n, 1) for n in cases.FWI_WEQ])
 def test_fwi_batched_shots(name, dtinv):
     """Shots as the slots of one batch handle (forward and adjoint passes of
     both shots in the same launches, residuals slot by slot): each shot's
