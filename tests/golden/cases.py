@@ -391,6 +391,11 @@ FWI_WEQ = {"fwi_vel_l2": "VEL_ISO", "fwi_elvti": "EL_VTI", "fwi_eltti": "EL_TTI"
            "fwi_veltti_l2": "VEL_TTI"}
 FWI_LNORMS = {1: ("fwi_el_v0", 1.0), 3: ("fwi_el_v0", 1.0), 4: ("fwi_el_v1_y", 1.0), 5: ("fwi_el_v1_y", 1e6),
               7: ("fwi_el_v0", 1e6), 8: ("fwi_el_v1_y", 1.0)}
+# LNORM 1 with measured data from sources of twice the synthetic amplitude:
+# measured - synthetic is then about the synthetic trace itself, so its sign
+# does not hinge on rounding wherever a trace is not near zero
+# (case, synthetic amplitude, measured amplitude)
+FWI_LNORM1_CLEAR = ("fwi_el_v0", 1.0, 2.0)
 
 
 def _fwi_models(nx=FWI_NX, ny=FWI_NY, dh=FWI_DH):

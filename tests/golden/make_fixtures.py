@@ -25,10 +25,11 @@ has the reference at /root/reference and its build in oracle/_ref).
 8. ref1/fwi_tf*.npz, fwi_workflow.npz: FWI gradients with TIME_FILT 1, 2 and a workflow.
 9. ref1/acq_*.npz: wavelet shapes, plane wave, receiver array and line, ASCII formats.
 11. ref1/fwi_vel_l2.npz ... fwi_veltti_l2.npz: FWI gradients on the other rheologies.
+12. ref1/fwi_lnorm1_clear.npz: LNORM 1 with measured data of twice the synthetic amplitude.
 10. ref1/c1.npz, c3_*.npz, c2_800.npz, c5_small.npz: the benchmark workloads
     (config 1 in full; the others at the sizes given in configs()).
 
-Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq] [configs] [fwi_weq]
+Usage:  python tests/golden/make_fixtures.py [examples] [ref1] [c1] [fwi] [signals] [snapranks] [fwi_lnorm] [fwi_eprecond] [fwi_tfilt] [acq] [configs] [fwi_weq] [fwi_lnorm1_clear]
 """
 from __future__ import annotations
 
@@ -366,6 +367,29 @@ def fwi_lnorm():
             print("ref1", name, "LNORM", ln, flush=True)
 
 
+def fwi_lnorm1_clear():
+    """LNORM 1 gradient files of the reference build with measured data from
+    sources of twice the synthetic amplitude (cases.FWI_LNORM1_CLEAR)."""
+    out = os.path.join(HERE, "ref1")
+    nx, ny = cases.FWI_NX, cases.FWI_NY
+    name, amp, amp_true = cases.FWI_LNORM1_CLEAR
+    if os.path.exists(os.path.join(out, "fwi_lnorm1_clear.npz")):
+        return
+    with tempfile.TemporaryDirectory() as td:
+        run_ref(cases.fwi_case(td, name, "true", amp=amp_true), td)
+        r = {}
+        for s in (1, 2):
+            for comp in ("vx", "vy"):
+                r[f"data_{comp}_shot{s}"] = read_su(os.path.join(td, "data", f"obs_{comp}.su.shot{s}"))
+        run_ref(cases.fwi_case(td, name, "init", lnorm=1, amp=amp), td)
+        for p in ("vp", "vs", "rho"):
+            for s in (1, 2):
+                r[f"grad_{p}_shot{s}"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}.shot{s}"), np.float32).reshape(nx, ny)
+            r[f"grad_{p}_sum"] = np.fromfile(os.path.join(td, "grad", f"g_gradient.{p}"), np.float32).reshape(nx, ny)
+        np.savez_compressed(os.path.join(out, "fwi_lnorm1_clear.npz"), **r)
+        print("ref1", name, "LNORM 1 with measured amplitude", amp_true, flush=True)
+
+
 def fwi_weq():
     """FWI gradients of the reference build on the other rheologies
     (tests/golden/cases.py FWI_WEQ): measured data and the summed gradients."""
@@ -422,8 +446,8 @@ def fwi_tfilt():
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["examples", "ref1", "c1", "fwi", "signals", "snapranks", "fwi_lnorm", "fwi_eprecond",
-                            "fwi_tfilt", "acq", "configs", "fwi_weq"]
+                            "fwi_tfilt", "acq", "configs", "fwi_weq", "fwi_lnorm1_clear"]
     for w in what:
         {"examples": examples, "ref1": ref1, "c1": c1, "fwi": fwi, "signals": signals, "snapranks": snapranks,
          "fwi_lnorm": fwi_lnorm, "fwi_eprecond": fwi_eprecond, "fwi_tfilt": fwi_tfilt, "acq": acq, "configs": configs,
-         "fwi_weq": fwi_weq}[w]()
+         "fwi_weq": fwi_weq, "fwi_lnorm1_clear": fwi_lnorm1_clear}[w]()
