@@ -4133,8 +4133,9 @@ int run_groups(sofi_dev *h, int nt_first, int nt_last)
 // kept for every later run of the handle and for every later batch of the
 // same rheology, FD order, grid, slot count and boundaries in the process.
 // Each candidate advances all slots by one step untimed (code loading, the
-// caches filled) and TUNE_STEPS timed steps; the results do not depend on
-// the configuration.
+// caches filled) and TUNE_STEPS timed steps.  The candidates run different
+// kernels (slot loops, slot blocks of 1, 2 or 4 slots), whose results agree
+// to rounding: the choice can change a result at that level.
 
 struct TuneKey {
     int weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic, mode;

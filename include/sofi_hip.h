@@ -65,8 +65,9 @@ int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
  * their own.  TTI stress updates then compute the velocity derivatives in
  * the kernel for the same slot blocks.  0 (the default) keeps the slot-loop
  * kernels of sofi_set_batch_launch.  Not for AC_ISO or 4th order in time
- * (the slot-loop kernels run).  The results do not depend on it.
- * No counterpart in the reference. */
+ * (the slot-loop kernels run).  The slot-block kernels are code of their
+ * own: their results agree with the slot-loop kernels' to rounding, not
+ * always bit for bit.  No counterpart in the reference. */
 int sofi_set_batch_block(sofi_dev *h, int slots);
 int sofi_get_batch_block(const sofi_dev *h);
 /* Resident groups of a batch: sofi_run advances `slots` slots through its
@@ -92,7 +93,9 @@ int sofi_set_batch_streams(sofi_dev *h, int streams);
  * of the same rheology, FD order, grid, slot count and boundaries in the
  * process.  Setting sofi_set_batch_block, or a non-zero value through
  * sofi_set_batch_launch / _resident / _streams, fixes the launch and turns
- * it off.  The results do not depend on it.  sofi_get_batch_config returns
+ * it off.  Its candidates run different kernels, so its choice can change
+ * a result at the rounding level; fixing the launch gives bit-for-bit
+ * repeatable runs.  sofi_get_batch_config returns
  * {slots per thread of the slot blocks (0: slot-loop kernels), resident
  * group, streams, slots per launch, slots per thread, tuned (0 / 1)}.  No
  * counterpart in the reference. */
