@@ -174,6 +174,8 @@ def hip():
     L.sofi_fwi_residual.argtypes = [_VP, C.POINTER(ResidualOpts), _FP, _FP, _DP]
     L.sofi_fwi_get_adjoint.argtypes = [_VP, C.c_int, _FP]
     L.sofi_fwi_adjoint_count.argtypes = [_VP]
+    L.sofi_run_many.argtypes = [C.POINTER(_VP), C.c_int, _IP, _IP, _IP]
+    L.sofi_get_many_config.argtypes = [_IP, _IP, _IP]
     _hip = L
     return L
 
@@ -201,5 +203,5 @@ def exported_symbols():
                            "sofi_get_batch_resident", "sofi_set_batch_streams", "sofi_set_batch_block",
                            "sofi_get_batch_block", "sofi_fwi_residual", "sofi_fwi_get_adjoint",
                            "sofi_fwi_adjoint_count", "sofi_fwi_residual_noalloc", "sofi_set_batch_tuning",
-                           "sofi_get_batch_config"],
+                           "sofi_get_batch_config", "sofi_run_many", "sofi_get_many_config"],
     }

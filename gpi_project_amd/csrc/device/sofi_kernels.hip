@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -664,10 +665,11 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
 // single slot of a one-shot handle, without the slot loop, whose run-time
 // trip count and per-slot offsets cost the one-shot kernels 10-45 % on
 // MI355X)
-template <int ND, bool ONE, bool NT = ONE>
-__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+template <int ND, bool ONE, bool NT>
+__device__ __forceinline__ void velocity_tile(const Grid &g, const Fields &f0, const Mat &m, const Src &s,
+                                              const Fuse &fz, int nt, int t)
 {
-    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
     const int j = 1 + ty * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -683,6 +685,11 @@ __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, S
         st1<NT>(f.vx, c, vx);
         st1<NT>(f.vy, c, vy);
     }
+}
+template <int ND, bool ONE, bool NT = ONE>
+__global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+{
+    velocity_tile<ND, ONE, NT>(g, f0, m, s, fz, nt, tile_id());
 }
 
 // VW cells per thread: the x stencils come from 2G+1 vectors of the row,
@@ -937,11 +944,11 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
 // cell's material coefficients read once (ONE: a one-shot handle's single
 // slot, as in k_velocity).
 template <int ND, int WEQ, bool ONE>
-__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+__device__ __forceinline__ void stress_tile(const Grid &g, const Fields &f0, const Mat &m, const Src &s,
+                                            const Fuse &fz, int nt, int t)
 {
-    if (ONE && fused_seis(g, f0, fz)) return;
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
-    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
     const int j = 1 + ty * BY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
@@ -1024,6 +1031,61 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
         st1<ONE>(f.sxx, c, sxx);
         st1<ONE>(f.syy, c, syy);
         st1<ONE>(f.sxy, c, sxy);
+    }
+}
+template <int ND, int WEQ, bool ONE>
+__global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+{
+    if (ONE && fused_seis(g, f0, fz)) return;
+    stress_tile<ND, WEQ, ONE>(g, f0, m, s, fz, nt, tile_id());
+}
+
+// ------------------------------------------------------ work-list launches
+
+// One handle's part of a work-list launch: its grid, fields, medium and
+// sources, and what its velocity / stress updates fuse (forces; explosive
+// sources and the free surface).  Fuse::ntb points at the handle's step
+// before the first step of the launch sequence, so that local step `it` of a
+// launch is the handle's step *ntb + it.
+struct Shot {
+    Grid g;
+    Fields f;
+    Mat m;
+    Src s;
+    Fuse fv, fs;
+};
+
+// The one-thread-per-cell update kernels over several one-shot handles on
+// grids of any size, one launch for all of them: item w = (handle, tile)
+// covers `rows` tiles down the tile column from that tile; block b takes
+// items b, b + gridDim.x, ... (a persistent grid when gridDim.x is below the
+// item count).  Items 0..n-1 belong to the handles that still advance (the
+// handles with the most steps come first).  Every tile runs the code of the
+// one-shot kernel, so each handle's results are those of its own launches,
+// bit for bit, whatever the grid size and rows.
+template <int ND, bool NT>
+__global__ __launch_bounds__(BX *BY) void k_velocity_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
+                                                       int n, int rows, int it)
+{
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int2 e = item[w];
+        const Shot &d = sh[e.x];
+        const int last = d.g.ntx * d.g.nty;
+        for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
+            velocity_tile<ND, true, NT>(d.g, d.f, d.m, d.s, d.fv, it, t);
+    }
+}
+
+template <int ND, int WEQ>
+__global__ __launch_bounds__(BX *BY) void k_stress_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
+                                                     int n, int rows, int it)
+{
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int2 e = item[w];
+        const Shot &d = sh[e.x];
+        const int last = d.g.ntx * d.g.nty;
+        for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
+            stress_tile<ND, WEQ, true>(d.g, d.f, d.m, d.s, d.fs, it, t);
     }
 }
 
@@ -2378,6 +2440,18 @@ struct sofi_dev {
     std::vector<hipEvent_t> xev;
     hipEvent_t evfork = nullptr;
     std::vector<int> src_inj_start, psrc_inj_start, rec_start;
+    // sofi_run_many: while a step of the handle is recorded for the work-list
+    // launches (wl), its update kernels' arguments go to wl_shot and its other
+    // launches to *defer, tagged with the stage they belong to (0 before the
+    // velocity update, 1 between the updates, 2 after the stress update)
+    struct Deferred {
+        int stage;
+        std::function<hipError_t(hipStream_t)> run;
+    };
+    bool wl = false;
+    Shot wl_shot{};
+    std::vector<Deferred> *defer = nullptr;
+    int dstage = 0;
 };
 
 namespace {
@@ -3215,6 +3289,19 @@ template <typename... P, typename... A>
 void launch(sofi_dev *h, void (*k)(P...), dim3 grid, dim3 block, A &&...a)
 {
     using Tup = std::tuple<std::decay_t<P>...>;
+    if (h->defer) {
+        // a step recorded for the work-list launches: issued later, in its stage
+        auto v = std::make_shared<Tup>(std::forward<A>(a)...);
+        h->defer->push_back({h->dstage, [k, grid, block, v](hipStream_t st) {
+            void *args[sizeof...(P) > 0 ? sizeof...(P) : 1];
+            std::apply([&](auto &...x) {
+                int q = 0;
+                ((args[q++] = (void *)&x), ...);
+            }, *v);
+            return hipLaunchKernel(reinterpret_cast<const void *>(k), grid, block, args, 0, st);
+        }});
+        return;
+    }
     if (!h->gbuild) {
         Tup v(std::forward<A>(a)...);
         void *args[sizeof...(P) > 0 ? sizeof...(P) : 1];
@@ -3378,7 +3465,10 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // the free surface follows the explosive / moment sources, so it is fused
     // when they are, or when there are none to inject this step
     const bool no_stress_src = nsrc == 0 || (h->psrc_inj.n == 0 && h->n_eq == 0);
-    fs.surf = !bm && !tti && (src_fused || slot_fused || no_stress_src) && d.free_surf && !sample_dv &&
+    // (in a work-list run, not in any step of a handle that samples div/curl,
+    // so that its update kernels' arguments are the same in every step)
+    const bool dv_any = h->wl ? d.seismo == 3 || d.seismo == 4 : sample_dv;
+    fs.surf = !bm && !tti && (src_fused || slot_fused || no_stress_src) && d.free_surf && !dv_any &&
               (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     fv.fwi_acc = store ? 1 : adj ? (d.velocity == 0 ? 2 : (conv ? 3 : 0)) : 0;
     fs.fwi_u = store && d.velocity == 1;
@@ -3386,14 +3476,14 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     fv.sblk = fs.sblk = 0;
     // vx / vy sampling rides on the stress launch of a one-shot handle
     // (SEISMO = 1, no TTI; the sampler addresses slot 0 of the handle's fields)
-    const bool seis_fused = sample && d.seismo == 1 && !tti && h->nb == 1;
+    const bool seis_fused = sample && d.seismo == 1 && !tti && h->nb == 1 && !h->wl;
     if (nsrc > 0 && h->any_force && !fv.nforce && !slot_fused && src_inj.n > 0)
         launch(h, k_force, dim3((src_inj.n + 63) / 64), dim3(64), g, f, m, h->s, src_inj, nt, ntb);
     if (adj && h->nadj > 0)
         launch(h, k_adjoint, dim3((h->adj_inj.n + 63) / 64), dim3(64), g, f, h->adj_inj, h->adj_sx,
                            h->adj_sy, d.nt, d.adjoint_type, nt, ntb);
     hipEvent_t *te = nullptr;
-    if (!inblk && h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
+    if (!inblk && !h->wl && h->time_every > 0 && nt % h->time_every == 0 && h->tev_used + 4 <= (int)h->tev.size()) {
         te = &h->tev[h->tev_used];
         h->tev_used += 4;
     }
@@ -3481,7 +3571,16 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         }
         int vw = vec_width(g, h->shape_v);
         dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
-        if (vw == 4 && one) launch(h, k_velocity_v<ND, 4, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        if (h->wl) {
+            // the work-list launch of sofi_run_many covers this update
+            h->wl_shot.g = gg;
+            h->wl_shot.f = fk;
+            h->wl_shot.m = m;
+            h->wl_shot.s = h->s;
+            h->wl_shot.fv = fv_l;
+            h->wl_shot.fv.ntb = h->ntb;
+            h->dstage = 1;
+        } else if (vw == 4 && one) launch(h, k_velocity_v<ND, 4, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (vw == 4) launch(h, k_velocity_v<ND, 4, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (vw == 2 && one) launch(h, k_velocity_v<ND, 2, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (vw == 2) launch(h, k_velocity_v<ND, 2, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
@@ -3520,6 +3619,12 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
             else if (one) launch(h, k_stress<ND, W, true>, tiles, blk, gg, fk, m, h->s, fss, nt);              \
             else launch(h, k_stress<ND, W, false>, tiles, blk, gg, fk, m, h->s, fss, nt);            \
         } while (0)
+        if (h->wl) {
+            h->wl_shot.fs = fss;
+            h->wl_shot.fs.ntb = h->ntb;
+            h->dstage = 2;
+            continue;
+        }
         switch (d.weq) {
           case SOFI_AC_ISO: {     // one thread per cell whatever the stress shape
               dim3 ta((g.nx + BX - 1) / BX * nty, nchunk);
@@ -4219,6 +4324,200 @@ int batch_tune(sofi_dev *h, int *nt, int nt_last)
     return 0;
 }
 
+// ------------------------------------------- work-list runs (sofi_run_many)
+
+// what keeps a handle out of a work-list run (null: nothing)
+const char *many_refusal(const sofi_dev *h)
+{
+    if (!h->medium_set) return "its medium is not set";
+    if (h->nb != 1) return "it is a batch handle";
+    if (h->d.mode == SOFI_MODE_FWI) return "it runs MODE=FWI";
+    if (h->fdt4) return "it runs 4th order in time";
+    switch (h->d.weq) {
+      case SOFI_EL_ISO:
+      case SOFI_VEL_ISO:
+      case SOFI_EL_VTI:
+      case SOFI_VEL_VTI: return nullptr;
+    }
+    return "AC_ISO and the TTI rheologies have no work-list kernels";
+}
+
+// a work-list launch: tile rows per item, blocks (0: one block per item)
+struct ManyCfg {
+    int rows, blocks;
+};
+std::mutex g_many_mu;
+std::map<std::tuple<int, int, int, int, int>, ManyCfg> g_many_cache;
+thread_local ManyCfg g_many_last{1, 0};
+thread_local int g_many_tuned = 0;
+
+struct ManyCtx {
+    hipStream_t st = nullptr;
+    std::vector<sofi_dev *> hs;     // in work-list order: the handles with the most steps first
+    std::vector<int> first, len;    // their first step and step count
+    std::vector<sofi_dev::Deferred> dl;
+    std::vector<Shot> tab;          // this step's descriptors
+    std::vector<Shot> sent;         // the descriptors on the device
+    Shot *dtab = nullptr;
+    int2 *ditem = nullptr;
+    int rows = 0;                   // tile rows per item of the uploaded items
+    std::vector<int> cum;           // cum[a]: items of the first a handles
+    hipError_t err = hipSuccess;
+};
+
+// the items of `rows` tile rows each, handle after handle, tile rows in
+// order and the tile columns of a row band side by side
+int many_items(ManyCtx &c, int rows)
+{
+    if (c.rows == rows) return 0;
+    std::vector<int2> it;
+    c.cum.assign(1, 0);
+    for (size_t p = 0; p < c.hs.size(); ++p) {
+        const Grid &g = c.hs[p]->g;
+        for (int ty = 0; ty < g.nty; ty += rows)
+            for (int tx = 0; tx < g.ntx; ++tx) it.push_back(make_int2((int)p, ty * g.ntx + tx));
+        c.cum.push_back((int)it.size());
+    }
+    // the launches that read the old items end first
+    HIPCHK(hipStreamSynchronize(c.st));
+    HIPCHK(hipMemcpy(c.ditem, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice));
+    c.rows = rows;
+    return 0;
+}
+
+// local step k (0-based) of every handle that still advances: each handle's
+// step recorded (its update kernels' arguments and its other launches), then
+// its launches before the velocity update, the velocity update of all of
+// them, the launches between, the stress update of all of them, the rest
+template <int ND>
+int many_step(ManyCtx &c, int k, ManyCfg cfg)
+{
+    int a = 0;
+    while (a < (int)c.hs.size() && c.len[a] > k) ++a;
+    c.dl.clear();
+    for (int p = 0; p < a; ++p) {
+        sofi_dev *h = c.hs[p];
+        memset(static_cast<void *>(&h->wl_shot), 0, sizeof(Shot));
+        h->wl = true;
+        h->defer = &c.dl;
+        h->dstage = 0;
+        const int rc = step(h, c.first[p] + k);
+        h->wl = false;
+        h->defer = nullptr;
+        if (rc) return rc;
+        if (h->dstage != 2) return fail("sofi_run_many: step %d of a handle recorded no update kernels", c.first[p] + k);
+        memcpy(static_cast<void *>(&c.tab[p]), &h->wl_shot, sizeof(Shot));
+    }
+    // the descriptors are uploaded when they change (in the first step: the
+    // arguments of a handle's update kernels are the same in every step of a
+    // work-list run), after the launches that read the old ones
+    if (c.sent.empty() || memcmp(c.sent.data(), c.tab.data(), (size_t)a * sizeof(Shot)) != 0) {
+        HIPCHK(hipStreamSynchronize(c.st));
+        HIPCHK(hipMemcpy(c.dtab, c.tab.data(), c.tab.size() * sizeof(Shot), hipMemcpyHostToDevice));
+        c.sent = c.tab;
+    }
+    if (many_items(c, cfg.rows)) return -1;
+    const int nit = c.cum[a];
+    const unsigned grid = (unsigned)(cfg.blocks > 0 && cfg.blocks < nit ? cfg.blocks : nit);
+    auto issue = [&](int stage) {
+        for (auto &e : c.dl)
+            if (e.stage == stage) {
+                const hipError_t r = e.run(c.st);
+                if (r != hipSuccess && c.err == hipSuccess) c.err = r;
+            }
+    };
+    const dim3 blk(BX, BY);
+    issue(0);
+    hipLaunchKernelGGL((k_velocity_w<ND, true>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+    issue(1);
+    switch (c.hs[0]->d.weq) {
+      case SOFI_EL_ISO:
+          hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_ISO>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+          break;
+      case SOFI_VEL_ISO:
+          hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_ISO>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+          break;
+      case SOFI_EL_VTI:
+          hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_VTI>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+          break;
+      default:
+          hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_VTI>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+    }
+    issue(2);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess && c.err == hipSuccess) c.err = e;
+    return 0;
+}
+
+constexpr int MANY_TUNE_STEPS = 2;
+
+// all steps of a work-list run: the launch fixed by the caller, cached for
+// this rheology, FD order, handle count and size, or measured first (each
+// candidate on MANY_TUNE_STEPS steps of the run after one untimed step;
+// results do not depend on it, every candidate runs the same per-cell code)
+template <int ND>
+int run_many_nd(ManyCtx &c, const int *launch)
+{
+    const sofi_dev *h0 = c.hs[0];
+    long long tiles = 0;
+    for (const sofi_dev *h : c.hs) tiles += (long long)h->g.ntx * h->g.nty;
+    int lg = 0;
+    while ((1ll << (lg + 1)) <= tiles) ++lg;
+    const auto key = std::make_tuple(ND, h0->d.weq, h0->g.L, (int)c.hs.size(), lg);
+    ManyCfg cfg{1, 0};
+    int k = 0;
+    g_many_tuned = 0;
+    bool cached = false;
+    if (launch) {
+        cfg = ManyCfg{launch[0], launch[1]};
+    } else {
+        std::lock_guard<std::mutex> lk(g_many_mu);
+        const auto f = g_many_cache.find(key);
+        if (f != g_many_cache.end()) {
+            cfg = f->second;
+            cached = true;
+            g_many_tuned = 1;
+        }
+    }
+    std::vector<ManyCfg> cand;
+    for (int rows : { 1, 2, 4 })
+        for (int blocks : { 0, 1024, 2048 }) cand.push_back(ManyCfg{rows, blocks});
+    if (!launch && !cached && c.len[0] >= 1 + (int)cand.size() * MANY_TUNE_STEPS) {
+        if (many_step<ND>(c, k++, cfg)) return -1;
+        std::vector<hipEvent_t> ev(2 * cand.size(), nullptr);
+        for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+        int rc = 0;
+        for (size_t q = 0; q < cand.size() && !rc; ++q) {
+            rc = many_items(c, cand[q].rows);     // outside the timed steps
+            if (!rc && hipEventRecord(ev[2 * q], c.st) != hipSuccess) rc = fail("sofi_run_many: hipEventRecord failed");
+            for (int s = 0; s < MANY_TUNE_STEPS && !rc; ++s) rc = many_step<ND>(c, k++, cand[q]);
+            if (!rc && hipEventRecord(ev[2 * q + 1], c.st) != hipSuccess) rc = fail("sofi_run_many: hipEventRecord failed");
+        }
+        if (!rc && hipEventSynchronize(ev.back()) != hipSuccess) rc = fail("sofi_run_many: hipEventSynchronize failed");
+        float best = 0.f;
+        for (size_t q = 0; q < cand.size() && !rc; ++q) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]) != hipSuccess) {
+                rc = fail("sofi_run_many: hipEventElapsedTime failed");
+                break;
+            }
+            if (q == 0 || ms < best) {
+                best = ms;
+                cfg = cand[q];
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(g_many_mu);
+        g_many_cache[key] = cfg;
+        g_many_tuned = 1;
+    }
+    for (; k < c.len[0]; ++k)
+        if (many_step<ND>(c, k, cfg)) return -1;
+    g_many_last = cfg;
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -4262,6 +4561,95 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
             if (h->nb == 1) return fail("Simulation is unstable: vy(NX/2, NY/2) is NaN at time step %d.", bad[k]);
             return fail("Simulation is unstable: vy(NX/2, NY/2) of shot slot %d is NaN at time step %d.", k, bad[k]);
         }
+    return 0;
+}
+
+int sofi_run_many(sofi_dev *const *hs, int n, const int *nt_first, const int *nt_last, const int *launch)
+{
+    if (!hs || n < 1 || !nt_first || !nt_last) return fail("sofi_run_many: null argument or no handles");
+    if (launch && (launch[0] < 1 || launch[0] > 64 || launch[1] < 0))
+        return fail("sofi_run_many: launch {%d rows, %d blocks}: rows 1..64, blocks >= 0", launch[0], launch[1]);
+    for (int i = 0; i < n; ++i) {
+        const sofi_dev *h = hs[i];
+        if (!h) return fail("sofi_run_many: handle %d is null", i);
+        for (int q = 0; q < i; ++q)
+            if (hs[q] == h) return fail("sofi_run_many: handle %d is handle %d again", i, q);
+        if (const char *why = many_refusal(h)) return fail("sofi_run_many: handle %d: %s", i, why);
+        if (h->dev != hs[0]->dev || h->g.nd != hs[0]->g.nd || h->d.weq != hs[0]->d.weq)
+            return fail("sofi_run_many: handle %d differs from handle 0 in device, FD order or rheology", i);
+        if (nt_first[i] < 1 || nt_last[i] > h->d.nt || nt_first[i] > nt_last[i])
+            return fail("sofi_run_many: handle %d: bad step range %d..%d (NT=%d)", i, nt_first[i], nt_last[i], h->d.nt);
+    }
+    ManyCtx c;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return nt_last[a] - nt_first[a] > nt_last[b] - nt_first[b];
+    });
+    long long tiles = 0;
+    for (int i : order) {
+        sofi_dev *h = hs[i];
+        if (sync_acq(h)) return -1;
+        h->gerr = hipSuccess;
+        c.hs.push_back(h);
+        c.first.push_back(nt_first[i]);
+        c.len.push_back(nt_last[i] - nt_first[i] + 1);
+        tiles += (long long)h->g.ntx * h->g.nty;
+    }
+    c.tab.assign(n, Shot{});
+    c.st = hs[0]->stream;
+    for (int i = 0; i < n; ++i) {
+        HIPCHK(hipEventRecord(hs[i]->ev0, hs[i]->stream));
+        HIPCHK(hipStreamWaitEvent(c.st, hs[i]->ev0, 0));
+    }
+    HIPCHK(hipMalloc(&c.dtab, sizeof(Shot) * (size_t)n));
+    if (hipMalloc(&c.ditem, sizeof(int2) * (size_t)tiles) != hipSuccess) {
+        (void)hipFree(c.dtab);
+        return fail("sofi_run_many: hipMalloc of %lld work items failed", tiles);
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess
+                 ? fail("sofi_run_many: hipEventCreate failed") : 0;
+    for (size_t p = 0; p < c.hs.size() && !rc; ++p)
+        hipLaunchKernelGGL(k_set_base, dim3(1), dim3(64), 0, c.st, c.hs[p]->ntb, c.first[p] - 1);
+    if (!rc && hipEventRecord(t0, c.st) != hipSuccess) rc = fail("sofi_run_many: hipEventRecord failed");
+    if (!rc) {
+        switch (hs[0]->g.nd) {
+          case 1: rc = run_many_nd<1>(c, launch); break;
+          case 2: rc = run_many_nd<2>(c, launch); break;
+          case 3: rc = run_many_nd<3>(c, launch); break;
+          case 4: rc = run_many_nd<4>(c, launch); break;
+          case 5: rc = run_many_nd<5>(c, launch); break;
+          case 6: rc = run_many_nd<6>(c, launch); break;
+          default: rc = fail("unsupported ND %d", hs[0]->g.nd);
+        }
+    }
+    float ms = 0.f;
+    if (hipEventRecord(t1, c.st) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+        hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+        rc = rc ? rc : fail("sofi_run_many: timing events failed");
+    (void)hipStreamSynchronize(c.st);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(c.ditem);
+    (void)hipFree(c.dtab);
+    if (rc) return -1;
+    if (c.err != hipSuccess) return fail("sofi_run_many: kernel launch failed: %s", hipGetErrorString(c.err));
+    for (int i = 0; i < n; ++i) {
+        sofi_dev *h = hs[i];
+        h->last_ms = ms;
+        int bad = 0;
+        HIPCHK(hipMemcpy(&bad, h->f.unstable, sizeof(int), hipMemcpyDeviceToHost));
+        if (bad) return fail("Simulation is unstable: vy(NX/2, NY/2) of handle %d is NaN at time step %d.", i, bad);
+    }
+    return 0;
+}
+
+int sofi_get_many_config(int *rows, int *blocks, int *tuned)
+{
+    if (rows) *rows = g_many_last.rows;
+    if (blocks) *blocks = g_many_last.blocks;
+    if (tuned) *tuned = g_many_tuned;
     return 0;
 }
 

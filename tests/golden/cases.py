@@ -296,6 +296,21 @@ def case_def(name: str) -> tuple[W.Case, dict]:
     return c, model
 
 
+def sized_case(td: str, name: str, weq: str, nx: int, ny: int, fdorder: int, L: int = 0, abs_type: int = 1,
+               free_surf: int = 0, source_type: int = 1, seismo: int = 4, seed: int = 0) -> str:
+    """One shot on an nx x ny grid (shots on grids of uneven size advanced
+    together): a layered model of the rheology, the source at the grid's
+    centre, the coverage cases' receiver pattern on this grid, snapshots."""
+    extra = (30.0, source_type) if source_type in (4, 5) else ()
+    c = W.Case(name=name, nx=nx, ny=ny, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdorder,
+               maxrelerror=1, L=L, fl=(4.0, 12.0, 30.0, 60.0, 90.0)[:max(L, 1)], f_ref=20.0,
+               abs_type=abs_type, fw=12, free_surf=free_surf, damping=8.0, fpml=20.0, vppml=3500.0,
+               source_type=source_type, sources=_src(nx, ny, DH, extra=extra), receivers=_rec_grid(nx, ny, DH),
+               seismo=seismo, ndt=2, snap=4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
+               idx=2, idy=3)
+    return W.write_case(td, c, _model(weq, seed, nx, ny, L=L))
+
+
 def build(name: str, td: str):
     if name in TOPO_CASES:
         return topo_case(name, td)
