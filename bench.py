@@ -378,6 +378,9 @@ def bench_many(args):
     from gpi_project_amd.sofi import Problem, Simulation, many_config, run_many
     td = tempfile.mkdtemp(prefix="sofi_many_")
     q4 = workloads.config4_requests(args.requests, side=(256, 1024), nt=(1000, 3000))
+    if args.many_weq:
+        q4 = [q for q in q4 if q.weq == args.many_weq]
+    passes = set(args.many_passes.split(","))
     paths = [workloads.config4_write(os.path.join(td, f"r{q.rid:03d}"), q) for q in q4]
     cwd = os.getcwd()
     probs = []
@@ -409,26 +412,29 @@ def bench_many(args):
         for s in sims:
             s.close()
 
-    # warm-up: code objects of both shapes
-    for launch in ("auto", "cell"):
-        s = Simulation(probs[0])
-        s.set_launch(launch)
-        s.load_shot(nsnap=0)
-        s.run(1, 50)
-        s.close()
     res = {}
-    sims = opened("auto")
-    t0 = time.perf_counter()
-    for s in sims:
-        s.run()
-    res["one at a time"] = time.perf_counter() - t0
-    close(sims)
-    sims = opened("auto")
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(args.workers) as ex:
-        list(ex.map(lambda s: s.run(), sims))
-    res["concurrently"] = time.perf_counter() - t0
-    close(sims)
+    if "alone" in passes or "concurrent" in passes:
+        # warm-up: code objects of both shapes
+        for launch in ("auto", "cell"):
+            s = Simulation(probs[0])
+            s.set_launch(launch)
+            s.load_shot(nsnap=0)
+            s.run(1, 50)
+            s.close()
+    if "alone" in passes:
+        sims = opened("auto")
+        t0 = time.perf_counter()
+        for s in sims:
+            s.run()
+        res["one at a time"] = time.perf_counter() - t0
+        close(sims)
+    if "concurrent" in passes:
+        sims = opened("auto")
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(args.workers) as ex:
+            list(ex.map(lambda s: s.run(), sims))
+        res["concurrently"] = time.perf_counter() - t0
+        close(sims)
     sims = opened("cell")
     cfg = {}
     t0 = time.perf_counter()
@@ -438,20 +444,22 @@ def bench_many(args):
         run_many([sims[k] for k in idx])
         cfg[f"FD{key[0]} {key[1]}"] = many_config()
 
-    with ThreadPoolExecutor(len(groups)) as ex:
+    with ThreadPoolExecutor(args.many_groups or len(groups)) as ex:
         list(ex.map(group_run, list(groups)))
     res["work-list"] = time.perf_counter() - t0
     got = sections(sims)
     close(sims)
-    sims = opened("cell")
-    for s in sims:
-        s.run()
-    ref = sections(sims)
-    close(sims)
+    same = None
+    if "alone" in passes:
+        sims = opened("cell")
+        for s in sims:
+            s.run()
+        ref = sections(sims)
+        close(sims)
+        same = all(np.array_equal(a[c], b[c]) for a, b in zip(got, ref) for c in b)
     for P in probs:
         P.close()
     shutil.rmtree(td, ignore_errors=True)
-    same = all(np.array_equal(a[c], b[c]) for a, b in zip(got, ref) for c in b)
     rate = {k: len(q4) / v for k, v in res.items()}
     for k, v in res.items():
         log(f"{k}: {v:.3f} s, {rate[k]:.2f} shots/s, {cell_steps / v / 1e9:.1f} GLUPS")
@@ -463,19 +471,21 @@ def bench_many(args):
         "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic: per-request seeded layered models (workloads.config4_requests, seed 5)",
         "config": {"workload": f"{len(q4)} requests, sides log-U[256,1024], FD4/8, NT U{{1000..3000}}, "
-                               "EL_ISO/VEL_ISO, all waiting; one sofi_run_many per FD order and rheology",
+                               f"{args.many_weq or 'EL_ISO/VEL_ISO'}, all waiting; one sofi_run_many per FD "
+                               "order and rheology",
                    "requests": len(q4), "cell_steps": cell_steps, "workers": args.workers},
-        "one_at_a_time_shots_per_s": round(rate["one at a time"], 4),
-        "concurrent_shots_per_s": round(rate["concurrently"], 4),
-        "speedup_vs_one_at_a_time": round(rate["work-list"] / rate["one at a time"], 3),
-        "speedup_vs_concurrent": round(rate["work-list"] / rate["concurrently"], 3),
+        "one_at_a_time_shots_per_s": round(rate["one at a time"], 4) if "one at a time" in rate else None,
+        "concurrent_shots_per_s": round(rate["concurrently"], 4) if "concurrently" in rate else None,
+        "speedup_vs_one_at_a_time":
+            round(rate["work-list"] / rate["one at a time"], 3) if "one at a time" in rate else None,
+        "speedup_vs_concurrent": round(rate["work-list"] / rate["concurrently"], 3) if "concurrently" in rate else None,
         "glups": round(cell_steps / res["work-list"] / 1e9, 3),
-        "one_at_a_time_glups": round(cell_steps / res["one at a time"] / 1e9, 3),
+        "one_at_a_time_glups": round(cell_steps / res["one at a time"] / 1e9, 3) if "one at a time" in rate else None,
         "launch": {k: {"rows": v[0], "blocks": v[1], "tuned": v[2]} for k, v in cfg.items()},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole work-list job (80 B per EL_ISO, 164 B per VEL_ISO cell-step)"},
-        "results_identical_to_one_shot_cell_runs": bool(same),
+        "results_identical_to_one_shot_cell_runs": same,
     }
     print(json.dumps(out), flush=True)
 
@@ -704,6 +714,11 @@ def main():
     ap.add_argument("--fwi-batch", type=int, default=8, help="FWI shots per batch handle (slots sharing the medium)")
     ap.add_argument("--fwi-batch-workers", type=int, default=1, help="FWI batch handles at once")
     ap.add_argument("--workers", type=int, default=4, help="concurrent handles of the served workload")
+    ap.add_argument("--many-weq", default="", help="c4-many: only the requests of this rheology")
+    ap.add_argument("--many-passes", default="alone,concurrent,worklist",
+                    help="c4-many: the passes to run (the work-list pass always runs)")
+    ap.add_argument("--many-groups", type=int, default=0,
+                    help="c4-many: work-list groups (FD order, rheology) run at once (0: all)")
     ap.add_argument("--pmc-child", metavar="CASE_DIR", help=argparse.SUPPRESS)
     ap.add_argument("--pmc-steps", type=int, default=20, help=argparse.SUPPRESS)
     args = ap.parse_args()
