@@ -481,7 +481,7 @@ def bench_many(args):
         "speedup_vs_concurrent": round(rate["work-list"] / rate["concurrently"], 3) if "concurrently" in rate else None,
         "glups": round(cell_steps / res["work-list"] / 1e9, 3),
         "one_at_a_time_glups": round(cell_steps / res["one at a time"] / 1e9, 3) if "one at a time" in rate else None,
-        "launch": {k: {"rows": v[0], "blocks": v[1], "tuned": v[2]} for k, v in cfg.items()},
+        "launch": {k: {"rows": v[0], "blocks": v[1], "waves": v[2], "tuned": v[3]} for k, v in cfg.items()},
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole work-list job (80 B per EL_ISO, 164 B per VEL_ISO cell-step)"},

@@ -175,7 +175,7 @@ def hip():
     L.sofi_fwi_get_adjoint.argtypes = [_VP, C.c_int, _FP]
     L.sofi_fwi_adjoint_count.argtypes = [_VP]
     L.sofi_run_many.argtypes = [C.POINTER(_VP), C.c_int, _IP, _IP, _IP]
-    L.sofi_get_many_config.argtypes = [_IP, _IP, _IP]
+    L.sofi_get_many_config.argtypes = [_IP, _IP, _IP, _IP]
     _hip = L
     return L
 

@@ -665,13 +665,14 @@ __device__ __forceinline__ void velocity_core(const Grid &g, Fields &f, const Ma
 // single slot of a one-shot handle, without the slot loop, whose run-time
 // trip count and per-slot offsets cost the one-shot kernels 10-45 % on
 // MI355X)
-template <int ND, bool ONE, bool NT>
+// tile t of BX x TBY cells (tiles row by row, g.ntx to a row)
+template <int ND, bool ONE, bool NT, int TBY = BY>
 __device__ __forceinline__ void velocity_tile(const Grid &g, const Fields &f0, const Mat &m, const Src &s,
                                               const Fuse &fz, int nt, int t)
 {
     const int ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
-    const int j = 1 + ty * BY + threadIdx.y;
+    const int j = 1 + ty * TBY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const int c = IDX(g, i, j);
     const float rip = ld1<NT>(m.rip, c), rjp = ld1<NT>(m.rjp, c);
@@ -943,14 +944,14 @@ __device__ __forceinline__ void stress_post(const Grid &g, Fields &f, const Mat 
 // blockIdx.y picks g.spt shot slots of a batch, updated in turn with the
 // cell's material coefficients read once (ONE: a one-shot handle's single
 // slot, as in k_velocity).
-template <int ND, int WEQ, bool ONE>
+template <int ND, int WEQ, bool ONE, int TBY = BY>
 __device__ __forceinline__ void stress_tile(const Grid &g, const Fields &f0, const Mat &m, const Src &s,
                                             const Fuse &fz, int nt, int t)
 {
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
     const int ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
-    const int j = 1 + ty * BY + threadIdx.y;
+    const int j = 1 + ty * TBY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const int c = IDX(g, i, j), P = g.pitch;
     // the cell's coefficients: EL_ISO f g fipjp; VEL_ISO + defect bases;
@@ -1056,36 +1057,37 @@ struct Shot {
 };
 
 // The one-thread-per-cell update kernels over several one-shot handles on
-// grids of any size, one launch for all of them: item w = (handle, tile)
-// covers `rows` tiles down the tile column from that tile; block b takes
-// items b, b + gridDim.x, ... (a persistent grid when gridDim.x is below the
-// item count).  Items 0..n-1 belong to the handles that still advance (the
-// handles with the most steps come first).  Every tile runs the code of the
-// one-shot kernel, so each handle's results are those of its own launches,
-// bit for bit, whatever the grid size and rows.
-template <int ND, bool NT>
-__global__ __launch_bounds__(BX *BY) void k_velocity_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
-                                                       int n, int rows, int it)
+// grids of any size, one launch for all of them, in tiles of BX x TBY cells
+// (TBY waves per block): item w = (handle, tile) covers `rows` tiles down the
+// tile column from that tile; block b takes items b, b + gridDim.x, ... (a
+// persistent grid when gridDim.x is below the item count).  Items 0..n-1
+// belong to the handles that still advance (the handles with the most steps
+// come first).  Every cell runs the code of the one-shot kernel, so each
+// handle's results are those of its own launches, bit for bit, whatever the
+// tile height, rows and grid size.
+template <int ND, bool NT, int TBY>
+__global__ __launch_bounds__(BX *TBY) void k_velocity_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
+                                                        int n, int rows, int it)
 {
     for (int w = blockIdx.x; w < n; w += gridDim.x) {
         const int2 e = item[w];
         const Shot &d = sh[e.x];
-        const int last = d.g.ntx * d.g.nty;
+        const int last = d.g.ntx * ((d.g.ny + TBY - 1) / TBY);
         for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
-            velocity_tile<ND, true, NT>(d.g, d.f, d.m, d.s, d.fv, it, t);
+            velocity_tile<ND, true, NT, TBY>(d.g, d.f, d.m, d.s, d.fv, it, t);
     }
 }
 
-template <int ND, int WEQ>
-__global__ __launch_bounds__(BX *BY) void k_stress_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
-                                                     int n, int rows, int it)
+template <int ND, int WEQ, int TBY>
+__global__ __launch_bounds__(BX *TBY) void k_stress_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
+                                                      int n, int rows, int it)
 {
     for (int w = blockIdx.x; w < n; w += gridDim.x) {
         const int2 e = item[w];
         const Shot &d = sh[e.x];
-        const int last = d.g.ntx * d.g.nty;
+        const int last = d.g.ntx * ((d.g.ny + TBY - 1) / TBY);
         for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
-            stress_tile<ND, WEQ, true>(d.g, d.f, d.m, d.s, d.fs, it, t);
+            stress_tile<ND, WEQ, true, TBY>(d.g, d.f, d.m, d.s, d.fs, it, t);
     }
 }
 
@@ -4342,13 +4344,14 @@ const char *many_refusal(const sofi_dev *h)
     return "AC_ISO and the TTI rheologies have no work-list kernels";
 }
 
-// a work-list launch: tile rows per item, blocks (0: one block per item)
+// a work-list launch: tiles per item (down a tile column), blocks (0: one
+// block per item), tile height in rows = waves per block (2, 4 or 8)
 struct ManyCfg {
-    int rows, blocks;
+    int rows, blocks, waves;
 };
 std::mutex g_many_mu;
 std::map<std::tuple<int, int, int, int, int>, ManyCfg> g_many_cache;
-thread_local ManyCfg g_many_last{1, 0};
+thread_local ManyCfg g_many_last{1, 0, BY};
 thread_local int g_many_tuned = 0;
 
 struct ManyCtx {
@@ -4360,21 +4363,22 @@ struct ManyCtx {
     std::vector<Shot> sent;         // the descriptors on the device
     Shot *dtab = nullptr;
     int2 *ditem = nullptr;
-    int rows = 0;                   // tile rows per item of the uploaded items
+    int rows = 0, waves = 0;        // tiles per item and tile height of the uploaded items
     std::vector<int> cum;           // cum[a]: items of the first a handles
     hipError_t err = hipSuccess;
 };
 
-// the items of `rows` tile rows each, handle after handle, tile rows in
-// order and the tile columns of a row band side by side
-int many_items(ManyCtx &c, int rows)
+// the items of `rows` tiles of BX x waves cells each, handle after handle,
+// bands of tile rows in order and the tile columns of a band side by side
+int many_items(ManyCtx &c, int rows, int waves)
 {
-    if (c.rows == rows) return 0;
+    if (c.rows == rows && c.waves == waves) return 0;
     std::vector<int2> it;
     c.cum.assign(1, 0);
     for (size_t p = 0; p < c.hs.size(); ++p) {
         const Grid &g = c.hs[p]->g;
-        for (int ty = 0; ty < g.nty; ty += rows)
+        const int nty = (g.ny + waves - 1) / waves;
+        for (int ty = 0; ty < nty; ty += rows)
             for (int tx = 0; tx < g.ntx; ++tx) it.push_back(make_int2((int)p, ty * g.ntx + tx));
         c.cum.push_back((int)it.size());
     }
@@ -4382,7 +4386,26 @@ int many_items(ManyCtx &c, int rows)
     HIPCHK(hipStreamSynchronize(c.st));
     HIPCHK(hipMemcpy(c.ditem, it.data(), it.size() * sizeof(int2), hipMemcpyHostToDevice));
     c.rows = rows;
+    c.waves = waves;
     return 0;
+}
+
+// one update launch of a work-list step in tiles of BX x TBY cells
+template <int ND, int TBY>
+void many_launch(hipStream_t st, int weq, bool stress, unsigned grid, const Shot *tab, const int2 *item, int n,
+                 int rows, int it)
+{
+    const dim3 blk(BX, TBY);
+    if (!stress) {
+        hipLaunchKernelGGL((k_velocity_w<ND, true, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it);
+        return;
+    }
+    switch (weq) {
+      case SOFI_EL_ISO: hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_ISO, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
+      case SOFI_VEL_ISO: hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_ISO, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
+      case SOFI_EL_VTI: hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_VTI, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
+      default: hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_VTI, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it);
+    }
 }
 
 // local step k (0-based) of every handle that still advances: each handle's
@@ -4416,7 +4439,7 @@ int many_step(ManyCtx &c, int k, ManyCfg cfg)
         HIPCHK(hipMemcpy(c.dtab, c.tab.data(), c.tab.size() * sizeof(Shot), hipMemcpyHostToDevice));
         c.sent = c.tab;
     }
-    if (many_items(c, cfg.rows)) return -1;
+    if (many_items(c, cfg.rows, cfg.waves)) return -1;
     const int nit = c.cum[a];
     const unsigned grid = (unsigned)(cfg.blocks > 0 && cfg.blocks < nit ? cfg.blocks : nit);
     auto issue = [&](int stage) {
@@ -4426,23 +4449,18 @@ int many_step(ManyCtx &c, int k, ManyCfg cfg)
                 if (r != hipSuccess && c.err == hipSuccess) c.err = r;
             }
     };
-    const dim3 blk(BX, BY);
+    auto update = [&](bool stress) {
+        const int weq = c.hs[0]->d.weq;
+        switch (cfg.waves) {
+          case 2: many_launch<ND, 2>(c.st, weq, stress, grid, c.dtab, c.ditem, nit, cfg.rows, k + 1); break;
+          case 8: many_launch<ND, 8>(c.st, weq, stress, grid, c.dtab, c.ditem, nit, cfg.rows, k + 1); break;
+          default: many_launch<ND, 4>(c.st, weq, stress, grid, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+        }
+    };
     issue(0);
-    hipLaunchKernelGGL((k_velocity_w<ND, true>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
+    update(false);
     issue(1);
-    switch (c.hs[0]->d.weq) {
-      case SOFI_EL_ISO:
-          hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_ISO>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
-          break;
-      case SOFI_VEL_ISO:
-          hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_ISO>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
-          break;
-      case SOFI_EL_VTI:
-          hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_VTI>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
-          break;
-      default:
-          hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_VTI>), dim3(grid), blk, 0, c.st, c.dtab, c.ditem, nit, cfg.rows, k + 1);
-    }
+    update(true);
     issue(2);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess && c.err == hipSuccess) c.err = e;
@@ -4464,12 +4482,12 @@ int run_many_nd(ManyCtx &c, const int *launch)
     int lg = 0;
     while ((1ll << (lg + 1)) <= tiles) ++lg;
     const auto key = std::make_tuple(ND, h0->d.weq, h0->g.L, (int)c.hs.size(), lg);
-    ManyCfg cfg{1, 0};
+    ManyCfg cfg{1, 0, BY};
     int k = 0;
     g_many_tuned = 0;
     bool cached = false;
     if (launch) {
-        cfg = ManyCfg{launch[0], launch[1]};
+        cfg = ManyCfg{launch[0], launch[1], launch[2]};
     } else {
         std::lock_guard<std::mutex> lk(g_many_mu);
         const auto f = g_many_cache.find(key);
@@ -4480,15 +4498,16 @@ int run_many_nd(ManyCtx &c, const int *launch)
         }
     }
     std::vector<ManyCfg> cand;
-    for (int rows : { 1, 2, 4 })
-        for (int blocks : { 0, 1024, 2048 }) cand.push_back(ManyCfg{rows, blocks});
+    for (int waves : { 2, 4, 8 })
+        for (int rows : { 1, 2, 4 })
+            for (int blocks : { 0, 2048 }) cand.push_back(ManyCfg{rows, blocks, waves});
     if (!launch && !cached && c.len[0] >= 1 + (int)cand.size() * MANY_TUNE_STEPS) {
         if (many_step<ND>(c, k++, cfg)) return -1;
         std::vector<hipEvent_t> ev(2 * cand.size(), nullptr);
         for (auto &e : ev) HIPCHK(hipEventCreate(&e));
         int rc = 0;
         for (size_t q = 0; q < cand.size() && !rc; ++q) {
-            rc = many_items(c, cand[q].rows);     // outside the timed steps
+            rc = many_items(c, cand[q].rows, cand[q].waves);     // outside the timed steps
             if (!rc && hipEventRecord(ev[2 * q], c.st) != hipSuccess) rc = fail("sofi_run_many: hipEventRecord failed");
             for (int s = 0; s < MANY_TUNE_STEPS && !rc; ++s) rc = many_step<ND>(c, k++, cand[q]);
             if (!rc && hipEventRecord(ev[2 * q + 1], c.st) != hipSuccess) rc = fail("sofi_run_many: hipEventRecord failed");
@@ -4567,8 +4586,9 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last)
 int sofi_run_many(sofi_dev *const *hs, int n, const int *nt_first, const int *nt_last, const int *launch)
 {
     if (!hs || n < 1 || !nt_first || !nt_last) return fail("sofi_run_many: null argument or no handles");
-    if (launch && (launch[0] < 1 || launch[0] > 64 || launch[1] < 0))
-        return fail("sofi_run_many: launch {%d rows, %d blocks}: rows 1..64, blocks >= 0", launch[0], launch[1]);
+    if (launch && (launch[0] < 1 || launch[0] > 64 || launch[1] < 0 || (launch[2] != 2 && launch[2] != 4 && launch[2] != 8)))
+        return fail("sofi_run_many: launch {%d rows, %d blocks, %d waves}: rows 1..64, blocks >= 0, waves 2, 4 or 8",
+                    launch[0], launch[1], launch[2]);
     for (int i = 0; i < n; ++i) {
         const sofi_dev *h = hs[i];
         if (!h) return fail("sofi_run_many: handle %d is null", i);
@@ -4594,7 +4614,7 @@ int sofi_run_many(sofi_dev *const *hs, int n, const int *nt_first, const int *nt
         c.hs.push_back(h);
         c.first.push_back(nt_first[i]);
         c.len.push_back(nt_last[i] - nt_first[i] + 1);
-        tiles += (long long)h->g.ntx * h->g.nty;
+        tiles += (long long)h->g.ntx * ((h->g.ny + 1) / 2);     // the most items: 1 tile of 2 rows each
     }
     c.tab.assign(n, Shot{});
     c.st = hs[0]->stream;
@@ -4645,10 +4665,11 @@ int sofi_run_many(sofi_dev *const *hs, int n, const int *nt_first, const int *nt
     return 0;
 }
 
-int sofi_get_many_config(int *rows, int *blocks, int *tuned)
+int sofi_get_many_config(int *rows, int *blocks, int *waves, int *tuned)
 {
     if (rows) *rows = g_many_last.rows;
     if (blocks) *blocks = g_many_last.blocks;
+    if (waves) *waves = g_many_last.waves;
     if (tuned) *tuned = g_many_tuned;
     return 0;
 }

@@ -355,23 +355,24 @@ def run_many(sims, nt_first=None, nt_last=None, launch=None) -> float:
     """Advance one-shot simulations on grids of any size together
     (sofi_run_many): sims[i] through steps nt_first[i]..nt_last[i] (default
     1..NT of each), each step's velocity and stress updates of all of them in
-    one work-list launch each.  launch = (rows, blocks) fixes the launch;
-    None lets the library measure it.  Returns the run's device time in ms."""
+    one work-list launch each.  launch = (rows, blocks, waves) fixes the
+    launch; None lets the library measure it.  Returns the run's device time
+    in ms."""
     n = len(sims)
     first = [1] * n if nt_first is None else list(nt_first)
     last = [s.desc.nt for s in sims] if nt_last is None else list(nt_last)
     L = N.hip()
     hs = (C.c_void_p * n)(*[s._h.value for s in sims])
-    lp = (C.c_int * 2)(*launch) if launch is not None else None
+    lp = (C.c_int * 3)(*launch) if launch is not None else None
     _chk_hip(L.sofi_run_many(hs, n, (C.c_int * n)(*first), (C.c_int * n)(*last), lp))
     return float(L.sofi_last_run_ms(sims[0]._h))
 
 
 def many_config():
-    """(rows, blocks, tuned) of this thread's last run_many."""
-    r, b, t = C.c_int(), C.c_int(), C.c_int()
-    _chk_hip(N.hip().sofi_get_many_config(C.byref(r), C.byref(b), C.byref(t)))
-    return r.value, b.value, bool(t.value)
+    """(rows, blocks, waves, tuned) of this thread's last run_many."""
+    r, b, w, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    _chk_hip(N.hip().sofi_get_many_config(C.byref(r), C.byref(b), C.byref(w), C.byref(t)))
+    return r.value, b.value, w.value, bool(t.value)
 
 
 class Batch(Simulation):

@@ -141,12 +141,12 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last);
  * update; sampling, snapshots and the sources and free surface that do not
  * ride in the update kernels are launches per handle.  A block takes item b,
  * b + blocks, ... (a persistent grid when blocks is below the item count),
- * and an item covers `rows` tiles of 64 x 4 cells down a tile column.
- * launch = {rows (1..64), blocks (0: one block per item)} fixes the launch;
- * NULL lets the first call of a rheology, FD order, handle count and size
- * measure rows 1, 2, 4 x blocks 0, 1024, 2048 on two steps each (after one
- * untimed step; the steps are part of the run) and keep the fastest in the
- * process.  Each handle's results equal those of sofi_run with its update
+ * and an item covers `rows` tiles of 64 x `waves` cells (`waves` wavefronts
+ * per block) down a tile column.  launch = {rows (1..64), blocks (0: one
+ * block per item), waves (2, 4 or 8)} fixes the launch; NULL lets the first
+ * call of a rheology, FD order, handle count and size measure waves 2, 4, 8
+ * x rows 1, 2, 4 x blocks 0, 2048 on two steps each (after one untimed step;
+ * the steps are part of the run) and keep the fastest in the process.  Each handle's results equal those of sofi_run with its update
  * kernels set to one thread per cell (sofi_set_launch), bit for bit, for any
  * launch, and the handle can go on with sofi_run afterwards.  The work runs
  * on the first handle's stream; the call waits for it and sets every
@@ -155,7 +155,7 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last);
 int sofi_run_many(sofi_dev *const *h, int n, const int *nt_first, const int *nt_last, const int *launch);
 /* The launch of this thread's last sofi_run_many and whether it came from
  * the measurement or the cache (1) or was fixed (0). */
-int sofi_get_many_config(int *rows, int *blocks, int *tuned);
+int sofi_get_many_config(int *rows, int *blocks, int *waves, int *tuned);
 
 /* Device time of the last sofi_run in milliseconds (HIP events on the
  * handle's stream). */

@@ -3,7 +3,7 @@ step's velocity update of all handles is one launch over a work-list of
 (handle, tile) items, and so is the stress update.  Each tile runs the
 one-shot cell kernel's code, so every handle's sections, snapshots and final
 wavefield equal its own sofi_run with one thread per cell bit for bit, for
-any launch (tile rows per item, persistent grid size) and for runs split
+any launch (tile height, tiles per item, persistent grid size) and for runs split
 across calls with handles of different step counts.  One handle of each set
 is a reference case, held to the reference's one-process run
 (tests/golden/ref1) through the work-list launches."""
@@ -84,7 +84,7 @@ def test_many_matches_one_shot_runs(name):
         alone.append(_results(sim, P))
         sim.close()
     # (launch, where each handle's first call ends); None: measured launch
-    for launch in (None, (1, 0), (2, 7), (4, 1024), (3, 1)):
+    for launch in (None, (1, 0, 4), (2, 7, 2), (4, 1024, 8), (3, 1, 4), (1, 0, 8)):
         sims = [_open(P) for _, _, P in probs]
         split = [max(1, t // (3 + k)) for k, t in enumerate(nt)]
         run_many(sims, [1] * len(sims), split, launch)
@@ -144,7 +144,9 @@ def test_many_refusals():
     with pytest.raises(SofiError, match="step range"):
         run_many(sims[:2], [1, 1], [10, probs[1][2].desc.nt + 1])
     with pytest.raises(SofiError, match="rows"):
-        run_many(sims[:2], launch=(0, 0))
+        run_many(sims[:2], launch=(0, 0, 4))
+    with pytest.raises(SofiError, match="waves"):
+        run_many(sims[:2], launch=(1, 0, 3))
     B = Batch(probs[0][2], 2)
     with pytest.raises(SofiError, match="batch"):
         run_many([sims[0], B])
