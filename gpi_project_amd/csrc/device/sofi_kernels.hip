@@ -1041,56 +1041,6 @@ __global__ __launch_bounds__(BX *BY) void k_stress(Grid g, Fields f0, Mat m, Src
     stress_tile<ND, WEQ, ONE>(g, f0, m, s, fz, nt, tile_id());
 }
 
-// ------------------------------------------------------ work-list launches
-
-// One handle's part of a work-list launch: its grid, fields, medium and
-// sources, and what its velocity / stress updates fuse (forces; explosive
-// sources and the free surface).  Fuse::ntb points at the handle's step
-// before the first step of the launch sequence, so that local step `it` of a
-// launch is the handle's step *ntb + it.
-struct Shot {
-    Grid g;
-    Fields f;
-    Mat m;
-    Src s;
-    Fuse fv, fs;
-};
-
-// The one-thread-per-cell update kernels over several one-shot handles on
-// grids of any size, one launch for all of them, in tiles of BX x TBY cells
-// (TBY waves per block): item w = (handle, tile) covers `rows` tiles down the
-// tile column from that tile; block b takes items b, b + gridDim.x, ... (a
-// persistent grid when gridDim.x is below the item count).  Items 0..n-1
-// belong to the handles that still advance (the handles with the most steps
-// come first).  Every cell runs the code of the one-shot kernel, so each
-// handle's results are those of its own launches, bit for bit, whatever the
-// tile height, rows and grid size.
-template <int ND, bool NT, int TBY>
-__global__ __launch_bounds__(BX *TBY) void k_velocity_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
-                                                        int n, int rows, int it)
-{
-    for (int w = blockIdx.x; w < n; w += gridDim.x) {
-        const int2 e = item[w];
-        const Shot &d = sh[e.x];
-        const int last = d.g.ntx * ((d.g.ny + TBY - 1) / TBY);
-        for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
-            velocity_tile<ND, true, NT, TBY>(d.g, d.f, d.m, d.s, d.fv, it, t);
-    }
-}
-
-template <int ND, int WEQ, int TBY>
-__global__ __launch_bounds__(BX *TBY) void k_stress_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
-                                                      int n, int rows, int it)
-{
-    for (int w = blockIdx.x; w < n; w += gridDim.x) {
-        const int2 e = item[w];
-        const Shot &d = sh[e.x];
-        const int last = d.g.ntx * ((d.g.ny + TBY - 1) / TBY);
-        for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
-            stress_tile<ND, WEQ, true, TBY>(d.g, d.f, d.m, d.s, d.fs, it, t);
-    }
-}
-
 // VW cells per thread (see k_velocity_v); inactive cells (past NX, or
 // frame cells that are never updated) keep their stresses and memory
 // variables
@@ -1453,13 +1403,13 @@ __global__ __launch_bounds__(BX *BY) void k_stress4(Grid g, Fields f, Mat m, Src
 // damping frame scales it once (abs_update_s_ac2, src/abs_update.c:28-31);
 // fused explosive sources add each amplitude twice (psource adds it to sxx
 // and to syy).
-template <int ND, bool ONE>
-__global__ __launch_bounds__(BX *BY) void k_stress_ac(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+template <int ND, bool ONE, int TBY = BY>
+__device__ __forceinline__ void stress_ac_tile(const Grid &g, const Fields &f0, const Mat &m, const Src &s,
+                                               const Fuse &fz, int nt, int t)
 {
-    if (ONE && fused_seis(g, f0, fz)) return;
-    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int ty = t / g.ntx, tx = t - ty * g.ntx;
     const int i = 1 + tx * BX + threadIdx.x;
-    const int j = 1 + ty * BY + threadIdx.y;
+    const int j = 1 + ty * TBY + threadIdx.y;
     if (i > g.nx || j > g.ny) return;
     const bool xf = in_xframe(g, i), yf = in_yframe(g, j), frame = xf || yf;
     if (frame && g.abs_type != 1 && g.abs_type != 2) return;
@@ -1501,6 +1451,64 @@ __global__ __launch_bounds__(BX *BY) void k_stress_ac(Grid g, Fields f0, Mat m, 
                 p += amp;
             }
         f.sxx[c] = p;
+    }
+}
+template <int ND, bool ONE>
+__global__ __launch_bounds__(BX *BY) void k_stress_ac(Grid g, Fields f0, Mat m, Src s, Fuse fz, int nt)
+{
+    if (ONE && fused_seis(g, f0, fz)) return;
+    stress_ac_tile<ND, ONE>(g, f0, m, s, fz, nt, tile_id());
+}
+
+// ------------------------------------------------------ work-list launches
+
+// One handle's part of a work-list launch: its grid, fields, medium and
+// sources, and what its velocity / stress updates fuse (forces; explosive
+// sources and the free surface).  Fuse::ntb points at the handle's step
+// before the first step of the launch sequence, so that local step `it` of a
+// launch is the handle's step *ntb + it.
+struct Shot {
+    Grid g;
+    Fields f;
+    Mat m;
+    Src s;
+    Fuse fv, fs;
+};
+
+// The one-thread-per-cell update kernels over several one-shot handles on
+// grids of any size, one launch for all of them, in tiles of BX x TBY cells
+// (TBY waves per block): item w = (handle, tile) covers `rows` tiles down the
+// tile column from that tile; block b takes items b, b + gridDim.x, ... (a
+// persistent grid when gridDim.x is below the item count).  Items 0..n-1
+// belong to the handles that still advance (the handles with the most steps
+// come first).  Every cell runs the code of the one-shot kernel, so each
+// handle's results are those of its own launches, bit for bit, whatever the
+// tile height, rows and grid size.
+template <int ND, bool NT, int TBY>
+__global__ __launch_bounds__(BX *TBY) void k_velocity_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
+                                                        int n, int rows, int it)
+{
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int2 e = item[w];
+        const Shot &d = sh[e.x];
+        const int last = d.g.ntx * ((d.g.ny + TBY - 1) / TBY);
+        for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
+            velocity_tile<ND, true, NT, TBY>(d.g, d.f, d.m, d.s, d.fv, it, t);
+    }
+}
+
+template <int ND, int WEQ, int TBY>
+__global__ __launch_bounds__(BX *TBY) void k_stress_w(const Shot *__restrict__ sh, const int2 *__restrict__ item,
+                                                      int n, int rows, int it)
+{
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        const int2 e = item[w];
+        const Shot &d = sh[e.x];
+        const int last = d.g.ntx * ((d.g.ny + TBY - 1) / TBY);
+        for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx) {
+            if constexpr (WEQ == SOFI_AC_ISO) stress_ac_tile<ND, true, TBY>(d.g, d.f, d.m, d.s, d.fs, it, t);
+            else stress_tile<ND, WEQ, true, TBY>(d.g, d.f, d.m, d.s, d.fs, it, t);
+        }
     }
 }
 
@@ -4339,9 +4347,10 @@ const char *many_refusal(const sofi_dev *h)
       case SOFI_EL_ISO:
       case SOFI_VEL_ISO:
       case SOFI_EL_VTI:
-      case SOFI_VEL_VTI: return nullptr;
+      case SOFI_VEL_VTI:
+      case SOFI_AC_ISO: return nullptr;
     }
-    return "AC_ISO and the TTI rheologies have no work-list kernels";
+    return "the TTI rheologies have no work-list kernels";
 }
 
 // a work-list launch: tiles per item (down a tile column), blocks (0: one
@@ -4404,6 +4413,7 @@ void many_launch(hipStream_t st, int weq, bool stress, unsigned grid, const Shot
       case SOFI_EL_ISO: hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_ISO, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
       case SOFI_VEL_ISO: hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_ISO, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
       case SOFI_EL_VTI: hipLaunchKernelGGL((k_stress_w<ND, SOFI_EL_VTI, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
+      case SOFI_AC_ISO: hipLaunchKernelGGL((k_stress_w<ND, SOFI_AC_ISO, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it); break;
       default: hipLaunchKernelGGL((k_stress_w<ND, SOFI_VEL_VTI, TBY>), dim3(grid), blk, 0, st, tab, item, n, rows, it);
     }
 }

@@ -134,8 +134,8 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last);
 
 /* Shots on grids of uneven size advanced together: handle i through its
  * steps nt_first[i]..nt_last[i], n one-shot forward handles of one device,
- * FD order and rheology (EL_ISO, VEL_ISO, EL_VTI or VEL_VTI; not 4th order
- * in time, not MODE=FWI), each with its own grid, medium, sources and
+ * FD order and rheology (EL_ISO, VEL_ISO, EL_VTI, VEL_VTI or AC_ISO; not 4th
+ * order in time, not MODE=FWI), each with its own grid, medium, sources and
  * receivers.  Every step's velocity update of all handles still advancing is
  * one launch over a work-list of (handle, tile) items, and so is the stress
  * update; sampling, snapshots and the sources and free surface that do not

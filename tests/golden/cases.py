@@ -306,8 +306,8 @@ def sized_case(td: str, name: str, weq: str, nx: int, ny: int, fdorder: int, L: 
                maxrelerror=1, L=L, fl=(4.0, 12.0, 30.0, 60.0, 90.0)[:max(L, 1)], f_ref=20.0,
                abs_type=abs_type, fw=12, free_surf=free_surf, damping=8.0, fpml=20.0, vppml=3500.0,
                source_type=source_type, sources=_src(nx, ny, DH, extra=extra), receivers=_rec_grid(nx, ny, DH),
-               seismo=seismo, ndt=2, snap=4, tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3),
-               idx=2, idy=3)
+               seismo=seismo, ndt=2, snap=seismo if weq == "AC_ISO" else 4,
+               tsnap=(NT_CASE * DT_CASE * 0.4, NT_CASE * DT_CASE, NT_CASE * DT_CASE * 0.3), idx=2, idy=3)
     return W.write_case(td, c, _model(weq, seed, nx, ny, L=L))
 
 

@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 # (reference case, [(weq, nx, ny, fdorder, L, abs_type, free_surf, source_type, seismo, seed)]):
 # one FD order and rheology per set, grids of different sizes, both absorbing
-# frames, forces, moment tensors, free surface, every SEISMO, periodic x
+# frames, forces, moment tensors, free surface, every SEISMO, periodic x, the
+# acoustic path
 SETS = {
     "el_fd6": (["el_fd6_h1_cpml_fs", "per_el_fd6_cpml_fs"],
                [("EL_ISO", 96, 200, 6, 0, 2, 0, 3, 1, 101), ("EL_ISO", 257, 75, 6, 0, 1, 1, 5, 2, 102)]),
@@ -30,6 +31,7 @@ SETS = {
                 [("VEL_ISO", 130, 140, 8, 2, 2, 0, 4, 1, 103), ("VEL_ISO", 64, 64, 8, 1, 1, 1, 1, 4, 104)]),
     "elvti_fd6": (["elvti_fd6_cpml"], [("EL_VTI", 200, 100, 6, 0, 2, 0, 2, 1, 105)]),
     "velvti_fd4": (["velvti_l2_fd4_cpml"], [("VEL_VTI", 90, 150, 4, 2, 1, 0, 1, 2, 106)]),
+    "ac_fd4": (["ac_fd4_cpml"], [("AC_ISO", 150, 90, 4, 0, 2, 0, 3, 1, 107), ("AC_ISO", 70, 180, 4, 0, 1, 0, 1, 2, 108)]),
 }
 
 
