@@ -693,6 +693,54 @@ __global__ __launch_bounds__(BX *BY) void k_velocity(Grid g, Fields f0, Mat m, S
     velocity_tile<ND, ONE, NT>(g, f0, m, s, fz, nt, tile_id());
 }
 
+// The one-shot velocity update (one slot, one thread per cell) with R
+// vertically adjacent cells per thread, in tiles of BX x R BY cells: the y
+// stencils of the R cells share their sxy and syy rows, 2 ND + R - 1 loads of
+// each instead of 2 ND R.  Each cell's sums are op_v's, in its order.  With R
+// = 2 the velocity kernel of the viscoelastic FD8 headline workload moves 6 %
+// more bytes per second than with one cell per thread (R = 3 and 4 gain less;
+// the same scheme made the stress kernel, 13 VGPRs larger, 12 % slower, and
+// one EL_TTI shot on 2000 x 1000 FD6 5 % slower, whose velocity update keeps
+// one cell per thread).
+constexpr int VROWS = 2;
+template <int ND, bool NT, int R>
+__global__ __launch_bounds__(BX *BY) void k_velocity_r(Grid g, Fields f, Mat m, Src s, Fuse fz, int nt)
+{
+    const int t = tile_id(), ty = t / g.ntx, tx = t - ty * g.ntx;
+    const int i = 1 + tx * BX + threadIdx.x;
+    const int j = 1 + ty * R * BY + R * threadIdx.y;
+    if (i > g.nx || j > g.ny) return;
+    const int nr = min(R, g.ny - j + 1);
+    const int c = IDX(g, i, j), P = g.pitch;
+    // rows j - ND .. j + ND + R - 2 of sxy and j - ND + 1 .. j + ND + R - 1 of syy
+    float qy[2 * ND + R - 1], qs[2 * ND + R - 1];
+#pragma unroll
+    for (int r = 0; r < 2 * ND + R - 1; ++r) {
+        const bool need = r < 2 * ND + nr - 1;
+        qy[r] = need ? f.sxy[c + (r - ND) * P] : 0.f;
+        qs[r] = need ? f.syy[c + (r - ND + 1) * P] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        if (q >= nr) break;
+        const int cq = c + q * P;
+        float a = 0.f, b = 0.f, d = 0.f, e = 0.f;
+#pragma unroll
+        for (int mm = 1; mm <= ND; ++mm) {
+            const float h = g.hc[mm];
+            a += h * (f.sxx[cq + mm] - f.sxx[cq - mm + 1]);
+            b += h * (f.sxy[cq + mm - 1] - f.sxy[cq - mm]);
+            d += h * (qy[ND + q + mm - 1] - qy[ND + q - mm]);
+            e += h * (qs[ND - 1 + q + mm] - qs[ND + q - mm]);
+        }
+        const float rip = ld1<NT>(m.rip, cq), rjp = ld1<NT>(m.rjp, cq);
+        float vx = ld1<NT>(f.vx, cq), vy = ld1<NT>(f.vy, cq);
+        velocity_core(g, f, m, s, fz, nt, i, j + q, cq, a, b, d, e, rip, rjp, vx, vy);
+        st1<NT>(f.vx, cq, vx);
+        st1<NT>(f.vy, cq, vy);
+    }
+}
+
 // VW cells per thread: the x stencils come from 2G+1 vectors of the row,
 // the y stencils from one vector per row; cells past NX keep their (zero)
 // padding values
@@ -3581,6 +3629,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         }
         int vw = vec_width(g, h->shape_v);
         dim3 tiles((g.nx + BX * vw - 1) / (BX * vw) * nty, nchunk);
+        const dim3 vtiles((unsigned)(g.ntx * ((g.ny + VROWS * BY - 1) / (VROWS * BY))));   // one slot, VROWS rows per thread
         if (h->wl) {
             // the work-list launch of sofi_run_many covers this update
             h->wl_shot.g = gg;
@@ -3595,7 +3644,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         else if (vw == 2 && one) launch(h, k_velocity_v<ND, 2, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (vw == 2) launch(h, k_velocity_v<ND, 2, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (one && tti) launch(h, k_velocity<ND, true, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
-        else if (one) launch(h, k_velocity<ND, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
+        else if (one) launch(h, k_velocity_r<ND, true, VROWS>, vtiles, blk, gg, fk, m, h->s, fv_l, nt);
         else launch(h, k_velocity<ND, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         wrap_x(h, gg, lo, gg.k_hi, f.vx, f.vy);
         vw = vec_width(g, h->shape_s);
