@@ -312,16 +312,17 @@ def bench_serve(args):
         fifo_end.append(t)
     lat_seq = np.array(fifo_end) - arrivals
 
-    def served(arr):
+    def served(arr, worklist=False):
         reqs = mk()
         for r, a in zip(reqs, arr):
             r.arrival = float(a)
-        srv = serve.Server(gpu_workers=args.workers, launch=args.launch)
+        srv = serve.Server(gpu_workers=args.workers, launch=args.launch, worklist=worklist)
         st = srv.serve(reqs)
         same = all(r.error is None and set(r.sections) == set(a.sections) and
                    all(np.array_equal(r.sections[s][c], a.sections[s][c]) for s in a.sections for c in a.sections[s])
                    for r, a in zip(reqs, alone))
         st["peak_hbm_bytes"] = srv.peak_used
+        st["lane_requests"] = srv.lane_requests
         return st, same
 
     st, same = served(arrivals)
@@ -330,6 +331,16 @@ def bench_serve(args):
     sat, same2 = served(np.zeros(len(q4)))
     log(f"saturated: {sat['span_s']:.2f} s, {sat['shots_per_s']:.2f} shots/s, {sat['glups']:.1f} GLUPS, "
         f"identical to alone: {same2}")
+    wl = wl_sat = None
+    same3 = same4 = True
+    if args.serve_worklist:
+        wl, same3 = served(arrivals, worklist=True)
+        log(f"served with work-list lanes: {wl['span_s']:.2f} s, {wl['shots_per_s']:.2f} shots/s, p50 "
+            f"{wl['p50_s']:.3f} s p99 {wl['p99_s']:.3f} s, {wl['lane_requests']} requests in lanes, identical "
+            f"to alone: {same3}")
+        wl_sat, same4 = served(np.zeros(len(q4)), worklist=True)
+        log(f"saturated with work-list lanes: {wl_sat['span_s']:.2f} s, {wl_sat['shots_per_s']:.2f} shots/s, "
+            f"{wl_sat['glups']:.1f} GLUPS, identical to alone: {same4}")
     shutil.rmtree(td, ignore_errors=True)
     ach = byts / sat["span_s"] / 1e9
     out = {
@@ -354,8 +365,14 @@ def bench_serve(args):
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "whole saturated job (80 B per EL_ISO, 164 B per VEL_ISO cell-step)"},
-        "results_identical_alone_and_in_mix": bool(same and same2),
+        "results_identical_alone_and_in_mix": bool(same and same2 and same3 and same4),
     }
+    if wl is not None:
+        out["worklist_lanes"] = {
+            "served_shots_per_s": round(wl["shots_per_s"], 4), "p50_latency_s": round(wl["p50_s"], 4),
+            "p99_latency_s": round(wl["p99_s"], 4), "saturated_shots_per_s": round(wl_sat["shots_per_s"], 4),
+            "saturated_glups": round(wl_sat["glups"], 3), "requests_in_lanes": wl["lane_requests"],
+            "speedup_vs_one_at_a_time": round(wl["shots_per_s"] / seq["shots_per_s"], 3)}
     print(json.dumps(out), flush=True)
 
 
@@ -714,6 +731,8 @@ def main():
     ap.add_argument("--fwi-batch", type=int, default=8, help="FWI shots per batch handle (slots sharing the medium)")
     ap.add_argument("--fwi-batch-workers", type=int, default=1, help="FWI batch handles at once")
     ap.add_argument("--workers", type=int, default=4, help="concurrent handles of the served workload")
+    ap.add_argument("--serve-worklist", action="store_true",
+                    help="c4: also serve with work-list lanes for the small one-thread-per-cell requests")
     ap.add_argument("--many-weq", default="", help="c4-many: only the requests of this rheology")
     ap.add_argument("--many-passes", default="alone,concurrent,worklist",
                     help="c4-many: the passes to run (the work-list pass always runs)")

@@ -21,6 +21,16 @@ device (one stream per handle).  Latency is completion time minus arrival
 time.  A request's results are those of the same request run alone: handles
 share nothing but the GPU, and a block boundary does not change a step.
 
+With worklist=True, small requests (one shot, at most lane_cells cells, a
+rheology whose update kernels run one thread per cell by default: VEL_ISO,
+VEL_VTI, AC_ISO) are admitted by footprint alone into a lane per FD order and
+rheology instead of a GPU worker.  A lane advances all its requests in blocks
+of lane_steps steps with sofi_run_many: each step's velocity and stress
+updates of all of them are one launch over a (handle, tile) work-list.
+Requests join a lane at the next block boundary and leave it when done, so
+requests that arrive while others run are batched into the same launches.
+Their results are still those of the request run alone, bit for bit.
+
 The reference has no serving layer (it runs the shots of one parameter file
 one after the other, src/sofi2D.c:320-395).
 """
@@ -126,17 +136,95 @@ class Server:
 
     def __init__(self, budget_bytes: int | None = None, gpu_workers: int = 4, setup_workers: int = 4,
                  launch: str = "auto", block_steps: int = 500,
-                 prepare: Callable | None = None, execute: Callable | None = None):
+                 prepare: Callable | None = None, execute: Callable | None = None,
+                 worklist: bool = False, lane_cells: int = 1 << 21, lane_steps: int = 250, lane_size: int = 32):
         self.budget = int(budget_bytes if budget_bytes is not None else 0.85 * device_free_bytes())
         self.gpu_workers = gpu_workers
         self.setup_workers = setup_workers
         self.prepare = prepare or gpu_prepare
         self.execute = execute or (lambda r, s, stop: gpu_execute(r, s, stop, launch, block_steps))
+        self.worklist = worklist and launch == "auto" and execute is None
+        self.lane_cells = lane_cells
+        self.lane_steps = lane_steps
+        self.lane_size = lane_size
+        self.lane_requests = 0                     # requests run in a lane
         self.used = 0
         self.peak_used = 0
         self.admit_order: list[int] = []
         self._cv = threading.Condition()
         self._cancel: set[int] = set()
+        self._lanes: dict = {}                     # (fdorder, weq) -> requests waiting to join
+        self._lane_n: dict = {}                    # (fdorder, weq) -> requests in the lane (joined or waiting)
+
+    # -- work-list lanes -------------------------------------------------------
+    def _lane_key(self, state):
+        """The lane of a prepared request, or None when it runs on a worker."""
+        if not self.worklist or state is None or not hasattr(state, "desc"):
+            return None
+        d = state.desc
+        if (state.nshots != 1 or d.mode != 1 or d.fdorder_time != 2 or d.nx * d.ny > self.lane_cells
+                or d.weq not in (0, 3, 5)):          # AC_ISO, VEL_ISO, VEL_VTI (one thread per cell by default)
+            return None
+        return (d.fdorder, d.weq)
+
+    def _lane(self, key, t0: float):
+        """Advance the lane's requests together until it has none left."""
+        from .sofi import Simulation, run_many
+        members = []                               # [req, problem, simulation, next step]
+
+        def finish(m, err=None):
+            req, P, sim = m[0], m[1], m[2]
+            if err is None and sim is not None:
+                req.sections[1] = {c: sim.section(c) for c in P.seismo_comps()}
+                req.cell_steps += float(P.desc.nx) * P.desc.ny * P.desc.nt
+            if sim is not None:
+                sim.close()
+            P.close()
+            req.end = time.perf_counter() - t0
+            with self._cv:
+                if req.rid in self._cancel:
+                    req.cancelled = True
+                    err = err or "cancelled"
+                req.error = err
+                self.used -= req.footprint
+                self._lane_n[key] -= 1
+                self._done += 1
+                self._cv.notify_all()
+
+        while True:
+            with self._cv:
+                new = self._lanes.get(key, [])
+                self._lanes[key] = []
+                if not new and not members:
+                    del self._lanes[key]
+                    return
+            for req, P in new:
+                try:
+                    sim = Simulation(P)
+                    sim.load_shot(nsnap=0)
+                    members.append([req, P, sim, 1])
+                except Exception as e:  # noqa: BLE001
+                    finish([req, P, None, 0], str(e))
+            for m in [m for m in members if m[0].rid in self._cancel]:
+                members.remove(m)
+                finish(m, "cancelled")
+            if not members:
+                continue
+            first = [m[3] for m in members]
+            last = [min(m[1].desc.nt, m[3] + self.lane_steps - 1) for m in members]
+            try:
+                ms = run_many([m[2] for m in members], first, last)
+            except Exception as e:  # noqa: BLE001
+                for m in members:
+                    finish(m, str(e))
+                members = []
+                continue
+            for m, l in zip(members, last):
+                m[0].gpu_ms += ms
+                m[3] = l + 1
+            for m in [m for m in members if m[3] > m[1].desc.nt]:
+                members.remove(m)
+                finish(m)
 
     def cancel(self, rid: int) -> None:
         """Stop request rid: dropped if it still waits, stopped at its next
@@ -214,8 +302,25 @@ class Server:
                         r.end = time.perf_counter() - t0
                         self._done += 1
                         continue
+                    # a small request into its work-list lane, by footprint alone
+                    key = self._lane_key(state)
+                    if (key is not None and self.used + r.footprint <= self.budget
+                            and self._lane_n.get(key, 0) < self.lane_size):
+                        ready.pop(k)
+                        self.used += r.footprint
+                        self.peak_used = max(self.peak_used, self.used)
+                        r.admitted = time.perf_counter() - t0
+                        self.admit_order.append(r.rid)
+                        self.lane_requests += 1
+                        self._lane_n[key] = self._lane_n.get(key, 0) + 1
+                        if key in self._lanes:
+                            self._lanes[key].append((r, state))
+                        else:
+                            self._lanes[key] = [(r, state)]
+                            threading.Thread(target=self._lane, args=(key, t0), daemon=True).start()
+                        continue
                     # first waiting request that fits, in arrival order (backfill)
-                    if self._running < self.gpu_workers and self.used + r.footprint <= self.budget:
+                    if key is None and self._running < self.gpu_workers and self.used + r.footprint <= self.budget:
                         ready.pop(k)
                         self.used += r.footprint
                         self.peak_used = max(self.peak_used, self.used)

@@ -55,6 +55,30 @@ def test_served_results_equal_results_alone():
         assert _same(a, b), a.rid
 
 
+LANE_SPECS = [(150, 200, 500, "VEL_ISO", 8), (180, 120, 400, "EL_ISO", 4), (90, 130, 420, "VEL_ISO", 8),
+              (120, 120, 450, "VEL_ISO", 4), (200, 100, 300, "VEL_ISO", 8), (64, 200, 350, "VEL_ISO", 4)]
+
+
+def test_worklist_lanes_give_results_alone():
+    """Small VEL_ISO requests served through work-list lanes (one lane per FD
+    order, joining at block boundaries of 64 steps while others run), the
+    EL_ISO request on a worker: every result equals the request alone."""
+    from gpi_project_amd import serve
+    td = tempfile.mkdtemp(prefix="sofi_serve_")
+    alone = _requests(td, LANE_SPECS)
+    serve.run_sequential(alone)
+    mix = _requests(td, LANE_SPECS)
+    for k, r in enumerate(mix):
+        r.arrival = 0.03 * k
+    srv = serve.Server(gpu_workers=2, worklist=True, lane_steps=64)
+    st = srv.serve(mix)
+    print(st, srv.admit_order, "in lanes:", srv.lane_requests)
+    assert all(r.error is None for r in mix), [(r.rid, r.error) for r in mix]
+    assert srv.lane_requests == sum(1 for s in LANE_SPECS if s[3] == "VEL_ISO") and srv.used == 0
+    for a, b in zip(alone, mix):
+        assert _same(a, b), a.rid
+
+
 def test_cancel_mid_flight_leaves_others_unchanged():
     from gpi_project_amd import serve
     td = tempfile.mkdtemp(prefix="sofi_serve_")

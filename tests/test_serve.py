@@ -118,3 +118,23 @@ def test_cancel_frees_memory_and_leaves_others_unchanged(when):
     assert st["completed"] == 2 and s.used == 0 and f.live == 0
     if when == "waiting":
         assert 1 not in s.admit_order
+
+
+def test_worklist_lane_selection():
+    """Which prepared requests a work-list server sends to a lane: one-shot
+    forward runs of at most lane_cells cells whose rheology runs one thread
+    per cell by default (AC_ISO 0, VEL_ISO 3, VEL_VTI 5), 2nd order in time;
+    everything else goes to a GPU worker."""
+    from types import SimpleNamespace as NS
+
+    def st(weq=3, nx=500, ny=400, nshots=1, mode=1, fdt=2, fd=8):
+        return NS(nshots=nshots, desc=NS(weq=weq, nx=nx, ny=ny, mode=mode, fdorder_time=fdt, fdorder=fd))
+
+    srv = Server(budget_bytes=1 << 30, worklist=True, lane_cells=1 << 20)
+    assert srv._lane_key(st()) == (8, 3)
+    assert srv._lane_key(st(weq=0, fd=4)) == (4, 0)
+    assert srv._lane_key(st(weq=5)) == (8, 5)
+    for bad in (st(weq=2), st(weq=4), st(weq=6), st(nx=2000, ny=1000), st(nshots=2), st(mode=0), st(fdt=4)):
+        assert srv._lane_key(bad) is None
+    assert Server(budget_bytes=1 << 30)._lane_key(st()) is None               # lanes off by default
+    assert Server(budget_bytes=1 << 30, worklist=True, launch="cell")._lane_key(st()) is None
