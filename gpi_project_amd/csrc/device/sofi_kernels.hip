@@ -3869,10 +3869,14 @@ int run_block(sofi_dev *h, int nt, int k)
 }
 
 // slots of a batch advanced together through a run's whole step range: the
-// largest group whose wavefields and the shared medium fit the budget of the
-// memory-side cache (256 MB Infinity Cache; 192 MB used), so that every step
-// of the group reads what the previous kernels left there; all slots when
-// even one does not fit (the medium is then shared by all of them instead)
+// largest group such that the shared medium and the wavefields of the groups
+// that run at once (one per stream) fit the memory-side cache (256 MB
+// Infinity Cache; 240 MB used), so that every step of a group reads what the
+// previous kernels left there; at least one slot; all slots when the medium
+// and one slot do not fit (the medium is then shared by all of them instead).
+// Measured on 16 EL_VTI shots of 2000 x 1000 FD6 on two streams: groups of 2
+// (233 MB at once) 70.5 GLUPS, of 3 (316 MB) 59.2, of 1 64.2; 16 EL_TTI
+// shots: groups of 1 (250 MB) 42.3, of 2 41.7, of 3 37.9.
 int resident_slots(const sofi_dev *h)
 {
     if (h->nb == 1) return 1;
@@ -3893,10 +3897,11 @@ int resident_slots(const sofi_dev *h)
       case SOFI_EL_TTI: arrays += 8; break;
       default: arrays += 8 + 8 * L; break;
     }
-    const size_t budget = (size_t)192 << 20, medium = arrays * pl;
+    const size_t budget = (size_t)240 << 20, medium = arrays * pl;
+    const size_t nst = h->str_mode > 0 ? (size_t)h->str_mode : 2;     // groups at once
     if (medium + slot > budget) return h->nb;
-    const size_t n = (budget - medium) / slot;
-    return n < (size_t)h->nb ? (int)n : h->nb;
+    const size_t n = (budget - medium) / (slot * nst);
+    return n < 1 ? 1 : n < (size_t)h->nb ? (int)n : h->nb;
 }
 
 // streams a batch's resident groups run on at once (1 when the batch is one

@@ -73,8 +73,10 @@ int sofi_get_batch_block(const sofi_dev *h);
 /* Resident groups of a batch: sofi_run advances `slots` slots through its
  * whole step range before the next group, so that a group's wavefields and
  * the shared medium stay in the 256 MB memory-side cache from one kernel to
- * the next (0, the default: the largest group that fits 192 MB, or all
- * slots when one slot does not fit).  The results do not depend on it.
+ * the next (0, the default: the largest group such that the medium and the
+ * groups that run at once on the batch's streams fit 240 MB, at least one
+ * slot; all slots when the medium and one slot do not fit).  The results do
+ * not depend on it.
  * sofi_get_batch_resident returns the group size the next run uses.  No
  * counterpart in the reference. */
 int sofi_set_batch_resident(sofi_dev *h, int slots);
