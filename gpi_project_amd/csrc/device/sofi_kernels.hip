@@ -4259,6 +4259,8 @@ namespace {
 // through the whole step range with the sampling state of the range's
 // start; groups go to S streams at once (the extra streams wait for the work
 // enqueued so far on the handle's stream, which waits for them at the end)
+constexpr int RUN_TURN = 32;
+
 int run_groups(sofi_dev *h, int nt_first, int nt_last)
 {
     const int G = resident_slots(h), lsamp0 = h->lsamp, ngroups = (h->nb + G - 1) / G;
@@ -4277,12 +4279,33 @@ int run_groups(sofi_dev *h, int nt_first, int nt_last)
         for (int q = 0; q < S - 1; ++q) HIPCHK(hipStreamWaitEvent(h->xs[q], h->evfork, 0));
     }
     int rc = 0;
-    for (int gi = 0; gi < ngroups && !rc; ++gi) {
-        h->cur_lo = gi * G;
-        h->cur_hi = h->cur_lo + G < h->nb ? h->cur_lo + G : h->nb;
-        h->cur = gi % S == 0 ? nullptr : h->xs[gi % S - 1];
-        h->lsamp = lsamp0;
-        rc = run_range(h, nt_first, nt_last);
+    if (S == 1) {
+        for (int gi = 0; gi < ngroups && !rc; ++gi) {
+            h->cur_lo = gi * G;
+            h->cur_hi = h->cur_lo + G < h->nb ? h->cur_lo + G : h->nb;
+            h->lsamp = lsamp0;
+            rc = run_range(h, nt_first, nt_last);
+        }
+    } else {
+        // the S groups that run at once (one per stream) are enqueued in turns
+        // of RUN_TURN steps each, so that no stream waits while the host
+        // enqueues another group's whole step range; each group keeps its own
+        // sampling state
+        std::vector<int> ls(ngroups, lsamp0);
+        for (int g0 = 0; g0 < ngroups && !rc; g0 += S) {
+            const int g1 = g0 + S < ngroups ? g0 + S : ngroups;
+            for (int a = nt_first; a <= nt_last && !rc; a += RUN_TURN) {
+                const int b = a + RUN_TURN - 1 < nt_last ? a + RUN_TURN - 1 : nt_last;
+                for (int gi = g0; gi < g1 && !rc; ++gi) {
+                    h->cur_lo = gi * G;
+                    h->cur_hi = h->cur_lo + G < h->nb ? h->cur_lo + G : h->nb;
+                    h->cur = gi % S == 0 ? nullptr : h->xs[gi % S - 1];
+                    h->lsamp = ls[gi];
+                    rc = run_range(h, a, b);
+                    ls[gi] = h->lsamp;
+                }
+            }
+        }
     }
     h->cur = nullptr;
     h->cur_lo = 0;

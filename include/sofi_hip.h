@@ -71,7 +71,8 @@ int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
 int sofi_set_batch_block(sofi_dev *h, int slots);
 int sofi_get_batch_block(const sofi_dev *h);
 /* Resident groups of a batch: sofi_run advances `slots` slots through its
- * whole step range before the next group, so that a group's wavefields and
+ * whole step range before the next group (the groups that run at once on the
+ * batch's streams in turns of 32 steps), so that a group's wavefields and
  * the shared medium stay in the 256 MB memory-side cache from one kernel to
  * the next (0, the default: the largest group such that the medium and the
  * groups that run at once on the batch's streams fit 240 MB, at least one
