@@ -304,6 +304,15 @@ struct Fuse {
     Rec rec;
 };
 
+// The explosive sources of the slots a slot-block stress launch updates,
+// added by the thread that updates the source's cell in the source's slot
+// after the update (psource, src/psource.c), in source order: slot k[w],
+// cell offset c[w] within the slot, source l[w] of the handle's list
+struct PFuse {
+    int n;
+    int k[FUSE_MAX], c[FUSE_MAX], l[FUSE_MAX];
+};
+
 // the time step of a launch: nt, or inside a captured block of steps the
 // block's local step nt plus its base step, which the device holds so that
 // one captured block serves every block of the run (the update kernels read
@@ -1810,7 +1819,8 @@ __global__ __launch_bounds__(BX *BY) void k_velocity_b(Grid g, Fields f0, Mat m,
 }
 
 template <int ND, int WEQ, int S>
-__global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, int nt, const int *ntb, int fwi_u)
+__global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, int nt, const int *ntb, int fwi_u, Src s,
+                                                     PFuse pf)
 {
     constexpr bool VISCO = WEQ == SOFI_VEL_ISO || WEQ == SOFI_VEL_VTI;
     const int t = blockIdx.x, ty = t / g.ntx, tx = t - ty * g.ntx;
@@ -1906,6 +1916,12 @@ __global__ __launch_bounds__(BX *BY) void k_stress_b(Grid g, Fields f0, Mat m, i
             visc_vti_final(g.dthalbe, c1, c2, c3, c4, sxyv, vxx, vyy, u, sx, sy, sz, fz, f, g.dt, c);
         }
         stress_post<ND, VISCO>(g, f, m, s0, fz, nt, i, j, c, sx, sy, sz);
+        for (int w = 0; w < pf.n; ++w)
+            if (pf.k[w] == k0 + q && pf.c[w] == c) {
+                const float amp = psource_amp(g, s, pf.l[w], step_of(ntb, nt));
+                sx += amp;
+                sy += amp;
+            }
         f.sxx[c] = sx;
         f.syy[c] = sy;
         f.sxy[c] = sz;
@@ -3518,6 +3534,27 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // as launches of their own
     const bool bm = h->nb > 1 && h->bs > 0 && !h->fdt4 && d.weq != SOFI_AC_ISO;
     const int span = hi0 - lo0, grp = !bm && h->grp > 0 && h->grp < span ? h->grp : span;
+    // slot blocks of ISO / VTI rheologies: the explosive sources of the
+    // slots (at most FUSE_MAX of them) ride in the stress launch
+    PFuse pf{};
+    bool pf_fused = false;
+    if (bm && !tti && nsrc > 0 && h->slot_fz.size() == (size_t)h->nb) {
+        pf_fused = true;
+        for (int k = lo0; k < hi0 && pf_fused; ++k) {
+            const Fuse &sf = h->slot_fz[k];
+            for (int q = 0; q < sf.npsrc; ++q) {
+                if (pf.n == FUSE_MAX || q >= FUSE_MAX) {
+                    pf_fused = false;
+                    break;
+                }
+                pf.k[pf.n] = k;
+                pf.c[pf.n] = sf.c[q];
+                pf.l[pf.n] = sf.soff + q;
+                ++pf.n;
+            }
+        }
+        if (!pf_fused) pf = PFuse{};
+    }
     const bool slot_fused = !bm && !tti && !h->fdt4 && h->slot_fusable && grp == 1 && nsrc > 0;
     fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
     // the free surface follows the explosive / moment sources, so it is fused
@@ -3602,10 +3639,10 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
                 (void)hipEventRecord(te[2], st);
             }
             switch (d.weq) {
-              case SOFI_EL_ISO: launch_blocks<ND, StrB<ND, SOFI_EL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
-              case SOFI_VEL_ISO: launch_blocks<ND, StrB<ND, SOFI_VEL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
-              case SOFI_EL_VTI: launch_blocks<ND, StrB<ND, SOFI_EL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
-              case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u); break;
+              case SOFI_EL_ISO: launch_blocks<ND, StrB<ND, SOFI_EL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u, h->s, pf); break;
+              case SOFI_VEL_ISO: launch_blocks<ND, StrB<ND, SOFI_VEL_ISO>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u, h->s, pf); break;
+              case SOFI_EL_VTI: launch_blocks<ND, StrB<ND, SOFI_EL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u, h->s, pf); break;
+              case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u, h->s, pf); break;
               default: {
                   Fuse f0{};
                   f0.fwi_u = fs.fwi_u;
@@ -3725,7 +3762,7 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         else launch(h, k_seis_dv<ND, false>, rg, rb, g, f, m, rr, ins, ntb, ndt);
     }
     if (nsrc > 0 && !src_fused && !slot_fused) {
-        if (psrc_inj.n > 0)
+        if (psrc_inj.n > 0 && !pf_fused)
             launch(h, k_psource, dim3((psrc_inj.n + 63) / 64), dim3(64), g, f, h->s, psrc_inj, nt, ntb);
         if (h->n_eq > 0)
             launch(h, k_eqsource, dim3(1), dim3(64), g, f, h->s, h->eq_list, h->n_eq, nt, ntb, lo0, hi0);
