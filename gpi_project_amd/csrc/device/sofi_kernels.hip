@@ -3555,6 +3555,10 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
         }
         if (!pf_fused) pf = PFuse{};
     }
+    const bool tti_pf = bm && tti && span == 1 && nsrc > 0 && h->slot_fz.size() == (size_t)h->nb &&
+                        h->slots[lo0].source_type == 1 && h->slot_fz[lo0].npsrc > 0 &&
+                        h->slot_fz[lo0].npsrc <= FUSE_MAX && (g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
+    pf_fused = pf_fused || tti_pf;
     const bool slot_fused = !bm && !tti && !h->fdt4 && h->slot_fusable && grp == 1 && nsrc > 0;
     fs.npsrc = src_fused && h->source_type == 1 ? nsrc : 0;
     // the free surface follows the explosive / moment sources, so it is fused
@@ -3644,11 +3648,21 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
               case SOFI_EL_VTI: launch_blocks<ND, StrB<ND, SOFI_EL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u, h->s, pf); break;
               case SOFI_VEL_VTI: launch_blocks<ND, StrB<ND, SOFI_VEL_VTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, nt, ntb, fs.fwi_u, h->s, pf); break;
               default: {
+                  // a group of one slot: its explosive sources ride in the
+                  // last stress kernel, as for a one-shot handle
                   Fuse f0{};
                   f0.fwi_u = fs.fwi_u;
                   f0.ntb = ntb;
-                  if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0, h->s, nt);
-                  else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, f0, h->s, nt);
+                  if (tti_pf) {
+                      const Fuse &sf = h->slot_fz[lo];
+                      memcpy(f0.c, sf.c, sizeof(sf.c));
+                      f0.soff = sf.soff;
+                      f0.npsrc = sf.npsrc;
+                  }
+                  Fuse fin = f0;
+                  if (g.abs_type == 1 && g.fw > 0 && frame_tile_count(g) > 0) fin.npsrc = 0;
+                  if (d.weq == SOFI_EL_TTI) launch_blocks<ND, TtiF<ND, SOFI_EL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, fin, h->s, nt);
+                  else launch_blocks<ND, TtiF<ND, SOFI_VEL_TTI>::template K>(h, gg, lo, gg.k_hi, S, f, m, fin, h->s, nt);
                   tti_frame<ND>(h, gg, lo, gg.k_hi, f, m, f0, nt);
               }
             }

@@ -105,7 +105,8 @@ def test_batch_slots_are_independent():
 ])
 def test_slot_blocks_match_each_other_and_one_shot_handles(weq, stype, fs, absn):
     """Slot blocks (all slots in one launch per kernel, 1, 2 or 4 slots per
-    thread): the three block sizes give the same bits, and every slot agrees
+    thread, or resident groups of 1 and 2 slots on two streams): all give the
+    same bits, and every slot agrees
     with a one-shot handle of its shot (a different kernel, whose compiled
     arithmetic contracts differently) to the 1e-4 parity bar; measured
     1.1e-5 at worst over 300 steps."""
@@ -127,9 +128,22 @@ def test_slot_blocks_match_each_other_and_one_shot_handles(weq, stype, fs, absn)
             B.run(98)
             got[S] = {(k, c): B.section_of(k, c) for k in range(n) for c in P.seismo_comps()}
             B.close()
+        # resident groups of 1 and 2 slots on two streams, enqueued in turns
+        # (one-slot TTI groups add their sources in the last stress kernel)
+        for res in (1, 2):
+            B = Batch(P, n)
+            B.set_block(1)
+            B.set_resident(res)
+            B.set_streams(2)
+            B.load_shots(list(range(1, n + 1)))
+            B.run(1, 97)
+            B.run(98)
+            got[("res", res)] = {(k, c): B.section_of(k, c) for k in range(n) for c in P.seismo_comps()}
+            B.close()
         P.close()
     for key, a in got[1].items():
         assert np.array_equal(a, got[2][key]) and np.array_equal(a, got[4][key]), (weq, key)
+        assert np.array_equal(a, got[("res", 1)][key]) and np.array_equal(a, got[("res", 2)][key]), (weq, key)
         k, c = key
         r = ref[k + 1][c]
         peak = max(float(np.abs(ref[k + 1][q]).max()) for q in ref[k + 1])
