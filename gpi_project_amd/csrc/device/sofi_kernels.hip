@@ -1530,6 +1530,8 @@ struct Shot {
     Mat m;
     Src s;
     Fuse fv, fs;
+    Rec r;          // its receivers (sampled by k_seis_w)
+    int seismo;
 };
 
 // The one-thread-per-cell update kernels over several one-shot handles on
@@ -1552,6 +1554,36 @@ __global__ __launch_bounds__(BX *TBY) void k_velocity_w(const Shot *__restrict__
         for (int r = 0, t = e.y; r < rows && t < last; ++r, t += d.g.ntx)
             velocity_tile<ND, true, NT, TBY>(d.g, d.f, d.m, d.s, d.fv, it, t);
     }
+}
+
+// The samplers of a work-list step, one launch for every handle that
+// samples in it: job q (blockIdx.y) samples the receivers of handle shot[q]
+// at sample index ins[q], div / curl (DV) or vx / vy / p (seismo_ssg,
+// src/seismo_ssg.c:40-64), as k_seis_dv / k_seis do for one handle.
+constexpr int SEIS_JOBS = 64;
+struct SeisJobs {
+    int shot[SEIS_JOBS], ins[SEIS_JOBS];
+};
+template <int ND, bool DV>
+__global__ void k_seis_w(const Shot *__restrict__ sh, SeisJobs jobs)
+{
+    const Shot &d = sh[jobs.shot[blockIdx.y]];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, ins = jobs.ins[blockIdx.y];
+    if (k >= d.r.n) return;
+    if (!DV) {
+        seis_sample(d.g, d.f, d.r, k, ins, d.seismo);
+        return;
+    }
+    const Grid &g = d.g;
+    const Fields f = d.f.at(d.r.shot[k], g);
+    const int i = d.r.i[k], j = d.r.j[k], c = IDX(g, i, j);
+    float vxx, vyx, vxy, vyy;
+    op_s<ND>(g, f.vx, f.vy, c, vxx, vyx, vxy, vyy);
+    const bool xf = in_xframe(g, i), yf = in_yframe(g, j);
+    if (g.abs_type == 1 && (xf || yf)) cpml_dv_read(g, d.m, f, i, j, xf, yf, vxx, vyx, vxy, vyy);
+    float *t = d.r.sec + d.r.off[k] + (ins - 1);
+    t[SOFI_C_DIV * d.r.cst[k]] = (float)((double)(vxx + vyy) * d.r.sq_ppi[k]);
+    t[SOFI_C_CURL * d.r.cst[k]] = (float)((double)(vxy - vyx) * d.r.sq_pu[k]);
 }
 
 template <int ND, int WEQ, int TBY>
@@ -2524,6 +2556,7 @@ struct sofi_dev {
     };
     bool wl = false;
     Shot wl_shot{};
+    int wl_dv = -1, wl_seis = -1;          // this step's div/curl and vx/vy/p sample index (-1: none)
     std::vector<Deferred> *defer = nullptr;
     int dstage = 0;
 };
@@ -3689,6 +3722,8 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
             h->wl_shot.s = h->s;
             h->wl_shot.fv = fv_l;
             h->wl_shot.fv.ntb = h->ntb;
+            h->wl_shot.r = h->r;
+            h->wl_shot.seismo = d.seismo;
             h->dstage = 1;
         } else if (vw == 4 && one) launch(h, k_velocity_v<ND, 4, true>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
         else if (vw == 4) launch(h, k_velocity_v<ND, 4, false>, tiles, blk, gg, fk, m, h->s, fv_l, nt);
@@ -3770,7 +3805,9 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     }
 #undef SOFI_STRESS
     if (te) (void)hipEventRecord(te[3], st);
-    if (sample_dv) {
+    if (sample_dv && h->wl) {
+        h->wl_dv = ins;                    // the work-list step's sampler launch
+    } else if (sample_dv) {
         dim3 rb(64), rg((rr.n + 63) / 64);
         if (tti) launch(h, k_seis_dv<ND, true>, rg, rb, g, f, m, rr, ins, ntb, ndt);
         else launch(h, k_seis_dv<ND, false>, rg, rb, g, f, m, rr, ins, ntb, ndt);
@@ -3791,7 +3828,9 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // the stresses' ghost columns wrap once the step's stresses are final
     // (exchange_s after the sources and the free surface, src/time_loop.c:273)
     wrap_x(h, g, lo0, hi0, d.weq == SOFI_AC_ISO ? nullptr : f.sxy, f.sxx);
-    if (sample && d.seismo != 3 && !seis_fused) {
+    if (sample && d.seismo != 3 && h->wl) {
+        h->wl_seis = ins;
+    } else if (sample && d.seismo != 3 && !seis_fused) {
         dim3 rb(64), rg((rr.n + 63) / 64);
         launch(h, k_seis, rg, rb, g, f, rr, ins, d.seismo, ntb, ndt);
     }
@@ -4562,6 +4601,7 @@ int many_step(ManyCtx &c, int k, ManyCfg cfg)
         h->wl = true;
         h->defer = &c.dl;
         h->dstage = 0;
+        h->wl_dv = h->wl_seis = -1;
         const int rc = step(h, c.first[p] + k);
         h->wl = false;
         h->defer = nullptr;
@@ -4595,11 +4635,36 @@ int many_step(ManyCtx &c, int k, ManyCfg cfg)
           default: many_launch<ND, 4>(c.st, weq, stress, grid, c.dtab, c.ditem, nit, cfg.rows, k + 1);
         }
     };
+    // the samplers of every handle that samples in this step, one launch each
+    // for div / curl (before the rest of the step, as k_seis_dv runs) and for
+    // vx / vy / p (after it, as k_seis runs)
+    auto sample = [&](bool dv) {
+        SeisJobs jobs{};
+        int nj = 0, nrec = 0;
+        for (int p = 0; p <= a; ++p) {
+            if (nj == SEIS_JOBS || (p == a && nj > 0)) {
+                const dim3 sg((unsigned)((nrec + 63) / 64), (unsigned)nj);
+                if (dv) hipLaunchKernelGGL((k_seis_w<ND, true>), sg, dim3(64), 0, c.st, c.dtab, jobs);
+                else hipLaunchKernelGGL((k_seis_w<ND, false>), sg, dim3(64), 0, c.st, c.dtab, jobs);
+                nj = nrec = 0;
+            }
+            if (p == a) break;
+            const sofi_dev *h = c.hs[p];
+            const int ins = dv ? h->wl_dv : h->wl_seis;
+            if (ins < 0 || h->r.n == 0) continue;
+            jobs.shot[nj] = p;
+            jobs.ins[nj] = ins;
+            nrec = std::max(nrec, h->r.n);
+            ++nj;
+        }
+    };
     issue(0);
     update(false);
     issue(1);
     update(true);
+    sample(true);
     issue(2);
+    sample(false);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess && c.err == hipSuccess) c.err = e;
     return 0;
