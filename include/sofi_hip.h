@@ -153,8 +153,9 @@ int sofi_run(sofi_dev *h, int nt_first, int nt_last);
  * kernels set to one thread per cell (sofi_set_launch), bit for bit, for any
  * launch, and the handle can go on with sofi_run afterwards.  The work runs
  * on the first handle's stream; the call waits for it and sets every
- * handle's sofi_last_run_ms to the whole run's time.  No counterpart in the
- * reference, which runs one shot after another (src/sofi2D.c:325-395). */
+ * handle's sofi_last_run_ms to the whole run's time (sofi_set_kernel_timing
+ * does not sample its steps).  No counterpart in the reference, which runs
+ * one shot after another (src/sofi2D.c:325-395). */
 int sofi_run_many(sofi_dev *const *h, int n, const int *nt_first, const int *nt_last, const int *launch);
 /* The launch of this thread's last sofi_run_many and whether it came from
  * the measurement or the cache (1) or was fixed (0). */
