@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 # (reference case, [(weq, nx, ny, fdorder, L, abs_type, free_surf, source_type, seismo, seed)]):
 # one FD order and rheology per set, grids of different sizes, both absorbing
 # frames, forces, moment tensors, free surface, every SEISMO, periodic x, the
-# acoustic path
+# acoustic path, FD orders 2 to 12
 SETS = {
     "el_fd6": (["el_fd6_h1_cpml_fs", "per_el_fd6_cpml_fs"],
                [("EL_ISO", 96, 200, 6, 0, 2, 0, 3, 1, 101), ("EL_ISO", 257, 75, 6, 0, 1, 1, 5, 2, 102)]),
@@ -32,6 +32,8 @@ SETS = {
     "elvti_fd6": (["elvti_fd6_cpml"], [("EL_VTI", 200, 100, 6, 0, 2, 0, 2, 1, 105)]),
     "velvti_fd4": (["velvti_l2_fd4_cpml"], [("VEL_VTI", 90, 150, 4, 2, 1, 0, 1, 2, 106)]),
     "ac_fd4": (["ac_fd4_cpml"], [("AC_ISO", 150, 90, 4, 0, 2, 0, 3, 1, 107), ("AC_ISO", 70, 180, 4, 0, 1, 0, 1, 2, 108)]),
+    "el_fd12": (["el_fd12_h4_cpml_fy_fs"], [("EL_ISO", 110, 170, 12, 0, 1, 0, 1, 4, 109)]),
+    "vel_fd2": (["vel_l1_fd2_cpml_fs"], [("VEL_ISO", 190, 80, 2, 1, 2, 0, 2, 2, 110)]),
 }
 
 
