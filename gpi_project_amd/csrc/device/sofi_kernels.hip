@@ -3563,8 +3563,9 @@ int step_nd(sofi_dev *h, int nt, const int *ntb)
     // 2, where a source cell would be skipped by every update kernel)
     const bool src_fused = h->fusable && h->source_type != 5 && (!tti || g.fw == 0 || g.abs_type == 1 || g.abs_type == 2);
     // a batch in slot blocks (h->bs slots per thread, all slots of the group
-    // in one launch per kernel); sources, free surface and sampling then run
-    // as launches of their own
+    // in one launch per kernel); forces, the free surface and sampling then
+    // run as launches of their own, explosive sources too unless the stress
+    // launch carries them (below)
     const bool bm = h->nb > 1 && h->bs > 0 && !h->fdt4 && d.weq != SOFI_AC_ISO;
     const int span = hi0 - lo0, grp = !bm && h->grp > 0 && h->grp < span ? h->grp : span;
     // slot blocks of ISO / VTI rheologies: the explosive sources of the

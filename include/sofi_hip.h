@@ -61,8 +61,9 @@ int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
  * cover all slots of the (resident) group in one launch each, one thread per
  * cell for `slots` consecutive slots (1, 2 or 4), the cell's material
  * coefficients read once for them and every slot's stencil loaded before any
- * slot is updated; sources, free surface and sampling run as launches of
- * their own.  TTI stress updates then compute the velocity derivatives in
+ * slot is updated; forces, free surface and sampling run as launches of
+ * their own, and so do explosive sources beyond 8 per group (fewer ride in
+ * the stress launch).  TTI stress updates then compute the velocity derivatives in
  * the kernel for the same slot blocks.  0 (the default) keeps the slot-loop
  * kernels of sofi_set_batch_launch.  Not for AC_ISO or 4th order in time
  * (the slot-loop kernels run).  The slot-block kernels are code of their
