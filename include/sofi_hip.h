@@ -63,10 +63,10 @@ int sofi_get_batch_launch(const sofi_dev *h, int *group, int *spt);
  * coefficients read once for them and every slot's stencil loaded before any
  * slot is updated; forces, free surface and sampling run as launches of
  * their own, and so do explosive sources beyond 8 per group (fewer ride in
- * the stress launch).  TTI stress updates then compute the velocity derivatives in
- * the kernel for the same slot blocks.  0 (the default) keeps the slot-loop
- * kernels of sofi_set_batch_launch.  Not for AC_ISO or 4th order in time
- * (the slot-loop kernels run).  The slot-block kernels are code of their
+ * the stress launch).  TTI stress updates then compute the velocity
+ * derivatives in the kernel for the same slot blocks.  0 (the default) keeps
+ * the slot-loop kernels of sofi_set_batch_launch.  Not for AC_ISO or 4th
+ * order in time (the slot-loop kernels run).  The slot-block kernels are code of their
  * own: their results agree with the slot-loop kernels' to rounding, not
  * always bit for bit.  No counterpart in the reference. */
 int sofi_set_batch_block(sofi_dev *h, int slots);
