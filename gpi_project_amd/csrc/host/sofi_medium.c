@@ -259,9 +259,10 @@ static int build_vel_iso(const sofi_params *p, sofi_model *m, float **para)
     for (int i = 1; i <= p->nxg; i++)
         for (int j = 1; j <= p->nyg; j++) {
             size_t k = (size_t)(i - 1) * p->nyg + (j - 1);
-            float vp = para[0][k], vs = para[1][k], r = para[2][k], qp = para[3][k], qs = para[4][k];
-            float tp = (float)(2.0 / (qp * L));
-            float ts = (float)(2.0 / (qs * L));
+            float vp = para[0][k], vs = para[1][k], r = para[2][k];
+            /* relaxation times from Q, or given (the internal model, para[5], para[6]) */
+            float tp = para[5] ? para[5][k] : (float)(2.0 / (para[3][k] * L));
+            float ts = para[6] ? para[6][k] : (float)(2.0 / (para[4][k] * L));
             float sumu = 0.0f, sumpi = 0.0f;
             for (int l = 1; l <= L; l++) {
                 sumu += ((ws * ws * pts[l] * pts[l] * ts) / (1.0 + ws * ws * pts[l] * pts[l]));
@@ -365,7 +366,13 @@ static void bond(float c11, float c13, float c33, float c55, float theta, float 
     *o35 = a5 * l23 * l1 - a6 * l2 * l13;
 }
 
-static int build_aniso(const sofi_params *p, sofi_model *m, float **para)
+/* The stiffness of an internal anisotropic model (READMOD = 0): the same in
+ * every cell; tau (visco) from the parameter TAU, theta in degrees (TTI). */
+typedef struct {
+    float c11, c13, c33, c55, rho, tau, theta;
+} sofi_builtin_aniso;
+
+static int build_aniso(const sofi_params *p, sofi_model *m, float **para, const sofi_builtin_aniso *bi)
 {
     int pitch_ = m->pitch, L = p->L, weq = p->weq;
     int visco = (weq == SOFI_VEL_VTI || weq == SOFI_VEL_TTI);
@@ -393,16 +400,34 @@ static int build_aniso(const sofi_params *p, sofi_model *m, float **para)
     for (int i = 1; i <= p->nxg; i++)
         for (int j = 1; j <= p->nyg; j++) {
             size_t k = (size_t)(i - 1) * p->nyg + (j - 1);
-            float vp = para[0][k], vs = para[1][k], r = para[2][k], eps = para[3][k], del = para[4][k];
+            float vp, vs, r, eps, del, a11, a13, a33, a55;
+            if (bi) {
+                /* velocities and Thomsen parameters of the given stiffness,
+                 * for the stability checks only */
+                r = bi->rho;
+                a11 = bi->c11;
+                a13 = bi->c13;
+                a33 = bi->c33;
+                a55 = bi->c55;
+                vp = (float)sqrt(a33 / r);
+                vs = (float)sqrt(a55 / r);
+                eps = (float)((a11 - a33) / (2.0 * a33));
+                del = (float)(((a13 + a55) * (double)(a13 + a55) - (a33 - a55) * (double)(a33 - a55)) /
+                              (2.0 * a33 * (a33 - a55)));
+            } else {
+                vp = para[0][k];
+                vs = para[1][k];
+                r = para[2][k];
+                eps = para[3][k];
+                del = para[4][k];
+                thomsen(r, vp, vs, eps, del, &a11, &a13, &a33, &a55);
+            }
             float vpmin, vpmax, vsmin, vsmax;
             aniso_minmax(m, vp, vs, eps, del, &vpmin, &vpmax, &vsmin, &vsmax);
-            float a11, a13, a33, a55;
-            thomsen(r, vp, vs, eps, del, &a11, &a13, &a33, &a55);
             float tau11 = 0, tau33 = 0, tau13 = 0, tau55 = 0;
             if (visco) {
-                float qp = para[iq][k], qs = para[iq + 1][k];
-                float tp = (float)(2.0 / (qp * L));
-                float ts = (float)(2.0 / (qs * L));
+                float tp = bi ? bi->tau : (float)(2.0 / (para[iq][k] * L));
+                float ts = bi ? bi->tau : (float)(2.0 / (para[iq + 1][k] * L));
                 float sumu = 0.0f, sumpi = 0.0f;
                 for (int l = 1; l <= L; l++) {
                     sumu += ((ws * ws * pts[l] * pts[l] * ts) / (1.0 + ws * ws * pts[l] * pts[l]));
@@ -427,7 +452,7 @@ static int build_aniso(const sofi_params *p, sofi_model *m, float **para)
             keep_model(m, i, j, vp, vs, r);
             if (tti) {
                 float o11, o13, o33, o55, o15, o35;
-                bond(a11, a13, a33, a55, para[it][k], &o11, &o13, &o33, &o55, &o15, &o35);
+                bond(a11, a13, a33, a55, bi ? bi->theta : para[it][k], &o11, &o13, &o33, &o55, &o15, &o35);
                 AT(c11, i, j) = o11;
                 AT(c33, i, j) = o33;
                 AT(c13, i, j) = o13;
@@ -659,6 +684,59 @@ static void build_absorb(const sofi_params *p, sofi_model *m)
 
 /* ------------------------------------------------------------- driver */
 
+/* ----------------------------------------- internal models (READMOD = 0)
+ *
+ * The reference's built-in test media, with their constants, handed to the
+ * same prepmod as a model read from files:
+ *   model_elastic      src/hh_elastic.c:37-88      a 100 m layer of vp 0,
+ *                      vs 1, rho 1 over vp 3500, vs 2000, rho 2000
+ *   model_visco        src/hh_visco.c:38-102       the same, tau_p = tau_s =
+ *                      0.01, and 2.0 for 95 m <= y <= 110 m
+ *   model_elastic_VTI  src/hh_elastic_VTI.c:39-72  c11 34.3, c33 22.7, c55
+ *                      5.4, c13 10.7 GPa, rho 2000 (Jones and Wang 1981)
+ *   model_visco_vti    src/hh_visco_VTI.c:41-123   the same, every tau = TAU
+ *   model_elastic_TTI  src/hh_elastic_TTI.c:42-104 the VTI medium rotated by 0
+ *   model_visco_tti    src/hh_visco_TTI.c:44-150   the visco VTI medium
+ *                      rotated by 30 degrees
+ * (their layer switches at j < -1 and y < -1 never fire). */
+static int build_internal(const sofi_params *p, sofi_model *m, float **para)
+{
+    const size_t n = (size_t)p->nxg * p->nyg;
+    const int vel = p->weq == SOFI_VEL_ISO;
+    if (p->weq == SOFI_EL_ISO || vel) {
+        for (int k = 0; k < 3; ++k) para[k] = (float *)malloc(n * sizeof(float));
+        if (vel) {
+            para[5] = (float *)malloc(n * sizeof(float));
+            para[6] = (float *)malloc(n * sizeof(float));
+        }
+        for (int k = 0; k < (vel ? 7 : 3); ++k)
+            if ((k < 3 || k >= 5) && !para[k]) return sofi_fail("out of memory building the internal model");
+        for (int i = 1; i <= p->nxg; i++)
+            for (int j = 1; j <= p->nyg; j++) {
+                const size_t k = (size_t)(i - 1) * p->nyg + (j - 1);
+                const float y = (float)j * p->dh;
+                const int top = y <= 100.0f;
+                para[0][k] = top ? 0.0f : 3500.0f;
+                para[1][k] = top ? 1.0f : 2000.0f;
+                para[2][k] = top ? 1.0f : 2000.0f;
+                if (vel) {
+                    const float t = (y >= 95.0f && y <= 110.0f) ? 2.0f : 0.01f;
+                    para[5][k] = t;
+                    para[6][k] = t;
+                }
+            }
+        return vel ? build_vel_iso(p, m, para) : build_el_iso(p, m, para);
+    }
+    sofi_builtin_aniso bi = { 34.3e9f, 10.7e9f, 22.7e9f, 5.4e9f, 2000.0f, p->tau, 0.0f };
+    switch (p->weq) {
+      case SOFI_EL_VTI:
+      case SOFI_VEL_VTI:
+      case SOFI_EL_TTI: return build_aniso(p, m, para, &bi);
+      case SOFI_VEL_TTI: bi.theta = 30.0f; return build_aniso(p, m, para, &bi);
+    }
+    return sofi_fail("Internal model for your chosen WEQ not implemented.");
+}
+
 int sofi_build_model(const sofi_params *p, sofi_model *m)
 {
     memset(m, 0, sizeof(*m));
@@ -680,6 +758,14 @@ int sofi_build_model(const sofi_params *p, sofi_model *m)
     }
     float *para[8] = { 0 };
     int rc = 0;
+    if (p->readmod == 0) {
+        rc = build_internal(p, m, para);
+        for (int k = 0; k < 8; ++k) free(para[k]);
+        if (rc) return rc;
+        if (p->fw > 0 && p->abs_type == 1) build_pml(p, m);
+        if (p->fw > 0 && p->abs_type == 2) build_absorb(p, m);
+        return 0;
+    }
     switch (p->weq) {
       case SOFI_AC_ISO: {
           const char *sfx[] = { "vp", "rho" };
@@ -702,25 +788,25 @@ int sofi_build_model(const sofi_params *p, sofi_model *m)
       case SOFI_EL_VTI: {
           const char *sfx[] = { "vp", "vs", "rho", "epsilon", "delta" };
           rc = read_model_files(p, 5, sfx, para);
-          if (!rc) rc = build_aniso(p, m, para);
+          if (!rc) rc = build_aniso(p, m, para, NULL);
           break;
       }
       case SOFI_VEL_VTI: {
           const char *sfx[] = { "vp", "vs", "rho", "epsilon", "delta", "qp", "qs" };
           rc = read_model_files(p, 7, sfx, para);
-          if (!rc) rc = build_aniso(p, m, para);
+          if (!rc) rc = build_aniso(p, m, para, NULL);
           break;
       }
       case SOFI_EL_TTI: {
           const char *sfx[] = { "vp", "vs", "rho", "epsilon", "delta", "theta" };
           rc = read_model_files(p, 6, sfx, para);
-          if (!rc) rc = build_aniso(p, m, para);
+          if (!rc) rc = build_aniso(p, m, para, NULL);
           break;
       }
       case SOFI_VEL_TTI: {
           const char *sfx[] = { "vp", "vs", "rho", "epsilon", "delta", "qp", "qs", "theta" };
           rc = read_model_files(p, 8, sfx, para);
-          if (!rc) rc = build_aniso(p, m, para);
+          if (!rc) rc = build_aniso(p, m, para, NULL);
           break;
       }
       default:

@@ -558,7 +558,13 @@ int sofi_check_supported(const sofi_params *p)
         return sofi_fail("Error in selection of FD coefficients: wrong FDORDER!");
     if (p->maxrelerror < 0 || p->maxrelerror > 4)
         return sofi_fail("Error in selection of FD coefficients: wrong choice of maximum relative error!");
-    if (p->readmod != 1) return sofi_fail("READMOD=0 (internal hh_* models) is not supported by this build.");
+    /* READMOD = 0: the internal models of src/hh_*.c (no acoustic one,
+     * src/readmod.c:60-84; under MODE=FWI the reference averages starting
+     * models that the internal models never set, src/readmod.c:86-97) */
+    if (p->readmod != 0 && p->readmod != 1) return sofi_fail("READMOD=%d: 0 (internal model) or 1 (model files).", p->readmod);
+    if (p->readmod == 0 && p->weq == SOFI_AC_ISO) return sofi_fail("Internal model for your chosen WEQ not implemented.");
+    if (p->readmod == 0 && p->mode == SOFI_MODE_FWI)
+        return sofi_fail("READMOD=0 with MODE=FWI: the internal models set no starting model for the gradient.");
     if (p->free_surf && (p->weq == SOFI_VEL_VTI || p->weq == SOFI_VEL_TTI))
         return sofi_fail("FREE_SURF with VEL_VTI/VEL_TTI dereferences unallocated tau arrays in the reference (src/surface.c:94-95); refused.");
     if (p->snap && p->snap_format != 2 && p->snap_format != 3)

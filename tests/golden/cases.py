@@ -166,6 +166,19 @@ PERIODIC_CASES = {
     "per_ac_fd4_cpml": ("AC_ISO", 4, 0, 0, 1, 0, 1, (), 2, 88),
 }
 
+# The reference's internal models (READMOD = 0, src/hh_*.c): a 100 m
+# near-vacuum layer over a half space (EL_ISO, VEL_ISO with a high-attenuation
+# band at 95-110 m) and a homogeneous Jones-Wang VTI medium, rotated by 0 and
+# 30 degrees for the TTI ones; TAU = 0.05 for the visco anisotropic ones.
+BUILTIN_CASES = {
+    "int_el_fd4_cpml": ("EL_ISO", 4, 0, 0, 1, 0, 1, (), 4, 91),
+    "int_vel_l2_fd6_abs": ("VEL_ISO", 6, 1, 2, 2, 0, 1, (), 4, 92),
+    "int_elvti_fd4_cpml": ("EL_VTI", 4, 0, 0, 1, 0, 1, (), 4, 93),
+    "int_velvti_l2_fd4_cpml": ("VEL_VTI", 4, 0, 2, 1, 0, 1, (), 4, 94),
+    "int_eltti_fd4_cpml": ("EL_TTI", 4, 0, 0, 1, 0, 1, (), 4, 95),
+    "int_veltti_l1_fd6_abs": ("VEL_TTI", 6, 1, 1, 2, 0, 1, (), 4, 96),
+}
+
 # EL_VTI/EL_TTI with a free surface: surface_elastic() reads pu (= c55) and
 # ppi (= c11 already multiplied by DT in prepmod), so the sxx correction is
 # off by ~1/DT and the run diverges; the reference aborts with "Simulation is
@@ -281,7 +294,8 @@ def decomposed_case(td: str, npx: int = 2, npy: int = 3):
 
 def case_def(name: str) -> tuple[W.Case, dict]:
     table = (CASES if name in CASES else FDT4_CASES if name in FDT4_CASES else AC_CASES if name in AC_CASES
-             else TOPO_CASES if name in TOPO_CASES else PERIODIC_CASES if name in PERIODIC_CASES else UNSTABLE)
+             else TOPO_CASES if name in TOPO_CASES else PERIODIC_CASES if name in PERIODIC_CASES
+             else BUILTIN_CASES if name in BUILTIN_CASES else UNSTABLE)
     weq, fdo, mre, L, absn, fs, st, extra, seismo, seed = table[name]
     model = _model(weq, seed, L=L)
     c = W.Case(name=name, nx=NX, ny=NY, dh=DH, dt=DT_CASE, time=NT_CASE * DT_CASE, weq=weq, fdorder=fdo,
@@ -293,6 +307,8 @@ def case_def(name: str) -> tuple[W.Case, dict]:
     if name in PERIODIC_CASES:
         src = [(10 * DH + 1.0, (NY // 2) * DH + 1.0, 0.0, 20.0, 1.0, *extra)]
         c = W.Case(**{**c.__dict__, "sources": src, "extra": {"BOUNDARY": 1}})
+    if name in BUILTIN_CASES:
+        c = W.Case(**{**c.__dict__, "extra": {"READMOD": 0, "TAU": 0.05}})
     return c, model
 
 

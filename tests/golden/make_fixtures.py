@@ -138,7 +138,7 @@ def ref1():
             print("ref1", t, flush=True)
     # coverage cases (FDORDER_TIME 2, then 4, the acoustic path, topography)
     for name in (list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES) + list(cases.TOPO_CASES) +
-                 list(cases.PERIODIC_CASES)):
+                 list(cases.PERIODIC_CASES) + list(cases.BUILTIN_CASES)):
         if os.path.exists(os.path.join(out, f"{name}.npz")):
             continue
         with tempfile.TemporaryDirectory() as td:

@@ -66,7 +66,7 @@ def test_tti_examples_vs_reference_1proc(name, examples_dir):
 
 @pytest.mark.parametrize("shape", ["cell", "vec2", "vec4"])
 @pytest.mark.parametrize("name", list(cases.CASES) + list(cases.AC_CASES) + list(cases.TOPO_CASES) +
-                         list(cases.PERIODIC_CASES))
+                         list(cases.PERIODIC_CASES) + list(cases.BUILTIN_CASES))
 def test_cases_vs_reference_1proc(name, shape):
     td = tempfile.mkdtemp(prefix="sofi_case_")
     js, c = cases.build(name, td)

@@ -23,7 +23,7 @@ BIN = os.path.join(ROOT, "oracle", "_ref", "sofi2D_ref_hip")
 MPIEXEC = "/opt/conda/bin/mpiexec"
 
 NAMES = (list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES) + list(cases.TOPO_CASES) +
-         list(cases.PERIODIC_CASES))
+         list(cases.PERIODIC_CASES) + list(cases.BUILTIN_CASES))
 
 
 def _read_su(path):

@@ -65,7 +65,7 @@ def test_tti_golden_depends_on_decomposition(examples_dir):
 
 
 @pytest.mark.parametrize("name", list(cases.CASES) + list(cases.FDT4_CASES) + list(cases.AC_CASES) +
-                         list(cases.TOPO_CASES) + list(cases.PERIODIC_CASES))
+                         list(cases.TOPO_CASES) + list(cases.PERIODIC_CASES) + list(cases.BUILTIN_CASES))
 def test_oracle_vs_reference_cases(name):
     td = tempfile.mkdtemp(prefix="sofi_case_")
     js, c = cases.build(name, td)
