@@ -4416,9 +4416,9 @@ int run_groups(sofi_dev *h, int nt_first, int nt_last)
 // kept for every later run of the handle and for every later batch of the
 // same rheology, FD order, grid, slot count and boundaries in the process.
 // Each candidate advances all slots by one step untimed (code loading, the
-// caches filled) and TUNE_STEPS timed steps.  The candidates run different
-// kernels (slot loops, slot blocks of 1, 2 or 4 slots), whose results agree
-// to rounding: the choice can change a result at that level.
+// caches filled) and TUNE_STEPS timed steps.  The candidates are of one
+// family of kernels (slot blocks; the slot-loop kernels for AC_ISO), whose
+// results are the same bits for every launch, so the choice changes no bit.
 
 struct TuneKey {
     int weq, nd, nx, ny, nb, abs_type, free_surf, L, periodic, mode;
@@ -4462,20 +4462,29 @@ int batch_tune(sofi_dev *h, int *nt, int nt_last)
     }
     const int nb = h->nb;
     std::vector<sofi_dev::BatchCfg> cand;
-    // the slot-loop kernels: one slot per launch in cache-sized resident
-    // groups on two streams; all slots per launch, up to four per thread
-    cand.push_back({ 0, 0, 2, 1, 1 });
-    cand.push_back({ 0, nb, 1, nb, nb >= 4 ? 4 : 1 });
+    // the candidates of one family of kernels, whose results are the same
+    // bits whatever the launch: slot blocks (1, 2 or 4 slots per thread) over
+    // all slots, or over cache-sized groups on two streams; for AC_ISO, which
+    // has no slot blocks, the slot-loop kernels (one slot per launch in
+    // cache-sized groups on two streams; all slots per launch, up to four
+    // per thread)
     if (d.weq != SOFI_AC_ISO) {
-        // slot blocks over all slots, or over cache-sized groups on two streams
         for (int bs : { 1, 2, 4 }) {
             if (bs > nb) continue;
             cand.push_back({ bs, nb, 1, 0, 0 });
             cand.push_back({ bs, 0, 2, 0, 0 });
         }
+    } else {
+        cand.push_back({ 0, 0, 2, 1, 1 });
+        cand.push_back({ 0, nb, 1, nb, nb >= 4 ? 4 : 1 });
     }
     const int need = (int)cand.size() * (TUNE_STEPS + 1);
-    if (nt_last - *nt + 1 < 2 * need) return 0;      // too short a run: the defaults
+    if (nt_last - *nt + 1 < 2 * need) {
+        // too short a run to time: the first candidate, so that the steps
+        // run the kernels of the family a later tuned run chooses from
+        if (h->bs != cand[0].bs) apply_cfg(h, cand[0]);
+        return 0;
+    }
     float best = 1e30f;
     sofi_dev::BatchCfg bestc = cand[0];
     for (const auto &c : cand) {

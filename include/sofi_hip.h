@@ -89,17 +89,19 @@ int sofi_get_batch_resident(const sofi_dev *h);
  * do not depend on it.  No counterpart in the reference. */
 int sofi_set_batch_streams(sofi_dev *h, int streams);
 /* The autotuner of a batch's launch (on by default): the first sofi_run of
- * at least 2 x 5 steps per candidate measures the candidates (slot-loop
- * kernels with one slot per launch in cache-sized resident groups on two
- * streams, or all slots per launch; slot blocks of 1, 2 or 4 slots per
- * thread over all slots or over cache-sized groups on two streams) on 5
- * steps each and keeps the fastest for the handle and for every later batch
- * of the same rheology, FD order, grid, slot count and boundaries in the
- * process.  Setting sofi_set_batch_block, or a non-zero value through
- * sofi_set_batch_launch / _resident / _streams, fixes the launch and turns
- * it off.  Its candidates run different kernels, so its choice can change
- * a result at the rounding level; fixing the launch gives bit-for-bit
- * repeatable runs.  sofi_get_batch_config returns
+ * at least 2 x 5 steps per candidate measures the candidates on 5 steps
+ * each and keeps the fastest for the handle and for every later batch of
+ * the same rheology, FD order, grid, slot count and boundaries in the
+ * process.  The candidates are slot blocks of 1, 2 or 4 slots per thread
+ * over all slots or over cache-sized resident groups on two streams; for
+ * AC_ISO, which has no slot blocks, the slot-loop kernels with one slot per
+ * launch in cache-sized groups on two streams, or all slots per launch.
+ * They are one family of kernels whose results are the same bits for every
+ * launch, so the choice changes no bit; a shorter sofi_run of a batch not
+ * yet tuned runs the first candidate.  Setting sofi_set_batch_block, or a
+ * non-zero value through sofi_set_batch_launch / _resident / _streams,
+ * fixes the launch and turns the tuner off (the default launch then is the
+ * slot-loop kernels).  sofi_get_batch_config returns
  * {slots per thread of the slot blocks (0: slot-loop kernels), resident
  * group, streams, slots per launch, slots per thread, tuned (0 / 1)}.  No
  * counterpart in the reference. */

@@ -151,3 +151,42 @@ def test_slot_blocks_match_each_other_and_one_shot_handles(weq, stype, fs, absn)
             assert float(np.abs(a - r).max()) < 1e-6 * peak
         else:
             assert rel_l2(a, r) <= 1e-4, (weq, key, rel_l2(a, r))
+
+
+@pytest.mark.parametrize("weq", ["EL_VTI", "VEL_ISO", "AC_ISO"])
+def test_tuner_choice_changes_no_bit(weq):
+    """The autotuner's candidates are one family of kernels (slot blocks; the
+    slot-loop kernels for AC_ISO, which has no slot blocks): a tuned batch
+    whose run opens with a call too short to tune (which runs the first
+    candidate) equals a batch of a fixed launch of that family bit for bit,
+    whatever the tuner chose."""
+    from gpi_project_amd.sofi import Batch, Problem
+    td = tempfile.mkdtemp(prefix="sofi_tune_")
+    n = 3
+    js = cases.multishot_case(td, weq, n, 1, 0, 1, seismo=2 if weq == "AC_ISO" else 4)
+    with chdir(td):
+        P = Problem(os.path.basename(js))
+        got, cfg = {}, {}
+        for mode in ("tuned", "fixed"):
+            B = Batch(P, n)
+            if mode == "fixed":
+                if weq == "AC_ISO":
+                    B.set_tuning(False)     # the default slot-loop launch
+                else:
+                    B.set_block(2)          # slot blocks of 2 over all slots
+            B.load_shots(list(range(1, n + 1)))
+            B.run(1, 9)                     # too short to time the candidates
+            short = B.config
+            B.run(10)
+            cfg[mode] = (short, B.config)
+            got[mode] = {(k, c): B.section_of(k, c) for k in range(n) for c in P.seismo_comps()}
+            B.close()
+        P.close()
+    print(weq, cfg)
+    short, full = cfg["tuned"]
+    assert not short["tuned"] and full["tuned"]
+    assert (short["slot_block"] > 0) == (weq != "AC_ISO") and (full["slot_block"] > 0) == (weq != "AC_ISO")
+    assert all(max(float(np.abs(got["tuned"][(k, c)]).max()) for c in range(5) if (k, c) in got["tuned"]) > 0
+               for k in range(n))
+    for key, a in got["tuned"].items():
+        assert np.array_equal(a, got["fixed"][key]), (weq, key, float(np.abs(a - got["fixed"][key]).max()))
