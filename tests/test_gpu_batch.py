@@ -184,7 +184,7 @@ def test_tuner_choice_changes_no_bit(weq):
         P.close()
     print(weq, cfg)
     short, full = cfg["tuned"]
-    assert not short["tuned"] and full["tuned"]
+    assert full["tuned"]        # (the short call is tuned too when an earlier batch of this key was)
     assert (short["slot_block"] > 0) == (weq != "AC_ISO") and (full["slot_block"] > 0) == (weq != "AC_ISO")
     assert all(max(float(np.abs(got["tuned"][(k, c)]).max()) for c in range(5) if (k, c) in got["tuned"]) > 0
                for k in range(n))
